@@ -1,0 +1,284 @@
+// assemble.hip -- influence-matrix assembly, right-hand sides and force
+// reductions for one frequency on gfx950.
+//
+// Assembly is one thread per (field i, source j) pair: a 256-thread block is
+// four wave64s; each wave owns one source panel j (wave-uniform, so the
+// source geometry and Newman frame are scalar loads) and 64 consecutive
+// field panels i, so each wave writes 64 consecutive complex128 entries of
+// column j of A and of S (column-major, coalesced 1 KiB stores).  The pair
+// loop of CALGREEN_MULTI and the entry formulas of BODINT_MULTI_LEFT /
+// RBC_MULTI_RIGHT are fused: no Green-function intermediates reach HBM.
+#include "green_dev.hpp"
+#include "hamsx_internal.hpp"
+
+namespace hamsx {
+
+// FLAG = DIST <= 50*PNSZ(J) (CalGreenFuncMulti.f90:134-139), formed without
+// FMA contraction so the decision matches the FP64 reference bit for bit.
+__device__ __forceinline__ int near_flag(double xi, double yi, double zi, double xj, double yj, double zj, double pn) {
+#pragma clang fp contract(off)
+  double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+  double dist = sqrt(dx * dx + dy * dy + dz * dz);
+  return dist <= 50.0 * pn ? 1 : 0;
+}
+
+template <bool DEEP>
+__global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2* __restrict__ A,
+                                                  double2* __restrict__ S, int* flags) {
+  const int n = m.n;
+  const int lane = threadIdx.x & 63;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
+  const int i = blockIdx.x * 64 + lane;
+  if (j >= n || i >= n) return;
+  const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
+  const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
+  double sij = 0.0, dij[3] = {0.0, 0.0, 0.0};
+  if (flag) d_newman(m.frame + (size_t)kFrameLen * j, m.ncn[j], xi, yi, zi, sij, dij);
+  PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
+  cd g[4];
+  int ovf = 0;
+  if (DEEP) {
+    d_infgreen(p, f.v, flag, g);
+  } else {
+    FreqParams fp{f.v, f.wk, f.w1, f.h, f.wvn};
+    d_fingreen(p, fp, flag, g, &ovf);
+  }
+  if (ovf) atomicOr(flags, 1);
+  const double ear = m.area[j], n1 = m.nrm[j], n2 = m.nrm[n + j], n3 = m.nrm[2 * n + j];
+  cd t, s;
+  if (flag) {
+    t = (mk(dij[0]) + g[1] * ear) * n1 + (mk(dij[1]) + g[2] * ear) * n2 + (mk(dij[2]) + g[3] * ear) * n3;
+    s = mk(sij) + g[0] * ear;
+  } else {
+    t = (g[1] * n1 + g[2] * n2 + g[3] * n3) * ear;
+    s = g[0] * ear;
+  }
+  if (i == j) t.r = 2.0 * kPi + t.r;  // AMAT(IEL,IEL) = 2*PI before the sum (AssbMatxMulti.f90:72-74)
+  const size_t o = (size_t)i + (size_t)n * j;
+  A[o] = make_double2(t.r, t.i);
+  S[o] = make_double2(s.r, s.i);
+}
+
+// ---------------------------------------------------------------------------
+// Incident wave (PatclVelct.f90:42-123).  DINP returns the velocity
+// components divided by CI; VINP the potential divided by CI.
+// ---------------------------------------------------------------------------
+__device__ inline void d_wave_consts(const FreqDev& f, double& w0, double& wk0) {
+  if (fabs(f.v) < HX_F(1.0e-8)) {
+    w0 = HX_F(1.0e-20);
+    wk0 = HX_F(1.0e-20);
+  } else if (fabs(f.v + 1.0) < HX_F(1.0e-8)) {
+    w0 = HX_F(1.0e20);
+    wk0 = HX_F(1.0e20);
+  } else {
+    w0 = f.w1;
+    wk0 = f.wk;
+  }
+}
+
+__device__ inline void d_dinp(double x, double y, double z, const FreqDev& f, const WaveDev& w, double beta, cd out[3]) {
+  double w0, wk0;
+  d_wave_consts(f, w0, wk0);
+  double sb, cb;
+  sincos(beta, &sb, &cb);
+  double wkx = wk0 * ((x - w.xw0) * cb + (y - w.xw1) * sb);
+  double se, ce;
+  sincos(wkx, &se, &ce);
+  double th, tv;
+  if (f.h < 0.0) {
+    th = exp(wk0 * z);
+    tv = th;
+  } else {
+    double sh = sinh(wk0 * f.h);
+    th = cosh(wk0 * (z + f.h)) / sh;
+    tv = sinh(wk0 * (z + f.h)) / sh;
+  }
+  double ah = w0 * w.amp * th, av = w0 * w.amp * tv;
+  // (a e^{i wkx}) / i = a (sin, -cos)
+  out[0] = cd{ah * se * cb, -(ah * ce * cb)};
+  out[1] = cd{ah * se * sb, -(ah * ce * sb)};
+  // (-i a e^{i wkx}) / i = -a e^{i wkx}
+  out[2] = cd{-(av * ce), -(av * se)};
+}
+
+__device__ inline cd d_vinp(double x, double y, double z, const FreqDev& f, const WaveDev& w, double beta) {
+  double w0, wk0;
+  d_wave_consts(f, w0, wk0);
+  double sb, cb;
+  sincos(beta, &sb, &cb);
+  double wkx = wk0 * ((x - w.xw0) * cb + (y - w.xw1) * sb);
+  double se, ce;
+  sincos(wkx, &se, &ce);
+  if (f.h < 0.0 || f.h > 500.0) {
+    // (-i AMP G / W0 e^{wk0 z} e^{i wkx}) / i
+    double a = -(w.amp * kG / w0 * exp(wk0 * z));
+    return cd{a * ce, a * se};
+  }
+  double dum = -w.amp * w0 / (wk0 * sinh(wk0 * f.h));
+  double a = dum * cosh(wk0 * (z + f.h));
+  return cd{a * ce, a * se};  // (i a e^{i wkx}) / i
+}
+
+// dum2(j, beta) = grad(phi_I) . n at panel j (DBC_MULTI_RIGHT, BodyIntgrMulti.f90:150-163)
+__global__ void k_dum2(MeshDev m, FreqDev f, WaveDev w, double2* dum2) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (j >= m.n) return;
+  cd d[3];
+  d_dinp(m.cx[j], m.cy[j], m.cz[j], f, w, w.beta[b], d);
+  cd v = d[0] * m.nrm[j] + d[1] * m.nrm[m.n + j] + d[2] * m.nrm[2 * m.n + j];
+  dum2[(size_t)b * m.n + j] = make_double2(v.r, v.i);
+}
+
+// Partial products of S with the generator columns over one chunk of source
+// panels (all in one body): 6 radiation columns of that body (n_1..n_6,
+// ASSB_RBC_MULTI) and nbeta diffraction columns (-dum2, ASSB_DBC_MULTI).
+constexpr int kBetaGroup = 8;
+__global__ __launch_bounds__(64) void k_rhs_partial(int n, const double* __restrict__ nrm, const double2* __restrict__ S,
+                                                    const double2* __restrict__ dum2, int nbeta, const int* chunk_j0,
+                                                    const int* chunk_j1, double2* __restrict__ partial) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.y;
+  const int g0 = blockIdx.z * kBetaGroup;
+  if (i >= n) return;
+  const int j0 = chunk_j0[c], j1 = chunk_j1[c];
+  const int nslot = 6 + nbeta;
+  cd rad[6], dif[kBetaGroup];
+  for (int k = 0; k < 6; k++) rad[k] = mk(0.0);
+  for (int k = 0; k < kBetaGroup; k++) dif[k] = mk(0.0);
+  const int ng = min(kBetaGroup, nbeta - g0);
+  for (int j = j0; j < j1; j++) {
+    double2 sv = S[(size_t)i + (size_t)n * j];
+    cd s{sv.x, sv.y};
+    if (blockIdx.z == 0)
+      for (int k = 0; k < 6; k++) rad[k] = rad[k] + s * nrm[(size_t)k * n + j];
+    for (int b = 0; b < ng; b++) {
+      double2 dv = dum2[(size_t)(g0 + b) * n + j];
+      dif[b] = dif[b] - s * cd{dv.x, dv.y};
+    }
+  }
+  double2* out = partial + (size_t)c * nslot * n;
+  if (blockIdx.z == 0)
+    for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = make_double2(rad[k].r, rad[k].i);
+  for (int b = 0; b < ng; b++) out[(size_t)(6 + g0 + b) * n + i] = make_double2(dif[b].r, dif[b].i);
+}
+
+// Sum the chunk partials in chunk order: X(:, 6b+m) over the chunks of body b,
+// X(:, 6nb+beta) over all chunks.
+__global__ void k_rhs_reduce(int n, int nbody, int nbeta, int nchunk, const int* chunk_body,
+                             const double2* __restrict__ partial, double2* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= n) return;
+  const int nslot = 6 + nbeta;
+  double re = 0.0, im = 0.0;
+  if (col < 6 * nbody) {
+    int b = col / 6, slot = col % 6;
+    for (int c = 0; c < nchunk; c++)
+      if (chunk_body[c] == b) {
+        double2 v = partial[((size_t)c * nslot + slot) * n + i];
+        re += v.x;
+        im += v.y;
+      }
+  } else {
+    int slot = 6 + (col - 6 * nbody);
+    for (int c = 0; c < nchunk; c++) {
+      double2 v = partial[((size_t)c * nslot + slot) * n + i];
+      re += v.x;
+      im += v.y;
+    }
+  }
+  X[(size_t)col * n + i] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// Force reductions (RFORCE_MULTI / EFORCE_MULTI, PotentWavForceMulti.f90:45-226).
+// One block per (body M, potential column r); 256 threads reduce over the
+// body's panels in a fixed order.
+//   r < 6nb : AMAS(6M+k, r) = sum Im(i rho phi DS n_k),
+//             BDMP(6M+k, r) = -sum Re(i rho w phi DS n_k)
+//   r >= 6nb: EXFC(6M+k, beta) = i w rho sum (phi_D [+ phi_I]) DS n_k
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_forces(MeshDev m, FreqDev f, WaveDev w, int nbody, const int* body_start,
+                                                const double2* __restrict__ X, double* __restrict__ out) {
+  __shared__ double red[12][256];
+  const int bm = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
+  const int n = m.n, nd = 6 * nbody;
+  const bool rad = r < nd;
+  double acc[12];
+  for (int k = 0; k < 12; k++) acc[k] = 0.0;
+  for (int e = body_start[bm] + t; e < body_start[bm + 1]; e += 256) {
+    double2 xv = X[(size_t)r * n + e];
+    cd phi{xv.x, xv.y};
+    if (!rad && w.isol == 1) phi = phi + d_vinp(m.cx[e], m.cy[e], m.cz[e], f, w, w.beta[r - nd]);
+    const double ds = m.area[e];
+    for (int k = 0; k < 6; k++) {
+      double nk = m.nrm[(size_t)k * n + e];
+      if (rad) {
+        acc[k] += w.rho * phi.r * ds * nk;            // Im(i rho phi DS n_k)
+        acc[6 + k] += w.rho * f.w1 * phi.i * ds * nk;  // -Re(i rho w phi DS n_k)
+      } else {
+        acc[k] += phi.r * ds * nk;
+        acc[6 + k] += phi.i * ds * nk;
+      }
+    }
+  }
+  for (int k = 0; k < 12; k++) red[k][t] = acc[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < 12; k++) red[k][t] += red[k][t + s];
+    __syncthreads();
+  }
+  if (t < 6) {
+    const int row = 6 * bm + t;
+    if (rad) {
+      out[(size_t)row + (size_t)nd * r] = red[t][0];
+      out[(size_t)nd * nd + row + (size_t)nd * r] = red[6 + t][0];
+    } else {
+      // i w rho (a + i b) = (-w rho b, w rho a)
+      const int beta = r - nd;
+      const double wr = f.w1 * w.rho;
+      double* ex = out + 2 * (size_t)nd * nd + 2 * ((size_t)row + (size_t)nd * beta);
+      ex[0] = -(wr * red[6 + t][0]);
+      ex[1] = wr * red[t][0];
+    }
+  }
+}
+
+__global__ void k_clear_flag(int* flags) { flags[0] = 0; }
+
+namespace launch {
+
+void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags) {
+  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
+  if (f.h < 0.0)
+    hipLaunchKernelGGL(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
+  else
+    hipLaunchKernelGGL(k_assemble<false>, grid, dim3(256), 0, s, m, f, A, S, flags);
+}
+
+void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
+         const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
+         const int* chunk_body_d, const int* chunk_j0_d, const int* chunk_j1_d, double2* X) {
+  (void)body_start_h;
+  (void)body_start_d;
+  const int n = m.n;
+  if (w.nbeta > 0) hipLaunchKernelGGL(k_dum2, dim3((n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, dum2);
+  int ngroups = w.nbeta > 0 ? (w.nbeta + kBetaGroup - 1) / kBetaGroup : 1;
+  hipLaunchKernelGGL(k_rhs_partial, dim3((n + 63) / 64, nchunk, ngroups), dim3(64), 0, s, n, m.nrm, S, dum2,
+                     w.nbeta, chunk_j0_d, chunk_j1_d, partial);
+  hipLaunchKernelGGL(k_rhs_reduce, dim3((n + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, n, nbody, w.nbeta,
+                     nchunk, chunk_body_d, partial, X);
+}
+
+void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
+            const double2* X, double* out) {
+  hipLaunchKernelGGL(k_forces, dim3(nbody, 6 * nbody + w.nbeta), dim3(256), 0, s, m, f, w, nbody, body_start_d, X,
+                     out);
+}
+
+}  // namespace launch
+}  // namespace hamsx

@@ -1,0 +1,752 @@
+// hamsx_api.cpp -- the C-ABI of include/hamsx.h.
+//
+// Host side: panel geometry (NormalProcess.f90), wave properties and the
+// dispersion roots (ImplementSubs.f90, WavDynSubs.f90), the Newman local
+// frames of every source panel, and the orchestration of one frequency
+// solve on a context's HIP stream.  Device side: assemble.hip, lu.hip.
+#include "hamsx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hamsx_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HX_TRY(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(HAMSX_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr double kPi = 3.141592653589793;   // Const_mod
+constexpr double kG = 9.80665;              // Const_mod
+constexpr double kGDisp = 9.807;            // DISPERSION's own gravity (WavDynSubs.f90:58)
+constexpr int kFrameLen = 24;
+
+double ipow(double a, int b) {  // REAL**INTEGER in flang's square-and-multiply order
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b >>= 1;
+    if (b == 0) break;
+    a *= a;
+  }
+  return r;
+}
+
+// DISPERSION (WavDynSubs.f90:40-127): first root k tanh(kh) = w^2/g by a
+// Newton-Halley iteration on y = kh, then the evanescent roots.
+void dispersion(double w, double h, int nrt, double* wvn) {
+  const double x = w * w * h / kGDisp;
+  double y, t;
+  if (x > 0.0 && x <= 2.0) {
+    y = std::sqrt(x) * (0.9994 + 0.1701 * x + (double)0.0305f * x * x);
+  } else {
+    t = x * std::exp(-2.0 * x);
+    y = x + 2.0 * t - 6.0 * t * t;
+  }
+  if (std::fabs(y - x) < (double)1.0e-10f) {
+    wvn[0] = x / h;
+  } else {
+    double exx = 1.0;
+    while (exx > 1.0e-10) {
+      double trial = y, dnm = trial * trial - x * x;
+      double fun = std::log((trial + x) / (trial - x)) / 2.0 - trial;
+      double dfun = -x / dnm - 1.0, d2fun = 2.0 * x * trial / (dnm * dnm);
+      y = trial - fun / dfun * (1.0 + (fun / dfun) * (d2fun / dfun) / 2.0);
+      exx = std::fabs(y - trial);
+    }
+    wvn[0] = y / h;
+  }
+  double u = 3.0 * x / (7.0 + 3.0 * x);
+  t = 0.0159 + 0.1032 * u + 4.3152 * u * u - 2.8768 * u * u * u;
+  for (int i = 2; i <= nrt; i++) {
+    double exx = 1.0;
+    while (exx > 1.0e-10) {
+      double trial = t;
+      y = (i - 1) * kPi - trial;
+      double dnm = y * y + x * x, fun = std::atan2(x, y) - trial;
+      double dfun = x / dnm - 1.0, d2fun = 2.0 * x * trial / (dnm * dnm);
+      t = trial - fun / dfun * (1.0 + (fun / dfun) * (d2fun / dfun) / 2.0);
+      exx = std::fabs(t - trial);
+    }
+    y = (i - 1) * kPi - t;
+    wvn[i - 1] = y / h;
+    t = t - kPi * x / (x * x + kPi * i * (kPi * (i - 1) - t));
+  }
+}
+
+double norm3(const double* v) { return std::sqrt(ipow(v[0], 2) + ipow(v[1], 2) + ipow(v[2], 2)); }
+
+// Newman local frame of one panel (SingularIntgrMulti.f90:130-181 triangle,
+// :326-400 quadrilateral): origin, unit vectors, projected vertices.
+void newman_frame(int ncn, const double* xv /* ncn x 3 */, double* fr) {
+  std::fill(fr, fr + kFrameLen, 0.0);
+  double *po = fr, *ui = fr + 3, *uj = fr + 6, *uk = fr + 9, *ksi = fr + 12, *eta = fr + 17;
+  if (ncn == 3) {
+    double p12[3], p13[3], po3[3], po1[3], po2[3];
+    for (int d = 0; d < 3; d++) {
+      p12[d] = xv[3 + d] - xv[d];
+      p13[d] = xv[6 + d] - xv[d];
+    }
+    double n12 = ipow(p12[0], 2) + ipow(p12[1], 2) + ipow(p12[2], 2);
+    double alfa = (p12[0] * p13[0] + p12[1] * p13[1] + p12[2] * p13[2]) / n12;
+    for (int d = 0; d < 3; d++) po[d] = xv[d] + alfa * p12[d];
+    for (int d = 0; d < 3; d++) po3[d] = xv[6 + d] - po[d];
+    double l12 = std::sqrt(n12), l3 = norm3(po3);
+    for (int d = 0; d < 3; d++) {
+      ui[d] = p12[d] / l12;
+      uj[d] = po3[d] / l3;
+    }
+    uk[0] = ui[1] * uj[2] - ui[2] * uj[1];
+    uk[1] = ui[2] * uj[0] - ui[0] * uj[2];
+    uk[2] = ui[0] * uj[1] - ui[1] * uj[0];
+    for (int d = 0; d < 3; d++) {
+      po1[d] = xv[d] - po[d];
+      po2[d] = xv[3 + d] - po[d];
+    }
+    ksi[0] = -std::copysign(1.0, alfa) * norm3(po1);
+    eta[0] = 0.0;
+    ksi[1] = 0.0;
+    eta[1] = norm3(po3);
+    ksi[2] = norm3(po2);
+    eta[2] = 0.0;
+    ksi[3] = ksi[0];
+    eta[3] = eta[0];
+    return;
+  }
+  double pov[4][3], pon[4][3], un[3];
+  for (int d = 0; d < 3; d++) po[d] = (xv[d] + xv[3 + d] + xv[6 + d] + xv[9 + d]) / 4.0;
+  for (int c = 0; c < 4; c++)
+    for (int d = 0; d < 3; d++) pov[c][d] = xv[3 * c + d] - po[d];
+  un[0] = pov[0][1] * pov[1][2] - pov[0][2] * pov[1][1];
+  un[1] = pov[0][2] * pov[1][0] - pov[0][0] * pov[1][2];
+  un[2] = pov[0][0] * pov[1][1] - pov[0][1] * pov[1][0];
+  double li = norm3(pov[0]), lk = norm3(un);
+  for (int d = 0; d < 3; d++) {
+    ui[d] = pov[0][d] / li;
+    uk[d] = un[d] / lk;
+  }
+  uj[0] = uk[1] * ui[2] - uk[2] * ui[1];
+  uj[1] = uk[2] * ui[0] - uk[0] * ui[2];
+  uj[2] = uk[0] * ui[1] - uk[1] * ui[0];
+  for (int c = 0; c < 4; c++) {
+    double dn = uk[0] * pov[c][0] + uk[1] * pov[c][1] + uk[2] * pov[c][2];
+    for (int d = 0; d < 3; d++) pon[c][d] = pov[c][d] - dn * uk[d];
+  }
+  for (int c = 0; c < 4; c++) {
+    ksi[c] = ui[0] * pon[c][0] + ui[1] * pon[c][1] + ui[2] * pon[c][2];
+    eta[c] = uj[0] * pon[c][0] + uj[1] * pon[c][1] + uj[2] * pon[c][2];
+  }
+  ksi[4] = ksi[0];
+  eta[4] = eta[0];
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {
+    if (count <= n && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return 0;
+    if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) {
+      p = nullptr;
+      return fail(HAMSX_ENOMEM, "hipMalloc of %zu bytes failed", sizeof(T) * count);
+    }
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct hamsx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int prec = HAMSX_PREC_FP64;
+  int refine_iters = 3;
+  // mesh
+  int n = 0, nbody = 0, nchunk = 0;
+  std::vector<int> body_start;
+  DevBuf<double> cx, cy, cz, area, pnsz, nrm, frame;
+  DevBuf<int> ncn, body, body_start_d, chunk_body, chunk_j0, chunk_j1;
+  // frequency and waves
+  hamsx::FreqDev freq{};
+  bool have_freq = false;
+  DevBuf<double> wvn, beta;
+  double* h_pinned = nullptr;  // staging: wvn (NK) + beta + results
+  size_t h_pinned_len = 0;
+  int nbeta = 0;
+  double amp = 1.0, xw0 = 0.0, xw1 = 0.0, rho = 1000.0;
+  int isol = 1;
+  bool have_waves = false;
+  // matrices and workspace
+  DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial;
+  DevBuf<float2> Af;
+  DevBuf<int> ipiv, perm, widx, wcnt, flags;
+  DevBuf<double> out, norms;
+  bool have_A = false, have_S = false, have_LU = false, have_X = false;
+  int last_iters = 0;
+  double last_resid = 0.0;
+
+  hamsx::MeshDev mesh() const {
+    return hamsx::MeshDev{n, cx.p, cy.p, cz.p, area.p, pnsz.p, nrm.p, ncn.p, frame.p, body.p};
+  }
+  hamsx::WaveDev waves() const { return hamsx::WaveDev{nbeta, beta.p, amp, xw0, xw1, rho, isol}; }
+  int nrhs() const { return 6 * nbody + nbeta; }
+  size_t out_len() const { return 2 * (size_t)(6 * nbody) * (6 * nbody) + 2 * (size_t)(6 * nbody) * nbeta; }
+};
+
+namespace {
+
+int ensure_pinned(hamsx_ctx* c, size_t len) {
+  if (len <= c->h_pinned_len) return 0;
+  HX_TRY(hipStreamSynchronize(c->stream));  // no copy may still read the old buffer
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  c->h_pinned = nullptr;
+  c->h_pinned_len = 0;
+  HX_TRY(hipHostMalloc((void**)&c->h_pinned, sizeof(double) * len, hipHostMallocDefault));
+  c->h_pinned_len = len;
+  return 0;
+}
+
+int set_device(hamsx_ctx* c) {
+  HX_TRY(hipSetDevice(c->device));
+  return 0;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(HAMSX_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int ensure_work(hamsx_ctx* c) {
+  const size_t n = c->n, nr = std::max(1, c->nrhs());
+  int rc;
+  if ((rc = c->X.ensure(n * nr)) || (rc = c->Xtmp.ensure(n * nr)) ||
+      (rc = c->dum2.ensure(n * std::max(1, c->nbeta))) ||
+      (rc = c->partial.ensure((size_t)c->nchunk * (6 + c->nbeta) * n)) || (rc = c->out.ensure(c->out_len() + 8)))
+    return rc;
+  if (c->prec == HAMSX_PREC_MIXED) {
+    if ((rc = c->B.ensure(n * nr)) || (rc = c->R.ensure(n * nr)) || (rc = c->D.ensure(n * nr)) ||
+        (rc = c->Af.ensure(n * n)) || (rc = c->norms.ensure(2 * nr)))
+      return rc;
+  }
+  return 0;
+}
+
+int do_rhs(hamsx_ctx* c) {
+  if (!c->have_S) return fail(HAMSX_ESTATE, "right-hand sides need hamsx_assemble first");
+  if (!c->have_waves) return fail(HAMSX_ESTATE, "right-hand sides need hamsx_set_waves first");
+  int rc = ensure_work(c);
+  if (rc) return rc;
+  hamsx::launch::rhs(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start.data(), c->body_start_d.p,
+                     c->S.p, c->dum2.p, c->partial.p, c->nchunk, c->chunk_body.p, c->chunk_j0.p, c->chunk_j1.p,
+                     c->X.p);
+  return check_launch();
+}
+
+int do_factor(hamsx_ctx* c) {
+  if (c->have_LU) return 0;
+  if (!c->have_A) return fail(HAMSX_ESTATE, "hamsx_factor needs hamsx_assemble first");
+  int rc = ensure_work(c);
+  if (rc) return rc;
+  const int n = c->n;
+  const int nsets0 = (n + hamsx::kTS - 1) / hamsx::kTS;
+  if ((rc = c->ipiv.ensure(n)) || (rc = c->perm.ensure(n)) ||
+      (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)))
+    return rc;
+  if (c->prec == HAMSX_PREC_FP64) {
+    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1);
+    c->have_A = false;  // factored in place
+  } else {
+    hamsx::launch::to_c64(c->stream, c->A.p, c->Af.p, (size_t)n * n);
+    hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
+                                 c->flags.p + 1);
+  }
+  c->have_LU = true;
+  return check_launch();
+}
+
+// X <- A^{-1} X for all 6nb + nbeta columns (with refinement in mixed mode)
+int do_solve(hamsx_ctx* c) {
+  if (c->have_X) return 0;
+  int rc = do_rhs(c);
+  if (rc) return rc;
+  if ((rc = do_factor(c))) return rc;
+  const int n = c->n, nr = c->nrhs();
+  if (c->prec == HAMSX_PREC_FP64) {
+    hamsx::launch::lu_solve(c->stream, c->A.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+  } else {
+    const size_t cnt = (size_t)n * nr;
+    HX_TRY(hipMemcpyAsync(c->B.p, c->X.p, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
+    hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    for (int it = 0; it < c->refine_iters; it++) {
+      hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->D.p, nr, c->norms.p);
+      hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr);
+      hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt);
+    }
+    hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->R.p, nr, c->norms.p);
+    c->last_iters = c->refine_iters;
+  }
+  c->have_X = true;
+  return check_launch();
+}
+
+int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
+  if (!dst || ncol <= 0) return 0;
+  HX_TRY(hipMemcpyAsync(dst, c->X.p + (size_t)c->n * col0, sizeof(double2) * (size_t)c->n * ncol,
+                        hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* info_out) {
+  const int nd = 6 * c->nbody;
+  const size_t len = c->out_len();
+  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 8);
+  if (rc) return rc;
+  double* hout = c->h_pinned + hamsx::kNK + c->nbeta;
+  hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p, c->out.p);
+  if ((rc = check_launch())) return rc;
+  HX_TRY(hipMemcpyAsync(hout, c->out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+  if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0)
+    HX_TRY(hipMemcpyAsync(hout + len + 2, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
+                          c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
+  if (bdmp) std::memcpy(bdmp, hout + (size_t)nd * nd, sizeof(double) * nd * nd);
+  if (exfc) std::memcpy(exfc, hout + 2 * (size_t)nd * nd, sizeof(double) * 2 * nd * c->nbeta);
+  int fl[2];
+  std::memcpy(fl, hout + len, sizeof fl);
+  if (info_out) *info_out = fl[1];
+  if (c->prec == HAMSX_PREC_MIXED) {
+    const double* nm = hout + len + 2;
+    double worst = 0.0;
+    for (int q = 0; q < c->nrhs(); q++) {
+      double rn = nm[q], xn = nm[c->nrhs() + q];
+      worst = std::max(worst, xn > 0 ? rn / xn : rn);
+    }
+    c->last_resid = worst;
+  }
+  if (fl[0] & 1) return fail(HAMSX_EDEVICE, "adaptive quadrature stack exhausted in the finite-depth kernel");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hamsx_version(void) { return "hamsx 0.1 (gfx950)"; }
+
+const char* hamsx_last_error(void) { return g_last_error.c_str(); }
+
+int hamsx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int hamsx_create(hamsx_ctx** out, int device) {
+  if (!out) return fail(HAMSX_EARG, "ctx pointer is NULL");
+  *out = nullptr;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return fail(HAMSX_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= nd) return fail(HAMSX_EARG, "device %d out of range (%d devices)", device, nd);
+  auto* c = new hamsx_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(HAMSX_EDEVICE, "stream creation failed on device %d", device);
+  }
+  int rc = c->flags.ensure(4);
+  if (rc == 0) rc = c->wvn.ensure(hamsx::kNK);
+  if (rc) {
+    hamsx_destroy(c);
+    return rc;
+  }
+  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 4, c->stream) != hipSuccess) {
+    hamsx_destroy(c);
+    return fail(HAMSX_EDEVICE, "flag initialisation failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int hamsx_destroy(hamsx_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto* b : {&c->cx, &c->cy, &c->cz, &c->area, &c->pnsz, &c->nrm, &c->frame, &c->wvn, &c->beta, &c->out,
+                  &c->norms})
+    b->release();
+  for (auto* b : {&c->ncn, &c->body, &c->body_start_d, &c->chunk_body, &c->chunk_j0, &c->chunk_j1, &c->ipiv,
+                  &c->perm, &c->widx, &c->wcnt, &c->flags})
+    b->release();
+  for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial}) b->release();
+  c->Af.release();
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int hamsx_set_precision(hamsx_ctx* c, int mode) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (mode != HAMSX_PREC_FP64 && mode != HAMSX_PREC_MIXED) return fail(HAMSX_EARG, "unknown precision mode %d", mode);
+  if (mode != c->prec) {
+    c->prec = mode;
+    c->have_LU = false;
+    c->have_X = false;
+  }
+  return 0;
+}
+
+int hamsx_panel_geometry(int nnode, const double* xyz, int nelem, const int* ncn, const int* ncon, const double* xr,
+                         double* cen, double* area, double* pnsz, double* dxyz) {
+  if (nnode <= 0 || nelem <= 0 || !xyz || !ncn || !ncon || !xr || !cen || !area || !pnsz || !dxyz)
+    return fail(HAMSX_EARG, "hamsx_panel_geometry: bad arguments");
+  for (int e = 0; e < nelem; e++) {
+    const int nc = ncn[e];
+    if (nc != 3 && nc != 4) return fail(HAMSX_EARG, "panel %d has %d vertices (3 or 4 expected)", e + 1, nc);
+    double p[4][3];
+    for (int c = 0; c < nc; c++) {
+      int nd = ncon[e + (size_t)nelem * c] - 1;
+      if (nd < 0 || nd >= nnode) return fail(HAMSX_EARG, "panel %d refers to node %d", e + 1, nd + 1);
+      for (int d = 0; d < 3; d++) p[c][d] = xyz[nd + (size_t)nnode * d];
+    }
+    for (int d = 0; d < 3; d++) {  // CalPanelCentre
+      double s = 0.0;
+      for (int c = 0; c < nc; c++) s = s + p[c][d];
+      cen[e + (size_t)nelem * d] = s / nc;
+    }
+    auto tri = [](const double* p1, const double* p2, const double* p3) {  // CalDeltaArea_Improved
+      double v21[3], v23[3], v13[3];
+      for (int d = 0; d < 3; d++) {
+        v21[d] = p1[d] - p2[d];
+        v23[d] = p3[d] - p2[d];
+        v13[d] = p3[d] - p1[d];
+      }
+      double a = norm3(v21), b = norm3(v13), c = norm3(v23);
+      return std::sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c))) / 4.0;
+    };
+    double ds = tri(p[0], p[1], p[2]);
+    if (nc == 4) ds = ds + tri(p[0], p[3], p[2]);
+    area[e] = ds;
+    double mx = 0.0;  // CalPanelChartLength
+    for (int c = 0; c < nc; c++) {
+      const double* a = p[c];
+      const double* b = p[(c + 1) % nc];
+      double v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+      double l = norm3(v);
+      mx = (c == 0) ? l : std::max(mx, l);
+    }
+    pnsz[e] = mx;
+    double un[3];  // CalTransNormals
+    if (nc == 3) {
+      double v21[3], v23[3];
+      for (int d = 0; d < 3; d++) {
+        v21[d] = p[0][d] - p[1][d];
+        v23[d] = p[2][d] - p[1][d];
+      }
+      un[0] = v21[1] * v23[2] - v21[2] * v23[1];
+      un[1] = v21[2] * v23[0] - v21[0] * v23[2];
+      un[2] = v21[0] * v23[1] - v21[1] * v23[0];
+    } else {
+      double v13[3], v24[3];
+      for (int d = 0; d < 3; d++) {
+        v13[d] = p[2][d] - p[0][d];
+        v24[d] = p[3][d] - p[1][d];
+      }
+      un[0] = v24[1] * v13[2] - v24[2] * v13[1];
+      un[1] = v24[2] * v13[0] - v24[0] * v13[2];
+      un[2] = v24[0] * v13[1] - v24[1] * v13[0];
+    }
+    double aun = norm3(un);
+    if (!(aun > 0.0)) return fail(HAMSX_EARG, "panel %d is degenerate (zero normal)", e + 1);
+    double n1 = un[0] / aun, n2 = un[1] / aun, n3 = un[2] / aun;
+    double x = cen[e], y = cen[e + (size_t)nelem], z = cen[e + 2 * (size_t)nelem];
+    dxyz[e] = n1;
+    dxyz[e + (size_t)nelem] = n2;
+    dxyz[e + 2 * (size_t)nelem] = n3;
+    dxyz[e + 3 * (size_t)nelem] = (y - xr[1]) * n3 - (z - xr[2]) * n2;  // CalRotNormals
+    dxyz[e + 4 * (size_t)nelem] = (z - xr[2]) * n1 - (x - xr[0]) * n3;
+    dxyz[e + 5 * (size_t)nelem] = (x - xr[0]) * n2 - (y - xr[1]) * n1;
+  }
+  return 0;
+}
+
+int hamsx_wave_properties(int inft, double value, double h, double* v, double* wk, double* w1, double* wvn) {
+  if (!v || !wk || !w1) return fail(HAMSX_EARG, "hamsx_wave_properties: NULL output");
+  if (!(value > 0.0)) return fail(HAMSX_EARG, "frequency value must be positive");
+  if (wvn) std::fill(wvn, wvn + HAMSX_NK, 0.0);
+  double tmp[HAMSX_NK] = {0};
+  double* roots = wvn ? wvn : tmp;
+  const bool deep = h <= 0.0;
+  switch (inft) {  // CalWaveProperts, ImplementSubs.f90:72-139
+    case 1:
+      *v = value;
+      *w1 = std::sqrt(kG * *v);
+      if (deep) *wk = *v;
+      else {
+        dispersion(*w1, h, HAMSX_NK, roots);
+        *wk = roots[0];
+      }
+      break;
+    case 2:
+      *wk = value;
+      *v = deep ? *wk : *wk * std::tanh(*wk * h);
+      *w1 = std::sqrt(kG * *v);
+      if (!deep) dispersion(*w1, h, HAMSX_NK, roots);
+      break;
+    case 3:
+    case 4:
+      *w1 = inft == 3 ? value : 2.0 * kPi / value;
+      *v = *w1 * *w1 / kG;
+      if (deep) *wk = *v;
+      else {
+        dispersion(*w1, h, HAMSX_NK, roots);
+        *wk = roots[0];
+      }
+      break;
+    case 5:
+      *wk = 2.0 * kPi / value;
+      *v = deep ? *wk : *wk * std::tanh(*wk * h);
+      *w1 = std::sqrt(kG * *v);
+      if (!deep) dispersion(*w1, h, HAMSX_NK, roots);
+      break;
+    default:
+      return fail(HAMSX_EARG, "input frequency type %d (1..5 expected)", inft);
+  }
+  return 0;
+}
+
+int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const int* ncn, const int* ncon,
+                   const double* cen, const double* area, const double* pnsz, const double* dxyz, int nbody,
+                   const int* body_start) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (nelem <= 0 || nnode <= 0 || nbody <= 0 || !xyz || !ncn || !ncon || !cen || !area || !pnsz || !dxyz ||
+      !body_start)
+    return fail(HAMSX_EARG, "hamsx_set_mesh: bad arguments");
+  if (body_start[0] != 0 || body_start[nbody] != nelem)
+    return fail(HAMSX_EARG, "body_start must run from 0 to nelem");
+  for (int b = 0; b < nbody; b++)
+    if (body_start[b + 1] <= body_start[b]) return fail(HAMSX_EARG, "body %d has no panels", b + 1);
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  const size_t n = nelem;
+  // Newman frames, body index and right-hand-side chunks on the host
+  std::vector<double> frame(n * kFrameLen);
+  std::vector<int> body(n);
+  for (size_t e = 0; e < n; e++) {
+    const int nc = ncn[e];
+    if (nc != 3 && nc != 4) return fail(HAMSX_EARG, "panel %zu has %d vertices", e + 1, nc);
+    double xv[12];
+    for (int q = 0; q < nc; q++) {
+      int nd = ncon[e + n * q] - 1;
+      if (nd < 0 || nd >= nnode) return fail(HAMSX_EARG, "panel %zu refers to node %d", e + 1, nd + 1);
+      for (int d = 0; d < 3; d++) xv[3 * q + d] = xyz[nd + (size_t)nnode * d];
+    }
+    newman_frame(nc, xv, frame.data() + e * kFrameLen);
+  }
+  std::vector<int> cb, c0, c1;
+  for (int b = 0; b < nbody; b++) {
+    for (int j = body_start[b]; j < body_start[b + 1]; j++) body[j] = b;
+    for (int j = body_start[b]; j < body_start[b + 1]; j += 256) {
+      cb.push_back(b);
+      c0.push_back(j);
+      c1.push_back(std::min(j + 256, body_start[b + 1]));
+    }
+  }
+  c->n = nelem;
+  c->nbody = nbody;
+  c->nchunk = (int)cb.size();
+  c->body_start.assign(body_start, body_start + nbody + 1);
+  if ((rc = c->cx.ensure(n)) || (rc = c->cy.ensure(n)) || (rc = c->cz.ensure(n)) || (rc = c->area.ensure(n)) ||
+      (rc = c->pnsz.ensure(n)) || (rc = c->nrm.ensure(6 * n)) || (rc = c->frame.ensure(n * kFrameLen)) ||
+      (rc = c->ncn.ensure(n)) || (rc = c->body.ensure(n)) || (rc = c->body_start_d.ensure(nbody + 1)) ||
+      (rc = c->chunk_body.ensure(cb.size())) || (rc = c->chunk_j0.ensure(cb.size())) ||
+      (rc = c->chunk_j1.ensure(cb.size())) || (rc = c->A.ensure(n * n)) || (rc = c->S.ensure(n * n)))
+    return rc;
+  HX_TRY(hipMemcpy(c->cx.p, cen, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->cy.p, cen + n, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->cz.p, cen + 2 * n, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->area.p, area, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->pnsz.p, pnsz, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->nrm.p, dxyz, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->frame.p, frame.data(), sizeof(double) * n * kFrameLen, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->ncn.p, ncn, sizeof(int) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->body.p, body.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->body_start_d.p, body_start, sizeof(int) * (nbody + 1), hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->chunk_body.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->chunk_j0.p, c0.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->chunk_j1.p, c1.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+  c->have_A = c->have_S = c->have_LU = c->have_X = false;
+  return 0;
+}
+
+int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, const double* wvn, int nk) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (h > 0.0 && (!wvn || nk < hamsx::kNK)) return fail(HAMSX_EARG, "finite depth needs %d dispersion roots", hamsx::kNK);
+  int rc = set_device(c);
+  if (rc) return rc;
+  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 8))) return rc;
+  // the staging slot is free: every earlier use ended in a stream synchronisation
+  HX_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < hamsx::kNK; k++) c->h_pinned[k] = (h > 0.0 && wvn) ? wvn[k] : 0.0;
+  HX_TRY(hipMemcpyAsync(c->wvn.p, c->h_pinned, sizeof(double) * hamsx::kNK, hipMemcpyHostToDevice, c->stream));
+  c->freq = hamsx::FreqDev{v, wk, w1, h, c->wvn.p};
+  c->have_freq = true;
+  c->have_A = c->have_S = c->have_LU = c->have_X = false;
+  return 0;
+}
+
+int hamsx_set_waves(hamsx_ctx* c, int nbeta, const double* beta, double amp, double xw0, double xw1, double rho,
+                    int isol) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (nbeta < 0 || (nbeta > 0 && !beta)) return fail(HAMSX_EARG, "bad heading list");
+  if (isol != 1 && isol != 2) return fail(HAMSX_EARG, "ISOL must be 1 or 2");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  if ((rc = c->beta.ensure(std::max(1, nbeta)))) return rc;
+  if (nbeta > 0) HX_TRY(hipMemcpy(c->beta.p, beta, sizeof(double) * nbeta, hipMemcpyHostToDevice));
+  c->nbeta = nbeta;
+  c->amp = amp;
+  c->xw0 = xw0;
+  c->xw1 = xw1;
+  c->rho = rho;
+  c->isol = isol;
+  c->have_waves = true;
+  c->have_X = false;
+  return 0;
+}
+
+int hamsx_assemble(hamsx_ctx* c) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_mesh first");
+  if (!c->have_freq) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_frequency first");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
+  if ((rc = check_launch())) return rc;
+  c->have_A = c->have_S = true;
+  c->have_LU = c->have_X = false;
+  return 0;
+}
+
+int hamsx_factor(hamsx_ctx* c) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  if ((rc = do_factor(c))) return rc;
+  int info = 0;
+  HX_TRY(hipMemcpyAsync(&info, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  return info;
+}
+
+int hamsx_solve_radiation(hamsx_ctx* c, double* mxpot) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  int rc = set_device(c);
+  if (rc || (rc = do_solve(c))) return rc;
+  return fetch_mxpot(c, 0, 6 * c->nbody, mxpot);
+}
+
+int hamsx_solve_diffraction(hamsx_ctx* c, double* mxpot) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  int rc = set_device(c);
+  if (rc || (rc = do_solve(c))) return rc;
+  return fetch_mxpot(c, 6 * c->nbody, c->nbeta, mxpot);
+}
+
+int hamsx_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  int rc = set_device(c);
+  if (rc || (rc = do_solve(c))) return rc;
+  int info = 0;
+  if ((rc = finish_forces(c, amas, bdmp, exfc, &info))) return rc;
+  return info;
+}
+
+int hamsx_solve_frequency(hamsx_ctx* c, double w1, double h, double* amas, double* bdmp, double* exfc) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_solve_frequency needs hamsx_set_waves first");
+  double v, wk, w, wvn[HAMSX_NK];
+  int rc = hamsx_wave_properties(3, w1, h, &v, &wk, &w, wvn);
+  if (rc || (rc = hamsx_set_frequency(c, v, wk, w, h, wvn, HAMSX_NK)) || (rc = hamsx_assemble(c))) return rc;
+  if ((rc = do_solve(c))) return rc;
+  int info = 0;
+  if ((rc = finish_forces(c, amas, bdmp, exfc, &info))) return rc;
+  return info;
+}
+
+int hamsx_get_matrix(hamsx_ctx* c, int which, double* out, int* ipiv) {
+  if (!c || !out) return fail(HAMSX_EARG, "bad arguments");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const size_t nn = (size_t)c->n * c->n;
+  if (which == 0 || which == 2) {
+    if (which == 0 && !c->have_A) return fail(HAMSX_ESTATE, "A is not available (not assembled, or factored in place)");
+    if (which == 2 && !c->have_LU) return fail(HAMSX_ESTATE, "no factorisation available");
+    if (which == 2 && c->prec == HAMSX_PREC_MIXED) {
+      std::vector<float2> tmp(nn);
+      HX_TRY(hipMemcpyAsync(tmp.data(), c->Af.p, sizeof(float2) * nn, hipMemcpyDeviceToHost, c->stream));
+      HX_TRY(hipStreamSynchronize(c->stream));
+      for (size_t k = 0; k < nn; k++) {
+        out[2 * k] = tmp[k].x;
+        out[2 * k + 1] = tmp[k].y;
+      }
+    } else {
+      HX_TRY(hipMemcpyAsync(out, c->A.p, sizeof(double2) * nn, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (which == 2 && ipiv) HX_TRY(hipMemcpyAsync(ipiv, c->ipiv.p, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+  } else if (which == 1) {
+    if (!c->have_S) return fail(HAMSX_ESTATE, "S is not available");
+    HX_TRY(hipMemcpyAsync(out, c->S.p, sizeof(double2) * nn, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    return fail(HAMSX_EARG, "which must be 0 (A), 1 (S) or 2 (LU)");
+  }
+  HX_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hamsx_refine_stats(hamsx_ctx* c, int* iters, double* resid) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (iters) *iters = c->last_iters;
+  if (resid) *resid = c->last_resid;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// hamsx_internal.hpp -- shared declarations between the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace hamsx {
+
+constexpr int kNK = 200;      // dispersion roots kept (WaveDyn_mod NK)
+constexpr int kPanelNB = 32;  // LU panel width
+constexpr int kTS = 128;      // rows per tournament block
+
+// Device-resident, frequency-invariant mesh (structure of arrays).
+struct MeshDev {
+  int n;
+  const double* cx;
+  const double* cy;
+  const double* cz;
+  const double* area;
+  const double* pnsz;
+  const double* nrm;    // nrm[m * n + j], m = 0..5
+  const int* ncn;
+  const double* frame;  // Newman local frame, kFrameLen doubles per panel
+  const int* body;      // body index of each panel
+};
+
+struct FreqDev {
+  double v, wk, w1, h;
+  const double* wvn;  // kNK roots on the device
+};
+
+struct WaveDev {
+  int nbeta;
+  const double* beta;  // radians, device
+  double amp, xw0, xw1, rho;
+  int isol;
+};
+
+namespace launch {
+// influence matrices A and S (complex128, column-major, leading dimension n)
+void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
+// right-hand sides X(n, 6*nbody + nbeta) = S * generator columns
+void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
+         const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
+         const int* chunk_body_d, const int* chunk_j0_d, const int* chunk_j1_d, double2* X);
+// LU with tournament pivoting; ipiv 1-based, perm = row permutation, info[0] = first zero pivot
+void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info);
+// X <- A^{-1} X using the factors, nrhs columns (leading dimension n)
+void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
+// AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
+void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
+            const double2* X, double* out);
+// single-precision copy of A for the mixed path, and the helpers of refinement
+void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
+void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info);
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
+void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
+              double* norms);
+void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count);
+}  // namespace launch
+
+}  // namespace hamsx

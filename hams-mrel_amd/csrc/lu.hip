@@ -1,0 +1,683 @@
+// lu.hip -- dense complex LU with partial (tournament) pivoting and the
+// triangular solves, gfx950.  Replaces ZGETRF/ZGETRS (AssbMatxMulti.f90:112,
+// :279, :346).
+//
+// Right-looking blocked factorisation, panel width NB = 32:
+//   1. pivot selection for the m x NB panel by a tournament (CALU): 128-row
+//      blocks are factored with partial pivoting in LDS, each keeps its NB
+//      pivot rows, and groups of four candidate sets are merged the same way
+//      until one set is left -- no grid-wide synchronisation;
+//   2. the selected rows are swapped to the top across all N columns (the
+//      swaps are also applied to a permutation vector used by the solves);
+//   3. the NB x NB diagonal block is factored without further pivoting and
+//      L21 = A21 U11^-1, U12 = L11^-1 A12 are formed row/column-parallel;
+//   4. the trailing update A22 -= L21 U12 is a complex GEMM on MFMA
+//      (v_mfma_f64_16x16x4f64 for complex128, v_mfma_f32_16x16x4f32 for
+//      complex64), 64 x 64 output tiles per 256-thread block.
+// The same templates serve the complex128 (plain) and complex64 (mixed
+// precision) paths.
+#include <hip/hip_runtime.h>
+
+#include "hamsx_internal.hpp"
+
+namespace hamsx {
+
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using c2 = double2;
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct CT<float> {
+  using c2 = float2;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i ai), Smith
+  if (fabs(ar) >= fabs(ai)) {
+    R q = ai / ar, d = ar + ai * q;
+    rr = R(1) / d;
+    ri = -q / d;
+  } else {
+    R q = ar / ai, d = ar * q + ai;
+    rr = q / d;
+    ri = R(-1) / d;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tournament pivot selection.  Level 0 (idx_in == nullptr): block b takes
+// rows [row0 + b*kTS, min(row0 + (b+1)*kTS, n)).  Level > 0: block b merges
+// candidate sets [b*4, b*4 + 4) of the previous level.  Each block factors
+// its candidates' panel entries with partial pivoting (pivot = first maximum
+// of |re| + |im|, as IZAMAX) and writes its ncols pivot rows in pivot order.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
+                                                int ncols, int row0, const int* __restrict__ idx_in,
+                                                const int* __restrict__ cnt_in, int nsets_in, int* __restrict__ idx_out,
+                                                int* __restrict__ cnt_out) {
+  __shared__ R vr[kPanelNB][kTS + 1];
+  __shared__ R vi[kPanelNB][kTS + 1];
+  __shared__ int idx[kTS];
+  __shared__ R redv[4];
+  __shared__ int redi[4];
+  const int t = threadIdx.x, b = blockIdx.x;
+  int mc = 0;
+  if (idx_in == nullptr) {
+    int r0 = row0 + b * kTS;
+    mc = min(kTS, n - r0);
+    if (t < mc) idx[t] = r0 + t;
+  } else {
+    // gather up to 4 sets of <= kPanelNB candidates each
+    __shared__ int off[5];
+    if (t == 0) {
+      int o = 0;
+      for (int q = 0; q < 4; q++) {
+        off[q] = o;
+        int sidx = b * 4 + q;
+        o += (sidx < nsets_in) ? cnt_in[sidx] : 0;
+      }
+      off[4] = o;
+    }
+    __syncthreads();
+    mc = off[4];
+    for (int q = 0; q < 4; q++) {
+      int sidx = b * 4 + q, c = off[q + 1] - off[q];
+      if (t < c) idx[off[q] + t] = idx_in[sidx * kPanelNB + t];
+    }
+  }
+  __syncthreads();
+  // load candidate rows of the panel
+  for (int e = t; e < mc * ncols; e += 256) {
+    int r = e % mc, c = e / mc;
+    auto v = A[(size_t)idx[r] + (size_t)lda * (col0 + c)];
+    vr[c][r] = v.x;
+    vi[c][r] = v.y;
+  }
+  __syncthreads();
+  const int nsel = min(ncols, mc);
+  for (int c = 0; c < nsel; c++) {
+    // argmax over rows c..mc-1 of column c; ties -> lowest row
+    R best = R(-1);
+    int bi = 0x7fffffff;
+    for (int r = c + t; r < mc; r += 256) {
+      R m = fabs(vr[c][r]) + fabs(vi[c][r]);
+      if (m > best || (m == best && r < bi)) {
+        best = m;
+        bi = r;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      R ob = __shfl_xor(best, o);
+      int oi = __shfl_xor(bi, o);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if ((t & 63) == 0) {
+      redv[t >> 6] = best;
+      redi[t >> 6] = bi;
+    }
+    __syncthreads();
+    best = redv[0];
+    bi = redi[0];
+    for (int q = 1; q < 4; q++)
+      if (redv[q] > best || (redv[q] == best && redi[q] < bi)) {
+        best = redv[q];
+        bi = redi[q];
+      }
+    const int p = bi;
+    const bool nonzero = best > R(0);
+    __syncthreads();
+    if (nonzero && p != c) {
+      if (t < ncols) {
+        R a = vr[t][c], bb = vi[t][c];
+        vr[t][c] = vr[t][p];
+        vi[t][c] = vi[t][p];
+        vr[t][p] = a;
+        vi[t][p] = bb;
+      }
+      if (t == 0) {
+        int q = idx[c];
+        idx[c] = idx[p];
+        idx[p] = q;
+      }
+    }
+    __syncthreads();
+    if (nonzero) {
+      R ir, ii;
+      crecip(vr[c][c], vi[c][c], ir, ii);
+      for (int r = c + 1 + t; r < mc; r += 256) {
+        R a = vr[c][r], bb = vi[c][r];
+        vr[c][r] = a * ir - bb * ii;
+        vi[c][r] = a * ii + bb * ir;
+      }
+    }
+    __syncthreads();
+    const int nr = mc - c - 1, nq = ncols - c - 1;
+    for (int e = t; e < nr * nq; e += 256) {
+      int r = c + 1 + e % nr, q = c + 1 + e / nr;
+      R lr = vr[c][r], li = vi[c][r], ur = vr[q][c], ui = vi[q][c];
+      vr[q][r] -= lr * ur - li * ui;
+      vi[q][r] -= lr * ui + li * ur;
+    }
+    __syncthreads();
+  }
+  if (t < nsel) idx_out[b * kPanelNB + t] = idx[t];
+  if (t == 0) cnt_out[b] = nsel;
+}
+
+// Turn the final pivot-row list into LAPACK-style sequential swaps
+// ipiv[col0 + c] (1-based) and record the first zero pivot in info.
+__global__ void k_pivots_to_swaps(const int* __restrict__ winners, int nsel, int col0, int* __restrict__ ipiv) {
+  if (threadIdx.x != 0) return;
+  int who[2 * kPanelNB], pos[2 * kPanelNB], nt = 0;  // rows whose position changed
+  for (int c = 0; c < nsel; c++) {
+    int w = winners[c], p = w;
+    for (int q = 0; q < nt; q++)
+      if (who[q] == w) {
+        p = pos[q];
+        break;
+      }
+    const int dst = col0 + c;
+    ipiv[dst] = p + 1;
+    if (p != dst) {
+      // row currently at dst moves to p
+      int rowat = dst;
+      for (int q = 0; q < nt; q++)
+        if (pos[q] == dst) {
+          rowat = who[q];
+          break;
+        }
+      bool f1 = false, f2 = false;
+      for (int q = 0; q < nt; q++) {
+        if (who[q] == w) {
+          pos[q] = dst;
+          f1 = true;
+        } else if (who[q] == rowat) {
+          pos[q] = p;
+          f2 = true;
+        }
+      }
+      if (!f1) {
+        who[nt] = w;
+        pos[nt++] = dst;
+      }
+      if (!f2) {
+        who[nt] = rowat;
+        pos[nt++] = p;
+      }
+    }
+  }
+}
+
+// Apply the panel's swaps to every column (and to the permutation vector).
+template <typename R>
+__global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0, int nsel,
+                            const int* __restrict__ ipiv, int* __restrict__ perm) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  for (int c = 0; c < nsel; c++) {
+    const int r = col0 + c, p = ipiv[r] - 1;
+    if (p == r) continue;
+    if (j < n) {
+      auto a = A[(size_t)r + (size_t)lda * j];
+      A[(size_t)r + (size_t)lda * j] = A[(size_t)p + (size_t)lda * j];
+      A[(size_t)p + (size_t)lda * j] = a;
+    } else {
+      int q = perm[r];
+      perm[r] = perm[p];
+      perm[p] = q;
+    }
+  }
+}
+
+// Diagonal block LU (no pivoting: its rows are the tournament winners in
+// pivot order), then L21 = A21 U11^-1 (blocks [0, nrb)) and U12 = L11^-1 A12
+// (blocks [nrb, nrb + ncb)), one row or column per thread.
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
+                                                      int nb, int nrb, int* __restrict__ info) {
+  __shared__ R dr[kPanelNB][kPanelNB + 1];
+  __shared__ R di[kPanelNB][kPanelNB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += 256) {
+    int r = e % nb, c = e / nb;
+    auto v = A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)];
+    dr[r][c] = v.x;
+    di[r][c] = v.y;
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; c++) {
+    R ir, ii;
+    const bool nz = (dr[c][c] != R(0) || di[c][c] != R(0));
+    if (nz) crecip(dr[c][c], di[c][c], ir, ii);
+    __syncthreads();
+    if (nz)
+      for (int r = c + 1 + t; r < nb; r += 256) {
+        R a = dr[r][c], b = di[r][c];
+        dr[r][c] = a * ir - b * ii;
+        di[r][c] = a * ii + b * ir;
+      }
+    __syncthreads();
+    const int m = nb - c - 1;
+    for (int e = t; e < m * m; e += 256) {
+      int r = c + 1 + e % m, q = c + 1 + e / m;
+      R lr = dr[r][c], li = di[r][c], ur = dr[c][q], ui = di[c][q];
+      dr[r][q] -= lr * ur - li * ui;
+      di[r][q] -= lr * ui + li * ur;
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    for (int e = t; e < nb * nb; e += 256) {
+      int r = e % nb, c = e / nb;
+      A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = {dr[r][c], di[r][c]};
+    }
+    if (t == 0 && info[0] == 0)
+      for (int c = 0; c < nb; c++)
+        if (dr[c][c] == R(0) && di[c][c] == R(0)) {
+          info[0] = col0 + c + 1;
+          break;
+        }
+  }
+  const int b = blockIdx.x;
+  if (b < nrb) {
+    const int i = col0 + nb + b * 256 + t;
+    if (i >= n) return;
+    R xr[kPanelNB], xi[kPanelNB];
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) {
+      if (c < nb) {
+        auto v = A[(size_t)i + (size_t)lda * (col0 + c)];
+        xr[c] = v.x;
+        xi[c] = v.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) {
+      if (c >= nb) break;
+      R ar = xr[c], ai = xi[c];
+#pragma unroll
+      for (int q = 0; q < kPanelNB; q++) {
+        if (q >= c) break;
+        ar -= xr[q] * dr[q][c] - xi[q] * di[q][c];
+        ai -= xr[q] * di[q][c] + xi[q] * dr[q][c];
+      }
+      R ir = R(0), ii = R(0);
+      if (dr[c][c] != R(0) || di[c][c] != R(0)) crecip(dr[c][c], di[c][c], ir, ii);
+      xr[c] = ar * ir - ai * ii;
+      xi[c] = ar * ii + ai * ir;
+    }
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++)
+      if (c < nb) A[(size_t)i + (size_t)lda * (col0 + c)] = {xr[c], xi[c]};
+  } else {
+    const int j = col0 + nb + (b - nrb) * 256 + t;
+    if (j >= n) return;
+    R yr[kPanelNB], yi[kPanelNB];
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) {
+      if (c < nb) {
+        auto v = A[(size_t)(col0 + c) + (size_t)lda * j];
+        yr[c] = v.x;
+        yi[c] = v.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) {
+      if (c >= nb) break;
+#pragma unroll
+      for (int q = 0; q < kPanelNB; q++) {
+        if (q >= c) break;
+        yr[c] -= dr[c][q] * yr[q] - di[c][q] * yi[q];
+        yi[c] -= dr[c][q] * yi[q] + di[c][q] * yr[q];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++)
+      if (c < nb) A[(size_t)(col0 + c) + (size_t)lda * j] = {yr[c], yi[c]};
+  }
+}
+
+// Trailing update C -= L U over the (n - r0) x (n - r0) block starting at
+// (r0, r0), K = kk <= 32.  MFMA 16x16x4 computes the transposed product
+// (U^T L^T) so that the 16 consecutive lanes of an accumulator row map to
+// 16 consecutive rows of column-major C.  Block tiles are 64 x 64, each of
+// the four waves owns a 32 x 32 quarter (2 x 2 MFMA tiles, real and
+// imaginary accumulators).  Block ids are remapped so that each of the eight
+// XCDs works on a contiguous band of tiles (L21 rows stay in its L2).
+template <typename R>
+__global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __restrict__ A, int lda, int n, int r0,
+                                                         int kk, int tiles_m, int ntiles) {
+  using v4 = typename CT<R>::v4;
+  __shared__ R lr_[kPanelNB][64], li_[kPanelNB][64];  // L21 tile: [k][row]
+  __shared__ R ur_[kPanelNB][64], ui_[kPanelNB][64];  // U12 tile: [k][col]
+  int bid = blockIdx.x;
+  {
+    const int per = (ntiles + 7) / 8, xcd = bid % 8, k = bid / 8;
+    const int rb = xcd * per + k;
+    if (rb >= ntiles) return;
+    bid = rb;
+  }
+  const int tm = bid % tiles_m, tn = bid / tiles_m;
+  const int i0 = r0 + tm * 64, j0 = r0 + tn * 64;
+  const int t = threadIdx.x;
+  for (int e = t; e < 64 * kPanelNB; e += 256) {
+    int r = e % 64, k = e / 64;
+    R a = R(0), b = R(0), c = R(0), d = R(0);
+    if (k < kk) {
+      if (i0 + r < n) {
+        auto v = A[(size_t)(i0 + r) + (size_t)lda * (r0 - kk + k)];
+        a = v.x;
+        b = v.y;
+      }
+      if (j0 + r < n) {
+        auto v = A[(size_t)(r0 - kk + k) + (size_t)lda * (j0 + r)];
+        c = v.x;
+        d = v.y;
+      }
+    }
+    lr_[k][r] = a;
+    li_[k][r] = b;
+    ur_[k][r] = c;
+    ui_[k][r] = d;
+  }
+  __syncthreads();
+  const int w = t >> 6, lane = t & 63;
+  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;  // wave quarter within the tile
+  const int l16 = lane & 15, kq = lane >> 4;
+  v4 accR[2][2], accI[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      accR[a][b] = v4{0, 0, 0, 0};
+      accI[a][b] = v4{0, 0, 0, 0};
+    }
+  for (int k0 = 0; k0 < kk; k0 += 4) {
+    const int k = k0 + kq;
+    R Ur[2], Ui[2], Lr[2], Li[2];
+    for (int b = 0; b < 2; b++) {  // A operand: U^T rows = C columns
+      Ur[b] = ur_[k][wj + b * 16 + l16];
+      Ui[b] = ui_[k][wj + b * 16 + l16];
+    }
+    for (int a = 0; a < 2; a++) {  // B operand: L^T columns = C rows
+      Lr[a] = lr_[k][wi + a * 16 + l16];
+      Li[a] = li_[k][wi + a * 16 + l16];
+    }
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        accR[a][b] = CT<R>::mfma(Ur[b], Lr[a], accR[a][b]);
+        accR[a][b] = CT<R>::mfma(-Ui[b], Li[a], accR[a][b]);
+        accI[a][b] = CT<R>::mfma(Ui[b], Lr[a], accI[a][b]);
+        accI[a][b] = CT<R>::mfma(Ur[b], Li[a], accI[a][b]);
+      }
+  }
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int q = 0; q < 4; q++) {
+        const int row = i0 + wi + a * 16 + l16;
+        const int col = j0 + wj + b * 16 + kq * 4 + q;
+        if (row < n && col < n) {
+          auto& c = A[(size_t)row + (size_t)lda * col];
+          auto v = c;
+          v.x -= accR[a][b][q];
+          v.y -= accI[a][b][q];
+          c = v;
+        }
+      }
+}
+
+__global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) perm[i] = i;
+  if (i == 0) info[0] = 0;
+}
+
+template <typename R>
+static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
+                        int* info) {
+  hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
+  // workspace: two ping-pong candidate buffers of nsets0 sets
+  const int nsets0 = (n + kTS - 1) / kTS;
+  int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
+  int* cnt[2] = {wcnt, wcnt + nsets0};
+  for (int col0 = 0; col0 < n; col0 += kPanelNB) {
+    const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
+    const int m = n - col0;
+    int nsets = (m + kTS - 1) / kTS, cur = 0;
+    hipLaunchKernelGGL(k_select<R>, dim3(nsets), dim3(256), 0, s, A, n, n, col0, nb, col0, (const int*)nullptr,
+                       (const int*)nullptr, 0, idx[cur], cnt[cur]);
+    while (nsets > 1) {
+      const int nout = (nsets + 3) / 4;
+      hipLaunchKernelGGL(k_select<R>, dim3(nout), dim3(256), 0, s, A, n, n, col0, nb, col0, idx[cur], cnt[cur], nsets,
+                         idx[1 - cur], cnt[1 - cur]);
+      cur = 1 - cur;
+      nsets = nout;
+    }
+    hipLaunchKernelGGL(k_pivots_to_swaps, dim3(1), dim3(64), 0, s, idx[cur], nb, col0, ipiv);
+    hipLaunchKernelGGL(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
+    const int rest = n - col0 - nb;
+    const int nrb = (rest + 255) / 256;
+    hipLaunchKernelGGL(k_panel_factor<R>, dim3(nrb * 2 > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
+                       info);
+    if (rest > 0) {
+      const int tiles = (rest + 63) / 64;
+      const int ntiles = tiles * tiles;
+      const int grid = ((ntiles + 7) / 8) * 8;
+      hipLaunchKernelGGL(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves, 32-row blocks: each launch solves one diagonal block
+// (every block of the grid recomputes it redundantly from LDS) and updates
+// its own chunk of the remaining rows.  X is complex128; the factors are
+// complex128 or complex64 (mixed path: FP32 factors, FP64 right-hand sides).
+// ---------------------------------------------------------------------------
+constexpr int kRhsMax = 64;
+template <typename R, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
+                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs) {
+  __shared__ double tr[kPanelNB][kPanelNB + 1], ti[kPanelNB][kPanelNB + 1];
+  __shared__ double yr[kPanelNB][kRhsMax], yi[kPanelNB][kRhsMax];
+  const int t = threadIdx.x;
+  for (int e = t; e < kb * kb; e += 256) {
+    int r = e % kb, c = e / kb;
+    auto v = LU[(size_t)(k0 + r) + (size_t)n * (k0 + c)];
+    tr[r][c] = (double)v.x;
+    ti[r][c] = (double)v.y;
+  }
+  for (int e = t; e < kb * nrhs; e += 256) {
+    int r = e % kb, q = e / kb;
+    double2 v = X[(size_t)(k0 + r) + (size_t)n * q];
+    yr[r][q] = v.x;
+    yi[r][q] = v.y;
+  }
+  __syncthreads();
+  // solve the diagonal block, one right-hand side per thread
+  for (int q = t; q < nrhs; q += 256) {
+    if (LOWER) {
+      for (int c = 0; c < kb; c++)
+        for (int r = c + 1; r < kb; r++) {
+          yr[r][q] -= tr[r][c] * yr[c][q] - ti[r][c] * yi[c][q];
+          yi[r][q] -= tr[r][c] * yi[c][q] + ti[r][c] * yr[c][q];
+        }
+    } else {
+      for (int c = kb - 1; c >= 0; c--) {
+        double a = tr[c][c], b = ti[c][c], xr, xi;
+        if (fabs(a) >= fabs(b)) {
+          double qq = b / a, dd = a + b * qq;
+          xr = (yr[c][q] + yi[c][q] * qq) / dd;
+          xi = (yi[c][q] - yr[c][q] * qq) / dd;
+        } else {
+          double qq = a / b, dd = a * qq + b;
+          xr = (yr[c][q] * qq + yi[c][q]) / dd;
+          xi = (yi[c][q] * qq - yr[c][q]) / dd;
+        }
+        yr[c][q] = xr;
+        yi[c][q] = xi;
+        for (int r = 0; r < c; r++) {
+          yr[r][q] -= tr[r][c] * xr - ti[r][c] * xi;
+          yi[r][q] -= tr[r][c] * xi + ti[r][c] * xr;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // diagonal rows go to Y, so no block of this launch reads a row another writes
+  if (blockIdx.x == 0)
+    for (int e = t; e < kb * nrhs; e += 256) {
+      int r = e % kb, q = e / kb;
+      Y[(size_t)(k0 + r) + (size_t)n * q] = make_double2(yr[r][q], yi[r][q]);
+    }
+  // update the other rows: below the block (LOWER) or above it (upper)
+  const int i = LOWER ? k0 + kb + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
+  if (LOWER ? i >= n : i >= k0) return;
+  typename CT<R>::c2 l[kPanelNB];
+#pragma unroll
+  for (int c = 0; c < kPanelNB; c++)
+    if (c < kb) l[c] = LU[(size_t)i + (size_t)n * (k0 + c)];
+  for (int q = 0; q < nrhs; q++) {
+    double ar = 0.0, ai = 0.0;
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++)
+      if (c < kb) {
+        double lr = (double)l[c].x, li = (double)l[c].y;
+        ar += lr * yr[c][q] - li * yi[c][q];
+        ai += lr * yi[c][q] + li * yr[c][q];
+      }
+    double2 v = X[(size_t)i + (size_t)n * q];
+    v.x -= ar;
+    v.y -= ai;
+    X[(size_t)i + (size_t)n * q] = v;
+  }
+}
+
+__global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
+                              int n, int nrhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i < n && q < nrhs) dst[(size_t)i + (size_t)n * q] = src[(size_t)perm[i] + (size_t)n * q];
+}
+
+template <typename R>
+static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const int* perm, double2* X,
+                       double2* Xtmp, int nrhs) {
+  for (int q0 = 0; q0 < nrhs; q0 += kRhsMax) {
+    const int nq = nrhs - q0 < kRhsMax ? nrhs - q0 : kRhsMax;
+    double2* Xq = X + (size_t)n * q0;
+    double2* Tq = Xtmp + (size_t)n * q0;
+    // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
+    // backward pass: off-diagonal rows updated in X, solved rows into T.
+    hipLaunchKernelGGL(k_gather_rows, dim3((n + 255) / 256, nq), dim3(256), 0, s, perm, Xq, Tq, n, nq);
+    for (int k0 = 0; k0 < n; k0 += kPanelNB) {
+      const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+      const int rest = n - k0 - kb;
+      const int g = rest > 0 ? (rest + 255) / 256 : 1;
+      hipLaunchKernelGGL((k_trsm_block<R, true>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Tq, Xq, nq);
+    }
+    for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
+      const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+      const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
+      hipLaunchKernelGGL((k_trsm_block<R, false>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Xq, Tq, nq);
+    }
+    (void)hipMemcpyAsync(Xq, Tq, sizeof(double2) * (size_t)n * nq, hipMemcpyDeviceToDevice, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision helpers: FP32 copy of A, FP64 residual R = B - A X with
+// per-column max norms, and X += D.
+// ---------------------------------------------------------------------------
+__global__ void k_to_c64(const double2* __restrict__ A, float2* __restrict__ Af, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < count; i += (size_t)gridDim.x * blockDim.x) Af[i] = make_float2((float)A[i].x, (float)A[i].y);
+}
+
+// R(:, q) = B(:, q) - A X(:, q): each block owns 64 rows and one column
+// chunk of A; partial products are accumulated with atomics.
+__global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ A, int n, const double2* __restrict__ X,
+                                                  double2* __restrict__ R, int nrhs, int jchunk) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q0 = (threadIdx.x >> 6);
+  const int j0 = blockIdx.y * jchunk, j1 = min(n, j0 + jchunk);
+  if (i >= n) return;
+  for (int q = q0; q < nrhs; q += 4) {
+    double ar = 0.0, ai = 0.0;
+    for (int j = j0; j < j1; j++) {
+      double2 a = A[(size_t)i + (size_t)n * j], x = X[(size_t)j + (size_t)n * q];
+      ar += a.x * x.x - a.y * x.y;
+      ai += a.x * x.y + a.y * x.x;
+    }
+    atomicAdd(&R[(size_t)i + (size_t)n * q].x, -ar);
+    atomicAdd(&R[(size_t)i + (size_t)n * q].y, -ai);
+  }
+}
+
+__global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms) {
+  __shared__ double red[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  double m = 0.0;
+  for (int i = t; i < n; i += 256) {
+    double2 v = V[(size_t)i + (size_t)n * q];
+    m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+  }
+  red[t] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  if (t == 0) norms[q] = red[0];
+}
+
+__global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    X[i].x += D[i].x;
+    X[i].y += D[i].y;
+  }
+}
+
+namespace launch {
+void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info) {
+  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info);
+}
+void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
+  lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
+}
+void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
+  hipLaunchKernelGGL(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
+}
+void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info) {
+  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info);
+}
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
+  lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs);
+}
+void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
+              double* norms) {
+  (void)hipMemcpyAsync(R, B, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
+  const int jchunk = 256;
+  hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64, (n + jchunk - 1) / jchunk), dim3(256), 0, s, A, n, X, R, nrhs,
+                     jchunk);
+  hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms);
+  hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs);
+}
+void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count) {
+  hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, X, D, count);
+}
+}  // namespace launch
+}  // namespace hamsx

@@ -1,0 +1,88 @@
+"""ctypes binding of libhamsx.so (the C-ABI declared in include/hamsx.h).
+
+The library is built in-tree (``make -C hams-mrel_amd``); importing this
+module fails loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhamsx.so")
+
+HAMSX_OK = 0
+HAMSX_EARG = -1
+HAMSX_EDEVICE = -2
+HAMSX_ENOMEM = -3
+HAMSX_ESTATE = -4
+HAMSX_ENOCONV = -5
+HAMSX_NK = 200
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+#: every symbol include/hamsx.h declares, with (restype, argtypes)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+SIGNATURES = {
+    "hamsx_version": (C.c_char_p, []),
+    "hamsx_last_error": (C.c_char_p, []),
+    "hamsx_device_count": (C.c_int, []),
+    "hamsx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "hamsx_destroy": (C.c_int, [_vp]),
+    "hamsx_set_precision": (C.c_int, [_vp, C.c_int]),
+    "hamsx_panel_geometry": (C.c_int, [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
+    "hamsx_wave_properties": (C.c_int, [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
+    "hamsx_set_mesh": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip]),
+    "hamsx_set_frequency": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_int]),
+    "hamsx_set_waves": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
+    "hamsx_assemble": (C.c_int, [_vp]),
+    "hamsx_factor": (C.c_int, [_vp]),
+    "hamsx_solve_radiation": (C.c_int, [_vp, _dp]),
+    "hamsx_solve_diffraction": (C.c_int, [_vp, _dp]),
+    "hamsx_forces": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "hamsx_solve_frequency": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, _dp]),
+    "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
+    "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
+}
+
+_lib = None
+
+
+class HamsxError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed with code {code}: {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libhamsx.so not built at {LIB_PATH}; run `make -C hams-mrel_amd` "
+                          "(or __graft_entry__.build()) first")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, where: str, allow_info: bool = False) -> int:
+    """Raise on negative codes; positive INFO values are returned (LAPACK style)."""
+    if rc < 0 or (rc > 0 and not allow_info):
+        msg = load().hamsx_last_error().decode(errors="replace")
+        raise HamsxError(rc, where, msg)
+    return rc
+
+
+def ptr(a, ctype=C.c_double):
+    """Pointer to the first element of a contiguous array (C or Fortran order);
+    complex arrays are passed as interleaved doubles."""
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        raise ValueError("array must be contiguous")
+    return C.cast(a.ctypes.data, C.POINTER(ctype))

@@ -1,0 +1,138 @@
+"""Host-side mirror of HAMS-MREL's per-frequency solve on top of libhamsx.
+
+The method names follow the reference's driver sequence
+(``src/HAMS_Prog.f90:613-733``): ``calwaveproperts`` -> ``assemble``
+(CALGREEN_MULTI + ASSB_LEFT_MULTI) -> ``factor`` (ZGETRF) ->
+``radiation_solver`` / ``diffraction_solver`` (ASSB_RBC/DBC_MULTI + ZGETRS)
+-> ``forces`` (RFORCE_MULTI / EFORCE_MULTI).  Errors raise
+:class:`hamsx._lib.HamsxError`; a zero pivot is reported as INFO > 0 the way
+ZGETRF reports it (the reference ignores it; here it is returned).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import Combined
+
+
+def wave_properties(inft: int, value: float, h: float):
+    """CalWaveProperts (``src/ImplementSubs.f90:40-151``): returns v, wk, w1, wvn[200]."""
+    lib = _lib.load()
+    v, wk, w1 = C.c_double(), C.c_double(), C.c_double()
+    wvn = np.zeros(_lib.HAMSX_NK)
+    _lib.check(lib.hamsx_wave_properties(inft, value, h, C.byref(v), C.byref(wk), C.byref(w1), _lib.ptr(wvn)),
+               "hamsx_wave_properties")
+    return v.value, wk.value, w1.value, wvn
+
+
+class FrequencySolver:
+    """One device context holding a mesh resident in HBM."""
+
+    def __init__(self, mesh: Combined, device: int = 0, precision: str = "fp64"):
+        self._lib = _lib.load()
+        self.mesh = mesh
+        h = C.c_void_p()
+        _lib.check(self._lib.hamsx_create(C.byref(h), device), "hamsx_create")
+        self._h = h
+        self.set_precision(precision)
+        m = mesh
+        _lib.check(self._lib.hamsx_set_mesh(
+            self._h, m.nelem, m.nnode, _lib.ptr(m.xyz), _lib.ptr(m.ncn, C.c_int), _lib.ptr(m.ncon, C.c_int),
+            _lib.ptr(m.cen), _lib.ptr(m.area), _lib.ptr(m.pnsz), _lib.ptr(m.dxyz), m.nbody,
+            _lib.ptr(m.body_start, C.c_int)), "hamsx_set_mesh")
+        self.nbeta = 0
+
+    # -- lifecycle -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.hamsx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration ---------------------------------------------------------
+    def set_precision(self, precision: str):
+        mode = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        _lib.check(self._lib.hamsx_set_precision(self._h, mode), "hamsx_set_precision")
+        self.precision = precision
+
+    def set_waves(self, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1):
+        b = np.ascontiguousarray(headings_rad, dtype=np.float64)
+        self.nbeta = len(b)
+        _lib.check(self._lib.hamsx_set_waves(self._h, len(b), _lib.ptr(b), amp, xw[0], xw[1], rho, isol),
+                   "hamsx_set_waves")
+
+    # -- the reference's per-frequency sequence --------------------------------
+    def calwaveproperts(self, w1: float, h: float, inft: int = 3):
+        v, wk, w, wvn = wave_properties(inft, w1, h)
+        _lib.check(self._lib.hamsx_set_frequency(self._h, v, wk, w, h, _lib.ptr(wvn), len(wvn)),
+                   "hamsx_set_frequency")
+        return v, wk, w, wvn
+
+    def assemble(self):
+        _lib.check(self._lib.hamsx_assemble(self._h), "hamsx_assemble")
+
+    def factor(self) -> int:
+        return _lib.check(self._lib.hamsx_factor(self._h), "hamsx_factor", allow_info=True)
+
+    def radiation_solver(self) -> np.ndarray:
+        n, nd = self.mesh.nelem, 6 * self.mesh.nbody
+        out = np.zeros((n, nd), dtype=np.complex128, order="F")
+        _lib.check(self._lib.hamsx_solve_radiation(self._h, _lib.ptr(out)),
+                   "hamsx_solve_radiation")
+        return out
+
+    def diffraction_solver(self) -> np.ndarray:
+        n = self.mesh.nelem
+        out = np.zeros((n, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        _lib.check(self._lib.hamsx_solve_diffraction(self._h, _lib.ptr(out)),
+                   "hamsx_solve_diffraction")
+        return out[:, :self.nbeta]
+
+    def forces(self):
+        nd = 6 * self.mesh.nbody
+        amas = np.zeros((nd, nd), order="F")
+        bdmp = np.zeros((nd, nd), order="F")
+        exfc = np.zeros((nd, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        info = _lib.check(self._lib.hamsx_forces(self._h, _lib.ptr(amas), _lib.ptr(bdmp),
+                                                 _lib.ptr(exfc)), "hamsx_forces", allow_info=True)
+        return amas, bdmp, exfc[:, :self.nbeta], info
+
+    def solve_frequency(self, w1: float, h: float):
+        """One whole frequency solve (one device synchronisation)."""
+        nd = 6 * self.mesh.nbody
+        amas = np.zeros((nd, nd), order="F")
+        bdmp = np.zeros((nd, nd), order="F")
+        exfc = np.zeros((nd, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        info = _lib.check(self._lib.hamsx_solve_frequency(self._h, w1, h, _lib.ptr(amas), _lib.ptr(bdmp),
+                                                          _lib.ptr(exfc)),
+                          "hamsx_solve_frequency", allow_info=True)
+        return amas, bdmp, exfc[:, :self.nbeta], info
+
+    def get_matrix(self, which: str):
+        n = self.mesh.nelem
+        code = {"A": 0, "S": 1, "LU": 2}[which]
+        out = np.zeros((n, n), dtype=np.complex128, order="F")
+        ipiv = np.zeros(n, dtype=np.int32)
+        _lib.check(self._lib.hamsx_get_matrix(self._h, code, _lib.ptr(out),
+                                              _lib.ptr(ipiv, C.c_int) if code == 2 else None),
+                   "hamsx_get_matrix")
+        return (out, ipiv) if code == 2 else out
+
+    def refine_stats(self):
+        it, res = C.c_int(), C.c_double()
+        _lib.check(self._lib.hamsx_refine_stats(self._h, C.byref(it), C.byref(res)), "hamsx_refine_stats")
+        return it.value, res.value

@@ -1,0 +1,136 @@
+/*
+ * hamsx.h -- C-ABI of the MI355X (gfx950) frequency-domain panel solver.
+ *
+ * The boundary replaces the per-frequency subroutine calls of HAMS-MREL's
+ * driver loop (src/HAMS_Prog.f90:613-733, multi-body; :255-304 single body).
+ * Plain pointers and sizes only; every array is caller-owned host memory in
+ * Fortran column-major order, complex values are interleaved (re, im) doubles
+ * (COMPLEX*16 layout), panel connectivity is 1-based as in the .pnl files.
+ *
+ * Return codes: 0 = ok; > 0 = LAPACK-style INFO (1-based index of a zero
+ * pivot in the factorisation); < 0 = one of the HAMSX_E* codes below.
+ * A context is bound to one device and one HIP stream; it is not re-entrant.
+ * Many contexts may exist in one process.
+ */
+#ifndef HAMSX_H
+#define HAMSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HAMSX_OK 0
+#define HAMSX_EARG (-1)      /* bad argument (size, pointer, mode) */
+#define HAMSX_EDEVICE (-2)   /* HIP runtime error */
+#define HAMSX_ENOMEM (-3)    /* device allocation failed */
+#define HAMSX_ESTATE (-4)    /* call out of sequence (e.g. factor before assemble) */
+#define HAMSX_ENOCONV (-5)   /* iterative refinement did not reach its tolerance */
+
+#define HAMSX_NK 200         /* roots of the dispersion relation kept (WaveDyn_mod NK) */
+
+/* precision modes for hamsx_set_precision */
+#define HAMSX_PREC_FP64 0    /* complex128 LU, the reference's arithmetic */
+#define HAMSX_PREC_MIXED 1   /* complex64 LU + FP64 iterative refinement */
+
+typedef struct hamsx_ctx hamsx_ctx;
+
+/* Library/version string; never NULL. */
+const char *hamsx_version(void);
+
+/* Last error message recorded by the calling thread (empty if none). */
+const char *hamsx_last_error(void);
+
+/* Number of visible GPUs (0 if none or the runtime is missing). */
+int hamsx_device_count(void);
+
+/* ---- context ---------------------------------------------------------- */
+int hamsx_create(hamsx_ctx **ctx, int device);
+int hamsx_destroy(hamsx_ctx *ctx);
+int hamsx_set_precision(hamsx_ctx *ctx, int mode);
+
+/* ---- host-side set-up routines (no device work) -------------------------
+ * Panel centroid, area (two Heron triangles per quad), maximum edge length
+ * and the six generalized normals about reference point xr[3].
+ * Replaces CalPanelCentre/CalPanelArea/CalPanelChartLength/CalTransNormals/
+ * CalRotNormals, src/NormalProcess.f90:46-441, as called by CalNormalsMulti,
+ * src/ReadPanelMeshMulti.f90:150-156.
+ *   node_xyz[nnode*3], ncn[nelem], ncon[nelem*4] (1-based);
+ *   cen[nelem*3], area[nelem], pnsz[nelem], dxyz[nelem*6] are outputs. */
+int hamsx_panel_geometry(int nnode, const double *node_xyz, int nelem, const int *ncn,
+                         const int *ncon, const double *xr, double *cen, double *area,
+                         double *pnsz, double *dxyz);
+
+/* Wave numbers for a frequency given as input type inft (1 deep-water wave
+ * number, 2 finite-depth wave number, 3 rad/s, 4 period, 5 wave length),
+ * depth h (< 0: infinite).  Outputs v = w^2/g, wk, w1 and the NK dispersion
+ * roots wvn[HAMSX_NK] (finite depth only; zeros otherwise).
+ * Replaces CalWaveProperts, src/ImplementSubs.f90:40-151 with DISPERSION,
+ * src/WavDynSubs.f90:40-127. */
+int hamsx_wave_properties(int inft, double value, double h, double *v, double *wk,
+                          double *w1, double *wvn);
+
+/* ---- mesh (device-resident, frequency-invariant) ------------------------
+ * The combined multi-body arrays of BuildCombinedArrays,
+ * src/CalGreenFuncMulti.f90:53-106: nodes XYZ_GLOBAL_MULTI_COMB(nnode,3),
+ * NCN/NCON_MULTI_COMB, centroids XYZ_GLOBAL_MULTI_COMB_P(nelem,3),
+ * DS/PNSZ_MULTI_COMB, DXYZ_MULTI_COMB(nelem,6).  Bodies occupy consecutive
+ * panel ranges: body b owns panels [body_start[b], body_start[b+1]) (0-based,
+ * body_start has nbody+1 entries).  Only NSYS = 1 (no geometric symmetry). */
+int hamsx_set_mesh(hamsx_ctx *ctx, int nelem, int nnode, const double *node_xyz,
+                   const int *ncn, const int *ncon, const double *cen, const double *area,
+                   const double *pnsz, const double *dxyz, int nbody, const int *body_start);
+
+/* ---- per-frequency state -------------------------------------------------
+ * The scalars CalWaveProperts leaves in WaveDyn_mod (V, WK, W1, H, WVN). */
+int hamsx_set_frequency(hamsx_ctx *ctx, double v, double wk, double w1, double h,
+                        const double *wvn, int nk);
+
+/* Incident-wave parameters read by ASSB_DBC_MULTI / EFORCE_MULTI from module
+ * state: headings beta[nbeta] in radians, wave amplitude, reference point
+ * XW(2), fluid density and ISOL (1: excitation includes the incident
+ * potential, 2: diffraction potential only). */
+int hamsx_set_waves(hamsx_ctx *ctx, int nbeta, const double *beta, double amp, double xw0,
+                    double xw1, double rho, int isol);
+
+/* CALGREEN_MULTI + ASSB_LEFT_MULTI (src/CalGreenFuncMulti.f90:112-183,
+ * src/AssbMatxMulti.f90:53-105) without ZGETRF: builds the influence matrix
+ * A(N,N) and the single-layer operator S(N,N) in device memory. */
+int hamsx_assemble(hamsx_ctx *ctx);
+
+/* ZGETRF of A (src/AssbMatxMulti.f90:110-114).  Returns INFO. */
+int hamsx_factor(hamsx_ctx *ctx);
+
+/* ASSB_RBC_MULTI + RADIATION_SOLVER_MULTI (src/AssbMatxMulti.f90:124-181,
+ * :257-317): mxpot[nelem*6*nbody] (complex) receives the radiation
+ * potentials; pass NULL to keep them on the device only. */
+int hamsx_solve_radiation(hamsx_ctx *ctx, double *mxpot);
+
+/* ASSB_DBC_MULTI + DIFFRACTION_SOLVER_MULTI (src/AssbMatxMulti.f90:187-251,
+ * :323-383) for every heading set by hamsx_set_waves: mxpot[nelem*nbeta]
+ * (complex) receives the diffraction potentials, or NULL. */
+int hamsx_solve_diffraction(hamsx_ctx *ctx, double *mxpot);
+
+/* RFORCE_MULTI / EFORCE_MULTI reductions (src/PotentWavForceMulti.f90:45-226):
+ * amas, bdmp [(6nb)*(6nb)] column-major, AMAS_MULTI_COMB(KK,:,:) layout;
+ * exfc [(6nb)*nbeta] complex, EXFC_MULTI_COMB(KK,:,:) transposed to
+ * (dof, heading).  Any output pointer may be NULL. */
+int hamsx_forces(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
+
+/* One whole frequency solve: wave properties of w1 (rad/s) at the mesh depth
+ * h, assembly, factorisation, radiation + all diffraction solves and the
+ * force reductions, with a single device synchronisation. */
+int hamsx_solve_frequency(hamsx_ctx *ctx, double w1, double h, double *amas, double *bdmp,
+                          double *exfc);
+
+/* Parity/debug: copy A (which = 0), S (1) or the LU factors (2) to host
+ * out[nelem*nelem] (complex); ipiv[nelem] (1-based) for which = 2, else NULL. */
+int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
+
+/* Refinement statistics of the last mixed-precision solve: iterations used
+ * and the final max-norm relative residual. */
+int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

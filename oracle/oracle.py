@@ -1,0 +1,164 @@
+"""ctypes wrapper of the C oracle (liboracle.so) and of the reference checker
+(oracle/_ref/libhams_ref.so, built from the reference's Fortran).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libhams_ref.so")
+NK = 200
+G = 9.80665
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_d = C.c_double
+_o = None
+_r = None
+
+
+def _p(a, t=C.c_double):
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        raise ValueError("array must be contiguous")
+    return C.cast(a.ctypes.data, C.POINTER(t))
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            raise ImportError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        o = C.CDLL(ORACLE_SO)
+        for name in ("orc_error", "orc_e1xa", "orc_eix", "orc_struve_h0", "orc_struve_h1", "orc_bessel_j0",
+                     "orc_bessel_j1"):
+            getattr(o, name).restype = _d
+            getattr(o, name).argtypes = [_d]
+        o.orc_enxa.restype = _d
+        o.orc_enxa.argtypes = [C.c_int, _d]
+        o.orc_limes.restype = _d
+        o.orc_limes.argtypes = [C.c_int, _dp, _ip, _ip, _dp]
+        o.orc_ik01a.argtypes = [_d, _dp, _dp]
+        o.orc_jy01b.argtypes = [_d, _dp, _dp, _dp, _dp]
+        o.orc_dispersion.argtypes = [_d, _d, C.c_int, _dp]
+        o.orc_infgreen3d.argtypes = [_d] * 7 + [C.c_int, _dp]
+        o.orc_fingreen3d.argtypes = [_d] * 7 + [_dp, C.c_int, _d, C.c_int, _dp]
+        o.orc_sglint.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        o.orc_dinp.argtypes = [_d] * 11 + [_dp]
+        o.orc_vinp.argtypes = [_d] * 11 + [_dp]
+        o.orc_panel_geometry.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+        o.orc_assemble.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _d, _d, _dp, C.c_int,
+                                   _dp, _dp, C.c_int]
+        o.orc_assemble_entry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp,
+                                         _d, _d, _dp, C.c_int, _dp, _dp]
+        o.orc_rhs.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _d, _d, _d, _d, _d, _d, _d, _dp]
+        o.orc_zgetrf.restype = C.c_int
+        o.orc_zgetrf.argtypes = [C.c_int, _dp, _ip]
+        o.orc_zgetrs.argtypes = [C.c_int, _dp, _ip, C.c_int, _dp]
+        o.orc_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _d, _d, _d, _d, _d, _d,
+                                 _d, _d, C.c_int, _dp, _dp, _dp]
+        o.orc_solve_frequency.restype = C.c_int
+        o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
+                                          _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
+        _o = o
+    return _o
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own Fortran routines (built here from /root/reference)."""
+    global _r
+    if _r is None:
+        r = C.CDLL(REF_SO)
+        for name in ("ref_error", "ref_e1xa", "ref_eix", "ref_struve_h0", "ref_struve_h1", "ref_bessel_j0",
+                     "ref_bessel_j1"):
+            getattr(r, name).restype = _d
+            getattr(r, name).argtypes = [_d]
+        r.ref_enxa.restype = _d
+        r.ref_enxa.argtypes = [C.c_int, _d]
+        r.ref_limes.restype = _d
+        r.ref_limes.argtypes = [C.c_int, _dp]
+        r.ref_ik01a.argtypes = [_d, _dp, _dp]
+        r.ref_jy01b.argtypes = [_d, _dp, _dp, _dp, _dp]
+        r.ref_dispersion.argtypes = [_d, _d, C.c_int, _dp]
+        r.ref_wave_properties.argtypes = [C.c_int, _d, _d, _dp, _dp, _dp, _dp]
+        r.ref_infgreen3d.argtypes = [_d] * 7 + [C.c_int, _dp]
+        r.ref_fingreen3d.argtypes = [_d] * 7 + [_dp, C.c_int, _d, C.c_int, _dp]
+        r.ref_set_mesh.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp]
+        r.ref_sglint.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        r.ref_dinp.argtypes = [_d] * 11 + [_dp]
+        r.ref_vinp.argtypes = [_d] * 11 + [_dp]
+        r.ref_geometry.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+        _r = r
+    return _r
+
+
+# ---------------------------------------------------------------------------
+# Convenience wrappers over combined meshes (hamsx.mesh.Combined layout)
+# ---------------------------------------------------------------------------
+def wave(w1: float, h: float):
+    """CalWaveProperts for a frequency in rad/s: v, wk, wvn."""
+    v = w1 * w1 / G
+    wvn = np.zeros(NK)
+    if h > 0:
+        lib().orc_dispersion(w1, h, NK, _p(wvn))
+        wk = wvn[0]
+    else:
+        wk = v
+    return v, wk, wvn
+
+
+def geometry(mesh, xr):
+    ne = mesh.nelem
+    xyz = np.asfortranarray(mesh.xyz, dtype=np.float64)
+    ncn = np.ascontiguousarray(mesh.ncn, dtype=np.int32)
+    ncon = np.asfortranarray(mesh.ncon, dtype=np.int32)
+    cen = np.zeros((ne, 3), order="F")
+    area, pnsz = np.zeros(ne), np.zeros(ne)
+    dxyz = np.zeros((ne, 6), order="F")
+    xr = np.asarray(xr, dtype=np.float64)
+    lib().orc_panel_geometry(mesh.nnode, _p(xyz), ne, _p(ncn, C.c_int), _p(ncon, C.c_int), _p(xr), _p(cen),
+                             _p(area), _p(pnsz), _p(dxyz))
+    return cen, area, pnsz, dxyz
+
+
+def assemble(m, w1: float, h: float, nthreads: int = 8):
+    v, wk, wvn = wave(w1, h)
+    n = m.nelem
+    A = np.zeros((n, n), dtype=np.complex128, order="F")
+    S = np.zeros((n, n), dtype=np.complex128, order="F")
+    lib().orc_assemble(n, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen), _p(m.area),
+                       _p(m.pnsz), _p(m.dxyz), v, h, _p(wvn), NK, _p(A), _p(S),
+                       nthreads)
+    return A, S
+
+
+def assemble_entry(m, i: int, j: int, w1: float, h: float):
+    v, wk, wvn = wave(w1, h)
+    a, s = np.zeros(2), np.zeros(2)
+    lib().orc_assemble_entry(i, j, m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+                             _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), v, h, _p(wvn), NK, _p(a), _p(s))
+    return complex(a[0], a[1]), complex(s[0], s[1])
+
+
+def solve_frequency(m, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1,
+                    nthreads: int = 8):
+    nd = 6 * m.nbody
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    amas = np.zeros((nd, nd), order="F")
+    bdmp = np.zeros((nd, nd), order="F")
+    exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
+    info = lib().orc_solve_frequency(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+                                     _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
+                                     _p(m.body_start, C.c_int), w1, h, len(beta), _p(beta), amp, xw[0], xw[1], rho,
+                                     isol, _p(amas), _p(bdmp), _p(exfc), nthreads)
+    return amas, bdmp, exfc[:, :len(beta)], info

@@ -1,0 +1,141 @@
+"""GPU parity of the HIP path against the C oracle (FP64 restatement of the
+reference, pinned to the reference's own Fortran in test_oracle_golden.py).
+
+Tolerances: influence-matrix entries 1e-12 normwise relative to max|A|
+(FP64 with FMA contraction on the device against the reference's unfused
+FP64); added mass, damping and excitation 1e-8 normwise relative per array
+(the plain FP64 path) and 1e-6 for the mixed-precision path.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from hamsx import configs
+from hamsx.mesh import combine, cylinder
+from hamsx.solver import FrequencySolver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_pair(na=24, nz=6, nr=3, gap=30.0):
+    bodies = [cylinder(4.0, 8.0, na, nz, nr, -gap / 2), cylinder(3.0, 6.0, na, nz, nr, gap / 2)]
+    xr = [np.array([-gap / 2, 0, 0]), np.array([gap / 2, 0, 0])]
+    return combine(bodies, xr)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def mesh_small():
+    return small_pair()
+
+
+@pytest.mark.parametrize("depth,w", [(-1.0, 1.0), (-1.0, 0.35), (20.0, 0.8), (50.0, 1.5), (200.0, 2.5)])
+def test_assembly_matches_oracle(mesh_small, depth, w):
+    m = mesh_small
+    say(f"[gpu] assembly n={m.nelem} depth={depth} w={w}")
+    with FrequencySolver(m) as s:
+        s.calwaveproperts(w, depth)
+        s.assemble()
+        A = s.get_matrix("A")
+        S = s.get_matrix("S")
+    Ao, So = oracle.assemble(m, w, depth)
+    ea, es = rel(A, Ao), rel(S, So)
+    say(f"    rel err A {ea:.3e}  S {es:.3e}")
+    assert ea < 1e-12 and es < 1e-12
+
+
+def test_lu_backward_error():
+    m = small_pair(na=40, nz=8, nr=4)   # 2 x 480 panels: three tournament levels
+    say(f"[gpu] LU n={m.nelem}")
+    with FrequencySolver(m) as s:
+        s.calwaveproperts(1.2, 40.0)
+        s.assemble()
+        A = s.get_matrix("A")
+        info = s.factor()
+        LU, ipiv = s.get_matrix("LU")
+    assert info == 0
+    n = m.nelem
+    PA = A.copy()
+    for k in range(n):
+        p = ipiv[k] - 1
+        if p != k:
+            PA[[k, p], :] = PA[[p, k], :]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    say(f"    ||PA - LU|| / ||A|| = {err:.3e}")
+    assert err < 1e-13
+
+
+@pytest.mark.parametrize("depth", [-1.0, 30.0])
+def test_frequency_solve_matches_oracle(mesh_small, depth):
+    m = mesh_small
+    heads = np.deg2rad([0.0, 45.0, 90.0])
+    for w in (0.5, 1.3):
+        say(f"[gpu] solve_frequency n={m.nelem} depth={depth} w={w}")
+        with FrequencySolver(m) as s:
+            s.set_waves(heads)
+            am, bd, ex, info = s.solve_frequency(w, depth)
+        ao, bo, eo, io = oracle.solve_frequency(m, w, depth, heads)
+        assert info == 0 and io == 0
+        errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+        say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e}")
+        assert max(errs) < 1e-8
+
+
+def test_step_sequence_equals_fused_call(mesh_small):
+    """The reference-named steps and the fused call give the same numbers."""
+    m = mesh_small
+    heads = np.deg2rad([0.0, 90.0])
+    say(f"[gpu] step sequence n={m.nelem}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        a1, b1, e1, _ = s.solve_frequency(0.9, 35.0)
+        s.calwaveproperts(0.9, 35.0)
+        s.assemble()
+        assert s.factor() == 0
+        rad = s.radiation_solver()
+        dif = s.diffraction_solver()
+        a2, b2, e2, _ = s.forces()
+    assert rad.shape == (m.nelem, 12) and dif.shape == (m.nelem, 2)
+    np.testing.assert_array_equal(a1, a2)
+    np.testing.assert_array_equal(b1, b2)
+    np.testing.assert_array_equal(e1, e2)
+
+
+@pytest.mark.parametrize("depth", [-1.0, 45.0])
+def test_mixed_precision_within_1e6(mesh_small, depth):
+    m = mesh_small
+    heads = np.deg2rad([0.0, 60.0])
+    w = 1.1
+    say(f"[gpu] mixed n={m.nelem} depth={depth}")
+    with FrequencySolver(m, precision="mixed") as s:
+        s.set_waves(heads)
+        am, bd, ex, info = s.solve_frequency(w, depth)
+        it, res = s.refine_stats()
+    ao, bo, eo, _ = oracle.solve_frequency(m, w, depth, heads)
+    errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+    say(f"    mixed rel err {errs}  refine iters {it} resid {res:.2e}")
+    assert max(errs) < 1e-6
+
+
+def test_single_cylinder_config():
+    case = configs.single_cylinder(na=24, nz=6, nr=3)
+    m = combine(case.bodies, case.xr)
+    heads = np.deg2rad(case.headings_deg)
+    say(f"[gpu] single cylinder n={m.nelem}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        am, bd, ex, info = s.solve_frequency(float(case.omegas[0]), case.depth)
+    ao, bo, eo, _ = oracle.solve_frequency(m, float(case.omegas[0]), case.depth, heads)
+    assert max(rel(am, ao), rel(bd, bo), rel(ex, eo)) < 1e-8
