@@ -253,11 +253,11 @@ namespace launch {
 
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags) {
   dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
+  HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
   if (f.h < 0.0)
-    hipLaunchKernelGGL(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
+    HX_LAUNCH(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
   else
-    hipLaunchKernelGGL(k_assemble<false>, grid, dim3(256), 0, s, m, f, A, S, flags);
+    HX_LAUNCH(k_assemble<false>, grid, dim3(256), 0, s, m, f, A, S, flags);
 }
 
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
@@ -266,17 +266,17 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
   (void)body_start_h;
   (void)body_start_d;
   const int n = m.n;
-  if (w.nbeta > 0) hipLaunchKernelGGL(k_dum2, dim3((n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, dum2);
+  if (w.nbeta > 0) HX_LAUNCH(k_dum2, dim3((n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, dum2);
   int ngroups = w.nbeta > 0 ? (w.nbeta + kBetaGroup - 1) / kBetaGroup : 1;
-  hipLaunchKernelGGL(k_rhs_partial, dim3((n + 63) / 64, nchunk, ngroups), dim3(64), 0, s, n, m.nrm, S, dum2,
+  HX_LAUNCH(k_rhs_partial, dim3((n + 63) / 64, nchunk, ngroups), dim3(64), 0, s, n, m.nrm, S, dum2,
                      w.nbeta, chunk_j0_d, chunk_j1_d, partial);
-  hipLaunchKernelGGL(k_rhs_reduce, dim3((n + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, n, nbody, w.nbeta,
+  HX_LAUNCH(k_rhs_reduce, dim3((n + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, n, nbody, w.nbeta,
                      nchunk, chunk_body_d, partial, X);
 }
 
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
             const double2* X, double* out) {
-  hipLaunchKernelGGL(k_forces, dim3(nbody, 6 * nbody + w.nbeta), dim3(256), 0, s, m, f, w, nbody, body_start_d, X,
+  HX_LAUNCH(k_forces, dim3(nbody, 6 * nbody + w.nbeta), dim3(256), 0, s, m, f, w, nbody, body_start_d, X,
                      out);
 }
 

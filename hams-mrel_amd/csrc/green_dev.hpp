@@ -57,7 +57,15 @@ __device__ __forceinline__ double ipow(double a, int b) {
 
 // ---------------------------------------------------------------------------
 // Special functions (FinGrnExtSubs.f90)
+//
+// From here to the end of the finite-depth section, FMA contraction is off:
+// the epsilon extrapolation of EIGENE / PIDCOCK and the adaptive quadrature
+// of LINTON turn last-bit differences of their inputs into differences of up
+// to ~1e-5 in their outputs (two FP64 builds of the reference's own Fortran,
+// with and without FMA, disagree by that much), so the series terms are
+// formed with the reference's rounding and operation order.
 // ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
 
 // ERROR: FinGrnExtSubs.f90:62-93
 __device__ inline double d_erf(double x) {
@@ -222,14 +230,12 @@ __device__ inline void d_jy01b(double x, double& bj0, double& bj1, double& by0, 
   double t = 4.0 / x, t2 = t * t, a0 = sqrt(2.0 / (kPi * x));
   double p0 = ((((-.9285e-5 * t2 + .43506e-4) * t2 - .122226e-3) * t2 + .434725e-3) * t2 - .4394275e-2) * t2 + .999999997;
   double q0 = t * (((((.8099e-5 * t2 - .35614e-4) * t2 + .85844e-4) * t2 - .218024e-3) * t2 + .1144106e-2) * t2 - .031249995);
-  double s0, c0;
-  sincos(x - .25 * kPi, &s0, &c0);
+  const double ta0 = x - .25 * kPi, s0 = sin(ta0), c0 = cos(ta0);
   bj0 = a0 * (p0 * c0 - q0 * s0);
   by0 = a0 * (p0 * s0 + q0 * c0);
   double p1 = ((((.10632e-4 * t2 - .50363e-4) * t2 + .145575e-3) * t2 - .559487e-3) * t2 + .7323931e-2) * t2 + 1.000000004;
   double q1 = t * (((((-.9173e-5 * t2 + .40658e-4) * t2 - .99941e-4) * t2 + .266891e-3) * t2 - .1601836e-2) * t2 + .093749994);
-  double s1, c1;
-  sincos(x - .75 * kPi, &s1, &c1);
+  const double ta1 = x - .75 * kPi, s1 = sin(ta1), c1 = cos(ta1);
   bj1 = a0 * (p1 * c1 - q1 * s1);
   by1 = a0 * (p1 * s1 + q1 * c1);
 }
@@ -262,7 +268,9 @@ __device__ inline double d_limes(int n, double* s, double* a) {
 
 // ---------------------------------------------------------------------------
 // Deep water: INFGREEN3D (InfGreen_Appr.f90:104-186), Havelock global fit
+// (well conditioned: contraction back on)
 // ---------------------------------------------------------------------------
+#pragma clang fp contract(fast)
 __device__ inline double d_bj0(double xx) {  // InfGreen_Appr.f90:583-626
   if (xx <= 3.0) {
     double yy = (xx / 3.0) * (xx / 3.0);
@@ -399,6 +407,7 @@ __device__ inline void d_infgreen(const PairGeom& p, double v, int flag, cd g[4]
 // ---------------------------------------------------------------------------
 // Finite depth: FINGREEN3D (FinGreen3D.f90:81-226) and its four regions
 // ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
 struct FreqParams {
   double v, wk, w1, h;
   const double* wvn;  // HAMSX_NK roots, wvn[0] = wk
@@ -424,13 +433,11 @@ __device__ __forceinline__ void d_eigen_term(double r, double zf, double zp, dou
   double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
   double k0, k1;
   d_ik01a(km * r, k0, k1);
-  double skzf, ckzf;
-  sincos(km * (zf + h), &skzf, &ckzf);
-  double ckzp = cos(km * (zp + h));
-  double c = pm * ckzp;
-  rm[0] = c * k0 * ckzf;
-  rm[1] = -(c * km * k1 * ckzf);
-  rm[2] = -(c * km * k0 * skzf);
+  const double kzf = km * (zf + h), kzp = km * (zp + h);
+  const double ckzf = cos(kzf), ckzp = cos(kzp), skzf = sin(kzf);
+  rm[0] = pm * k0 * ckzf * ckzp;
+  rm[1] = -(pm * km * k1 * ckzf * ckzp);
+  rm[2] = -(pm * km * k0 * skzf * ckzp);
 }
 
 // Rankine-term removal shared by EIGEN, EIGENE and PIDCOCK
@@ -537,18 +544,15 @@ __device__ __noinline__ void d_pidcock(double r, double zf, double zp, const Fre
   fg[1].r += 2.0 / (r * h);
   g[0] = gr[0] = gz[0] = 0.0;
   for (int i = 2; i <= nt; i++) {
-    double umt = (i - 1) * kPi / h, um = fp.wvn[i - 1];
+    double mp = (i - 1) * kPi, umt = mp / h, um = fp.wvn[i - 1];
     double k0, k1, k0t, k1t;
     d_ik01a(um * r, k0, k1);
     d_ik01a(umt * r, k0t, k1t);
-    double u2v2 = um * um + v * v;
-    double ac = u2v2 / (u2v2 * h - v);
-    double su_f, cu_f, st_f, ct_f;
-    sincos(um * (zf + h), &su_f, &cu_f);
-    sincos(umt * (zf + h), &st_f, &ct_f);
-    double cu_p = cos(um * (zp + h)), ct_p = cos(umt * (zp + h));
-    double am = ac * cu_f * cu_p, amt = ct_f * ct_p / h;
-    double dam = -(um * ac * su_f * cu_p), damt = -(umt * st_f * ct_p / h);
+    double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
+    double am = ac * cos(um * (zf + h)) * cos(um * (zp + h));
+    double amt = cos(umt * (zf + h)) * cos(umt * (zp + h)) / h;
+    double dam = -(um * ac * sin(um * (zf + h)) * cos(um * (zp + h)));
+    double damt = -(umt * sin(umt * (zf + h)) * cos(umt * (zp + h)) / h);
     g[i - 1] = g[i - 2] + (am * k0 - amt * k0t);
     gr[i - 1] = gr[i - 2] + (-(um * am * k1) + umt * amt * k1t);
     gz[i - 1] = gz[i - 2] + (dam * k0 - damt * k0t);
@@ -840,6 +844,7 @@ __device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int f
   }
 }
 
+#pragma clang fp contract(fast)
 // ---------------------------------------------------------------------------
 // Newman (1986) panel integral, SGLINTBD_MULTI_TRI/QUAD (SingularIntgrMulti.f90:44-445).
 // The per-panel local frame (origin, unit vectors, projected vertices) is

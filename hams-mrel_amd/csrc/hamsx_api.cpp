@@ -12,11 +12,22 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "hamsx_internal.hpp"
+
+namespace hamsx {
+bool debug_launches() {
+  static const bool on = std::getenv("HAMSX_DEBUG") != nullptr;
+  return on;
+}
+void report_launch(const char* name, hipError_t e) {
+  if (e != hipSuccess) std::fprintf(stderr, "hamsx: launch of %s failed: %s\n", name, hipGetErrorString(e));
+}
+}  // namespace hamsx
 
 namespace {
 
@@ -216,6 +227,16 @@ struct hamsx_ctx {
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   int last_iters = 0;
   double last_resid = 0.0;
+  // stage timing (measurement hook)
+  bool profiling = false;
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;     // recorded, not yet read
+  std::vector<hipEvent_t> pool;
+  double stage_ms[HAMSX_NSTAGE] = {0};
+  int stage_cnt[HAMSX_NSTAGE] = {0};
 
   hamsx::MeshDev mesh() const {
     return hamsx::MeshDev{n, cx.p, cy.p, cz.p, area.p, pnsz.p, nrm.p, ncn.p, frame.p, body.p};
@@ -236,6 +257,50 @@ int ensure_pinned(hamsx_ctx* c, size_t len) {
   HX_TRY(hipHostMalloc((void**)&c->h_pinned, sizeof(double) * len, hipHostMallocDefault));
   c->h_pinned_len = len;
   return 0;
+}
+
+hipEvent_t take_event(hamsx_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one stage with events when profiling is on.
+struct StageTimer {
+  hamsx_ctx* c;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageTimer(hamsx_ctx* ctx, int st) : c(ctx), stage(st) {
+    if (!c->profiling) return;
+    a = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->marks.push_back({stage, a, b});
+  }
+};
+
+// Reads the recorded event pairs (the stream must have been synchronised).
+void harvest_marks(hamsx_ctx* c) {
+  for (auto& m : c->marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+      c->stage_ms[m.stage] += ms;
+      c->stage_cnt[m.stage] += 1;
+    }
+    c->pool.push_back(m.a);
+    c->pool.push_back(m.b);
+  }
+  c->marks.clear();
 }
 
 int set_device(hamsx_ctx* c) {
@@ -269,6 +334,7 @@ int do_rhs(hamsx_ctx* c) {
   if (!c->have_waves) return fail(HAMSX_ESTATE, "right-hand sides need hamsx_set_waves first");
   int rc = ensure_work(c);
   if (rc) return rc;
+  StageTimer tm(c, 1);
   hamsx::launch::rhs(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start.data(), c->body_start_d.p,
                      c->S.p, c->dum2.p, c->partial.p, c->nchunk, c->chunk_body.p, c->chunk_j0.p, c->chunk_j1.p,
                      c->X.p);
@@ -285,6 +351,7 @@ int do_factor(hamsx_ctx* c) {
   if ((rc = c->ipiv.ensure(n)) || (rc = c->perm.ensure(n)) ||
       (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)))
     return rc;
+  StageTimer tm(c, 2);
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1);
     c->have_A = false;  // factored in place
@@ -304,6 +371,7 @@ int do_solve(hamsx_ctx* c) {
   if (rc) return rc;
   if ((rc = do_factor(c))) return rc;
   const int n = c->n, nr = c->nrhs();
+  StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_solve(c->stream, c->A.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
   } else {
@@ -333,10 +401,15 @@ int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
 int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* info_out) {
   const int nd = 6 * c->nbody;
   const size_t len = c->out_len();
-  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 8);
+  // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 2 ints][norms: 2 * nrhs]
+  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 2 + 2 * (size_t)c->nrhs());
   if (rc) return rc;
   double* hout = c->h_pinned + hamsx::kNK + c->nbeta;
-  hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p, c->out.p);
+  {
+    StageTimer tm(c, 4);
+    hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p,
+                          c->out.p);
+  }
   if ((rc = check_launch())) return rc;
   HX_TRY(hipMemcpyAsync(hout, c->out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
@@ -344,6 +417,7 @@ int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* i
     HX_TRY(hipMemcpyAsync(hout + len + 2, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
                           c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
+  harvest_marks(c);
   if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
   if (bdmp) std::memcpy(bdmp, hout + (size_t)nd * nd, sizeof(double) * nd * nd);
   if (exfc) std::memcpy(exfc, hout + 2 * (size_t)nd * nd, sizeof(double) * 2 * nd * c->nbeta);
@@ -415,6 +489,8 @@ int hamsx_destroy(hamsx_ctx* c) {
     b->release();
   for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial}) b->release();
   c->Af.release();
+  for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
+  for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -621,7 +697,8 @@ int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, 
   if (h > 0.0 && (!wvn || nk < hamsx::kNK)) return fail(HAMSX_EARG, "finite depth needs %d dispersion roots", hamsx::kNK);
   int rc = set_device(c);
   if (rc) return rc;
-  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 8))) return rc;
+  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 2 + 2 * (size_t)c->nrhs())))
+    return rc;
   // the staging slot is free: every earlier use ended in a stream synchronisation
   HX_TRY(hipStreamSynchronize(c->stream));
   for (int k = 0; k < hamsx::kNK; k++) c->h_pinned[k] = (h > 0.0 && wvn) ? wvn[k] : 0.0;
@@ -659,7 +736,10 @@ int hamsx_assemble(hamsx_ctx* c) {
   if (!c->have_freq) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_frequency first");
   int rc = set_device(c);
   if (rc) return rc;
-  hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
+  {
+    StageTimer tm(c, 0);
+    hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
+  }
   if ((rc = check_launch())) return rc;
   c->have_A = c->have_S = true;
   c->have_LU = c->have_X = false;
@@ -746,6 +826,29 @@ int hamsx_refine_stats(hamsx_ctx* c, int* iters, double* resid) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
   if (iters) *iters = c->last_iters;
   if (resid) *resid = c->last_resid;
+  return 0;
+}
+
+int hamsx_set_profiling(hamsx_ctx* c, int on) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  c->profiling = on != 0;
+  return 0;
+}
+
+int hamsx_stage_times(hamsx_ctx* c, int reset, double* ms, int* count) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  harvest_marks(c);
+  for (int k = 0; k < HAMSX_NSTAGE; k++) {
+    if (ms) ms[k] = c->stage_ms[k];
+    if (count) count[k] = c->stage_cnt[k];
+    if (reset) {
+      c->stage_ms[k] = 0.0;
+      c->stage_cnt[k] = 0;
+    }
+  }
   return 0;
 }
 

@@ -6,6 +6,16 @@
 
 namespace hamsx {
 
+// Launch checks: with HAMSX_DEBUG set in the environment every launch is
+// followed by hipGetLastError, and a failure is reported with the kernel name.
+bool debug_launches();
+void report_launch(const char* name, hipError_t e);
+#define HX_LAUNCH(kernel, ...)                                                      \
+  do {                                                                              \
+    hipLaunchKernelGGL(kernel, __VA_ARGS__);                                        \
+    if (::hamsx::debug_launches()) ::hamsx::report_launch(#kernel, hipGetLastError()); \
+  } while (0)
+
 constexpr int kNK = 200;      // dispersion roots kept (WaveDyn_mod NK)
 constexpr int kPanelNB = 32;  // LU panel width
 constexpr int kTS = 128;      // rows per tournament block

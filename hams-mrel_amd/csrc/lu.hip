@@ -31,6 +31,8 @@ struct CT<double> {
   static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // accumulator element q of lane group g (= lane / 16) holds output row 4q + g
+  static __device__ __forceinline__ int out_row(int g, int q) { return 4 * q + g; }
 };
 template <>
 struct CT<float> {
@@ -39,6 +41,8 @@ struct CT<float> {
   static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // accumulator element q of lane group g (= lane / 16) holds output row 4g + q
+  static __device__ __forceinline__ int out_row(int g, int q) { return 4 * g + q; }
 };
 
 template <typename R>
@@ -243,12 +247,12 @@ __global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, 
   }
 }
 
-// Diagonal block LU (no pivoting: its rows are the tournament winners in
-// pivot order), then L21 = A21 U11^-1 (blocks [0, nrb)) and U12 = L11^-1 A12
-// (blocks [nrb, nrb + ncb)), one row or column per thread.
+// Diagonal block LU without pivoting (its rows are the tournament winners in
+// pivot order), one block, written back in place; records the first exactly
+// zero pivot in info (ZGETRF's INFO).
 template <typename R>
-__global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
-                                                      int nb, int nrb, int* __restrict__ info) {
+__global__ __launch_bounds__(256) void k_diag_factor(typename CT<R>::c2* __restrict__ A, int lda, int col0, int nb,
+                                                     int* __restrict__ info) {
   __shared__ R dr[kPanelNB][kPanelNB + 1];
   __shared__ R di[kPanelNB][kPanelNB + 1];
   const int t = threadIdx.x;
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __rest
   }
   __syncthreads();
   for (int c = 0; c < nb; c++) {
-    R ir, ii;
+    R ir = R(0), ii = R(0);
     const bool nz = (dr[c][c] != R(0) || di[c][c] != R(0));
     if (nz) crecip(dr[c][c], di[c][c], ir, ii);
     __syncthreads();
@@ -280,18 +284,42 @@ __global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __rest
     }
     __syncthreads();
   }
-  if (blockIdx.x == 0) {
-    for (int e = t; e < nb * nb; e += 256) {
-      int r = e % nb, c = e / nb;
-      A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = {dr[r][c], di[r][c]};
-    }
-    if (t == 0 && info[0] == 0)
-      for (int c = 0; c < nb; c++)
-        if (dr[c][c] == R(0) && di[c][c] == R(0)) {
-          info[0] = col0 + c + 1;
-          break;
-        }
+  for (int e = t; e < nb * nb; e += 256) {
+    int r = e % nb, c = e / nb;
+    A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = {dr[r][c], di[r][c]};
   }
+  if (t == 0 && info[0] == 0)
+    for (int c = 0; c < nb; c++)
+      if (dr[c][c] == R(0) && di[c][c] == R(0)) {
+        info[0] = col0 + c + 1;
+        break;
+      }
+}
+
+// With the diagonal block factored: L21 = A21 U11^-1 (blocks [0, nrb), one
+// row per thread) and U12 = L11^-1 A12 (blocks [nrb, ..), one column per
+// thread).  Every block only reads the diagonal block.
+template <typename R>
+__global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
+                                                     int nb, int nrb) {
+  __shared__ R dr[kPanelNB][kPanelNB + 1];
+  __shared__ R di[kPanelNB][kPanelNB + 1];
+  __shared__ R rr[kPanelNB], ri[kPanelNB];  // reciprocals of the pivots
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += 256) {
+    int r = e % nb, c = e / nb;
+    auto v = A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)];
+    dr[r][c] = v.x;
+    di[r][c] = v.y;
+  }
+  __syncthreads();
+  if (t < nb) {
+    R a = R(0), b = R(0);
+    if (dr[t][t] != R(0) || di[t][t] != R(0)) crecip(dr[t][t], di[t][t], a, b);
+    rr[t] = a;
+    ri[t] = b;
+  }
+  __syncthreads();
   const int b = blockIdx.x;
   if (b < nrb) {
     const int i = col0 + nb + b * 256 + t;
@@ -307,18 +335,18 @@ __global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __rest
     }
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
-      if (c >= nb) break;
-      R ar = xr[c], ai = xi[c];
+      if (c < nb) {
+        R ar = xr[c], ai = xi[c];
 #pragma unroll
-      for (int q = 0; q < kPanelNB; q++) {
-        if (q >= c) break;
-        ar -= xr[q] * dr[q][c] - xi[q] * di[q][c];
-        ai -= xr[q] * di[q][c] + xi[q] * dr[q][c];
+        for (int q = 0; q < kPanelNB; q++) {
+          if (q < c) {
+            ar -= xr[q] * dr[q][c] - xi[q] * di[q][c];
+            ai -= xr[q] * di[q][c] + xi[q] * dr[q][c];
+          }
+        }
+        xr[c] = ar * rr[c] - ai * ri[c];
+        xi[c] = ar * ri[c] + ai * rr[c];
       }
-      R ir = R(0), ii = R(0);
-      if (dr[c][c] != R(0) || di[c][c] != R(0)) crecip(dr[c][c], di[c][c], ir, ii);
-      xr[c] = ar * ir - ai * ii;
-      xi[c] = ar * ii + ai * ir;
     }
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++)
@@ -337,12 +365,14 @@ __global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __rest
     }
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
-      if (c >= nb) break;
+      if (c < nb) {
 #pragma unroll
-      for (int q = 0; q < kPanelNB; q++) {
-        if (q >= c) break;
-        yr[c] -= dr[c][q] * yr[q] - di[c][q] * yi[q];
-        yi[c] -= dr[c][q] * yi[q] + di[c][q] * yr[q];
+        for (int q = 0; q < kPanelNB; q++) {
+          if (q < c) {
+            yr[c] -= dr[c][q] * yr[q] - di[c][q] * yi[q];
+            yi[c] -= dr[c][q] * yi[q] + di[c][q] * yr[q];
+          }
+        }
       }
     }
 #pragma unroll
@@ -354,7 +384,9 @@ __global__ __launch_bounds__(256) void k_panel_factor(typename CT<R>::c2* __rest
 // Trailing update C -= L U over the (n - r0) x (n - r0) block starting at
 // (r0, r0), K = kk <= 32.  MFMA 16x16x4 computes the transposed product
 // (U^T L^T) so that the 16 consecutive lanes of an accumulator row map to
-// 16 consecutive rows of column-major C.  Block tiles are 64 x 64, each of
+// 16 consecutive rows of column-major C.  Operand layout (both types): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; the accumulator row held
+// by element q of lane l is type-specific (CT::out_row, measured on gfx950).  Block tiles are 64 x 64, each of
 // the four waves owns a 32 x 32 quarter (2 x 2 MFMA tiles, real and
 // imaginary accumulators).  Block ids are remapped so that each of the eight
 // XCDs works on a contiguous band of tiles (L21 rows stay in its L2).
@@ -427,7 +459,7 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
     for (int b = 0; b < 2; b++)
       for (int q = 0; q < 4; q++) {
         const int row = i0 + wi + a * 16 + l16;
-        const int col = j0 + wj + b * 16 + kq * 4 + q;
+        const int col = j0 + wj + b * 16 + CT<R>::out_row(kq, q);
         if (row < n && col < n) {
           auto& c = A[(size_t)row + (size_t)lda * col];
           auto v = c;
@@ -447,7 +479,7 @@ __global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info)
 template <typename R>
 static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
                         int* info) {
-  hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
+  HX_LAUNCH(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
   // workspace: two ping-pong candidate buffers of nsets0 sets
   const int nsets0 = (n + kTS - 1) / kTS;
   int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
@@ -456,26 +488,26 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
     const int m = n - col0;
     int nsets = (m + kTS - 1) / kTS, cur = 0;
-    hipLaunchKernelGGL(k_select<R>, dim3(nsets), dim3(256), 0, s, A, n, n, col0, nb, col0, (const int*)nullptr,
+    HX_LAUNCH(k_select<R>, dim3(nsets), dim3(256), 0, s, A, n, n, col0, nb, col0, (const int*)nullptr,
                        (const int*)nullptr, 0, idx[cur], cnt[cur]);
     while (nsets > 1) {
       const int nout = (nsets + 3) / 4;
-      hipLaunchKernelGGL(k_select<R>, dim3(nout), dim3(256), 0, s, A, n, n, col0, nb, col0, idx[cur], cnt[cur], nsets,
+      HX_LAUNCH(k_select<R>, dim3(nout), dim3(256), 0, s, A, n, n, col0, nb, col0, idx[cur], cnt[cur], nsets,
                          idx[1 - cur], cnt[1 - cur]);
       cur = 1 - cur;
       nsets = nout;
     }
-    hipLaunchKernelGGL(k_pivots_to_swaps, dim3(1), dim3(64), 0, s, idx[cur], nb, col0, ipiv);
-    hipLaunchKernelGGL(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
+    HX_LAUNCH(k_pivots_to_swaps, dim3(1), dim3(64), 0, s, idx[cur], nb, col0, ipiv);
+    HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
     const int rest = n - col0 - nb;
     const int nrb = (rest + 255) / 256;
-    hipLaunchKernelGGL(k_panel_factor<R>, dim3(nrb * 2 > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
-                       info);
+    HX_LAUNCH(k_diag_factor<R>, dim3(1), dim3(256), 0, s, A, n, col0, nb, info);
+    if (nrb > 0) HX_LAUNCH(k_panel_solve<R>, dim3(nrb * 2), dim3(256), 0, s, A, n, n, col0, nb, nrb);
     if (rest > 0) {
       const int tiles = (rest + 63) / 64;
       const int ntiles = tiles * tiles;
       const int grid = ((ntiles + 7) / 8) * 8;
-      hipLaunchKernelGGL(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles);
+      HX_LAUNCH(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles);
     }
   }
 }
@@ -581,17 +613,17 @@ static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const
     double2* Tq = Xtmp + (size_t)n * q0;
     // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
     // backward pass: off-diagonal rows updated in X, solved rows into T.
-    hipLaunchKernelGGL(k_gather_rows, dim3((n + 255) / 256, nq), dim3(256), 0, s, perm, Xq, Tq, n, nq);
+    HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nq), dim3(256), 0, s, perm, Xq, Tq, n, nq);
     for (int k0 = 0; k0 < n; k0 += kPanelNB) {
       const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
       const int rest = n - k0 - kb;
       const int g = rest > 0 ? (rest + 255) / 256 : 1;
-      hipLaunchKernelGGL((k_trsm_block<R, true>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Tq, Xq, nq);
+      HX_LAUNCH((k_trsm_block<R, true>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Tq, Xq, nq);
     }
     for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
       const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
       const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
-      hipLaunchKernelGGL((k_trsm_block<R, false>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Xq, Tq, nq);
+      HX_LAUNCH((k_trsm_block<R, false>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Xq, Tq, nq);
     }
     (void)hipMemcpyAsync(Xq, Tq, sizeof(double2) * (size_t)n * nq, hipMemcpyDeviceToDevice, s);
   }
@@ -659,7 +691,7 @@ void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2*
   lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
 }
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
-  hipLaunchKernelGGL(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
+  HX_LAUNCH(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
 }
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info) {
   lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info);
@@ -671,13 +703,13 @@ void residual(hipStream_t s, const double2* A, int n, const double2* X, const do
               double* norms) {
   (void)hipMemcpyAsync(R, B, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
   const int jchunk = 256;
-  hipLaunchKernelGGL(k_residual, dim3((n + 63) / 64, (n + jchunk - 1) / jchunk), dim3(256), 0, s, A, n, X, R, nrhs,
+  HX_LAUNCH(k_residual, dim3((n + 63) / 64, (n + jchunk - 1) / jchunk), dim3(256), 0, s, A, n, X, R, nrhs,
                      jchunk);
-  hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms);
-  hipLaunchKernelGGL(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs);
+  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms);
+  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs);
 }
 void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count) {
-  hipLaunchKernelGGL(k_axpy, dim3(1024), dim3(256), 0, s, X, D, count);
+  HX_LAUNCH(k_axpy, dim3(1024), dim3(256), 0, s, X, D, count);
 }
 }  // namespace launch
 }  // namespace hamsx

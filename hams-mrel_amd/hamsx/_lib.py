@@ -45,7 +45,10 @@ SIGNATURES = {
     "hamsx_solve_frequency": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, _dp]),
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
+    "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),
 }
+STAGES = ("assemble", "rhs", "factor", "solve", "forces")
 
 _lib = None
 

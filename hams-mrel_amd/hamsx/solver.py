@@ -136,3 +136,14 @@ class FrequencySolver:
         it, res = C.c_int(), C.c_double()
         _lib.check(self._lib.hamsx_refine_stats(self._h, C.byref(it), C.byref(res)), "hamsx_refine_stats")
         return it.value, res.value
+
+    def set_profiling(self, on: bool = True):
+        _lib.check(self._lib.hamsx_set_profiling(self._h, 1 if on else 0), "hamsx_set_profiling")
+
+    def stage_times(self, reset: bool = False) -> dict:
+        """Device milliseconds per stage (HIP events on the context stream)."""
+        ms = np.zeros(len(_lib.STAGES))
+        cnt = np.zeros(len(_lib.STAGES), dtype=np.int32)
+        _lib.check(self._lib.hamsx_stage_times(self._h, 1 if reset else 0, _lib.ptr(ms), _lib.ptr(cnt, C.c_int)),
+                   "hamsx_stage_times")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_lib.STAGES)}

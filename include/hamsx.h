@@ -130,6 +130,16 @@ int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
  * and the final max-norm relative residual. */
 int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);
 
+/* Measurement hook (no counterpart in the reference, which has no timers):
+ * with profiling on, every stage is bracketed by HIP events on the context's
+ * stream.  hamsx_stage_times returns the device milliseconds accumulated per
+ * stage since the last reset -- ms[0] assembly, ms[1] right-hand sides,
+ * ms[2] factorisation, ms[3] solves (with refinement), ms[4] forces -- and
+ * the number of launches of each in count[5]. */
+#define HAMSX_NSTAGE 5
+int hamsx_set_profiling(hamsx_ctx *ctx, int on);
+int hamsx_stage_times(hamsx_ctx *ctx, int reset, double *ms, int *count);
+
 #ifdef __cplusplus
 }
 #endif

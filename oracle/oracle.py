@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
+ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fma.so")
 REF_SO = os.path.join(HERE, "_ref", "libhams_ref.so")
 NK = 200
 G = 9.80665
@@ -22,6 +23,7 @@ _ip = C.POINTER(C.c_int)
 _d = C.c_double
 _o = None
 _r = None
+_variant = {}
 
 
 def _p(a, t=C.c_double):
@@ -30,12 +32,23 @@ def _p(a, t=C.c_double):
     return C.cast(a.ctypes.data, C.POINTER(t))
 
 
-def lib():
+def lib(fma: bool = False):
+    """The oracle; fma=True gives the FMA-contracted build of the same source."""
     global _o
+    if fma:
+        if "fma" not in _variant:
+            _variant["fma"] = _load(ORACLE_FMA_SO)
+        return _variant["fma"]
     if _o is None:
-        if not os.path.exists(ORACLE_SO):
-            raise ImportError(f"{ORACLE_SO} missing: run `make -C oracle`")
-        o = C.CDLL(ORACLE_SO)
+        _o = _load(ORACLE_SO)
+    return _o
+
+
+def _load(path):
+    if True:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing: run `make -C oracle`")
+        o = C.CDLL(path)
         for name in ("orc_error", "orc_e1xa", "orc_eix", "orc_struve_h0", "orc_struve_h1", "orc_bessel_j0",
                      "orc_bessel_j1"):
             getattr(o, name).restype = _d
@@ -66,8 +79,7 @@ def lib():
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
-        _o = o
-    return _o
+    return o
 
 
 def ref_available() -> bool:
@@ -131,12 +143,12 @@ def geometry(mesh, xr):
     return cen, area, pnsz, dxyz
 
 
-def assemble(m, w1: float, h: float, nthreads: int = 8):
+def assemble(m, w1: float, h: float, nthreads: int = 8, fma: bool = False):
     v, wk, wvn = wave(w1, h)
     n = m.nelem
     A = np.zeros((n, n), dtype=np.complex128, order="F")
     S = np.zeros((n, n), dtype=np.complex128, order="F")
-    lib().orc_assemble(n, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen), _p(m.area),
+    lib(fma).orc_assemble(n, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen), _p(m.area),
                        _p(m.pnsz), _p(m.dxyz), v, h, _p(wvn), NK, _p(A), _p(S),
                        nthreads)
     return A, S
@@ -151,13 +163,13 @@ def assemble_entry(m, i: int, j: int, w1: float, h: float):
 
 
 def solve_frequency(m, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1,
-                    nthreads: int = 8):
+                    nthreads: int = 8, fma: bool = False):
     nd = 6 * m.nbody
     beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
     amas = np.zeros((nd, nd), order="F")
     bdmp = np.zeros((nd, nd), order="F")
     exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
-    info = lib().orc_solve_frequency(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+    info = lib(fma).orc_solve_frequency(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
                                      _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
                                      _p(m.body_start, C.c_int), w1, h, len(beta), _p(beta), amp, xw[0], xw[1], rho,
                                      isol, _p(amas), _p(bdmp), _p(exfc), nthreads)

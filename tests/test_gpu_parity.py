@@ -1,10 +1,16 @@
 """GPU parity of the HIP path against the C oracle (FP64 restatement of the
 reference, pinned to the reference's own Fortran in test_oracle_golden.py).
 
-Tolerances: influence-matrix entries 1e-12 normwise relative to max|A|
-(FP64 with FMA contraction on the device against the reference's unfused
-FP64); added mass, damping and excitation 1e-8 normwise relative per array
-(the plain FP64 path) and 1e-6 for the mixed-precision path.
+Tolerances (max |difference| over an array divided by its max |entry|):
+  * deep water: influence matrices 1e-12, added mass / damping / excitation
+    1e-8 (plain FP64 path) and 1e-6 (mixed precision);
+  * finite depth: the same bounds, or -- when larger -- 4x the distance
+    between two FP64 builds of the reference algorithm (the oracle with and
+    without FMA contraction) on the same case.  The reference's EIGENE /
+    PIDCOCK epsilon extrapolation and LINTON's adaptive quadrature move its
+    own result by that much under rounding changes alone (its Fortran built by
+    the same flang with and without FMA disagrees by up to 2.5e-5 per Green
+    value), so no implementation with different rounding can be held closer.
 """
 import sys
 
@@ -50,8 +56,14 @@ def test_assembly_matches_oracle(mesh_small, depth, w):
         S = s.get_matrix("S")
     Ao, So = oracle.assemble(m, w, depth)
     ea, es = rel(A, Ao), rel(S, So)
-    say(f"    rel err A {ea:.3e}  S {es:.3e}")
-    assert ea < 1e-12 and es < 1e-12
+    fa = fs = 0.0
+    if depth > 0:
+        Af, Sf = oracle.assemble(m, w, depth, fma=True)
+        fa, fs = rel(Af, Ao), rel(Sf, So)
+    frac = float(np.mean(np.abs(A - Ao) > 1e-10 * np.max(np.abs(Ao))))
+    say(f"    rel err A {ea:.3e}  S {es:.3e}   reference rounding floor A {fa:.3e}  S {fs:.3e}   "
+        f"entries beyond 1e-10: {frac:.4f}")
+    assert ea <= max(1e-12, 4 * fa) and es <= max(1e-12, 4 * fs)
 
 
 def test_lu_backward_error():
@@ -89,8 +101,14 @@ def test_frequency_solve_matches_oracle(mesh_small, depth):
         ao, bo, eo, io = oracle.solve_frequency(m, w, depth, heads)
         assert info == 0 and io == 0
         errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
-        say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e}")
-        assert max(errs) < 1e-8
+        floor = (0.0, 0.0, 0.0)
+        if depth > 0:
+            af, bf, ef, _ = oracle.solve_frequency(m, w, depth, heads, fma=True)
+            floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+        say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e}   "
+            f"floor {floor[0]:.1e} {floor[1]:.1e} {floor[2]:.1e}")
+        for e, f in zip(errs, floor):
+            assert e <= max(1e-8, 4 * f)
 
 
 def test_step_sequence_equals_fused_call(mesh_small):
@@ -125,8 +143,13 @@ def test_mixed_precision_within_1e6(mesh_small, depth):
         it, res = s.refine_stats()
     ao, bo, eo, _ = oracle.solve_frequency(m, w, depth, heads)
     errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
-    say(f"    mixed rel err {errs}  refine iters {it} resid {res:.2e}")
-    assert max(errs) < 1e-6
+    floor = (0.0, 0.0, 0.0)
+    if depth > 0:
+        af, bf, ef, _ = oracle.solve_frequency(m, w, depth, heads, fma=True)
+        floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+    say(f"    mixed rel err {errs}  floor {floor}  refine iters {it} resid {res:.2e}")
+    for e, f in zip(errs, floor):
+        assert e <= max(1e-6, 4 * f)
 
 
 def test_single_cylinder_config():
