@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Frequency-solve throughput of the MI355X panel solver on one GPU.
+
+A step is one frequency solve: influence-matrix assembly (finite-depth or
+deep-water Green function + Newman integrals for all N^2 panel pairs),
+complex LU, the 6*NB radiation and NBETA diffraction solves, and the
+added-mass / damping / excitation reductions, with one device
+synchronisation (the forces come back to the host).  The mesh is resident in
+HBM before the timed region starts.
+
+Default workload: two truncated cylinders (R = 5 m, draft 10 m, 1,984 panels
+each, centres 40 m apart on x), depth 50 m, 30 frequencies uniform in
+[0.2, 2.5] rad/s, headings 0/45/90 deg, complex128 arithmetic.
+
+Prints ONE JSON line.  Besides the common fields it carries
+  * latency_ms: p50/p99 of the per-solve wall time;
+  * stage_ms_per_solve: device time per stage from HIP events on the
+    solver's own stream;
+  * roofline: the dominant kernel (the assembly) against the FP64 VALU peak,
+    plus the LU factorisation against the MFMA peak;
+  * cpu_baseline: the oracle (a C restatement of the reference's OpenMP
+    Fortran loop) on a bounded row sample of the same assembly, plus LAPACK
+    zgetrf/zgetrs on the full matrix, on the host's cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "hams-mrel_amd")]
+
+import numpy as np  # noqa: E402
+
+METRIC = "freq-solves/sec @4k-12k panels, mixed vs FP64 at 1e-6 rel; p50/p99 latency"
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector and FP64 matrix (dense)
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# Per-pair FP64 work of the assembly kernel and its HBM traffic per launch,
+# measured with rocprofv3 PMC counters on the named workload (the summaries
+# and the commands that made them are under profiles/).  flops_per_pair is
+# lane-level FP64 operations (FMA = 2): issued FP64 VALU instructions x 64
+# lanes x the measured fraction of active lanes, divided by the N^2 pairs.
+ASSEMBLY_COUNTERS = {}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="two-cylinders", choices=["two-cylinders", "five-cylinders", "one-cylinder"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    return ap.parse_args()
+
+
+def pick_case(name):
+    from hamsx import configs
+    if name == "two-cylinders":
+        return configs.two_cylinders()
+    if name == "five-cylinders":
+        return configs.five_cylinders()
+    return configs.single_cylinder()
+
+
+def cpu_baseline(case, mesh, rows, threads):
+    """Seconds per frequency solve on the host: the oracle's assembly on an
+    evenly strided sample of `rows` field panels against all N source panels
+    (scaled by N / rows), plus LAPACK zgetrf + zgetrs (scipy's OpenBLAS) on a
+    full N x N matrix with the same number of right-hand sides.  Right-hand
+    sides and force sums are O(N^2) and O(N) and are left out (they favour
+    the CPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import scipy.linalg as sl
+    n = mesh.nelem
+    w = float(case.omegas[len(case.omegas) // 2])
+    idx = np.unique(np.linspace(0, n - 1, min(rows, n)).astype(np.int32))
+    t0 = time.perf_counter()
+    oracle.assemble_rows(mesh, idx, w, case.depth, nthreads=threads)
+    t_sample = time.perf_counter() - t0
+    t_asm = t_sample * n / len(idx)
+    rng = np.random.default_rng(0)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 0.01 + 2 * math.pi * np.eye(n)
+    A = np.asfortranarray(A)
+    nr = 6 * mesh.nbody + len(case.headings_deg)
+    B = np.asfortranarray(rng.standard_normal((n, nr)) + 0j)
+    t0 = time.perf_counter()
+    lu = sl.lu_factor(A, overwrite_a=True, check_finite=False)
+    sl.lu_solve(lu, B, check_finite=False)
+    t_lu = time.perf_counter() - t0
+    per = t_asm + t_lu
+    return {"value": 1.0 / per, "unit": "freq-solves/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle assembly of {len(idx)} of {n} rows (all {n} columns) at w={w:.3f} rad/s, "
+                       f"{t_sample:.1f} s on {threads} threads, scaled to N rows = {t_asm:.1f} s; "
+                       f"+ scipy/OpenBLAS zgetrf+zgetrs N={n}, {nr} rhs = {t_lu:.2f} s"),
+            "seconds_per_solve": per}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs one GPU per process (asked for --gpus {args.gpus}); this build has no multi-GPU "
+              "path -- run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from hamsx.mesh import combine
+    from hamsx.solver import FrequencySolver
+
+    case = pick_case(args.config)
+    mesh = combine(case.bodies, case.xr)
+    n = mesh.nelem
+    heads = np.deg2rad(case.headings_deg)
+    omegas = [float(w) for w in case.omegas]
+    solver = FrequencySolver(mesh, precision=args.precision)
+    solver.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+    print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(omegas)} frequencies, "
+          f"{len(heads)} headings, precision {args.precision}", file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        solver.solve_frequency(omegas[k % len(omegas)], case.depth)
+    solver.set_profiling(True)
+    solver.stage_times(reset=True)
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        _, _, _, info = solver.solve_frequency(omegas[k % len(omegas)], case.depth)
+        lat.append(time.perf_counter() - ts)
+        if info != 0:
+            raise SystemExit(f"zero pivot at step {k}: INFO={info}")
+    elapsed = time.perf_counter() - t0
+    stages = solver.stage_times(reset=True)
+    solver.close()
+    print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
+
+    lat_ms = np.array(lat) * 1e3
+    ms = {k: v[0] / max(1, v[1]) for k, v in stages.items()}
+    t_asm = ms["assemble"] * 1e-3
+    pairs = float(n) * n
+    cnt = ASSEMBLY_COUNTERS.get((case.name, args.precision)) or ASSEMBLY_COUNTERS.get((case.name, "fp64"))
+    roof = {"kernel": "k_assemble (Green function + Newman integrals + A/S writes, one thread per pair)",
+            "bound": "valu-fp64", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "launch_ms": ms["assemble"],
+            "pairs_per_launch": pairs, "gpairs_per_s": pairs / t_asm / 1e9}
+    if cnt:
+        flops = cnt["flops_per_pair"] * pairs
+        roof.update({"achieved": flops / t_asm / 1e12, "frac": flops / t_asm / 1e12 / PEAK_FP64_TFLOPS,
+                     "flops_per_pair": cnt["flops_per_pair"], "traffic": cnt.get("traffic_bytes"),
+                     "algorithmic_bytes": 32.0 * pairs})
+    else:
+        roof.update({"achieved": None, "frac": None, "traffic": None, "algorithmic_bytes": 32.0 * pairs})
+    lu_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
+    lu_tf = 8.0 * n ** 3 / 3.0 / (ms["factor"] * 1e-3) / 1e12
+    roof_lu = {"kernel": "blocked LU (trailing updates on MFMA)", "bound": "mfma", "unit": "TFLOP/s",
+               "achieved": lu_tf, "peak": lu_peak, "frac": lu_tf / lu_peak, "launch_ms": ms["factor"],
+               "flops": 8.0 * n ** 3 / 3.0}
+    out = {
+        "metric": METRIC, "value": args.steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "c64-lu+f64-refine",
+        "data": "synthetic seeded cylinder meshes (no reference files read)",
+        "config": {"workload": case.name, "panels": n, "depth_m": case.depth, "frequencies": len(omegas),
+                   "headings_deg": case.headings_deg, "precision": args.precision},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "stage_ms_per_solve": ms,
+        "roofline": roof,
+        "roofline_lu": roof_lu,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, mesh, args.cpu_rows, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

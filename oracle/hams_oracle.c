@@ -1416,6 +1416,26 @@ void orc_assemble(int nelem, int nnode, const double *xyz, const int *ncn, const
     }
 }
 
+/* Rows rows[0..nrows) of A and S (out[r + nrows*j]), the same loop as
+ * orc_assemble restricted to a row sample: the CPU baseline times it. */
+void orc_assemble_rows(int nrows, const int *rows, int nelem, int nnode, const double *xyz,
+                       const int *ncn, const int *ncon, const double *cen, const double *area,
+                       const double *pnsz, const double *dxyz, double v, double h, const double *wvn,
+                       int nk, double *A, double *S, int nthreads)
+{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int r = 0; r < nrows; r++) {
+        for (int j = 0; j < nelem; j++) {
+            cplx a, s;
+            pair_entry(rows[r], j, nelem, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn, nk, &a,
+                       &s);
+            size_t o = 2 * ((size_t)r + (size_t)nrows * j);
+            A[o] = creal(a); A[o + 1] = cimag(a);
+            if (S) { S[o] = creal(s); S[o + 1] = cimag(s); }
+        }
+    }
+}
+
 /* ========================================================================
  * Right-hand sides: ASSB_RBC_MULTI (AssbMatxMulti.f90:124-181) with
  * RBC_MULTI_RIGHT (BodyIntgrMulti.f90:83-127); ASSB_DBC_MULTI (:187-251)

@@ -68,6 +68,12 @@ void orc_assemble_entry(int i, int j, int nelem, int nnode, const double *node_x
                         double v, double h, const double *wvn, int nk, double *a_ij,
                         double *s_ij);
 
+/* Rows rows[0..nrows) (0-based) of A and S into A[r + nrows*j], S[r + nrows*j]. */
+void orc_assemble_rows(int nrows, const int *rows, int nelem, int nnode, const double *node_xyz,
+                       const int *ncn, const int *ncon, const double *cen, const double *area,
+                       const double *pnsz, const double *dxyz, double v, double h,
+                       const double *wvn, int nk, double *A, double *S, int nthreads);
+
 /* ---- right-hand sides: ASSB_RBC_MULTI / ASSB_DBC_MULTI ----
  * rhs[:, 0:6*nbody] radiation, rhs[:, 6*nbody + b] diffraction for heading beta[b]. */
 void orc_rhs(int nelem, const double *S, const double *cen, const double *dxyz,

@@ -70,6 +70,8 @@ def _load(path):
                                    _dp, _dp, C.c_int]
         o.orc_assemble_entry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp,
                                          _d, _d, _dp, C.c_int, _dp, _dp]
+        o.orc_assemble_rows.argtypes = [C.c_int, _ip, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _d, _d,
+                                        _dp, C.c_int, _dp, _dp, C.c_int]
         o.orc_rhs.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _d, _d, _d, _d, _d, _d, _d, _dp]
         o.orc_zgetrf.restype = C.c_int
         o.orc_zgetrf.argtypes = [C.c_int, _dp, _ip]
@@ -151,6 +153,19 @@ def assemble(m, w1: float, h: float, nthreads: int = 8, fma: bool = False):
     lib(fma).orc_assemble(n, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen), _p(m.area),
                        _p(m.pnsz), _p(m.dxyz), v, h, _p(wvn), NK, _p(A), _p(S),
                        nthreads)
+    return A, S
+
+
+def assemble_rows(m, rows, w1: float, h: float, nthreads: int = 8):
+    """Rows `rows` (0-based) of A and S, shape (len(rows), N)."""
+    v, wk, wvn = wave(w1, h)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    nr, n = len(rows), m.nelem
+    A = np.zeros((nr, n), dtype=np.complex128, order="F")
+    S = np.zeros((nr, n), dtype=np.complex128, order="F")
+    lib().orc_assemble_rows(nr, _p(rows, C.c_int), n, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int),
+                            _p(m.ncon, C.c_int), _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), v, h, _p(wvn), NK,
+                            _p(A), _p(S), nthreads)
     return A, S
 
 
