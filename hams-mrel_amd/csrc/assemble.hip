@@ -22,14 +22,12 @@ __device__ __forceinline__ int near_flag(double xi, double yi, double zi, double
   return dist <= 50.0 * pn ? 1 : 0;
 }
 
-template <bool DEEP>
-__global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2* __restrict__ A,
-                                                  double2* __restrict__ S, int* flags) {
+// One pair: the Green function (deep water, or finite depth in region REG),
+// the Newman integrals when FLAG = 1, and the entries A(i,j), S(i,j).
+template <bool DEEP, int REG>
+__device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, int i, int j, double2* __restrict__ A,
+                                           double2* __restrict__ S, int* flags) {
   const int n = m.n;
-  const int lane = threadIdx.x & 63;
-  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
-  const int i = blockIdx.x * 64 + lane;
-  if (j >= n || i >= n) return;
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
   const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
   const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
@@ -42,7 +40,7 @@ __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2*
     d_infgreen(p, f.v, flag, g);
   } else {
     FreqParams fp{f.v, f.wk, f.w1, f.h, f.wvn};
-    d_fingreen(p, fp, flag, g, &ovf);
+    d_fingreen_r<REG>(p, fp, flag, g, &ovf);
   }
   if (ovf) atomicOr(flags, 1);
   const double ear = m.area[j], n1 = m.nrm[j], n2 = m.nrm[n + j], n3 = m.nrm[2 * n + j];
@@ -58,6 +56,78 @@ __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2*
   const size_t o = (size_t)i + (size_t)n * j;
   A[o] = make_double2(t.r, t.i);
   S[o] = make_double2(s.r, s.i);
+}
+
+// Grid form: a 256-thread block is four wave64s; each wave owns one source
+// panel j (wave-uniform, so the source geometry and Newman frame are scalar
+// loads) and 64 consecutive field panels i, writing 64 consecutive complex128
+// entries of column j of A and of S.  Used for deep water, whose Green
+// function has no long region-dependent series.
+template <bool DEEP>
+__global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2* __restrict__ A,
+                                                  double2* __restrict__ S, int* flags) {
+  const int lane = threadIdx.x & 63;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
+  const int i = blockIdx.x * 64 + lane;
+  if (j >= m.n || i >= m.n) return;
+  pair_entry<DEEP, kRegAny>(m, f, i, j, A, S, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Finite depth, binned form.  The pairs are counting-sorted once per
+// (mesh, h) by key = 2 * fin_subkey + FLAG into a list of packed (j << 16 | i)
+// indices, each bin padded to a multiple of 64 with kNoPair, so every wave
+// runs one region with one series length and one FLAG.  One kernel per
+// region walks its contiguous slice of the list.
+// ---------------------------------------------------------------------------
+constexpr unsigned kNoPair = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int pair_key(const MeshDev& m, double h, int i, int j) {
+  const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
+  const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
+  PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
+  return 2 * fin_subkey(p.rr, h) + flag;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(MeshDev m, double h, int* __restrict__ count) {
+  __shared__ int hist[kFinBins];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
+  if (i < m.n && j < m.n) atomicAdd(&hist[pair_key(m, h, i, j)], 1);
+  __syncthreads();
+  if (hist[threadIdx.x]) atomicAdd(&count[threadIdx.x], hist[threadIdx.x]);
+}
+
+// cursor[] starts at each bin's (padded) offset and ends at offset + count.
+__global__ __launch_bounds__(256) void k_bin_scatter(MeshDev m, double h, int* __restrict__ cursor,
+                                                     unsigned* __restrict__ list) {
+  __shared__ int hist[kFinBins], base[kFinBins];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool live = i < m.n && j < m.n;
+  int key = 0, rank = 0;
+  if (live) {
+    key = pair_key(m, h, i, j);
+    rank = atomicAdd(&hist[key], 1);
+  }
+  __syncthreads();
+  if (hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hist[threadIdx.x]);
+  __syncthreads();
+  if (live) list[base[key] + rank] = ((unsigned)j << 16) | (unsigned)i;
+}
+
+template <int REG>
+__global__ __launch_bounds__(256) void k_assemble_list(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
+                                                       int count, double2* __restrict__ A, double2* __restrict__ S,
+                                                       int* flags) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const unsigned e = list[t];
+  if (e == kNoPair) return;
+  pair_entry<false, REG>(m, f, (int)(e & 0xFFFFu), (int)(e >> 16), A, S, flags);
 }
 
 // ---------------------------------------------------------------------------
@@ -258,6 +328,43 @@ void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, dou
     HX_LAUNCH(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
   else
     HX_LAUNCH(k_assemble<false>, grid, dim3(256), 0, s, m, f, A, S, flags);
+}
+
+void bin_count(hipStream_t s, const MeshDev& m, double h, int* count) {
+  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  HX_LAUNCH(k_bin_count, grid, dim3(256), 0, s, m, h, count);
+}
+
+void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list) {
+  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  HX_LAUNCH(k_bin_scatter, grid, dim3(256), 0, s, m, h, cursor, list);
+}
+
+void assemble_binned(hipStream_t s, const MeshDev& m, const FreqDev& f, const unsigned* list,
+                     const size_t region_begin[5], const size_t region_end[5], double2* A, double2* S, int* flags) {
+  HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
+  // most expensive regions first, so the cheap ones fill the tail
+  const int order[4] = {kRegLinton, kRegPidcock, kRegEigene, kRegEigen};
+  for (int r : order) {
+    const size_t cnt = region_end[r] - region_begin[r];
+    if (cnt == 0) continue;
+    const dim3 grid((unsigned)((cnt + 255) / 256));
+    const unsigned* l = list + region_begin[r];
+    switch (r) {
+      case kRegLinton:
+        HX_LAUNCH(k_assemble_list<kRegLinton>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
+        break;
+      case kRegPidcock:
+        HX_LAUNCH(k_assemble_list<kRegPidcock>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
+        break;
+      case kRegEigene:
+        HX_LAUNCH(k_assemble_list<kRegEigene>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
+        break;
+      default:
+        HX_LAUNCH(k_assemble_list<kRegEigen>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
+        break;
+    }
+  }
 }
 
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "hamsx_internal.hpp"
+
 namespace hamsx {
 
 constexpr double kPi = 3.141592653589793;        // Const_mod (WavDynMods.f90:146)
@@ -806,8 +808,45 @@ __device__ __noinline__ void d_linton(double r, double zf, double zp, const Freq
   fg[2] = (-fp4) * g2;
 }
 
-// FINGREEN3D (:81-155) with FINGREEN3D_R's region split (:183-191)
-__device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int flag, cd g[4], int* ovf) {
+// Regions of FINGREEN3D_R (:183-191) by R/h, and the binning key the
+// assembly sorts pairs by: region, then the series length NT of EIGENE /
+// PIDCOCK, so that the lanes of a wave run the same code for the same number
+// of terms.  The key is a function of the geometry and h only.
+__device__ __forceinline__ int fin_region(double rr, double h) {
+  double rhp = fabs(rr / h);
+  if (rhp < 0.0005) return kRegLinton;
+  if (rhp < 0.05) return kRegPidcock;
+  if (rhp < 0.5) return kRegEigene;
+  return kRegEigen;
+}
+
+// Sub-key (see hamsx_internal.hpp), clamped so that any NT maps inside its
+// region's range.
+__device__ __forceinline__ int fin_subkey(double rr, double h) {
+  const int reg = fin_region(rr, h);
+  int nt;
+  switch (reg) {
+    case kRegLinton:
+      return 0;
+    case kRegPidcock: {
+#pragma clang fp contract(off)
+      nt = (int)(HX_F(-1010.10) * fabs(rr / h) + HX_F(100.50));
+      return kSubPidcock + min(kSubEigene - kSubPidcock - 1, max(0, 99 - nt));
+    }
+    case kRegEigene: {
+#pragma clang fp contract(off)
+      nt = (int)(HX_F(-88.89) * fabs(rr / h) + HX_F(54.45));
+      return kSubEigene + min(kSubEigen - kSubEigene - 1, max(0, 50 - nt));
+    }
+    default:
+      return kSubEigen;
+  }
+}
+// FINGREEN3D (:81-155) with FINGREEN3D_R's region split (:183-191).  REG
+// fixes the region at compile time for pairs binned by fin_region; kRegAny
+// decides per pair.
+template <int REG>
+__device__ inline void d_fingreen_r(const PairGeom& p, const FreqParams& fp, int flag, cd g[4], int* ovf) {
   double r13 = p.r1 * p.r1 * p.r1, r3 = p.r * p.r * p.r;
   for (int k = 0; k < 4; k++) g[k] = mk(0.0);
   double sgn = 1.0;
@@ -816,12 +855,12 @@ __device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int f
     sgn = -1.0;
   } else {
     cd fg[3];
-    double rhp = fabs(p.rr / fp.h);
-    if (rhp < 0.0005)
+    const int reg = REG == kRegAny ? fin_region(p.rr, fp.h) : REG;
+    if (reg == kRegLinton)
       d_linton(p.rr, p.zsrc, p.zfld, fp, fg, ovf);
-    else if (rhp < 0.05)
+    else if (reg == kRegPidcock)
       d_pidcock(p.rr, p.zsrc, p.zfld, fp, fg);
-    else if (rhp < 0.5)
+    else if (reg == kRegEigene)
       d_eigene(p.rr, p.zsrc, p.zfld, fp, fg);
     else
       d_eigen(p.rr, p.zsrc, p.zfld, fp, fg);
@@ -842,6 +881,10 @@ __device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int f
     g[2].r -= p.ypq / r3;
     g[3].r -= (p.zsrc - p.zfld) / r3;
   }
+}
+
+__device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int flag, cd g[4], int* ovf) {
+  d_fingreen_r<kRegAny>(p, fp, flag, g, ovf);
 }
 
 #pragma clang fp contract(fast)

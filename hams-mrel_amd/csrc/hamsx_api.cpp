@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -223,6 +224,12 @@ struct hamsx_ctx {
   DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial;
   DevBuf<float2> Af;
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
+  // finite-depth pair bins, valid for bin_n panels at depth bin_h
+  DevBuf<unsigned> plist;
+  DevBuf<int> bin_cnt;
+  int bin_n = 0;
+  double bin_h = 0.0;
+  size_t reg_begin[5] = {0}, reg_end[5] = {0};
   DevBuf<double> out, norms;
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   int last_iters = 0;
@@ -311,6 +318,54 @@ int set_device(hamsx_ctx* c) {
 int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(HAMSX_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// Diagnostic switch: HAMSX_ASSEMBLE_GRID=1 assembles finite depth with the
+// unbinned grid kernel (same numbers, no sort; used to cross-check the bins).
+bool force_grid_assembly() {
+  const char* e = std::getenv("HAMSX_ASSEMBLE_GRID");
+  return e && *e && *e != '0';
+}
+
+// Counting sort of the N^2 pairs by (region, NT, FLAG) for the current depth;
+// done once per (mesh, h) and reused by every frequency at that depth.
+int ensure_bins(hamsx_ctx* c) {
+  using namespace hamsx;
+  const double h = c->freq.h;
+  if (c->bin_n == c->n && c->bin_h == h) return 0;
+  c->bin_n = 0;
+  int rc;
+  if ((rc = c->bin_cnt.ensure(kFinBins))) return rc;
+  HX_TRY(hipMemsetAsync(c->bin_cnt.p, 0, sizeof(int) * kFinBins, c->stream));
+  launch::bin_count(c->stream, c->mesh(), h, c->bin_cnt.p);
+  if ((rc = check_launch())) return rc;
+  std::vector<int> cnt(kFinBins), start(kFinBins);
+  HX_TRY(hipMemcpyAsync(cnt.data(), c->bin_cnt.p, sizeof(int) * kFinBins, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  size_t off = 0;
+  for (int r = 0; r < 5; r++) c->reg_begin[r] = c->reg_end[r] = 0;
+  bool seen[5] = {false, false, false, false, false};
+  for (int k = 0; k < kFinBins; k++) {
+    start[k] = (int)off;
+    const size_t padded = ((size_t)cnt[k] + 63) / 64 * 64;
+    if (cnt[k] > 0) {
+      const int r = fin_key_region(k);
+      if (!seen[r]) c->reg_begin[r] = off;
+      seen[r] = true;
+      c->reg_end[r] = off + padded;
+    }
+    off += padded;
+  }
+  if (off > (size_t)INT32_MAX) return fail(HAMSX_EARG, "too many panel pairs for the binned assembly");
+  if ((rc = c->plist.ensure(std::max<size_t>(off, 1)))) return rc;
+  HX_TRY(hipMemsetAsync(c->plist.p, 0xFF, sizeof(unsigned) * off, c->stream));
+  HX_TRY(hipMemcpyAsync(c->bin_cnt.p, start.data(), sizeof(int) * kFinBins, hipMemcpyHostToDevice, c->stream));
+  launch::bin_scatter(c->stream, c->mesh(), h, c->bin_cnt.p, c->plist.p);
+  if ((rc = check_launch())) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));  // start[] is read by the copy above
+  c->bin_n = c->n;
+  c->bin_h = h;
   return 0;
 }
 
@@ -485,8 +540,9 @@ int hamsx_destroy(hamsx_ctx* c) {
                   &c->norms})
     b->release();
   for (auto* b : {&c->ncn, &c->body, &c->body_start_d, &c->chunk_body, &c->chunk_j0, &c->chunk_j1, &c->ipiv,
-                  &c->perm, &c->widx, &c->wcnt, &c->flags})
+                  &c->perm, &c->widx, &c->wcnt, &c->flags, &c->bin_cnt})
     b->release();
+  c->plist.release();
   for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial}) b->release();
   c->Af.release();
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
@@ -666,6 +722,7 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
     }
   }
   c->n = nelem;
+  c->bin_n = 0;
   c->nbody = nbody;
   c->nchunk = (int)cb.size();
   c->body_start.assign(body_start, body_start + nbody + 1);
@@ -736,9 +793,15 @@ int hamsx_assemble(hamsx_ctx* c) {
   if (!c->have_freq) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_frequency first");
   int rc = set_device(c);
   if (rc) return rc;
+  const bool binned = c->freq.h >= 0.0 && c->n <= 0xFFFF && !force_grid_assembly();
+  if (binned && (rc = ensure_bins(c))) return rc;
   {
     StageTimer tm(c, 0);
-    hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
+    if (binned)
+      hamsx::launch::assemble_binned(c->stream, c->mesh(), c->freq, c->plist.p, c->reg_begin, c->reg_end, c->A.p,
+                                     c->S.p, c->flags.p);
+    else
+      hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
   }
   if ((rc = check_launch())) return rc;
   c->have_A = c->have_S = true;

@@ -34,6 +34,20 @@ struct MeshDev {
   const int* body;      // body index of each panel
 };
 
+// Regions of the finite-depth Green function by R/h (FinGreen3D.f90:183-191)
+// and the key the binned assembly sorts pairs by: key = 2 * subkey + FLAG,
+// subkey ascending by region -- LINTON 0; PIDCOCK 1 + (99 - NT) for NT in
+// 49..99; EIGENE 52 + (50 - NT) for NT in 10..50; EIGEN 93.  The key depends
+// on the geometry and h only.
+enum FinRegion { kRegAny = 0, kRegLinton = 1, kRegPidcock = 2, kRegEigene = 3, kRegEigen = 4 };
+constexpr int kSubPidcock = 1, kSubEigene = 52, kSubEigen = 93;
+constexpr int kFinBins = 256;
+static_assert(2 * kSubEigen + 2 <= kFinBins, "key space");
+inline constexpr int fin_key_region(int key) {
+  return key / 2 < kSubPidcock ? kRegLinton : key / 2 < kSubEigene ? kRegPidcock
+                                            : key / 2 < kSubEigen ? kRegEigene : kRegEigen;
+}
+
 struct FreqDev {
   double v, wk, w1, h;
   const double* wvn;  // kNK roots on the device
@@ -49,6 +63,13 @@ struct WaveDev {
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
+// finite depth, binned: count[kFinBins] += pairs per key; then scatter the
+// packed (j << 16 | i) pair indices to list[cursor[key]++]
+void bin_count(hipStream_t s, const MeshDev& m, double h, int* count);
+void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list);
+// region r (1 LINTON, 2 PIDCOCK, 3 EIGENE, 4 EIGEN) owns list[region_begin[r], region_end[r])
+void assemble_binned(hipStream_t s, const MeshDev& m, const FreqDev& f, const unsigned* list,
+                     const size_t region_begin[5], const size_t region_end[5], double2* A, double2* S, int* flags);
 // right-hand sides X(n, 6*nbody + nbeta) = S * generator columns
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
          const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
