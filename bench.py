@@ -57,7 +57,21 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB three times (the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    x = torch.ones(mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": x.numel() * 8}), flush=True)
 
 
 def pick_case(name):
@@ -105,6 +119,9 @@ def cpu_baseline(case, mesh, rows, threads):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs one GPU per process (asked for --gpus {args.gpus}); this build has no multi-GPU "
               "path -- run with --gpus 1", file=sys.stderr)

@@ -268,6 +268,57 @@ __device__ inline double d_limes(int n, double* s, double* a) {
   return rl;
 }
 
+// The same Limes with s[] and a[] held in registers: the two sweeps of each
+// column pair are fused into one pass (a_new[i+1] is formed just before
+// s_new[i] needs it, which reads only entries the unfused order has not yet
+// overwritten), and the pass is unrolled over the fixed array length in
+// straight-line chunks of kLimesChunk steps, so every index is static and the
+// divisions of neighbouring steps can overlap.  A chunk that straddles the
+// live length also updates entries beyond it; those entries are dead (no
+// later live step reads an s entry above n - 2k or an a entry above
+// n - 2k + 1), and the selection scan is predicated on the live length.  Each
+// live table entry is the same expression on the same operands as in
+// d_limes, and the scan visits them in the same order: the result is
+// identical.  Needs n <= NMAX - 1.
+constexpr int kLimesChunk = 8;
+template <int NMAX>
+__device__ __forceinline__ double d_limes_reg(int n, double (&s)[NMAX], double (&a)[NMAX]) {
+  double rl = 0.0, amax = 0.0;
+  bool done = false;
+#pragma unroll
+  for (int i = 0; i < NMAX; i++) {
+    if (i == n) rl = s[i];
+    a[i] = 0.0;
+  }
+  for (int k = 1;; k++) {
+    const int ls = n - 2 * k;  // last s index of this column pair; the a sweep ends at ls + 1
+    if (ls + 1 < 0) break;
+    {
+      double d = s[1] - s[0];
+      a[0] = (s[1] != s[0]) ? a[1] + 1.0 / d : a[1];
+    }
+    if (ls < 0) break;
+#pragma unroll
+    for (int c = 0; c < NMAX - 2; c += kLimesChunk) {
+      if (c <= ls) {
+#pragma unroll
+        for (int i = c; i < c + kLimesChunk && i < NMAX - 2; i++) {
+          double d = s[i + 2] - s[i + 1];
+          a[i + 1] = (s[i + 2] != s[i + 1]) ? a[i + 2] + 1.0 / d : a[i + 2];
+          const double si1 = s[i + 1];
+          s[i] = (a[i + 1] != a[i]) ? si1 + 1.0 / (a[i + 1] - a[i]) : si1;
+          const bool ev = !done && i <= ls && fabs(a[i]) > amax;
+          amax = ev ? fabs(a[i]) : amax;
+          rl = ev ? s[i] : rl;
+          done = done || (ev && si1 == s[i]);
+        }
+        if (done) return rl;
+      }
+    }
+  }
+  return rl;
+}
+
 // ---------------------------------------------------------------------------
 // Deep water: INFGREEN3D (InfGreen_Appr.f90:104-186), Havelock global fit
 // (well conditioned: contraction back on)
@@ -467,9 +518,11 @@ __device__ inline void d_eigen(double r, double zf, double zp, const FreqParams&
 constexpr int kEigeneMax = 51;    // NT <= INT(-88.89*0.05 + 54.45) = 50
 constexpr int kPidcockMax = 100;  // NT <= INT(-1010.10*0.0005 + 100.50) = 99
 
-// EIGENE: 0.05 <= R/h < 0.5, epsilon-accelerated (:413-506)
+// EIGENE: 0.05 <= R/h < 0.5, epsilon-accelerated (:413-506).  The three
+// partial-sum series are written once to a per-lane array and then
+// extrapolated one at a time by the register-resident Limes.
 __device__ __noinline__ void d_eigene(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
-  double g[kEigeneMax], gr[kEigeneMax], gz[kEigeneMax], aux[kEigeneMax];
+  double ser[3][kEigeneMax];
   int nt;
   {
 #pragma clang fp contract(off)
@@ -477,17 +530,29 @@ __device__ __noinline__ void d_eigene(double r, double zf, double zp, const Freq
   }
   if (nt > kEigeneMax) nt = kEigeneMax;
   d_wave_mode(r, zf, zp, fp, false, fg);
-  g[0] = gr[0] = gz[0] = 0.0;
+  double g = 0.0, gr = 0.0, gz = 0.0;
+  ser[0][0] = ser[1][0] = ser[2][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     double rm[3];
     d_eigen_term(r, zf, zp, fp.wvn[i - 1], fp.h, rm);
-    g[i - 1] = g[i - 2] + rm[0];
-    gr[i - 1] = gr[i - 2] + rm[1];
-    gz[i - 1] = gz[i - 2] + rm[2];
+    g = g + rm[0];
+    gr = gr + rm[1];
+    gz = gz + rm[2];
+    ser[0][i - 1] = g;
+    ser[1][i - 1] = gr;
+    ser[2][i - 1] = gz;
   }
-  fg[0].r += d_limes(nt - 1, g, aux);
-  fg[1].r += d_limes(nt - 1, gr, aux);
-  fg[2].r += d_limes(nt - 1, gz, aux);
+  double res[3];
+#pragma unroll 1
+  for (int q = 0; q < 3; q++) {
+    double sv[kEigeneMax], av[kEigeneMax];
+#pragma unroll
+    for (int i = 0; i < kEigeneMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
+    res[q] = d_limes_reg<kEigeneMax>(nt - 1, sv, av);
+  }
+  fg[0].r += res[0];
+  fg[1].r += res[1];
+  fg[2].r += res[2];
   d_remove_rankine(r, zf, zp, fg);
 }
 
@@ -530,9 +595,10 @@ __device__ inline void d_image_sums(double r, double zf, double zp, double h, do
   out[2] = (-s1z + s2z) / (h * h);
 }
 
-// PIDCOCK: 0.0005 <= R/h < 0.05 (:558-660)
+// PIDCOCK: 0.0005 <= R/h < 0.05 (:558-660).  Partial sums go to a per-lane
+// array once; the three epsilon tables then run in registers.
 __device__ __noinline__ void d_pidcock(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
-  double g[kPidcockMax], gr[kPidcockMax], gz[kPidcockMax], aux[kPidcockMax];
+  double ser[3][kPidcockMax];
   const double ga = 0.5772156649015328;
   double h = fp.h, v = fp.v;
   int nt;
@@ -544,7 +610,8 @@ __device__ __noinline__ void d_pidcock(double r, double zf, double zp, const Fre
   d_wave_mode(r, zf, zp, fp, true, fg);
   fg[0].r += 2.0 / h * (ga + log(r / 4.0 / h));
   fg[1].r += 2.0 / (r * h);
-  g[0] = gr[0] = gz[0] = 0.0;
+  double g = 0.0, gr = 0.0, gz = 0.0;
+  ser[0][0] = ser[1][0] = ser[2][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     double mp = (i - 1) * kPi, umt = mp / h, um = fp.wvn[i - 1];
     double k0, k1, k0t, k1t;
@@ -555,18 +622,26 @@ __device__ __noinline__ void d_pidcock(double r, double zf, double zp, const Fre
     double amt = cos(umt * (zf + h)) * cos(umt * (zp + h)) / h;
     double dam = -(um * ac * sin(um * (zf + h)) * cos(um * (zp + h)));
     double damt = -(umt * sin(umt * (zf + h)) * cos(umt * (zp + h)) / h);
-    g[i - 1] = g[i - 2] + (am * k0 - amt * k0t);
-    gr[i - 1] = gr[i - 2] + (-(um * am * k1) + umt * amt * k1t);
-    gz[i - 1] = gz[i - 2] + (dam * k0 - damt * k0t);
+    g = g + (am * k0 - amt * k0t);
+    gr = gr + (-(um * am * k1) + umt * amt * k1t);
+    gz = gz + (dam * k0 - damt * k0t);
+    ser[0][i - 1] = g;
+    ser[1][i - 1] = gr;
+    ser[2][i - 1] = gz;
   }
-  double s0 = 4.0 * d_limes(nt - 1, g, aux);
-  double s1 = 4.0 * d_limes(nt - 1, gr, aux);
-  double s2 = 4.0 * d_limes(nt - 1, gz, aux);
+  double res[3];
+#pragma unroll 1
+  for (int q = 0; q < 3; q++) {
+    double sv[kPidcockMax], av[kPidcockMax];
+#pragma unroll
+    for (int i = 0; i < kPidcockMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
+    res[q] = 4.0 * d_limes_reg<kPidcockMax>(nt - 1, sv, av);
+  }
   double im[3];
   d_image_sums(r, zf, zp, h, im);
-  fg[0].r = fg[0].r + s0 + im[0];
-  fg[1].r = fg[1].r + s1 + im[1];
-  fg[2].r = fg[2].r + s2 + im[2];
+  fg[0].r = fg[0].r + res[0] + im[0];
+  fg[1].r = fg[1].r + res[1] + im[1];
+  fg[2].r = fg[2].r + res[2] + im[2];
   d_remove_rankine(r, zf, zp, fg);
 }
 
