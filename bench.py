@@ -51,7 +51,10 @@ def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1, help="untimed solves per context before timing")
+    ap.add_argument("--streams", type=int, default=4, help="frequency solves kept in flight (one context and HIP "
+                    "stream each)")
+    ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
     ap.add_argument("--config", default="two-cylinders", choices=["two-cylinders", "five-cylinders", "one-cylinder"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -134,25 +137,48 @@ def main():
     n = mesh.nelem
     heads = np.deg2rad(case.headings_deg)
     omegas = [float(w) for w in case.omegas]
-    solver = FrequencySolver(mesh, precision=args.precision)
-    solver.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+    nctx = max(1, args.streams)
+    solvers = []
+    for _ in range(nctx):
+        sv = FrequencySolver(mesh, precision=args.precision)
+        sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        solvers.append(sv)
     print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(omegas)} frequencies, "
-          f"{len(heads)} headings, precision {args.precision}", file=sys.stderr, flush=True)
-    for k in range(args.warmup):
-        solver.solve_frequency(omegas[k % len(omegas)], case.depth)
-    solver.set_profiling(True)
-    solver.stage_times(reset=True)
+          f"{len(heads)} headings, precision {args.precision}, {nctx} solves in flight", file=sys.stderr, flush=True)
+    # per-stage device time of isolated solves (one context, nothing else on the GPU)
+    solvers[0].solve_frequency(omegas[0], case.depth)
+    solvers[0].set_profiling(True)
+    solvers[0].stage_times(reset=True)
+    for k in range(args.stage_steps):
+        solvers[0].solve_frequency(omegas[(k * 7) % len(omegas)], case.depth)
+    stages = solvers[0].stage_times(reset=True)
+    solvers[0].set_profiling(False)
+    for k, sv in enumerate(solvers):
+        for _ in range(args.warmup):
+            sv.solve_frequency(omegas[k % len(omegas)], case.depth)
+    # timed: args.steps frequency solves, nctx in flight (one HIP stream per context)
     lat = []
+    t_sub = [0.0] * nctx
+    submitted = done = 0
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        _, _, _, info = solver.solve_frequency(omegas[k % len(omegas)], case.depth)
-        lat.append(time.perf_counter() - ts)
+    while submitted < min(nctx, args.steps):
+        t_sub[submitted] = time.perf_counter()
+        solvers[submitted].submit(omegas[submitted % len(omegas)], case.depth)
+        submitted += 1
+    while done < args.steps:
+        k = done % nctx
+        _, _, _, info = solvers[k].collect()
+        lat.append(time.perf_counter() - t_sub[k])
         if info != 0:
-            raise SystemExit(f"zero pivot at step {k}: INFO={info}")
+            raise SystemExit(f"zero pivot in solve {done}: INFO={info}")
+        done += 1
+        if submitted < args.steps:
+            t_sub[k] = time.perf_counter()
+            solvers[k].submit(omegas[submitted % len(omegas)], case.depth)
+            submitted += 1
     elapsed = time.perf_counter() - t0
-    stages = solver.stage_times(reset=True)
-    solver.close()
+    for sv in solvers:
+        sv.close()
     print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
 
     lat_ms = np.array(lat) * 1e3
@@ -182,7 +208,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "c64-lu+f64-refine",
         "data": "synthetic seeded cylinder meshes (no reference files read)",
         "config": {"workload": case.name, "panels": n, "depth_m": case.depth, "frequencies": len(omegas),
-                   "headings_deg": case.headings_deg, "precision": args.precision},
+                   "headings_deg": case.headings_deg, "precision": args.precision, "in_flight": nctx},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
         "roofline": roof,

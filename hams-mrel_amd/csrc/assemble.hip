@@ -22,6 +22,45 @@ __device__ __forceinline__ int near_flag(double xi, double yi, double zi, double
   return dist <= 50.0 * pn ? 1 : 0;
 }
 
+__device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int js, int jf) {
+  const double* t = f.tab;
+  const size_t mn = (size_t)kModes * n;
+  const double* p = t + kTabHead;
+  return FreqParams{f.v,     f.wk,       f.w1,           f.h,    f.wvn,          t,  t + kModes, t + 2 * kModes,
+                    t[3 * kModes], p, p + mn, p + 2 * mn, p + 3 * mn, p + 4 * mn, p + 4 * mn + n, n, js, jf};
+}
+
+// Mode tables: one thread per (panel, mode); panel 0's threads also write
+// the per-mode scalars, and thread (0, 0) N_0.
+__global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
+#pragma clang fp contract(off)
+  const int panel = blockIdx.x * 256 + threadIdx.x, mode = blockIdx.y, n = m.n;
+  if (panel >= n) return;
+  double* t = f.tab;
+  const size_t mn = (size_t)kModes * n;
+  double* p = t + kTabHead;
+  double cz, sz, ct, st;
+  mode_panel(mode, f.wvn, f.h, m.cz[panel], cz, sz, ct, st);
+  const size_t o = (size_t)mode * n + panel;
+  p[o] = cz;
+  p[mn + o] = sz;
+  p[2 * mn + o] = ct;
+  p[3 * mn + o] = st;
+  if (mode == 0) {
+    const double z = m.cz[panel];
+    p[4 * mn + panel] = cosh(f.wk * (z + f.h));
+    p[4 * mn + n + panel] = sinh(f.wk * (z + f.h));
+  }
+  if (panel == 0) {
+    double pm, ac, ut;
+    mode_scalars(mode, f.wvn, f.v, f.h, pm, ac, ut);
+    t[mode] = pm;
+    t[kModes + mode] = ac;
+    t[2 * kModes + mode] = ut;
+    if (mode == 0) t[3 * kModes] = wave_nm0(f.wk, f.h);
+  }
+}
+
 // One pair: the Green function (deep water, or finite depth in region REG),
 // the Newman integrals when FLAG = 1, and the entries A(i,j), S(i,j).
 template <bool DEEP, int REG>
@@ -39,7 +78,7 @@ __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, i
   if (DEEP) {
     d_infgreen(p, f.v, flag, g);
   } else {
-    FreqParams fp{f.v, f.wk, f.w1, f.h, f.wvn};
+    const FreqParams fp = freq_params(f, n, j, i);
     d_fingreen_r<REG>(p, fp, flag, g, &ovf);
   }
   if (ovf) atomicOr(flags, 1);
@@ -328,6 +367,10 @@ void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, dou
     HX_LAUNCH(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
   else
     HX_LAUNCH(k_assemble<false>, grid, dim3(256), 0, s, m, f, A, S, flags);
+}
+
+void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f) {
+  HX_LAUNCH(k_mode_tables, dim3((m.n + 255) / 256, kModes), dim3(256), 0, s, m, f);
 }
 
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count) {

@@ -464,13 +464,48 @@ __device__ inline void d_infgreen(const PairGeom& p, double v, int flag, cd g[4]
 struct FreqParams {
   double v, wk, w1, h;
   const double* wvn;  // HAMSX_NK roots, wvn[0] = wk
+  // Per-frequency tables (mode_tables below): every factor of a series term
+  // that depends only on the frequency, the mode and one panel's z.
+  const double *pm, *ac, *ut;  // [kModes] per mode
+  double nm0;                  // N_0 of the propagating mode
+  const double *cz, *sz;       // [kModes][n]: cos, sin of k_m (z + h)
+  const double *ct, *st;       // [kModes][n]: cos, sin of u_m (z + h), u_m = m pi / h
+  const double *chz, *shz;     // [n]: cosh, sinh of k (z + h)
+  int n;                       // table row stride (panels)
+  int js, jf;                  // table columns of zf (source panel) and zp (field panel)
 };
 
+// The table entries, each formed by the expression the reference evaluates
+// in place (EIGEN :300-337, EIGENE :433-474, PIDCOCK :598-640), so a table
+// lookup returns the value the in-place evaluation would.
+__device__ __forceinline__ void mode_scalars(int mode, const double* wvn, double v, double h, double& pm, double& ac,
+                                             double& ut) {
+  const double km = wvn[mode];
+  const double twokmh = 2.0 * km * h;
+  pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
+  ac = (km * km + v * v) / ((km * km + v * v) * h - v);
+  const double mp = mode * kPi;
+  ut = mp / h;
+}
+__device__ __forceinline__ void mode_panel(int mode, const double* wvn, double h, double z, double& cz, double& sz,
+                                           double& ct, double& st) {
+  const double km = wvn[mode];
+  const double kz = km * (z + h);
+  cz = cos(kz);
+  sz = sin(kz);
+  const double mp = mode * kPi, umt = mp / h;
+  ct = cos(umt * (z + h));
+  st = sin(umt * (z + h));
+}
+__device__ __forceinline__ double wave_nm0(double wk, double h) {
+  return h / 2.0 * (1.0 + sinh(2.0 * wk * h) / (2.0 * wk * h));
+}
+
 // Propagating mode (EIGEN :300-313, EIGENE :433-446, PIDCOCK :598-606)
-__device__ inline void d_wave_mode(double r, double zf, double zp, const FreqParams& fp, bool pidcock, cd fg[3]) {
-  double wk = fp.wk, h = fp.h;
-  double nm0 = h / 2.0 * (1.0 + sinh(2.0 * wk * h) / (2.0 * wk * h));
-  double chf = cosh(wk * (zf + h)), chp = cosh(wk * (zp + h)), shf = sinh(wk * (zf + h));
+__device__ inline void d_wave_mode(double r, const FreqParams& fp, bool pidcock, cd fg[3]) {
+  double wk = fp.wk;
+  double nm0 = fp.nm0;
+  double chf = fp.chz[fp.js], chp = fp.chz[fp.jf], shf = fp.shz[fp.js];
   double p0 = kPi / nm0 * chf * chp;
   double dp0 = pidcock ? wk * kPi / nm0 * shf * chp : wk * (kPi / nm0 * shf * chp);
   double j0, j1, y0, y1;
@@ -480,14 +515,13 @@ __device__ inline void d_wave_mode(double r, double zf, double zp, const FreqPar
   fg[2] = cd{-dp0 * y0, dp0 * j0};
 }
 
-// Evanescent term of EIGEN / EIGENE (:321-337)
-__device__ __forceinline__ void d_eigen_term(double r, double zf, double zp, double km, double h, double rm[3]) {
-  double twokmh = 2.0 * km * h;
-  double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
+// Evanescent term m (0-based mode index) of EIGEN / EIGENE (:321-337)
+__device__ __forceinline__ void d_eigen_term(double r, int mode, const FreqParams& fp, double rm[3]) {
+  const double km = fp.wvn[mode], pm = fp.pm[mode];
   double k0, k1;
   d_ik01a(km * r, k0, k1);
-  const double kzf = km * (zf + h), kzp = km * (zp + h);
-  const double ckzf = cos(kzf), ckzp = cos(kzp), skzf = sin(kzf);
+  const size_t row = (size_t)mode * fp.n;
+  const double ckzf = fp.cz[row + fp.js], ckzp = fp.cz[row + fp.jf], skzf = fp.sz[row + fp.js];
   rm[0] = pm * k0 * ckzf * ckzp;
   rm[1] = -(pm * km * k1 * ckzf * ckzp);
   rm[2] = -(pm * km * k0 * skzf * ckzp);
@@ -503,10 +537,10 @@ __device__ __forceinline__ void d_remove_rankine(double r, double zf, double zp,
 
 // EIGEN: R/h >= 0.5, at most 10 terms (:281-362)
 __device__ inline void d_eigen(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
-  d_wave_mode(r, zf, zp, fp, false, fg);
+  d_wave_mode(r, fp, false, fg);
   for (int i = 2; i <= 10; i++) {
     double rm[3];
-    d_eigen_term(r, zf, zp, fp.wvn[i - 1], fp.h, rm);
+    d_eigen_term(r, i - 1, fp, rm);
     fg[0].r += rm[0];
     fg[1].r += rm[1];
     fg[2].r += rm[2];
@@ -521,7 +555,7 @@ constexpr int kPidcockMax = 100;  // NT <= INT(-1010.10*0.0005 + 100.50) = 99
 // EIGENE: 0.05 <= R/h < 0.5, epsilon-accelerated (:413-506).  The three
 // partial-sum series are written once to a per-lane array and then
 // extrapolated one at a time by the register-resident Limes.
-__device__ __noinline__ void d_eigene(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
+__device__ __forceinline__ void d_eigene(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
   double ser[3][kEigeneMax];
   int nt;
   {
@@ -529,12 +563,12 @@ __device__ __noinline__ void d_eigene(double r, double zf, double zp, const Freq
     nt = (int)(HX_F(-88.89) * fabs(r / fp.h) + HX_F(54.45));
   }
   if (nt > kEigeneMax) nt = kEigeneMax;
-  d_wave_mode(r, zf, zp, fp, false, fg);
+  d_wave_mode(r, fp, false, fg);
   double g = 0.0, gr = 0.0, gz = 0.0;
   ser[0][0] = ser[1][0] = ser[2][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     double rm[3];
-    d_eigen_term(r, zf, zp, fp.wvn[i - 1], fp.h, rm);
+    d_eigen_term(r, i - 1, fp, rm);
     g = g + rm[0];
     gr = gr + rm[1];
     gz = gz + rm[2];
@@ -597,31 +631,35 @@ __device__ inline void d_image_sums(double r, double zf, double zp, double h, do
 
 // PIDCOCK: 0.0005 <= R/h < 0.05 (:558-660).  Partial sums go to a per-lane
 // array once; the three epsilon tables then run in registers.
-__device__ __noinline__ void d_pidcock(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
+__device__ __forceinline__ void d_pidcock(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
   double ser[3][kPidcockMax];
   const double ga = 0.5772156649015328;
-  double h = fp.h, v = fp.v;
+  double h = fp.h;
   int nt;
   {
 #pragma clang fp contract(off)
     nt = (int)(HX_F(-1010.10) * fabs(r / h) + HX_F(100.50));
   }
   if (nt > kPidcockMax) nt = kPidcockMax;
-  d_wave_mode(r, zf, zp, fp, true, fg);
+  d_wave_mode(r, fp, true, fg);
   fg[0].r += 2.0 / h * (ga + log(r / 4.0 / h));
   fg[1].r += 2.0 / (r * h);
   double g = 0.0, gr = 0.0, gz = 0.0;
   ser[0][0] = ser[1][0] = ser[2][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
-    double mp = (i - 1) * kPi, umt = mp / h, um = fp.wvn[i - 1];
+    const int mode = i - 1;
+    const size_t row = (size_t)mode * fp.n;
+    double umt = fp.ut[mode], um = fp.wvn[mode];
     double k0, k1, k0t, k1t;
     d_ik01a(um * r, k0, k1);
     d_ik01a(umt * r, k0t, k1t);
-    double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-    double am = ac * cos(um * (zf + h)) * cos(um * (zp + h));
-    double amt = cos(umt * (zf + h)) * cos(umt * (zp + h)) / h;
-    double dam = -(um * ac * sin(um * (zf + h)) * cos(um * (zp + h)));
-    double damt = -(umt * sin(umt * (zf + h)) * cos(umt * (zp + h)) / h);
+    double ac = fp.ac[mode];
+    const double cf = fp.cz[row + fp.js], cp = fp.cz[row + fp.jf], sf = fp.sz[row + fp.js];
+    const double tcf = fp.ct[row + fp.js], tcp = fp.ct[row + fp.jf], tsf = fp.st[row + fp.js];
+    double am = ac * cf * cp;
+    double amt = tcf * tcp / h;
+    double dam = -(um * ac * sf * cp);
+    double damt = -(umt * tsf * tcp / h);
     g = g + (am * k0 - amt * k0t);
     gr = gr + (-(um * am * k1) + umt * amt * k1t);
     gz = gz + (dam * k0 - damt * k0t);

@@ -226,12 +226,14 @@ struct hamsx_ctx {
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins, valid for bin_n panels at depth bin_h
   DevBuf<unsigned> plist;
+  DevBuf<double> modetab;  // per-frequency finite-depth mode tables
   DevBuf<int> bin_cnt;
   int bin_n = 0;
   double bin_h = 0.0;
   size_t reg_begin[5] = {0}, reg_end[5] = {0};
   DevBuf<double> out, norms;
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
+  bool pending = false;  // a submitted frequency solve not yet collected
   int last_iters = 0;
   double last_resid = 0.0;
   // stage timing (measurement hook)
@@ -453,8 +455,9 @@ int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
   return 0;
 }
 
-int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* info_out) {
-  const int nd = 6 * c->nbody;
+// Enqueues the force reductions and the copies of results, flags and
+// refinement norms into the pinned staging area (no host wait).
+int enqueue_forces(hamsx_ctx* c) {
   const size_t len = c->out_len();
   // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 2 ints][norms: 2 * nrhs]
   int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 2 + 2 * (size_t)c->nrhs());
@@ -471,6 +474,14 @@ int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* i
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0)
     HX_TRY(hipMemcpyAsync(hout + len + 2, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
                           c->stream));
+  return 0;
+}
+
+// Waits for the stream and unpacks the staging area written by enqueue_forces.
+int complete_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* info_out) {
+  const int nd = 6 * c->nbody;
+  const size_t len = c->out_len();
+  const double* hout = c->h_pinned + hamsx::kNK + c->nbeta;
   HX_TRY(hipStreamSynchronize(c->stream));
   harvest_marks(c);
   if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
@@ -490,6 +501,16 @@ int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* i
   }
   if (fl[0] & 1) return fail(HAMSX_EDEVICE, "adaptive quadrature stack exhausted in the finite-depth kernel");
   return 0;
+}
+
+int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* info_out) {
+  int rc = enqueue_forces(c);
+  if (rc) return rc;
+  return complete_forces(c, amas, bdmp, exfc, info_out);
+}
+
+int busy(hamsx_ctx*) {
+  return fail(HAMSX_ESTATE, "a submitted frequency solve is pending on this context: call hamsx_collect first");
 }
 
 }  // namespace
@@ -543,6 +564,7 @@ int hamsx_destroy(hamsx_ctx* c) {
                   &c->perm, &c->widx, &c->wcnt, &c->flags, &c->bin_cnt})
     b->release();
   c->plist.release();
+  c->modetab.release();
   for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial}) b->release();
   c->Af.release();
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
@@ -555,6 +577,7 @@ int hamsx_destroy(hamsx_ctx* c) {
 
 int hamsx_set_precision(hamsx_ctx* c, int mode) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (mode != HAMSX_PREC_FP64 && mode != HAMSX_PREC_MIXED) return fail(HAMSX_EARG, "unknown precision mode %d", mode);
   if (mode != c->prec) {
     c->prec = mode;
@@ -687,6 +710,7 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
                    const double* cen, const double* area, const double* pnsz, const double* dxyz, int nbody,
                    const int* body_start) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (nelem <= 0 || nnode <= 0 || nbody <= 0 || !xyz || !ncn || !ncon || !cen || !area || !pnsz || !dxyz ||
       !body_start)
     return fail(HAMSX_EARG, "hamsx_set_mesh: bad arguments");
@@ -751,6 +775,7 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
 
 int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, const double* wvn, int nk) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (h > 0.0 && (!wvn || nk < hamsx::kNK)) return fail(HAMSX_EARG, "finite depth needs %d dispersion roots", hamsx::kNK);
   int rc = set_device(c);
   if (rc) return rc;
@@ -769,6 +794,7 @@ int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, 
 int hamsx_set_waves(hamsx_ctx* c, int nbeta, const double* beta, double amp, double xw0, double xw1, double rho,
                     int isol) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (nbeta < 0 || (nbeta > 0 && !beta)) return fail(HAMSX_EARG, "bad heading list");
   if (isol != 1 && isol != 2) return fail(HAMSX_EARG, "ISOL must be 1 or 2");
   int rc = set_device(c);
@@ -789,14 +815,19 @@ int hamsx_set_waves(hamsx_ctx* c, int nbeta, const double* beta, double amp, dou
 
 int hamsx_assemble(hamsx_ctx* c) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_mesh first");
   if (!c->have_freq) return fail(HAMSX_ESTATE, "hamsx_assemble needs hamsx_set_frequency first");
   int rc = set_device(c);
   if (rc) return rc;
-  const bool binned = c->freq.h >= 0.0 && c->n <= 0xFFFF && !force_grid_assembly();
+  const bool finite = c->freq.h >= 0.0;
+  const bool binned = finite && c->n <= 0xFFFF && !force_grid_assembly();
   if (binned && (rc = ensure_bins(c))) return rc;
+  if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->n)))) return rc;
+  c->freq.tab = finite ? c->modetab.p : nullptr;
   {
     StageTimer tm(c, 0);
+    if (finite) hamsx::launch::mode_tables(c->stream, c->mesh(), c->freq);
     if (binned)
       hamsx::launch::assemble_binned(c->stream, c->mesh(), c->freq, c->plist.p, c->reg_begin, c->reg_end, c->A.p,
                                      c->S.p, c->flags.p);
@@ -811,6 +842,7 @@ int hamsx_assemble(hamsx_ctx* c) {
 
 int hamsx_factor(hamsx_ctx* c) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   int rc = set_device(c);
   if (rc) return rc;
   if ((rc = do_factor(c))) return rc;
@@ -822,6 +854,7 @@ int hamsx_factor(hamsx_ctx* c) {
 
 int hamsx_solve_radiation(hamsx_ctx* c, double* mxpot) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   int rc = set_device(c);
   if (rc || (rc = do_solve(c))) return rc;
   return fetch_mxpot(c, 0, 6 * c->nbody, mxpot);
@@ -829,6 +862,7 @@ int hamsx_solve_radiation(hamsx_ctx* c, double* mxpot) {
 
 int hamsx_solve_diffraction(hamsx_ctx* c, double* mxpot) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   int rc = set_device(c);
   if (rc || (rc = do_solve(c))) return rc;
   return fetch_mxpot(c, 6 * c->nbody, c->nbeta, mxpot);
@@ -836,6 +870,7 @@ int hamsx_solve_diffraction(hamsx_ctx* c, double* mxpot) {
 
 int hamsx_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   int rc = set_device(c);
   if (rc || (rc = do_solve(c))) return rc;
   int info = 0;
@@ -845,6 +880,7 @@ int hamsx_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc) {
 
 int hamsx_solve_frequency(hamsx_ctx* c, double w1, double h, double* amas, double* bdmp, double* exfc) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
   if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_solve_frequency needs hamsx_set_waves first");
   double v, wk, w, wvn[HAMSX_NK];
   int rc = hamsx_wave_properties(3, w1, h, &v, &wk, &w, wvn);
@@ -855,8 +891,43 @@ int hamsx_solve_frequency(hamsx_ctx* c, double w1, double h, double* amas, doubl
   return info;
 }
 
+int hamsx_submit_frequency(hamsx_ctx* c, double w1, double h) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_submit_frequency needs hamsx_set_waves first");
+  double v, wk, w, wvn[HAMSX_NK];
+  int rc = hamsx_wave_properties(3, w1, h, &v, &wk, &w, wvn);
+  if (rc || (rc = hamsx_set_frequency(c, v, wk, w, h, wvn, HAMSX_NK)) || (rc = hamsx_assemble(c))) return rc;
+  if ((rc = do_solve(c)) || (rc = enqueue_forces(c))) return rc;
+  c->pending = true;
+  return 0;
+}
+
+int hamsx_query(hamsx_ctx* c) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (!c->pending) return 1;
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(HAMSX_EDEVICE, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+int hamsx_collect(hamsx_ctx* c, double* amas, double* bdmp, double* exfc) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (!c->pending) return fail(HAMSX_ESTATE, "nothing submitted on this context");
+  int rc = set_device(c);
+  if (rc) return rc;
+  c->pending = false;
+  int info = 0;
+  if ((rc = complete_forces(c, amas, bdmp, exfc, &info))) return rc;
+  return info;
+}
+
 int hamsx_get_matrix(hamsx_ctx* c, int which, double* out, int* ipiv) {
   if (!c || !out) return fail(HAMSX_EARG, "bad arguments");
+  if (c->pending) return busy(c);
   int rc = set_device(c);
   if (rc) return rc;
   const size_t nn = (size_t)c->n * c->n;

@@ -51,7 +51,15 @@ inline constexpr int fin_key_region(int key) {
 struct FreqDev {
   double v, wk, w1, h;
   const double* wvn;  // kNK roots on the device
+  double* tab;        // finite depth: per-frequency mode tables (layout below)
 };
+
+// Per-frequency mode tables of the finite-depth series, kModes modes:
+// [pm | ac | ut] (kModes each), nm0 (1), padding to kTabHead, then
+// [cz | sz | ct | st] (kModes x n each) and [chz | shz] (n each).
+constexpr int kModes = 100;
+constexpr size_t kTabHead = 3 * kModes + 8;
+inline size_t mode_tab_len(int n) { return kTabHead + (size_t)4 * kModes * n + 2 * (size_t)n; }
 
 struct WaveDev {
   int nbeta;
@@ -63,6 +71,8 @@ struct WaveDev {
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
+// finite depth: fill f.tab for the current frequency (before either assembly)
+void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f);
 // finite depth, binned: count[kFinBins] += pairs per key; then scatter the
 // packed (j << 16 | i) pair indices to list[cursor[key]++]
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count);

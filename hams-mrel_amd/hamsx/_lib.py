@@ -43,6 +43,9 @@ SIGNATURES = {
     "hamsx_solve_diffraction": (C.c_int, [_vp, _dp]),
     "hamsx_forces": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_solve_frequency": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, _dp]),
+    "hamsx_submit_frequency": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "hamsx_query": (C.c_int, [_vp]),
+    "hamsx_collect": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),

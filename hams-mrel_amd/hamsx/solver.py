@@ -122,6 +122,25 @@ class FrequencySolver:
                           "hamsx_solve_frequency", allow_info=True)
         return amas, bdmp, exfc[:, :self.nbeta], info
 
+    # -- non-blocking form (several contexts in flight) ------------------------
+    def submit(self, w1: float, h: float):
+        """Enqueue one whole frequency solve on this context's stream."""
+        _lib.check(self._lib.hamsx_submit_frequency(self._h, w1, h), "hamsx_submit_frequency")
+
+    def query(self) -> bool:
+        """True when the submitted solve has finished (or none is pending)."""
+        return _lib.check(self._lib.hamsx_query(self._h), "hamsx_query", allow_info=True) == 1
+
+    def collect(self):
+        """Wait for the submitted solve: (amas, bdmp, exfc, info)."""
+        nd = 6 * self.mesh.nbody
+        amas = np.zeros((nd, nd), order="F")
+        bdmp = np.zeros((nd, nd), order="F")
+        exfc = np.zeros((nd, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        info = _lib.check(self._lib.hamsx_collect(self._h, _lib.ptr(amas), _lib.ptr(bdmp), _lib.ptr(exfc)),
+                          "hamsx_collect", allow_info=True)
+        return amas, bdmp, exfc[:, :self.nbeta], info
+
     def get_matrix(self, which: str):
         n = self.mesh.nelem
         code = {"A": 0, "S": 1, "LU": 2}[which]

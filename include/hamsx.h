@@ -122,6 +122,19 @@ int hamsx_forces(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
 int hamsx_solve_frequency(hamsx_ctx *ctx, double w1, double h, double *amas, double *bdmp,
                           double *exfc);
 
+/* Non-blocking form of hamsx_solve_frequency for pipelines that keep
+ * several solves in flight on several contexts (one stream each): submit
+ * enqueues the whole solve and returns; query returns 1 when it has finished
+ * (0 while running); collect waits for it, fills amas/bdmp/exfc as
+ * hamsx_solve_frequency does and returns its INFO.  While a solve is pending
+ * every other call on the context except hamsx_query, hamsx_collect and
+ * hamsx_destroy returns HAMSX_ESTATE.  No counterpart in the reference,
+ * whose frequency loop (src/HAMS_Prog.f90:613-733) is strictly serial; the
+ * first submit at a new depth sorts the pairs and waits for the device once. */
+int hamsx_submit_frequency(hamsx_ctx *ctx, double w1, double h);
+int hamsx_query(hamsx_ctx *ctx);
+int hamsx_collect(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
+
 /* Parity/debug: copy A (which = 0), S (1) or the LU factors (2) to host
  * out[nelem*nelem] (complex); ipiv[nelem] (1-based) for which = 2, else NULL. */
 int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);

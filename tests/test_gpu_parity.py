@@ -180,3 +180,38 @@ def test_single_cylinder_config():
         am, bd, ex, info = s.solve_frequency(float(case.omegas[0]), case.depth)
     ao, bo, eo, _ = oracle.solve_frequency(m, float(case.omegas[0]), case.depth, heads)
     assert max(rel(am, ao), rel(bd, bo), rel(ex, eo)) < 1e-8
+
+
+def test_submitted_solves_in_flight_equal_blocking_solves(mesh_small):
+    """Solves kept in flight on three contexts (three streams) give the same
+    numbers bit for bit as the blocking call on one context, and a pending
+    context refuses other work."""
+    from hamsx._lib import HamsxError
+    m = mesh_small
+    heads = np.deg2rad([0.0, 30.0])
+    cases = [(0.6, 40.0), (1.4, 40.0), (0.9, -1.0), (2.0, 25.0)]
+    say(f"[gpu] submit/collect n={m.nelem}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        ref = [s.solve_frequency(w, h) for w, h in cases]
+    ctxs = [FrequencySolver(m) for _ in range(3)]
+    try:
+        for c in ctxs:
+            c.set_waves(heads)
+        got = [None] * len(cases)
+        ctxs[0].submit(*cases[0])
+        ctxs[1].submit(*cases[1])
+        ctxs[2].submit(*cases[2])
+        with pytest.raises(HamsxError):
+            ctxs[0].assemble()
+        got[0] = ctxs[0].collect()
+        ctxs[0].submit(*cases[3])
+        got[1] = ctxs[1].collect()
+        got[2] = ctxs[2].collect()
+        got[3] = ctxs[0].collect()
+    finally:
+        for c in ctxs:
+            c.close()
+    for r, g in zip(ref, got):
+        for a, b in zip(r, g):
+            np.testing.assert_array_equal(a, b)
