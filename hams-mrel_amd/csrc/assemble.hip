@@ -158,8 +158,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshDev m, double h, int* _
   if (live) list[base[key] + rank] = ((unsigned)j << 16) | (unsigned)i;
 }
 
+// EIGENE's register-resident epsilon tables take ~200 VGPRs: asking for two
+// waves per SIMD keeps the kernel within 256 registers (a few spills)
+// instead of one wave per SIMD.
 template <int REG>
-__global__ __launch_bounds__(256) void k_assemble_list(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REG == kRegEigene || REG == kRegPidcock ? 2 : 1))) void k_assemble_list(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
                                                        int count, double2* __restrict__ A, double2* __restrict__ S,
                                                        int* flags) {
   const int t = blockIdx.x * 256 + threadIdx.x;
