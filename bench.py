@@ -16,8 +16,9 @@ Prints ONE JSON line.  Besides the common fields it carries
   * latency_ms: p50/p99 of the per-solve wall time;
   * stage_ms_per_solve: device time per stage from HIP events on the
     solver's own stream;
-  * roofline: the dominant kernel (the assembly) against the FP64 VALU peak,
-    plus the LU factorisation against the MFMA peak;
+  * roofline: the kernel with the largest device time per solve against its
+    bound (the finite-depth EIGENE region kernel: FP64 VALU), plus the LU
+    factorisation against the MFMA peak;
   * cpu_baseline: the oracle (a C restatement of the reference's OpenMP
     Fortran loop) on a bounded row sample of the same assembly, plus LAPACK
     zgetrf/zgetrs on the full matrix, on the host's cores.
@@ -39,12 +40,73 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector and FP64 matrix (dense)
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# Per-pair FP64 work of the assembly kernel and its HBM traffic per launch,
-# measured with rocprofv3 PMC counters on the named workload (the summaries
-# and the commands that made them are under profiles/).  flops_per_pair is
-# lane-level FP64 operations (FMA = 2): issued FP64 VALU instructions x 64
-# lanes x the measured fraction of active lanes, divided by the N^2 pairs.
-ASSEMBLY_COUNTERS = {}
+# FP64 work and HBM traffic per panel pair of each finite-depth region
+# kernel, measured with rocprofv3 counters on the named workload (summary and
+# commands: profiles/pmc_assembly_two_cylinders_fp64.md).  flops = lane-level
+# FP64 operations issued (FMA = 2, hardware transcendental = 1):
+# SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 x 64 x the active-lane fraction
+# SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU).  Traffic = FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0, the factors that make a 256 MiB device copy read 256 MiB
+# and write 256 MiB.  The algorithmic bytes of every region are 32 per pair
+# (A and S entries written once).
+REGION_COUNTERS = {
+    "cylinders-2x1984-h50": {
+        "asm_linton": {"flops_per_pair": 571424.7, "traffic_per_pair": (37738365 + 53079083) / 57728},
+        "asm_pidcock": {"flops_per_pair": 326377.2, "traffic_per_pair": (10817558282 + 15217550549) / 1079424},
+        "asm_eigene": {"flops_per_pair": 119803.3, "traffic_per_pair": (8720059880 + 7938387605) / 6735360},
+        "asm_eigen": {"flops_per_pair": 7985.8, "traffic_per_pair": (39528137 + 251920384) / 7872512},
+    },
+}
+REGIONS = ("asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
+
+
+def region_pairs(mesh, depth):
+    """Pairs per finite-depth region (R/h < 5e-4, < 0.05, < 0.5, >= 0.5),
+    classified with the kernels' own expression |sqrt(dx^2 + dy^2) / h|."""
+    if depth < 0:
+        return None
+    c = mesh.cen
+    out = np.zeros(4, dtype=np.int64)
+    for i0 in range(0, mesh.nelem, 1024):
+        dx = c[:, 0][None, :] - c[i0:i0 + 1024, 0][:, None]
+        dy = c[:, 1][None, :] - c[i0:i0 + 1024, 1][:, None]
+        rhp = np.abs(np.sqrt(dx * dx + dy * dy) / depth)
+        out += [(rhp < 5e-4).sum(), ((rhp >= 5e-4) & (rhp < 0.05)).sum(), ((rhp >= 0.05) & (rhp < 0.5)).sum(),
+                (rhp >= 0.5).sum()]
+    return dict(zip(REGIONS, out.tolist()))
+
+
+def roofline(case, mesh, ms, precision):
+    """Roofline of the kernel with the largest device time per solve."""
+    n = mesh.nelem
+    cands = [k for k in REGIONS + ("factor", "solve") if ms.get(k, 0.0) > 0.0]
+    dom = max(cands, key=lambda k: ms[k])
+    if dom == "factor":
+        flops = 8.0 * n ** 3 / 3.0
+        peak = PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
+        return {"kernel": "LU factorisation (tournament-pivoted panels + MFMA trailing updates)", "bound": "mfma",
+                "achieved": flops / (ms[dom] * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
+                "frac": flops / (ms[dom] * 1e-3) / 1e12 / peak, "traffic": None, "launch_ms": ms[dom],
+                "work": "8 N^3 / 3 real flops"}
+    if dom == "solve":
+        nbytes = 16.0 * n * n
+        return {"kernel": "triangular solves", "bound": "hbm", "achieved": nbytes / (ms[dom] * 1e-3) / 1e9,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": nbytes / (ms[dom] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                "traffic": None, "launch_ms": ms[dom], "work": "16 N^2 bytes (the factors read once)"}
+    pairs = region_pairs(mesh, case.depth)[dom]
+    cnt = REGION_COUNTERS.get(case.name, {}).get(dom)
+    out = {"kernel": f"k_assemble_list<{dom[4:].upper()}> (finite-depth Green function region + Newman + A/S "
+                     "entries, one thread per pair)",
+           "bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "launch_ms": ms[dom], "pairs": pairs,
+           "algorithmic_bytes": 32.0 * pairs}
+    if cnt:
+        flops = cnt["flops_per_pair"] * pairs
+        out.update({"achieved": flops / (ms[dom] * 1e-3) / 1e12,
+                    "frac": flops / (ms[dom] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                    "traffic": cnt["traffic_per_pair"] * pairs, "flops_per_pair": cnt["flops_per_pair"]})
+    else:
+        out.update({"achieved": None, "frac": None, "traffic": None})
+    return out
 
 
 def parse():
@@ -183,24 +245,11 @@ def main():
 
     lat_ms = np.array(lat) * 1e3
     ms = {k: v[0] / max(1, v[1]) for k, v in stages.items()}
-    t_asm = ms["assemble"] * 1e-3
-    pairs = float(n) * n
-    cnt = ASSEMBLY_COUNTERS.get((case.name, args.precision)) or ASSEMBLY_COUNTERS.get((case.name, "fp64"))
-    roof = {"kernel": "k_assemble (Green function + Newman integrals + A/S writes, one thread per pair)",
-            "bound": "valu-fp64", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "launch_ms": ms["assemble"],
-            "pairs_per_launch": pairs, "gpairs_per_s": pairs / t_asm / 1e9}
-    if cnt:
-        flops = cnt["flops_per_pair"] * pairs
-        roof.update({"achieved": flops / t_asm / 1e12, "frac": flops / t_asm / 1e12 / PEAK_FP64_TFLOPS,
-                     "flops_per_pair": cnt["flops_per_pair"], "traffic": cnt.get("traffic_bytes"),
-                     "algorithmic_bytes": 32.0 * pairs})
-    else:
-        roof.update({"achieved": None, "frac": None, "traffic": None, "algorithmic_bytes": 32.0 * pairs})
+    roof = roofline(case, mesh, ms, args.precision)
     lu_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
     lu_tf = 8.0 * n ** 3 / 3.0 / (ms["factor"] * 1e-3) / 1e12
-    roof_lu = {"kernel": "blocked LU (trailing updates on MFMA)", "bound": "mfma", "unit": "TFLOP/s",
-               "achieved": lu_tf, "peak": lu_peak, "frac": lu_tf / lu_peak, "launch_ms": ms["factor"],
-               "flops": 8.0 * n ** 3 / 3.0}
+    roof_lu = {"kernel": "LU factorisation", "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
+               "frac": lu_tf / lu_peak, "launch_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,

@@ -386,30 +386,25 @@ void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigne
   HX_LAUNCH(k_bin_scatter, grid, dim3(256), 0, s, m, h, cursor, list);
 }
 
-void assemble_binned(hipStream_t s, const MeshDev& m, const FreqDev& f, const unsigned* list,
-                     const size_t region_begin[5], const size_t region_end[5], double2* A, double2* S, int* flags) {
-  HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
-  // most expensive regions first, so the cheap ones fill the tail
-  const int order[4] = {kRegLinton, kRegPidcock, kRegEigene, kRegEigen};
-  for (int r : order) {
-    const size_t cnt = region_end[r] - region_begin[r];
-    if (cnt == 0) continue;
-    const dim3 grid((unsigned)((cnt + 255) / 256));
-    const unsigned* l = list + region_begin[r];
-    switch (r) {
-      case kRegLinton:
-        HX_LAUNCH(k_assemble_list<kRegLinton>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
-        break;
-      case kRegPidcock:
-        HX_LAUNCH(k_assemble_list<kRegPidcock>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
-        break;
-      case kRegEigene:
-        HX_LAUNCH(k_assemble_list<kRegEigene>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
-        break;
-      default:
-        HX_LAUNCH(k_assemble_list<kRegEigen>, grid, dim3(256), 0, s, m, f, l, (int)cnt, A, S, flags);
-        break;
-    }
+void clear_flags(hipStream_t s, int* flags) { HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags); }
+
+void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, const unsigned* list,
+                     size_t count, double2* A, double2* S, int* flags) {
+  if (count == 0) return;
+  const dim3 grid((unsigned)((count + 255) / 256));
+  switch (region) {
+    case kRegLinton:
+      HX_LAUNCH(k_assemble_list<kRegLinton>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S, flags);
+      break;
+    case kRegPidcock:
+      HX_LAUNCH(k_assemble_list<kRegPidcock>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S, flags);
+      break;
+    case kRegEigene:
+      HX_LAUNCH(k_assemble_list<kRegEigene>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S, flags);
+      break;
+    default:
+      HX_LAUNCH(k_assemble_list<kRegEigen>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S, flags);
+      break;
   }
 }
 

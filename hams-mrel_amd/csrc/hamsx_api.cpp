@@ -828,10 +828,15 @@ int hamsx_assemble(hamsx_ctx* c) {
   {
     StageTimer tm(c, 0);
     if (finite) hamsx::launch::mode_tables(c->stream, c->mesh(), c->freq);
-    if (binned)
-      hamsx::launch::assemble_binned(c->stream, c->mesh(), c->freq, c->plist.p, c->reg_begin, c->reg_end, c->A.p,
-                                     c->S.p, c->flags.p);
-    else
+    if (binned) {
+      hamsx::launch::clear_flags(c->stream, c->flags.p);
+      // most expensive regions first, so the cheap ones fill the tail
+      for (int r : {hamsx::kRegLinton, hamsx::kRegPidcock, hamsx::kRegEigene, hamsx::kRegEigen}) {
+        StageTimer tr(c, 4 + r);
+        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, c->plist.p + c->reg_begin[r],
+                                       c->reg_end[r] - c->reg_begin[r], c->A.p, c->S.p, c->flags.p);
+      }
+    } else
       hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
   }
   if ((rc = check_launch())) return rc;

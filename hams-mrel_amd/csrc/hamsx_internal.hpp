@@ -77,9 +77,11 @@ void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f);
 // packed (j << 16 | i) pair indices to list[cursor[key]++]
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count);
 void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list);
-// region r (1 LINTON, 2 PIDCOCK, 3 EIGENE, 4 EIGEN) owns list[region_begin[r], region_end[r])
-void assemble_binned(hipStream_t s, const MeshDev& m, const FreqDev& f, const unsigned* list,
-                     const size_t region_begin[5], const size_t region_end[5], double2* A, double2* S, int* flags);
+// one region's kernel (1 LINTON, 2 PIDCOCK, 3 EIGENE, 4 EIGEN) over its slice
+// of the sorted pair list
+void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, const unsigned* list,
+                     size_t count, double2* A, double2* S, int* flags);
+void clear_flags(hipStream_t s, int* flags);
 // right-hand sides X(n, 6*nbody + nbeta) = S * generator columns
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
          const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,

@@ -1,0 +1,318 @@
+! hamsx_mod.f90 -- Fortran 2003 (iso_c_binding) binding of libhamsx.so, the
+! MI355X (gfx950) frequency-domain panel solver, for a HAMS-MREL style host.
+!
+! Part 1 declares every C entry point of include/hamsx.h.  Part 2 wraps them
+! in subroutines named after the per-frequency calls of HAMS-MREL's driver
+! (src/HAMS_Prog.f90:613-733) and taking the driver's own arrays:
+!
+!   GPU_INIT                 replaces nothing (opens a device context)
+!   SET_MESH_MULTI_GPU       after BuildCombinedArrays (CalGreenFuncMulti.f90:53-106)
+!   CALWAVEPROPERTS_GPU      CalWaveProperts (ImplementSubs.f90:40-151)
+!   CALGREEN_ASSB_LEFT_GPU   CALGREEN_MULTI + ASSB_LEFT_MULTI incl. ZGETRF
+!                            (CalGreenFuncMulti.f90:112-183, AssbMatxMulti.f90:53-114)
+!   RADIATION_SOLVER_GPU     ASSB_RBC_MULTI + RADIATION_SOLVER_MULTI (AssbMatxMulti.f90:124-317)
+!   DIFFRACTION_SOLVER_GPU   ASSB_DBC_MULTI + DIFFRACTION_SOLVER_MULTI (AssbMatxMulti.f90:187-383)
+!   FORCES_GPU               RFORCE_MULTI / EFORCE_MULTI sums (PotentWavForceMulti.f90:45-226)
+!   PANEL_GEOMETRY_GPU       NormalProcess for one body (NormalProcess.f90:46-441)
+!
+! The influence matrices and their factors stay in device memory; only the
+! potentials (optional) and the force coefficients come back.  Every wrapper
+! stops the program with the library's message on an error, the way the
+! reference stops on input errors.
+module hamsx_mod
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: HAMSX_OK = 0, HAMSX_NK = 200
+  integer(c_int), parameter, public :: HAMSX_PREC_FP64 = 0, HAMSX_PREC_MIXED = 1
+
+  public :: hamsx_version, hamsx_last_error, hamsx_device_count, hamsx_create, hamsx_destroy, &
+            hamsx_set_precision, hamsx_panel_geometry, hamsx_wave_properties, hamsx_set_mesh, &
+            hamsx_set_frequency, hamsx_set_waves, hamsx_assemble, hamsx_factor, hamsx_solve_radiation, &
+            hamsx_solve_diffraction, hamsx_forces, hamsx_solve_frequency, hamsx_submit_frequency, &
+            hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
+            hamsx_stage_times
+  public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
+            CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
+            PANEL_GEOMETRY_GPU, hamsx_error_message
+
+  ! ---------------------------------------------------------------- part 1
+  interface
+    function hamsx_version() bind(C, name="hamsx_version") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function hamsx_last_error() bind(C, name="hamsx_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function hamsx_device_count() bind(C, name="hamsx_device_count") result(n)
+      import :: c_int
+      integer(c_int) :: n
+    end function
+    function hamsx_create(ctx, device) bind(C, name="hamsx_create") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device
+      integer(c_int) :: rc
+    end function
+    function hamsx_destroy(ctx) bind(C, name="hamsx_destroy") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_precision(ctx, mode) bind(C, name="hamsx_set_precision") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      integer(c_int) :: rc
+    end function
+    function hamsx_panel_geometry(nnode, node_xyz, nelem, ncn, ncon, xr, cen, area, pnsz, dxyz) &
+        bind(C, name="hamsx_panel_geometry") result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: nnode, nelem
+      real(c_double), intent(in) :: node_xyz(*), xr(3)
+      integer(c_int), intent(in) :: ncn(*), ncon(*)
+      real(c_double), intent(out) :: cen(*), area(*), pnsz(*), dxyz(*)
+      integer(c_int) :: rc
+    end function
+    function hamsx_wave_properties(inft, value, h, v, wk, w1, wvn) bind(C, name="hamsx_wave_properties") result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: inft
+      real(c_double), value :: value, h
+      real(c_double), intent(out) :: v, wk, w1, wvn(*)
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_mesh(ctx, nelem, nnode, node_xyz, ncn, ncon, cen, area, pnsz, dxyz, nbody, body_start) &
+        bind(C, name="hamsx_set_mesh") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nelem, nnode, nbody
+      real(c_double), intent(in) :: node_xyz(*), cen(*), area(*), pnsz(*), dxyz(*)
+      integer(c_int), intent(in) :: ncn(*), ncon(*), body_start(*)
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_frequency(ctx, v, wk, w1, h, wvn, nk) bind(C, name="hamsx_set_frequency") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: v, wk, w1, h
+      real(c_double), intent(in) :: wvn(*)
+      integer(c_int), value :: nk
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_waves(ctx, nbeta, beta, amp, xw0, xw1, rho, isol) bind(C, name="hamsx_set_waves") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbeta, isol
+      real(c_double), intent(in) :: beta(*)
+      real(c_double), value :: amp, xw0, xw1, rho
+      integer(c_int) :: rc
+    end function
+    function hamsx_assemble(ctx) bind(C, name="hamsx_assemble") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function hamsx_factor(ctx) bind(C, name="hamsx_factor") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function hamsx_solve_radiation(ctx, mxpot) bind(C, name="hamsx_solve_radiation") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, mxpot
+      integer(c_int) :: rc
+    end function
+    function hamsx_solve_diffraction(ctx, mxpot) bind(C, name="hamsx_solve_diffraction") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, mxpot
+      integer(c_int) :: rc
+    end function
+    function hamsx_forces(ctx, amas, bdmp, exfc) bind(C, name="hamsx_forces") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, amas, bdmp, exfc
+      integer(c_int) :: rc
+    end function
+    function hamsx_solve_frequency(ctx, w1, h, amas, bdmp, exfc) bind(C, name="hamsx_solve_frequency") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx, amas, bdmp, exfc
+      real(c_double), value :: w1, h
+      integer(c_int) :: rc
+    end function
+    function hamsx_submit_frequency(ctx, w1, h) bind(C, name="hamsx_submit_frequency") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: w1, h
+      integer(c_int) :: rc
+    end function
+    function hamsx_query(ctx) bind(C, name="hamsx_query") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function hamsx_collect(ctx, amas, bdmp, exfc) bind(C, name="hamsx_collect") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, amas, bdmp, exfc
+      integer(c_int) :: rc
+    end function
+    function hamsx_get_matrix(ctx, which, out, ipiv) bind(C, name="hamsx_get_matrix") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, out, ipiv
+      integer(c_int), value :: which
+      integer(c_int) :: rc
+    end function
+    function hamsx_refine_stats(ctx, iters, resid) bind(C, name="hamsx_refine_stats") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: resid
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_profiling(ctx, on) bind(C, name="hamsx_set_profiling") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      integer(c_int) :: rc
+    end function
+    function hamsx_stage_times(ctx, reset, ms, count) bind(C, name="hamsx_stage_times") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reset
+      real(c_double), intent(out) :: ms(9)
+      integer(c_int), intent(out) :: count(9)
+      integer(c_int) :: rc
+    end function
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+  end interface
+
+contains
+
+  ! ---------------------------------------------------------------- part 2
+  function hamsx_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, k
+    p = hamsx_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [max(n, 1)])
+    allocate(character(len=n) :: msg)
+    do k = 1, n
+      msg(k:k) = chars(k)
+    end do
+  end function
+
+  subroutine check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc < 0) then
+      write(*, '(A,A,A,I0,A,A)') ' hamsx: ', where, ' failed (', rc, '): ', hamsx_error_message()
+      stop 1
+    end if
+  end subroutine
+
+  ! Open a context on GPU `device` (0-based) in the given precision mode.
+  subroutine GPU_INIT(ctx, device, precision)
+    type(c_ptr), intent(out) :: ctx
+    integer, intent(in) :: device, precision
+    call check(hamsx_create(ctx, int(device, c_int)), "hamsx_create")
+    call check(hamsx_set_precision(ctx, int(precision, c_int)), "hamsx_set_precision")
+  end subroutine
+
+  subroutine GPU_FINALIZE(ctx)
+    type(c_ptr), intent(inout) :: ctx
+    call check(hamsx_destroy(ctx), "hamsx_destroy")
+    ctx = c_null_ptr
+  end subroutine
+
+  ! NormalProcess for one body (CalPanelCentre/Area/ChartLength/Normals).
+  subroutine PANEL_GEOMETRY_GPU(NNODE, XYZ, NELEM, NCN, NCON, XR, XYZ_P, DS, PNSZ, DXYZ_P)
+    integer, intent(in) :: NNODE, NELEM, NCN(NELEM), NCON(NELEM, 4)
+    real(c_double), intent(in) :: XYZ(NNODE, 3), XR(3)
+    real(c_double), intent(out) :: XYZ_P(NELEM, 3), DS(NELEM), PNSZ(NELEM), DXYZ_P(NELEM, 6)
+    integer(c_int) :: ncn_c(NELEM), ncon_c(NELEM, 4)
+    ncn_c = int(NCN, c_int)
+    ncon_c = int(NCON, c_int)
+    call check(hamsx_panel_geometry(int(NNODE, c_int), XYZ, int(NELEM, c_int), ncn_c, ncon_c, XR, XYZ_P, DS, &
+                                    PNSZ, DXYZ_P), "hamsx_panel_geometry")
+  end subroutine
+
+  ! The combined multi-body arrays of BuildCombinedArrays; NELEM_B(NB) are the
+  ! panel counts of the bodies in the order they were concatenated.
+  subroutine SET_MESH_MULTI_GPU(ctx, TNELEM, TNNODE, XYZ_COMB, NCN_COMB, NCON_COMB, XYZ_P_COMB, DS_COMB, &
+                                  PNSZ_COMB, DXYZ_COMB, NB, NELEM_B)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: TNELEM, TNNODE, NB, NELEM_B(NB), NCN_COMB(TNELEM), NCON_COMB(TNELEM, 4)
+    real(c_double), intent(in) :: XYZ_COMB(TNNODE, 3), XYZ_P_COMB(TNELEM, 3), DS_COMB(TNELEM), PNSZ_COMB(TNELEM), &
+                                  DXYZ_COMB(TNELEM, 6)
+    integer(c_int) :: starts(NB + 1), ncn_c(TNELEM), ncon_c(TNELEM, 4)
+    integer :: b
+    starts(1) = 0
+    do b = 1, NB
+      starts(b + 1) = starts(b) + int(NELEM_B(b), c_int)
+    end do
+    ncn_c = int(NCN_COMB, c_int)
+    ncon_c = int(NCON_COMB, c_int)
+    call check(hamsx_set_mesh(ctx, int(TNELEM, c_int), int(TNNODE, c_int), XYZ_COMB, ncn_c, ncon_c, XYZ_P_COMB, &
+                              DS_COMB, PNSZ_COMB, DXYZ_COMB, int(NB, c_int), starts), "hamsx_set_mesh")
+  end subroutine
+
+  ! Headings (radians), AMP, XW(2), RHO and ISOL as read from ControlFile.in.
+  subroutine SET_WAVES_MULTI_GPU(ctx, NBETA, WVHD, AMP, XW, RHO, ISOL)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: NBETA, ISOL
+    real(c_double), intent(in) :: WVHD(NBETA), AMP, XW(2), RHO
+    call check(hamsx_set_waves(ctx, int(NBETA, c_int), WVHD, AMP, XW(1), XW(2), RHO, int(ISOL, c_int)), &
+               "hamsx_set_waves")
+  end subroutine
+
+  ! INFT/OUFT as in ControlFile.in; returns V, WK, W1 and WVN(NK) and makes
+  ! them the context's current frequency.
+  subroutine CALWAVEPROPERTS_GPU(ctx, INFT, WFREQ, H, V, WK, W1, WVN)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: INFT
+    real(c_double), intent(in) :: WFREQ, H
+    real(c_double), intent(out) :: V, WK, W1, WVN(HAMSX_NK)
+    call check(hamsx_wave_properties(int(INFT, c_int), WFREQ, H, V, WK, W1, WVN), "hamsx_wave_properties")
+    call check(hamsx_set_frequency(ctx, V, WK, W1, H, WVN, HAMSX_NK), "hamsx_set_frequency")
+  end subroutine
+
+  ! Influence matrices on the device and their LU factors; INFO as ZGETRF.
+  subroutine CALGREEN_ASSB_LEFT_GPU(ctx, INFO)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(out) :: INFO
+    integer(c_int) :: rc
+    call check(hamsx_assemble(ctx), "hamsx_assemble")
+    rc = hamsx_factor(ctx)
+    call check(rc, "hamsx_factor")
+    INFO = int(rc)
+  end subroutine
+
+  ! MXPOT(TNELEM, 6*NB) receives the radiation potentials.
+  subroutine RADIATION_SOLVER_GPU(ctx, MXPOT)
+    type(c_ptr), intent(in) :: ctx
+    complex(c_double_complex), intent(inout), target, contiguous :: MXPOT(:, :)
+    call check(hamsx_solve_radiation(ctx, c_loc(MXPOT)), "hamsx_solve_radiation")
+  end subroutine
+
+  ! MXPOT(TNELEM, NBETA) receives the diffraction potentials.
+  subroutine DIFFRACTION_SOLVER_GPU(ctx, MXPOT)
+    type(c_ptr), intent(in) :: ctx
+    complex(c_double_complex), intent(inout), target, contiguous :: MXPOT(:, :)
+    call check(hamsx_solve_diffraction(ctx, c_loc(MXPOT)), "hamsx_solve_diffraction")
+  end subroutine
+
+  ! AMAS/BDMP(6NB, 6NB) as AMAS_MULTI_COMB(KK,:,:); EXFC(6NB, NBETA).
+  subroutine FORCES_GPU(ctx, AMAS, BDMP, EXFC)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out), target, contiguous :: AMAS(:, :), BDMP(:, :)
+    complex(c_double_complex), intent(out), target, contiguous :: EXFC(:, :)
+    call check(hamsx_forces(ctx, c_loc(AMAS), c_loc(BDMP), c_loc(EXFC)), "hamsx_forces")
+  end subroutine
+
+end module hamsx_mod

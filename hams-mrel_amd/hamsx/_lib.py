@@ -51,7 +51,7 @@ SIGNATURES = {
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),
 }
-STAGES = ("assemble", "rhs", "factor", "solve", "forces")
+STAGES = ("assemble", "rhs", "factor", "solve", "forces", "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
 
 _lib = None
 

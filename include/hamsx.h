@@ -147,9 +147,11 @@ int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);
  * with profiling on, every stage is bracketed by HIP events on the context's
  * stream.  hamsx_stage_times returns the device milliseconds accumulated per
  * stage since the last reset -- ms[0] assembly, ms[1] right-hand sides,
- * ms[2] factorisation, ms[3] solves (with refinement), ms[4] forces -- and
- * the number of launches of each in count[5]. */
-#define HAMSX_NSTAGE 5
+ * ms[2] factorisation, ms[3] solves (with refinement), ms[4] forces; within
+ * the finite-depth assembly, ms[5..8] the region kernels LINTON, PIDCOCK,
+ * EIGENE and EIGEN (R/h < 5e-4, < 0.05, < 0.5, >= 0.5) -- and the number of
+ * launches of each in count[HAMSX_NSTAGE]. */
+#define HAMSX_NSTAGE 9
 int hamsx_set_profiling(hamsx_ctx *ctx, int on);
 int hamsx_stage_times(hamsx_ctx *ctx, int reset, double *ms, int *count);
 

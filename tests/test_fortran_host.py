@@ -1,0 +1,76 @@
+"""The Fortran (iso_c_binding) host: hams-mrel_amd/fortran/hamsx_pnl_solve
+reads a .pnl mesh and runs the reference driver's per-frequency sequence
+through hamsx_mod's wrappers."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from hamsx.deck import read_pnl, write_pnl
+from hamsx.mesh import combine, cylinder
+
+DRIVER = os.path.join(ROOT, "hams-mrel_amd", "fortran", "hamsx_pnl_solve")
+
+
+def write_case(tmp_path, depth, freqs, heads_deg, prec=0):
+    pnl = tmp_path / "cyl.pnl"
+    write_pnl(str(pnl), cylinder(4.0, 6.0, 20, 5, 3))
+    inp = tmp_path / "driver.in"
+    inp.write_text(f"'{pnl}'\n0.0 0.0 0.0\n{depth}\n{len(freqs)}\n{' '.join(map(str, freqs))}\n"
+                   f"{len(heads_deg)}\n{' '.join(map(str, heads_deg))}\n{prec}\n")
+    return pnl, inp
+
+
+def parse(out, nbeta):
+    toks = out.split()
+    res, k = [], 0
+    while k < len(toks):
+        assert toks[k] == "W"
+        w = float(toks[k + 1])
+        k += 2
+        assert toks[k] == "AMAS"
+        am = np.array(toks[k + 1:k + 37], float).reshape(6, 6, order="F")
+        k += 37
+        assert toks[k] == "BDMP"
+        bd = np.array(toks[k + 1:k + 37], float).reshape(6, 6, order="F")
+        k += 37
+        assert toks[k] == "EXFC"
+        v = np.array(toks[k + 1:k + 1 + 12 * nbeta], float)
+        ex = (v[0::2] + 1j * v[1::2]).reshape(6, nbeta, order="F")
+        k += 1 + 12 * nbeta
+        res.append((w, am, bd, ex))
+    return res
+
+
+def test_driver_built_and_reports_library_errors(tmp_path):
+    """Without a GPU the driver stops with the library's own message."""
+    assert os.access(DRIVER, os.X_OK), "build the Fortran host: make -C hams-mrel_amd"
+    _, inp = write_case(tmp_path, 30.0, [1.0], [0.0])
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    p = subprocess.run([DRIVER, str(inp)], capture_output=True, text=True, timeout=120, env=env)
+    if p.returncode == 0:
+        pytest.skip("a GPU is visible despite HIP_VISIBLE_DEVICES=-1")
+    assert "hamsx_create failed" in p.stdout + p.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_host_equals_python_host(tmp_path):
+    from hamsx.solver import FrequencySolver
+    freqs, heads = [0.7, 1.6], [0.0, 60.0]
+    for depth in (-1.0, 30.0):
+        pnl, inp = write_case(tmp_path, depth, freqs, heads)
+        print(f"[gpu] fortran host depth={depth}", flush=True)
+        p = subprocess.run([DRIVER, str(inp)], capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stdout + p.stderr
+        got = parse(p.stdout, len(heads))
+        mesh = combine([read_pnl(str(pnl))], [np.zeros(3)])
+        with FrequencySolver(mesh) as s:
+            s.set_waves(np.deg2rad(heads))
+            for (w, am, bd, ex), wq in zip(got, freqs):
+                a2, b2, e2, info = s.solve_frequency(wq, depth)
+                assert info == 0 and w == wq
+                np.testing.assert_array_equal(am, a2)
+                np.testing.assert_array_equal(bd, b2)
+                np.testing.assert_array_equal(ex, e2)
