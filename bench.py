@@ -117,15 +117,24 @@ def parse():
     ap.add_argument("--streams", type=int, default=4, help="frequency solves kept in flight (one context and HIP "
                     "stream each)")
     ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
-    ap.add_argument("--config", default="two-cylinders", choices=["two-cylinders", "five-cylinders", "one-cylinder"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--config", default="two-cylinders",
+                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving"])
+    ap.add_argument("--requests", type=int, default=40, help="serving: requests in the Poisson stream")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrival rate, requests/s")
+    ap.add_argument("--seed", type=int, default=7, help="serving: seed of the request stream")
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default fp64 (serving: mixed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
-    return ap.parse_args()
+    args = ap.parse_args()
+    args.precision_set = args.precision is not None
+    if args.precision is None:
+        args.precision = "fp64"
+    return args
 
 
 def copy_probe(mib):
@@ -182,6 +191,50 @@ def cpu_baseline(case, mesh, rows, threads):
             "seconds_per_solve": per}
 
 
+def serving(args):
+    """Poisson stream of seeded uneven requests (1-9 cylinders/boxes, 500-12,000
+    panels, w ~ U[0.1, 3], heading ~ U[0, 180), depth infinite or U[30, 200])
+    through the continuous-batching server; value = completed requests per
+    second of makespan, latency = completion minus arrival."""
+    from hamsx import configs
+    from hamsx.mesh import combine
+    from hamsx.serving import Request, Server, poisson_arrivals
+    rng = np.random.default_rng(args.seed)
+    arr = poisson_arrivals(args.requests, args.rate, rng)
+    reqs, sizes = [], []
+    for k in range(args.requests):
+        case = configs.random_request(rng)
+        mesh = combine(case.bodies, case.xr)
+        sizes.append(mesh.nelem)
+        reqs.append(Request(k, mesh, float(case.omegas[0]), case.depth, np.deg2rad(case.headings_deg),
+                            arrival=float(arr[k])))
+    precision = args.precision if args.precision_set else "mixed"
+    print(f"[bench] serving {args.requests} requests, {min(sizes)}-{max(sizes)} panels, rate {args.rate}/s, "
+          f"{args.streams} slots, {precision}", file=sys.stderr, flush=True)
+    with Server(slots=args.streams, precision=precision) as srv:
+        warm = Request(-1, reqs[0].mesh, 1.0, reqs[0].depth, reqs[0].headings_rad)
+        srv.run([warm])
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+    bad = [r.rid for r in res if r.info != 0]
+    if bad:
+        raise SystemExit(f"zero pivots in requests {bad}")
+    lat = np.array([r.latency for r in res]) * 1e3
+    span = max(r.done for r in res) - min(r.arrival for r in res)
+    out = {
+        "metric": METRIC, "value": len(res) / span, "unit": "freq-solves/s", "n_gpus": 1, "steps": len(res),
+        "warmup": 1, "ms_per_step": span / len(res) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64-lu+f64-refine" if precision == "mixed" else "f64",
+        "data": f"synthetic seeded requests (seed {args.seed})",
+        "config": {"workload": f"serving-poisson-{args.rate:g}hz-seed{args.seed}", "requests": len(res),
+                   "panels_min": min(sizes), "panels_max": max(sizes), "panels_mean": float(np.mean(sizes)),
+                   "slots": args.streams, "precision": precision, "wall_s": wall},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -191,6 +244,9 @@ def main():
         print(f"bench.py runs one GPU per process (asked for --gpus {args.gpus}); this build has no multi-GPU "
               "path -- run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == "serving":
+        serving(args)
+        return
     from hamsx.mesh import combine
     from hamsx.solver import FrequencySolver
 

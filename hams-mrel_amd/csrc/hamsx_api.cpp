@@ -221,7 +221,7 @@ struct hamsx_ctx {
   int isol = 1;
   bool have_waves = false;
   // matrices and workspace
-  DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial;
+  DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag;
   DevBuf<float2> Af;
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins, valid for bin_n panels at depth bin_h
@@ -404,18 +404,20 @@ int do_factor(hamsx_ctx* c) {
   int rc = ensure_work(c);
   if (rc) return rc;
   const int n = c->n;
-  const int nsets0 = (n + hamsx::kTS - 1) / hamsx::kTS;
+  const int nsets0 = (n + hamsx::kSelRows - 1) / hamsx::kSelRows;
   if ((rc = c->ipiv.ensure(n)) || (rc = c->perm.ensure(n)) ||
-      (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)))
+      (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)) ||
+      (rc = c->ludiag.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB)))
     return rc;
   StageTimer tm(c, 2);
   if (c->prec == HAMSX_PREC_FP64) {
-    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1);
+    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
+                             c->ludiag.p);
     c->have_A = false;  // factored in place
   } else {
     hamsx::launch::to_c64(c->stream, c->A.p, c->Af.p, (size_t)n * n);
     hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                                 c->flags.p + 1);
+                                 c->flags.p + 1, c->ludiag.p);
   }
   c->have_LU = true;
   return check_launch();
@@ -565,7 +567,8 @@ int hamsx_destroy(hamsx_ctx* c) {
     b->release();
   c->plist.release();
   c->modetab.release();
-  for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial}) b->release();
+  for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial, &c->ludiag})
+    b->release();
   c->Af.release();
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
   for (auto e : c->pool) (void)hipEventDestroy(e);

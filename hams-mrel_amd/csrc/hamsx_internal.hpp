@@ -18,7 +18,7 @@ void report_launch(const char* name, hipError_t e);
 
 constexpr int kNK = 200;      // dispersion roots kept (WaveDyn_mod NK)
 constexpr int kPanelNB = 32;  // LU panel width
-constexpr int kTS = 128;      // rows per tournament block
+constexpr int kSelRows = 256; // candidate rows per tournament block (one per thread)
 
 // Device-resident, frequency-invariant mesh (structure of arrays).
 struct MeshDev {
@@ -86,8 +86,10 @@ void clear_flags(hipStream_t s, int* flags);
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
          const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
          const int* chunk_body_d, const int* chunk_j0_d, const int* chunk_j1_d, double2* X);
-// LU with tournament pivoting; ipiv 1-based, perm = row permutation, info[0] = first zero pivot
-void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info);
+// LU with tournament pivoting; ipiv 1-based, perm = row permutation, info[0] = first zero pivot;
+// diag: kPanelNB x kPanelNB complex128 scratch for the factored diagonal block
+void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
+               double2* diag);
 // X <- A^{-1} X using the factors, nrhs columns (leading dimension n)
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
 // AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
@@ -95,7 +97,8 @@ void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w,
             const double2* X, double* out);
 // single-precision copy of A for the mixed path, and the helpers of refinement
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
-void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info);
+void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
+                   double2* diag);
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
 void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
               double* norms);

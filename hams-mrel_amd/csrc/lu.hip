@@ -59,127 +59,141 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 }
 
 // ---------------------------------------------------------------------------
-// Tournament pivot selection.  Level 0 (idx_in == nullptr): block b takes
-// rows [row0 + b*kTS, min(row0 + (b+1)*kTS, n)).  Level > 0: block b merges
-// candidate sets [b*4, b*4 + 4) of the previous level.  Each block factors
-// its candidates' panel entries with partial pivoting (pivot = first maximum
-// of |re| + |im|, as IZAMAX) and writes its ncols pivot rows in pivot order.
+// Tournament pivot selection.  One thread per candidate row, kSelRows per
+// block.  Level 0 (idx_in == nullptr): block b takes rows
+// [row0 + b*kSelRows, ...).  Level > 0: block b merges candidate sets
+// [b*kMerge, b*kMerge + kMerge) of the previous level.  Each block runs
+// partial pivoting on its candidates' panel entries (pivot = largest
+// |re| + |im| as IZAMAX, ties to the first candidate), each candidate's NB
+// entries in registers: the argmax is a wave shuffle reduction plus one LDS
+// exchange, the pivot row is broadcast through LDS, and a retired pivot row
+// keeps its L entries (left of its step) and U entries.  Outputs the chosen
+// rows in pivot order; the final level (diag != nullptr) also writes the
+// factored NB x NB diagonal block L\U in pivot order and ZGETRF's INFO.
 // ---------------------------------------------------------------------------
+constexpr int kMerge = kSelRows / kPanelNB;
 template <typename R>
 __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
                                                 int ncols, int row0, const int* __restrict__ idx_in,
                                                 const int* __restrict__ cnt_in, int nsets_in, int* __restrict__ idx_out,
-                                                int* __restrict__ cnt_out) {
-  __shared__ R vr[kPanelNB][kTS + 1];
-  __shared__ R vi[kPanelNB][kTS + 1];
-  __shared__ int idx[kTS];
-  __shared__ R redv[4];
-  __shared__ int redi[4];
+                                                int* __restrict__ cnt_out, typename CT<R>::c2* __restrict__ diag,
+                                                int* __restrict__ info) {
+  __shared__ int sidx[kSelRows];
+  __shared__ int off[kMerge + 1];
+  __shared__ double redv[2][4];
+  __shared__ int redi[2][4];
+  __shared__ double prr[2][kPanelNB], pri[2][kPanelNB], rcr[2], rci[2];
+  __shared__ int sel[kPanelNB];
   const int t = threadIdx.x, b = blockIdx.x;
-  int mc = 0;
+  int mc;
   if (idx_in == nullptr) {
-    int r0 = row0 + b * kTS;
-    mc = min(kTS, n - r0);
-    if (t < mc) idx[t] = r0 + t;
+    const int r0 = row0 + b * kSelRows;
+    mc = min(kSelRows, n - r0);
+    if (t < mc) sidx[t] = r0 + t;
   } else {
-    // gather up to 4 sets of <= kPanelNB candidates each
-    __shared__ int off[5];
     if (t == 0) {
       int o = 0;
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < kMerge; q++) {
         off[q] = o;
-        int sidx = b * 4 + q;
-        o += (sidx < nsets_in) ? cnt_in[sidx] : 0;
+        const int si = b * kMerge + q;
+        o += (si < nsets_in) ? cnt_in[si] : 0;
       }
-      off[4] = o;
+      off[kMerge] = o;
     }
     __syncthreads();
-    mc = off[4];
-    for (int q = 0; q < 4; q++) {
-      int sidx = b * 4 + q, c = off[q + 1] - off[q];
-      if (t < c) idx[off[q] + t] = idx_in[sidx * kPanelNB + t];
+    mc = off[kMerge];
+    for (int q = 0; q < kMerge; q++) {
+      const int si = b * kMerge + q, c = off[q + 1] - off[q];
+      if (t < c) sidx[off[q] + t] = idx_in[si * kPanelNB + t];
     }
   }
   __syncthreads();
-  // load candidate rows of the panel
-  for (int e = t; e < mc * ncols; e += 256) {
-    int r = e % mc, c = e / mc;
-    auto v = A[(size_t)idx[r] + (size_t)lda * (col0 + c)];
-    vr[c][r] = v.x;
-    vi[c][r] = v.y;
+  bool active = t < mc;
+  double vr[kPanelNB], vi[kPanelNB];  // computed in FP64 for either factor type
+  {
+    const size_t g = active ? (size_t)sidx[t] : 0;
+#pragma unroll
+    for (int q = 0; q < kPanelNB; q++) {
+      vr[q] = 0.0;
+      vi[q] = 0.0;
+      if (active && q < ncols) {
+        auto v = A[g + (size_t)lda * (col0 + q)];
+        vr[q] = (double)v.x;
+        vi[q] = (double)v.y;
+      }
+    }
   }
-  __syncthreads();
   const int nsel = min(ncols, mc);
-  for (int c = 0; c < nsel; c++) {
-    // argmax over rows c..mc-1 of column c; ties -> lowest row
-    R best = R(-1);
-    int bi = 0x7fffffff;
-    for (int r = c + t; r < mc; r += 256) {
-      R m = fabs(vr[c][r]) + fabs(vi[c][r]);
-      if (m > best || (m == best && r < bi)) {
-        best = m;
-        bi = r;
+  int mystep = -1, zero_at = 0;
+#pragma unroll
+  for (int c = 0; c < kPanelNB; c++) {
+    if (c < nsel) {
+      const int buf = c & 1;
+      double m = active ? fabs(vr[c]) + fabs(vi[c]) : -1.0;
+      int bi = active ? t : 0x7fffffff;
+      for (int o = 32; o > 0; o >>= 1) {
+        const double om = __shfl_xor(m, o);
+        const int oi = __shfl_xor(bi, o);
+        if (om > m || (om == m && oi < bi)) {
+          m = om;
+          bi = oi;
+        }
       }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      R ob = __shfl_xor(best, o);
-      int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
+      if ((t & 63) == 0) {
+        redv[buf][t >> 6] = m;
+        redi[buf][t >> 6] = bi;
       }
-    }
-    if ((t & 63) == 0) {
-      redv[t >> 6] = best;
-      redi[t >> 6] = bi;
-    }
-    __syncthreads();
-    best = redv[0];
-    bi = redi[0];
-    for (int q = 1; q < 4; q++)
-      if (redv[q] > best || (redv[q] == best && redi[q] < bi)) {
-        best = redv[q];
-        bi = redi[q];
+      __syncthreads();
+      double best = redv[buf][0];
+      int p = redi[buf][0];
+      for (int w = 1; w < 4; w++)
+        if (redv[buf][w] > best || (redv[buf][w] == best && redi[buf][w] < p)) {
+          best = redv[buf][w];
+          p = redi[buf][w];
+        }
+      const bool nonzero = best > 0.0;
+      if (t == p) {
+#pragma unroll
+        for (int q = 0; q < kPanelNB; q++)
+          if (q >= c) {
+            prr[buf][q] = vr[q];
+            pri[buf][q] = vi[q];
+          }
+        double a = 0.0, bb = 0.0;
+        if (nonzero) crecip(vr[c], vi[c], a, bb);
+        rcr[buf] = a;
+        rci[buf] = bb;
+        sel[c] = t;
+        mystep = c;
+        active = false;
       }
-    const int p = bi;
-    const bool nonzero = best > R(0);
-    __syncthreads();
-    if (nonzero && p != c) {
-      if (t < ncols) {
-        R a = vr[t][c], bb = vi[t][c];
-        vr[t][c] = vr[t][p];
-        vi[t][c] = vi[t][p];
-        vr[t][p] = a;
-        vi[t][p] = bb;
+      __syncthreads();
+      if (active && nonzero) {
+        const double ir = rcr[buf], ii = rci[buf];
+        const double lr = vr[c] * ir - vi[c] * ii, li = vr[c] * ii + vi[c] * ir;
+        vr[c] = lr;
+        vi[c] = li;
+#pragma unroll
+        for (int q = 0; q < kPanelNB; q++)
+          if (q > c && q < ncols) {
+            const double ur = prr[buf][q], ui = pri[buf][q];
+            vr[q] -= lr * ur - li * ui;
+            vi[q] -= lr * ui + li * ur;
+          }
       }
-      if (t == 0) {
-        int q = idx[c];
-        idx[c] = idx[p];
-        idx[p] = q;
-      }
+      if (!nonzero && zero_at == 0) zero_at = c + 1;
     }
-    __syncthreads();
-    if (nonzero) {
-      R ir, ii;
-      crecip(vr[c][c], vi[c][c], ir, ii);
-      for (int r = c + 1 + t; r < mc; r += 256) {
-        R a = vr[c][r], bb = vi[c][r];
-        vr[c][r] = a * ir - bb * ii;
-        vi[c][r] = a * ii + bb * ir;
-      }
-    }
-    __syncthreads();
-    const int nr = mc - c - 1, nq = ncols - c - 1;
-    for (int e = t; e < nr * nq; e += 256) {
-      int r = c + 1 + e % nr, q = c + 1 + e / nr;
-      R lr = vr[c][r], li = vi[c][r], ur = vr[q][c], ui = vi[q][c];
-      vr[q][r] -= lr * ur - li * ui;
-      vi[q][r] -= lr * ui + li * ur;
-    }
-    __syncthreads();
   }
-  if (t < nsel) idx_out[b * kPanelNB + t] = idx[t];
+  __syncthreads();
+  if (t < nsel) idx_out[b * kPanelNB + t] = sidx[sel[t]];
   if (t == 0) cnt_out[b] = nsel;
+  if (diag != nullptr) {
+    if (mystep >= 0)
+#pragma unroll
+      for (int q = 0; q < kPanelNB; q++)
+        if (q < ncols) diag[mystep + kPanelNB * q] = {(R)vr[q], (R)vi[q]};
+    if (t == 0 && zero_at && info[0] == 0) info[0] = col0 + zero_at;
+  }
 }
 
 // Turn the final pivot-row list into LAPACK-style sequential swaps
@@ -247,70 +261,23 @@ __global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, 
   }
 }
 
-// Diagonal block LU without pivoting (its rows are the tournament winners in
-// pivot order), one block, written back in place; records the first exactly
-// zero pivot in info (ZGETRF's INFO).
-template <typename R>
-__global__ __launch_bounds__(256) void k_diag_factor(typename CT<R>::c2* __restrict__ A, int lda, int col0, int nb,
-                                                     int* __restrict__ info) {
-  __shared__ R dr[kPanelNB][kPanelNB + 1];
-  __shared__ R di[kPanelNB][kPanelNB + 1];
-  const int t = threadIdx.x;
-  for (int e = t; e < nb * nb; e += 256) {
-    int r = e % nb, c = e / nb;
-    auto v = A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)];
-    dr[r][c] = v.x;
-    di[r][c] = v.y;
-  }
-  __syncthreads();
-  for (int c = 0; c < nb; c++) {
-    R ir = R(0), ii = R(0);
-    const bool nz = (dr[c][c] != R(0) || di[c][c] != R(0));
-    if (nz) crecip(dr[c][c], di[c][c], ir, ii);
-    __syncthreads();
-    if (nz)
-      for (int r = c + 1 + t; r < nb; r += 256) {
-        R a = dr[r][c], b = di[r][c];
-        dr[r][c] = a * ir - b * ii;
-        di[r][c] = a * ii + b * ir;
-      }
-    __syncthreads();
-    const int m = nb - c - 1;
-    for (int e = t; e < m * m; e += 256) {
-      int r = c + 1 + e % m, q = c + 1 + e / m;
-      R lr = dr[r][c], li = di[r][c], ur = dr[c][q], ui = di[c][q];
-      dr[r][q] -= lr * ur - li * ui;
-      di[r][q] -= lr * ui + li * ur;
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < nb * nb; e += 256) {
-    int r = e % nb, c = e / nb;
-    A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = {dr[r][c], di[r][c]};
-  }
-  if (t == 0 && info[0] == 0)
-    for (int c = 0; c < nb; c++)
-      if (dr[c][c] == R(0) && di[c][c] == R(0)) {
-        info[0] = col0 + c + 1;
-        break;
-      }
-}
-
-// With the diagonal block factored: L21 = A21 U11^-1 (blocks [0, nrb), one
-// row per thread) and U12 = L11^-1 A12 (blocks [nrb, ..), one column per
-// thread).  Every block only reads the diagonal block.
+// With the diagonal block factored (diag, from the final tournament level):
+// L21 = A21 U11^-1 (blocks [0, nrb), one row per thread) and U12 = L11^-1 A12
+// (blocks [nrb, ..), one column per thread); block 0 also stores the factored
+// diagonal block into A.  No block reads what another writes.
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
-                                                     int nb, int nrb) {
+                                                     int nb, int nrb, const typename CT<R>::c2* __restrict__ diag) {
   __shared__ R dr[kPanelNB][kPanelNB + 1];
   __shared__ R di[kPanelNB][kPanelNB + 1];
   __shared__ R rr[kPanelNB], ri[kPanelNB];  // reciprocals of the pivots
   const int t = threadIdx.x;
   for (int e = t; e < nb * nb; e += 256) {
     int r = e % nb, c = e / nb;
-    auto v = A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)];
+    auto v = diag[r + kPanelNB * c];
     dr[r][c] = v.x;
     di[r][c] = v.y;
+    if (blockIdx.x == 0) A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = v;
   }
   __syncthreads();
   if (t < nb) {
@@ -478,22 +445,23 @@ __global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info)
 
 template <typename R>
 static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
-                        int* info) {
+                        int* info, typename CT<R>::c2* diag) {
   HX_LAUNCH(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
   // workspace: two ping-pong candidate buffers of nsets0 sets
-  const int nsets0 = (n + kTS - 1) / kTS;
+  const int nsets0 = (n + kSelRows - 1) / kSelRows;
   int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
   int* cnt[2] = {wcnt, wcnt + nsets0};
+  using c2 = typename CT<R>::c2;
   for (int col0 = 0; col0 < n; col0 += kPanelNB) {
     const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
     const int m = n - col0;
-    int nsets = (m + kTS - 1) / kTS, cur = 0;
+    int nsets = (m + kSelRows - 1) / kSelRows, cur = 0;
     HX_LAUNCH(k_select<R>, dim3(nsets), dim3(256), 0, s, A, n, n, col0, nb, col0, (const int*)nullptr,
-                       (const int*)nullptr, 0, idx[cur], cnt[cur]);
+              (const int*)nullptr, 0, idx[cur], cnt[cur], nsets == 1 ? diag : (c2*)nullptr, info);
     while (nsets > 1) {
-      const int nout = (nsets + 3) / 4;
+      const int nout = (nsets + kMerge - 1) / kMerge;
       HX_LAUNCH(k_select<R>, dim3(nout), dim3(256), 0, s, A, n, n, col0, nb, col0, idx[cur], cnt[cur], nsets,
-                         idx[1 - cur], cnt[1 - cur]);
+                idx[1 - cur], cnt[1 - cur], nout == 1 ? diag : (c2*)nullptr, info);
       cur = 1 - cur;
       nsets = nout;
     }
@@ -501,8 +469,8 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
     const int rest = n - col0 - nb;
     const int nrb = (rest + 255) / 256;
-    HX_LAUNCH(k_diag_factor<R>, dim3(1), dim3(256), 0, s, A, n, col0, nb, info);
-    if (nrb > 0) HX_LAUNCH(k_panel_solve<R>, dim3(nrb * 2), dim3(256), 0, s, A, n, n, col0, nb, nrb);
+    HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
+              (const c2*)diag);
     if (rest > 0) {
       const int tiles = (rest + 63) / 64;
       const int ntiles = tiles * tiles;
@@ -513,88 +481,112 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves, 32-row blocks: each launch solves one diagonal block
-// (every block of the grid recomputes it redundantly from LDS) and updates
-// its own chunk of the remaining rows.  X is complex128; the factors are
-// complex128 or complex64 (mixed path: FP32 factors, FP64 right-hand sides).
+// Triangular solves, 32-row diagonal blocks.  Each launch solves one diagonal
+// block for a group of kTrsmRhs right-hand sides and updates the remaining
+// rows of that group:
+//   * diagonal solve, redundantly in every block of the group: a half-wave
+//     holds one right-hand side, lane r one row; at step c the row-c value is
+//     broadcast by a shuffle and rows below (lower, unit diagonal) or above
+//     (upper, divided by the diagonal first) subtract T(r, c) times it;
+//   * update: one thread per remaining row accumulates T(i, 0:kb) y(0:kb, q)
+//     for the group's right-hand sides.
+// X holds the right-hand sides (complex128); the factors are complex128 or
+// complex64 (mixed path: FP32 factors, FP64 right-hand sides).  Solved
+// diagonal rows go to Y, so no block reads a row another block writes.
 // ---------------------------------------------------------------------------
-constexpr int kRhsMax = 64;
+constexpr int kTrsmRhs = 8;
 template <typename R, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
                                                     double2* __restrict__ X, double2* __restrict__ Y, int nrhs) {
-  __shared__ double tr[kPanelNB][kPanelNB + 1], ti[kPanelNB][kPanelNB + 1];
-  __shared__ double yr[kPanelNB][kRhsMax], yi[kPanelNB][kRhsMax];
-  const int t = threadIdx.x;
+  __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
+  __shared__ double dr[kPanelNB], di[kPanelNB];                      // 1 / T(c, c) (upper)
+  __shared__ double yr[kPanelNB][kTrsmRhs], yi[kPanelNB][kTrsmRhs];  // solved block rows
+  const int t = threadIdx.x, q0 = blockIdx.y * kTrsmRhs;
+  const int nq = min(kTrsmRhs, nrhs - q0);
   for (int e = t; e < kb * kb; e += 256) {
-    int r = e % kb, c = e / kb;
+    const int r = e % kb, c = e / kb;
     auto v = LU[(size_t)(k0 + r) + (size_t)n * (k0 + c)];
-    tr[r][c] = (double)v.x;
-    ti[r][c] = (double)v.y;
-  }
-  for (int e = t; e < kb * nrhs; e += 256) {
-    int r = e % kb, q = e / kb;
-    double2 v = X[(size_t)(k0 + r) + (size_t)n * q];
-    yr[r][q] = v.x;
-    yi[r][q] = v.y;
+    tr[c][r] = (double)v.x;
+    ti[c][r] = (double)v.y;
   }
   __syncthreads();
-  // solve the diagonal block, one right-hand side per thread
-  for (int q = t; q < nrhs; q += 256) {
+  if (!LOWER && t < kb) {  // reciprocal of the diagonal (Smith)
+    double a = tr[t][t], b = ti[t][t];
+    if (fabs(a) >= fabs(b)) {
+      double qq = b / a, dd = a + b * qq;
+      dr[t] = 1.0 / dd;
+      di[t] = -qq / dd;
+    } else {
+      double qq = a / b, dd = a * qq + b;
+      dr[t] = qq / dd;
+      di[t] = -1.0 / dd;
+    }
+  }
+  __syncthreads();
+  {
+    const int lane = t & 63, half = lane >> 5, r = lane & 31;
+    const int ql = (t >> 6) * 2 + half;  // right-hand side of this half-wave within the group
+    const bool live = ql < nq && r < kb;
+    double vr = 0.0, vi = 0.0;
+    if (live) {
+      double2 v = X[(size_t)(k0 + r) + (size_t)n * (q0 + ql)];
+      vr = v.x;
+      vi = v.y;
+    }
     if (LOWER) {
-      for (int c = 0; c < kb; c++)
-        for (int r = c + 1; r < kb; r++) {
-          yr[r][q] -= tr[r][c] * yr[c][q] - ti[r][c] * yi[c][q];
-          yi[r][q] -= tr[r][c] * yi[c][q] + ti[r][c] * yr[c][q];
+      for (int c = 0; c < kb; c++) {
+        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
+        if (r > c) {
+          const double a = tr[c][r], b = ti[c][r];
+          vr -= a * cr - b * ci;
+          vi -= a * ci + b * cr;
         }
+      }
     } else {
       for (int c = kb - 1; c >= 0; c--) {
-        double a = tr[c][c], b = ti[c][c], xr, xi;
-        if (fabs(a) >= fabs(b)) {
-          double qq = b / a, dd = a + b * qq;
-          xr = (yr[c][q] + yi[c][q] * qq) / dd;
-          xi = (yi[c][q] - yr[c][q] * qq) / dd;
-        } else {
-          double qq = a / b, dd = a * qq + b;
-          xr = (yr[c][q] * qq + yi[c][q]) / dd;
-          xi = (yi[c][q] * qq - yr[c][q]) / dd;
+        if (r == c) {
+          const double a = vr, b = vi;
+          vr = a * dr[c] - b * di[c];
+          vi = a * di[c] + b * dr[c];
         }
-        yr[c][q] = xr;
-        yi[c][q] = xi;
-        for (int r = 0; r < c; r++) {
-          yr[r][q] -= tr[r][c] * xr - ti[r][c] * xi;
-          yi[r][q] -= tr[r][c] * xi + ti[r][c] * xr;
+        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
+        if (r < c) {
+          const double a = tr[c][r], b = ti[c][r];
+          vr -= a * cr - b * ci;
+          vi -= a * ci + b * cr;
         }
       }
     }
+    if (ql < kTrsmRhs && r < kPanelNB) {
+      yr[r][ql] = live ? vr : 0.0;
+      yi[r][ql] = live ? vi : 0.0;
+    }
+    if (live && blockIdx.x == 0) Y[(size_t)(k0 + r) + (size_t)n * (q0 + ql)] = make_double2(vr, vi);
   }
   __syncthreads();
-  // diagonal rows go to Y, so no block of this launch reads a row another writes
-  if (blockIdx.x == 0)
-    for (int e = t; e < kb * nrhs; e += 256) {
-      int r = e % kb, q = e / kb;
-      Y[(size_t)(k0 + r) + (size_t)n * q] = make_double2(yr[r][q], yi[r][q]);
-    }
   // update the other rows: below the block (LOWER) or above it (upper)
   const int i = LOWER ? k0 + kb + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
   if (LOWER ? i >= n : i >= k0) return;
-  typename CT<R>::c2 l[kPanelNB];
+  double ar[kTrsmRhs], ai[kTrsmRhs];
 #pragma unroll
-  for (int c = 0; c < kPanelNB; c++)
-    if (c < kb) l[c] = LU[(size_t)i + (size_t)n * (k0 + c)];
-  for (int q = 0; q < nrhs; q++) {
-    double ar = 0.0, ai = 0.0;
+  for (int q = 0; q < kTrsmRhs; q++) ar[q] = ai[q] = 0.0;
+  for (int c = 0; c < kb; c++) {
+    auto v = LU[(size_t)i + (size_t)n * (k0 + c)];
+    const double lr = (double)v.x, li = (double)v.y;
 #pragma unroll
-    for (int c = 0; c < kPanelNB; c++)
-      if (c < kb) {
-        double lr = (double)l[c].x, li = (double)l[c].y;
-        ar += lr * yr[c][q] - li * yi[c][q];
-        ai += lr * yi[c][q] + li * yr[c][q];
-      }
-    double2 v = X[(size_t)i + (size_t)n * q];
-    v.x -= ar;
-    v.y -= ai;
-    X[(size_t)i + (size_t)n * q] = v;
+    for (int q = 0; q < kTrsmRhs; q++) {
+      ar[q] += lr * yr[c][q] - li * yi[c][q];
+      ai[q] += lr * yi[c][q] + li * yr[c][q];
+    }
   }
+#pragma unroll
+  for (int q = 0; q < kTrsmRhs; q++)
+    if (q < nq) {
+      double2 v = X[(size_t)i + (size_t)n * (q0 + q)];
+      v.x -= ar[q];
+      v.y -= ai[q];
+      X[(size_t)i + (size_t)n * (q0 + q)] = v;
+    }
 }
 
 __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
@@ -607,26 +599,23 @@ __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __res
 template <typename R>
 static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const int* perm, double2* X,
                        double2* Xtmp, int nrhs) {
-  for (int q0 = 0; q0 < nrhs; q0 += kRhsMax) {
-    const int nq = nrhs - q0 < kRhsMax ? nrhs - q0 : kRhsMax;
-    double2* Xq = X + (size_t)n * q0;
-    double2* Tq = Xtmp + (size_t)n * q0;
-    // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
-    // backward pass: off-diagonal rows updated in X, solved rows into T.
-    HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nq), dim3(256), 0, s, perm, Xq, Tq, n, nq);
-    for (int k0 = 0; k0 < n; k0 += kPanelNB) {
-      const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
-      const int rest = n - k0 - kb;
-      const int g = rest > 0 ? (rest + 255) / 256 : 1;
-      HX_LAUNCH((k_trsm_block<R, true>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Tq, Xq, nq);
-    }
-    for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
-      const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
-      const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
-      HX_LAUNCH((k_trsm_block<R, false>), dim3(g), dim3(256), 0, s, LU, n, k0, kb, Xq, Tq, nq);
-    }
-    (void)hipMemcpyAsync(Xq, Tq, sizeof(double2) * (size_t)n * nq, hipMemcpyDeviceToDevice, s);
+  if (nrhs <= 0) return;
+  const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
+  // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
+  // backward pass: off-diagonal rows updated in X, solved rows into T.
+  HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, perm, X, Xtmp, n, nrhs);
+  for (int k0 = 0; k0 < n; k0 += kPanelNB) {
+    const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+    const int rest = n - k0 - kb;
+    const int g = rest > 0 ? (rest + 255) / 256 : 1;
+    HX_LAUNCH((k_trsm_block<R, true>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, Xtmp, X, nrhs);
   }
+  for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
+    const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+    const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
+    HX_LAUNCH((k_trsm_block<R, false>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, X, Xtmp, nrhs);
+  }
+  (void)hipMemcpyAsync(X, Xtmp, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,8 +673,9 @@ __global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, s
 }
 
 namespace launch {
-void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info) {
-  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info);
+void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
+               double2* diag) {
+  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info, diag);
 }
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
   lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
@@ -693,8 +683,9 @@ void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2*
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
   HX_LAUNCH(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
 }
-void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info) {
-  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info);
+void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
+                   double2* diag) {
+  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag));
 }
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
   lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs);

@@ -33,17 +33,21 @@ class FrequencySolver:
 
     def __init__(self, mesh: Combined, device: int = 0, precision: str = "fp64"):
         self._lib = _lib.load()
-        self.mesh = mesh
         h = C.c_void_p()
         _lib.check(self._lib.hamsx_create(C.byref(h), device), "hamsx_create")
         self._h = h
         self.set_precision(precision)
+        self.nbeta = 0
+        self.set_mesh(mesh)
+
+    def set_mesh(self, mesh: Combined):
+        """Upload a (new) mesh; the context keeps it resident until replaced."""
         m = mesh
         _lib.check(self._lib.hamsx_set_mesh(
             self._h, m.nelem, m.nnode, _lib.ptr(m.xyz), _lib.ptr(m.ncn, C.c_int), _lib.ptr(m.ncon, C.c_int),
             _lib.ptr(m.cen), _lib.ptr(m.area), _lib.ptr(m.pnsz), _lib.ptr(m.dxyz), m.nbody,
             _lib.ptr(m.body_start, C.c_int)), "hamsx_set_mesh")
-        self.nbeta = 0
+        self.mesh = mesh
 
     # -- lifecycle -------------------------------------------------------------
     def close(self):
