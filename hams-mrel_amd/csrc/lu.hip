@@ -111,16 +111,17 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
   bool active = t < mc;
   double vr[kPanelNB], vi[kPanelNB];  // computed in FP64 for either factor type
   {
-    const size_t g = active ? (size_t)sidx[t] : 0;
+    // unconditional loads at clamped, in-bounds addresses (all issued before
+    // the first use), masked afterwards
+    const size_t g = (size_t)sidx[active ? t : 0];
+    typename CT<R>::c2 v[kPanelNB];
+#pragma unroll
+    for (int q = 0; q < kPanelNB; q++) v[q] = A[g + (size_t)lda * (col0 + min(q, ncols - 1))];
 #pragma unroll
     for (int q = 0; q < kPanelNB; q++) {
-      vr[q] = 0.0;
-      vi[q] = 0.0;
-      if (active && q < ncols) {
-        auto v = A[g + (size_t)lda * (col0 + q)];
-        vr[q] = (double)v.x;
-        vi[q] = (double)v.y;
-      }
+      const bool use = active && q < ncols;
+      vr[q] = use ? (double)v[q].x : 0.0;
+      vi[q] = use ? (double)v[q].y : 0.0;
     }
   }
   const int nsel = min(ncols, mc);
@@ -293,12 +294,10 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
     if (i >= n) return;
     R xr[kPanelNB], xi[kPanelNB];
 #pragma unroll
-    for (int c = 0; c < kPanelNB; c++) {
-      if (c < nb) {
-        auto v = A[(size_t)i + (size_t)lda * (col0 + c)];
-        xr[c] = v.x;
-        xi[c] = v.y;
-      }
+    for (int c = 0; c < kPanelNB; c++) {  // unconditional, clamped: all loads in flight together
+      auto v = A[(size_t)i + (size_t)lda * (col0 + min(c, nb - 1))];
+      xr[c] = v.x;
+      xi[c] = v.y;
     }
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
@@ -324,11 +323,9 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
     R yr[kPanelNB], yi[kPanelNB];
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
-      if (c < nb) {
-        auto v = A[(size_t)(col0 + c) + (size_t)lda * j];
-        yr[c] = v.x;
-        yi[c] = v.y;
-      }
+      auto v = A[(size_t)(col0 + min(c, nb - 1)) + (size_t)lda * j];
+      yr[c] = v.x;
+      yi[c] = v.y;
     }
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
@@ -373,25 +370,27 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
   const int tm = bid % tiles_m, tn = bid / tiles_m;
   const int i0 = r0 + tm * 64, j0 = r0 + tn * 64;
   const int t = threadIdx.x;
-  for (int e = t; e < 64 * kPanelNB; e += 256) {
-    int r = e % 64, k = e / 64;
-    R a = R(0), b = R(0), c = R(0), d = R(0);
-    if (k < kk) {
-      if (i0 + r < n) {
-        auto v = A[(size_t)(i0 + r) + (size_t)lda * (r0 - kk + k)];
-        a = v.x;
-        b = v.y;
-      }
-      if (j0 + r < n) {
-        auto v = A[(size_t)(r0 - kk + k) + (size_t)lda * (j0 + r)];
-        c = v.x;
-        d = v.y;
-      }
+  {
+    // unconditional loads at clamped, in-bounds addresses, all in flight
+    // together; out-of-range entries are zeroed when staged to LDS
+    constexpr int kPer = 64 * kPanelNB / 256;
+    typename CT<R>::c2 lv[kPer], uv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = t + 256 * u, r = e % 64, k = e / 64;
+      const int kc = r0 - kk + min(k, kk - 1);
+      lv[u] = A[(size_t)min(i0 + r, n - 1) + (size_t)lda * kc];
+      uv[u] = A[(size_t)kc + (size_t)lda * min(j0 + r, n - 1)];
     }
-    lr_[k][r] = a;
-    li_[k][r] = b;
-    ur_[k][r] = c;
-    ui_[k][r] = d;
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = t + 256 * u, r = e % 64, k = e / 64;
+      const bool lk = k < kk && i0 + r < n, uk = k < kk && j0 + r < n;
+      lr_[k][r] = lk ? lv[u].x : R(0);
+      li_[k][r] = lk ? lv[u].y : R(0);
+      ur_[k][r] = uk ? uv[u].x : R(0);
+      ui_[k][r] = uk ? uv[u].y : R(0);
+    }
   }
   __syncthreads();
   const int w = t >> 6, lane = t & 63;
@@ -422,17 +421,32 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
         accI[a][b] = CT<R>::mfma(Ur[b], Li[a], accI[a][b]);
       }
   }
+  // read the 16 C entries first (clamped addresses, all loads in flight),
+  // then write back the in-range ones
+  typename CT<R>::c2 cv[2][2][4];
+#pragma unroll
   for (int a = 0; a < 2; a++)
+#pragma unroll
     for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = min(i0 + wi + a * 16 + l16, n - 1);
+        const int col = min(j0 + wj + b * 16 + CT<R>::out_row(kq, q), n - 1);
+        cv[a][b][q] = A[(size_t)row + (size_t)lda * col];
+      }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
       for (int q = 0; q < 4; q++) {
         const int row = i0 + wi + a * 16 + l16;
         const int col = j0 + wj + b * 16 + CT<R>::out_row(kq, q);
         if (row < n && col < n) {
-          auto& c = A[(size_t)row + (size_t)lda * col];
-          auto v = c;
+          auto v = cv[a][b][q];
           v.x -= accR[a][b][q];
           v.y -= accI[a][b][q];
-          c = v;
+          A[(size_t)row + (size_t)lda * col] = v;
         }
       }
 }
@@ -570,13 +584,19 @@ __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __
   double ar[kTrsmRhs], ai[kTrsmRhs];
 #pragma unroll
   for (int q = 0; q < kTrsmRhs; q++) ar[q] = ai[q] = 0.0;
-  for (int c = 0; c < kb; c++) {
-    auto v = LU[(size_t)i + (size_t)n * (k0 + c)];
-    const double lr = (double)v.x, li = (double)v.y;
+  // issue all the row's factor loads before the arithmetic
+  typename CT<R>::c2 l[kPanelNB];
 #pragma unroll
-    for (int q = 0; q < kTrsmRhs; q++) {
-      ar[q] += lr * yr[c][q] - li * yi[c][q];
-      ai[q] += lr * yi[c][q] + li * yr[c][q];
+  for (int c = 0; c < kPanelNB; c++) l[c] = LU[(size_t)i + (size_t)n * (k0 + min(c, kb - 1))];
+#pragma unroll
+  for (int c = 0; c < kPanelNB; c++) {
+    if (c < kb) {
+      const double lr = (double)l[c].x, li = (double)l[c].y;
+#pragma unroll
+      for (int q = 0; q < kTrsmRhs; q++) {
+        ar[q] += lr * yr[c][q] - li * yi[c][q];
+        ai[q] += lr * yi[c][q] + li * yr[c][q];
+      }
     }
   }
 #pragma unroll
