@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <complex>
 #include <vector>
 
 #include "hamsx_internal.hpp"
@@ -205,6 +206,10 @@ struct hamsx_ctx {
   hipStream_t stream = nullptr;
   int prec = HAMSX_PREC_FP64;
   int refine_iters = 3;
+  // linear solver: HAMSX_SOLVER_LU or HAMSX_SOLVER_GMRES (restart, tolerance, iteration cap)
+  int solver = HAMSX_SOLVER_LU;
+  int gm_restart = 25, gm_maxiter = 100;
+  double gm_tol = 1.0e-4;
   // mesh
   int n = 0, nbody = 0, nchunk = 0;
   std::vector<int> body_start;
@@ -221,12 +226,17 @@ struct hamsx_ctx {
   int isol = 1;
   bool have_waves = false;
   // matrices and workspace
-  DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag;
+  DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag, mvw;
   DevBuf<float2> Af;
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins, valid for bin_n panels at depth bin_h
   DevBuf<unsigned> plist;
   DevBuf<double> modetab;  // per-frequency finite-depth mode tables
+  // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
+  DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
+  DevBuf<double> gNorm, gScale;
+  DevBuf<int> pPerm, pIpiv;
+  std::vector<size_t> pOff;
   DevBuf<int> bin_cnt;
   int bin_n = 0;
   double bin_h = 0.0;
@@ -380,7 +390,8 @@ int ensure_work(hamsx_ctx* c) {
     return rc;
   if (c->prec == HAMSX_PREC_MIXED) {
     if ((rc = c->B.ensure(n * nr)) || (rc = c->R.ensure(n * nr)) || (rc = c->D.ensure(n * nr)) ||
-        (rc = c->Af.ensure(n * n)) || (rc = c->norms.ensure(2 * nr)))
+        (rc = c->Af.ensure(n * n)) || (rc = c->norms.ensure(2 * nr)) ||
+        (rc = c->mvw.ensure(hamsx::launch::matvec_workspace((int)n, (int)nr))))
       return rc;
   }
   return 0;
@@ -423,11 +434,228 @@ int do_factor(hamsx_ctx* c) {
   return check_launch();
 }
 
+// ---------------------------------------------------------------------------
+// Restarted GMRES (ISOLV = 2 path of the reference: ZBLOCK_GMRES_SOLVE /
+// ZGMRES_SOLVE, src/IterativeSolvers.f90:324-562, block-diagonal LU
+// preconditioner SETUP_BLOCK_PRECONDITIONER :190-307), right-preconditioned,
+// x0 = 0, all right-hand sides advancing together with one device
+// synchronisation per Arnoldi step.  The operator is the assembled dense
+// matrix (the reference's H-matrix samples that same matrix,
+// src/HMatrix.f90:633-665).  Rotations are the standard complex Givens
+// rotations with real cosine, which annihilate the subdiagonal for complex
+// Hessenberg entries.
+// ---------------------------------------------------------------------------
+namespace {
+using cplx = std::complex<double>;
+
+int gmres_precond_setup(hamsx_ctx* c) {
+  const int nb = c->nbody;
+  c->pOff.assign(nb + 1, 0);
+  size_t maxb = 0;
+  for (int b = 0; b < nb; b++) {
+    const size_t sz = (size_t)(c->body_start[b + 1] - c->body_start[b]);
+    c->pOff[b + 1] = c->pOff[b] + sz * sz;
+    maxb = std::max(maxb, sz);
+  }
+  int rc;
+  const int nsets0 = ((int)maxb + hamsx::kSelRows - 1) / hamsx::kSelRows;
+  if ((rc = c->pLU.ensure(c->pOff[nb])) || (rc = c->pPerm.ensure(c->n)) || (rc = c->pIpiv.ensure(maxb)) ||
+      (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)) ||
+      (rc = c->ludiag.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB)))
+    return rc;
+  for (int b = 0; b < nb; b++) {
+    const int r0 = c->body_start[b], sz = c->body_start[b + 1] - r0;
+    double2* blk = c->pLU.p + c->pOff[b];
+    hamsx::launch::block_copy(c->stream, c->A.p, c->n, r0, sz, blk);
+    hamsx::launch::lu_factor(c->stream, blk, sz, c->pIpiv.p, c->pPerm.p + r0, c->widx.p, c->wcnt.p,
+                             c->flags.p + 1, c->ludiag.p);
+  }
+  return check_launch();
+}
+
+// Z = M^-1 V (n x nrhs), body by body
+void gmres_precond_apply(hamsx_ctx* c, const double2* V, double2* Z, int nrhs) {
+  for (int b = 0; b < c->nbody; b++) {
+    const int r0 = c->body_start[b], sz = c->body_start[b + 1] - r0;
+    hamsx::launch::rows_pack(c->stream, V, c->n, r0, sz, nrhs, c->gP.p);
+    hamsx::launch::lu_solve(c->stream, c->pLU.p + c->pOff[b], sz, c->pPerm.p + r0, c->gP.p, c->gPt.p, nrhs);
+    hamsx::launch::rows_unpack(c->stream, c->gP.p, c->n, r0, sz, nrhs, Z);
+  }
+}
+
+void givens(cplx a, cplx b, double& cs, cplx& sn) {
+  const double ab = std::abs(a), bb = std::abs(b);
+  if (bb == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (ab == 0.0) {
+    cs = 0.0;
+    sn = 1.0;
+  } else {
+    const double d = std::sqrt(ab * ab + bb * bb);
+    cs = ab / d;
+    sn = (a / ab) * std::conj(b) / d;
+  }
+}
+
+int do_solve_gmres(hamsx_ctx* c) {
+  const int n = c->n, nr = c->nrhs(), m = c->gm_restart;
+  const size_t blk = (size_t)n * nr;
+  int rc;
+  if ((rc = c->gV.ensure(blk * (m + 1))) || (rc = c->gW.ensure(blk)) || (rc = c->gZ.ensure(blk)) ||
+      (rc = c->gU.ensure(blk)) || (rc = c->gB.ensure(blk)) || (rc = c->gDots.ensure((size_t)2 * (m + 1) * nr)) ||
+      (rc = c->gNorm.ensure(nr)) || (rc = c->gScale.ensure(nr)) ||
+      (rc = c->mvw.ensure(hamsx::launch::matvec_workspace(n, nr))))
+    return rc;
+  size_t maxb = 0;
+  for (int b = 0; b < c->nbody; b++) maxb = std::max(maxb, (size_t)(c->body_start[b + 1] - c->body_start[b]));
+  if ((rc = c->gP.ensure(maxb * nr)) || (rc = c->gPt.ensure(maxb * nr))) return rc;
+  if ((rc = gmres_precond_setup(c))) return rc;
+  // B = right-hand sides, X = 0
+  HX_TRY(hipMemcpyAsync(c->gB.p, c->X.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
+  HX_TRY(hipMemsetAsync(c->X.p, 0, sizeof(double2) * blk, c->stream));
+  std::vector<double> bnorm(nr), beta(nr), hn(nr), scale(nr);
+  std::vector<cplx> H((size_t)(m + 1) * m * nr), g((size_t)(m + 1) * nr), sn((size_t)m * nr);
+  std::vector<double> cs((size_t)m * nr);
+  std::vector<double2> d12((size_t)2 * (m + 1) * nr), ybuf((size_t)m * nr);
+  std::vector<int> mact(nr), done(nr, 0);
+  hamsx::launch::col_norm2(c->stream, c->gB.p, n, nr, c->gNorm.p);
+  HX_TRY(hipMemcpyAsync(bnorm.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nr; q++) done[q] = bnorm[q] < 1.0e-30;
+  int iter = 0;
+  double worst = 0.0;
+  while (iter < c->gm_maxiter) {
+    // R = B - A X (into gW), beta = ||R||
+    if (iter == 0)
+      HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
+    else
+      hamsx::launch::matvec(c->stream, c->A.p, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+    hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
+    HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+    HX_TRY(hipStreamSynchronize(c->stream));
+    bool all = true;
+    worst = 0.0;
+    for (int q = 0; q < nr; q++) {
+      if (!done[q] && beta[q] / bnorm[q] < c->gm_tol) done[q] = 1;
+      if (!done[q]) all = false;
+      if (bnorm[q] >= 1.0e-30) worst = std::max(worst, beta[q] / bnorm[q]);
+      scale[q] = done[q] ? 0.0 : 1.0 / beta[q];
+      mact[q] = 0;
+      for (int j = 0; j <= m; j++) g[(size_t)j * nr + q] = 0.0;
+      g[q] = done[q] ? 0.0 : beta[q];
+    }
+    if (all) break;
+    HX_TRY(hipMemcpyAsync(c->gScale.p, scale.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    hamsx::launch::col_scale(c->stream, c->gW.p, n, nr, c->gScale.p, c->gV.p);
+    std::vector<int> active(nr);
+    for (int q = 0; q < nr; q++) active[q] = !done[q];
+    int j = 0;
+    for (; j < m && iter < c->gm_maxiter; j++) {
+      iter++;
+      double2* Vj = c->gV.p + blk * j;
+      gmres_precond_apply(c, Vj, c->gZ.p, nr);
+      hamsx::launch::matvec(c->stream, c->A.p, n, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, c->mvw.p);
+      double2* d1 = c->gDots.p;
+      double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
+      hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1);
+      hamsx::launch::block_update(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1, nullptr);
+      hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d2);
+      hamsx::launch::block_update(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d2, nullptr);
+      hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
+      HX_TRY(hipMemcpyAsync(d12.data(), c->gDots.p, sizeof(double2) * 2 * (m + 1) * nr, hipMemcpyDeviceToHost,
+                            c->stream));
+      HX_TRY(hipMemcpyAsync(hn.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+      HX_TRY(hipStreamSynchronize(c->stream));
+      bool any = false;
+      for (int q = 0; q < nr; q++) {
+        scale[q] = 0.0;
+        if (!active[q]) continue;
+        cplx* h = &H[((size_t)q * m + j) * (m + 1)];  // column j of q's Hessenberg matrix
+        for (int i = 0; i <= j; i++) {
+          const double2 a = d12[(size_t)i * nr + q], b = d12[(size_t)(m + 1) * nr + (size_t)i * nr + q];
+          h[i] = cplx(a.x + b.x, a.y + b.y);
+        }
+        h[j + 1] = hn[q];
+        for (int i = 0; i < j; i++) {
+          const double ci = cs[(size_t)i * nr + q];
+          const cplx si = sn[(size_t)i * nr + q];
+          const cplx t = ci * h[i] + si * h[i + 1];
+          h[i + 1] = -std::conj(si) * h[i] + ci * h[i + 1];
+          h[i] = t;
+        }
+        double cj;
+        cplx sj;
+        givens(h[j], h[j + 1], cj, sj);
+        cs[(size_t)j * nr + q] = cj;
+        sn[(size_t)j * nr + q] = sj;
+        h[j] = cj * h[j] + sj * h[j + 1];
+        h[j + 1] = 0.0;
+        cplx& gj = g[(size_t)j * nr + q];
+        cplx& gj1 = g[(size_t)(j + 1) * nr + q];
+        const cplx t = cj * gj + sj * gj1;
+        gj1 = -std::conj(sj) * gj + cj * gj1;
+        gj = t;
+        mact[q] = j + 1;
+        if (std::abs(gj1) / bnorm[q] < c->gm_tol) {
+          active[q] = 0;  // converged in this cycle: its later columns are zero
+        } else {
+          any = true;
+          scale[q] = hn[q] > 1.0e-30 ? 1.0 / hn[q] : 0.0;
+        }
+      }
+      if (!any) {
+        j++;
+        break;
+      }
+      HX_TRY(hipMemcpyAsync(c->gScale.p, scale.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+      hamsx::launch::col_scale(c->stream, c->gW.p, n, nr, c->gScale.p, c->gV.p + blk * (j + 1));
+    }
+    // y = H^-1 g per right-hand side; X += M^-1 (V y)
+    for (size_t k = 0; k < ybuf.size(); k++) ybuf[k] = make_double2(0.0, 0.0);
+    for (int q = 0; q < nr; q++) {
+      const int mq = mact[q];
+      std::vector<cplx> y(mq);
+      for (int i = 0; i < mq; i++) y[i] = g[(size_t)i * nr + q];
+      for (int i = mq - 1; i >= 0; i--) {
+        const cplx hii = H[((size_t)q * m + i) * (m + 1) + i];
+        y[i] = std::abs(hii) < 1.0e-30 ? cplx(0.0) : y[i] / hii;
+        for (int k = 0; k < i; k++) y[k] -= H[((size_t)q * m + i) * (m + 1) + k] * y[i];
+      }
+      for (int i = 0; i < mq; i++) ybuf[(size_t)i * nr + q] = make_double2(y[i].real(), y[i].imag());
+    }
+    HX_TRY(hipMemcpyAsync(c->gDots.p, ybuf.data(), sizeof(double2) * (size_t)m * nr, hipMemcpyHostToDevice,
+                          c->stream));
+    hamsx::launch::block_combine(c->stream, c->gV.p, j, n, nr, c->gDots.p, c->gU.p);
+    gmres_precond_apply(c, c->gU.p, c->gZ.p, nr);
+    hamsx::launch::axpy_update(c->stream, c->X.p, c->gZ.p, blk);
+    HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above are reused next cycle
+  }
+  // final true residual for the statistics
+  hamsx::launch::matvec(c->stream, c->A.p, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+  hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
+  HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  worst = 0.0;
+  for (int q = 0; q < nr; q++)
+    if (bnorm[q] >= 1.0e-30) worst = std::max(worst, beta[q] / bnorm[q]);
+  c->last_iters = iter;
+  c->last_resid = worst;
+  return check_launch();
+}
+}  // namespace
+
 // X <- A^{-1} X for all 6nb + nbeta columns (with refinement in mixed mode)
 int do_solve(hamsx_ctx* c) {
   if (c->have_X) return 0;
   int rc = do_rhs(c);
   if (rc) return rc;
+  if (c->solver == HAMSX_SOLVER_GMRES) {
+    StageTimer tm(c, 3);
+    if ((rc = do_solve_gmres(c))) return rc;
+    c->have_X = true;
+    return 0;
+  }
   if ((rc = do_factor(c))) return rc;
   const int n = c->n, nr = c->nrhs();
   StageTimer tm(c, 3);
@@ -438,11 +666,11 @@ int do_solve(hamsx_ctx* c) {
     HX_TRY(hipMemcpyAsync(c->B.p, c->X.p, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
     hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
     for (int it = 0; it < c->refine_iters; it++) {
-      hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->D.p, nr, c->norms.p);
+      hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p);
       hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr);
       hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt);
     }
-    hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->R.p, nr, c->norms.p);
+    hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p);
     c->last_iters = c->refine_iters;
   }
   c->have_X = true;
@@ -566,8 +794,13 @@ int hamsx_destroy(hamsx_ctx* c) {
                   &c->perm, &c->widx, &c->wcnt, &c->flags, &c->bin_cnt})
     b->release();
   c->plist.release();
+  for (auto* b : {&c->gV, &c->gW, &c->gZ, &c->gU, &c->gB, &c->gP, &c->gPt, &c->gDots, &c->pLU}) b->release();
+  c->gNorm.release();
+  c->gScale.release();
+  c->pPerm.release();
+  c->pIpiv.release();
   c->modetab.release();
-  for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial, &c->ludiag})
+  for (auto* b : {&c->A, &c->S, &c->X, &c->Xtmp, &c->B, &c->R, &c->D, &c->dum2, &c->partial, &c->ludiag, &c->mvw})
     b->release();
   c->Af.release();
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
@@ -961,6 +1194,23 @@ int hamsx_get_matrix(hamsx_ctx* c, int which, double* out, int* ipiv) {
     return fail(HAMSX_EARG, "which must be 0 (A), 1 (S) or 2 (LU)");
   }
   HX_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hamsx_set_solver(hamsx_ctx* c, int solver, double tol, int restart, int maxiter) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (solver != HAMSX_SOLVER_LU && solver != HAMSX_SOLVER_GMRES) return fail(HAMSX_EARG, "unknown solver %d", solver);
+  if (solver == HAMSX_SOLVER_GMRES) {
+    if (!(tol > 0.0) || restart < 1 || restart > 200 || maxiter < 1)
+      return fail(HAMSX_EARG, "GMRES needs tol > 0, 1 <= restart <= 200, maxiter >= 1");
+    if (c->prec != HAMSX_PREC_FP64) return fail(HAMSX_EARG, "GMRES runs in FP64 precision mode");
+    c->gm_tol = tol;
+    c->gm_restart = restart;
+    c->gm_maxiter = maxiter;
+  }
+  c->solver = solver;
+  c->have_X = false;
   return 0;
 }
 

@@ -100,9 +100,24 @@ void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
                    double2* diag);
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
+// Y = beta B + alpha A X (B may be null), A read once; P: matvec_workspace(n, nrhs) complex128
+size_t matvec_workspace(int n, int nrhs);
+void matvec(hipStream_t s, const double2* A, int n, const double2* X, int nrhs, const double2* B, double alpha,
+            double beta, double2* Y, double2* P);
+// R = B - A X with per-column max norms of R (norms[0:nrhs]) and X (norms[nrhs:2 nrhs])
 void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
-              double* norms);
+              double* norms, double2* P);
 void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count);
+// restarted GMRES helpers (krylov.hip); blocks are n x nrhs, V holds nv of them
+void block_dots(hipStream_t s, const double2* V, int nv, const double2* W, int n, int nrhs, double2* dots);
+void block_update(hipStream_t s, const double2* V, int nv, double2* W, int n, int nrhs, const double2* h,
+                  double2* hsum);
+void col_norm2(hipStream_t s, const double2* W, int n, int nrhs, double* norms);
+void col_scale(hipStream_t s, const double2* W, int n, int nrhs, const double* scale, double2* Y);
+void block_combine(hipStream_t s, const double2* V, int m, int n, int nrhs, const double2* y, double2* U);
+void rows_pack(hipStream_t s, const double2* X, int n, int r0, int nb, int nrhs, double2* P);
+void rows_unpack(hipStream_t s, const double2* P, int n, int r0, int nb, int nrhs, double2* X);
+void block_copy(hipStream_t s, const double2* A, int n, int r0, int nb, double2* B);
 }  // namespace launch
 
 }  // namespace hamsx

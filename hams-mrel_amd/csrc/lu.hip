@@ -647,24 +647,69 @@ __global__ void k_to_c64(const double2* __restrict__ A, float2* __restrict__ Af,
   for (; i < count; i += (size_t)gridDim.x * blockDim.x) Af[i] = make_float2((float)A[i].x, (float)A[i].y);
 }
 
-// R(:, q) = B(:, q) - A X(:, q): each block owns 64 rows and one column
-// chunk of A; partial products are accumulated with atomics.
-__global__ __launch_bounds__(256) void k_residual(const double2* __restrict__ A, int n, const double2* __restrict__ X,
-                                                  double2* __restrict__ R, int nrhs, int jchunk) {
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int q0 = (threadIdx.x >> 6);
-  const int j0 = blockIdx.y * jchunk, j1 = min(n, j0 + jchunk);
-  if (i >= n) return;
-  for (int q = q0; q < nrhs; q += 4) {
-    double ar = 0.0, ai = 0.0;
-    for (int j = j0; j < j1; j++) {
-      double2 a = A[(size_t)i + (size_t)n * j], x = X[(size_t)j + (size_t)n * q];
-      ar += a.x * x.x - a.y * x.y;
-      ai += a.x * x.y + a.y * x.x;
+// Y = beta B + alpha A X for up to kMvRhs right-hand sides per pass, reading
+// every entry of A once (the residual of the mixed path; HBM-bound).  Block
+// (rb, chunk, group): 64 rows, kMvChunk columns split over the four waves,
+// the wave-uniform X entries come through scalar loads.  Partial sums per
+// column chunk go to P and are added in a fixed order by k_matvec_reduce, so
+// the result does not depend on scheduling.
+constexpr int kMvRhs = 16, kMvChunk = 256;
+__global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restrict__ A, int n,
+                                                        const double2* __restrict__ X, int nrhs,
+                                                        double2* __restrict__ P) {
+  __shared__ double2 red[3][64][kMvRhs];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane, ic = min(i, n - 1);
+  const int q0 = blockIdx.z * kMvRhs, nq = min(kMvRhs, nrhs - q0);
+  const int j0 = blockIdx.y * kMvChunk + w * (kMvChunk / 4);
+  const int j1 = min(n, j0 + kMvChunk / 4);
+  double ar[kMvRhs], ai[kMvRhs];
+#pragma unroll
+  for (int q = 0; q < kMvRhs; q++) ar[q] = ai[q] = 0.0;
+  for (int j = j0; j < j1; j++) {
+    const double2 a = A[(size_t)ic + (size_t)n * j];
+#pragma unroll
+    for (int q = 0; q < kMvRhs; q++) {  // clamped column: loads stay unconditional
+      const double2 x = X[(size_t)j + (size_t)n * (q0 + min(q, nq - 1))];
+      ar[q] += a.x * x.x - a.y * x.y;
+      ai[q] += a.x * x.y + a.y * x.x;
     }
-    atomicAdd(&R[(size_t)i + (size_t)n * q].x, -ar);
-    atomicAdd(&R[(size_t)i + (size_t)n * q].y, -ai);
   }
+  if (w > 0)
+#pragma unroll
+    for (int q = 0; q < kMvRhs; q++) red[w - 1][lane][q] = make_double2(ar[q], ai[q]);
+  __syncthreads();
+  if (w == 0 && i < n) {
+    const size_t base = (size_t)blockIdx.y * n * nrhs;
+#pragma unroll
+    for (int q = 0; q < kMvRhs; q++)
+      if (q < nq) {
+        double sr = ar[q], si = ai[q];
+        for (int v = 0; v < 3; v++) {
+          sr += red[v][lane][q].x;
+          si += red[v][lane][q].y;
+        }
+        P[base + (size_t)i + (size_t)n * (q0 + q)] = make_double2(sr, si);
+      }
+  }
+}
+
+__global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n, int nrhs,
+                                const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * nrhs;
+  if (e >= tot) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunk; c++) {
+    const double2 v = P[(size_t)c * tot + e];
+    sr += v.x;
+    si += v.y;
+  }
+  double yr = alpha * sr, yi = alpha * si;
+  if (B) {
+    yr += beta * B[e].x;
+    yi += beta * B[e].y;
+  }
+  Y[e] = make_double2(yr, yi);
 }
 
 __global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms) {
@@ -710,12 +755,21 @@ void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* w
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
   lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs);
 }
+size_t matvec_workspace(int n, int nrhs) {
+  return (size_t)((n + kMvChunk - 1) / kMvChunk) * n * nrhs;
+}
+void matvec(hipStream_t s, const double2* A, int n, const double2* X, int nrhs, const double2* B, double alpha,
+            double beta, double2* Y, double2* P) {
+  const int nchunk = (n + kMvChunk - 1) / kMvChunk;
+  HX_LAUNCH(k_matvec_partial, dim3((n + 63) / 64, nchunk, (nrhs + kMvRhs - 1) / kMvRhs), dim3(256), 0, s, A, n, X,
+            nrhs, P);
+  const size_t tot = (size_t)n * nrhs;
+  HX_LAUNCH(k_matvec_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, P, nchunk, n, nrhs, B, alpha,
+            beta, Y);
+}
 void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
-              double* norms) {
-  (void)hipMemcpyAsync(R, B, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
-  const int jchunk = 256;
-  HX_LAUNCH(k_residual, dim3((n + 63) / 64, (n + jchunk - 1) / jchunk), dim3(256), 0, s, A, n, X, R, nrhs,
-                     jchunk);
+              double* norms, double2* P) {
+  matvec(s, A, n, X, nrhs, B, -1.0, 1.0, R, P);
   HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms);
   HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs);
 }

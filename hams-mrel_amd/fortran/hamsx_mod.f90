@@ -26,12 +26,14 @@ module hamsx_mod
 
   integer(c_int), parameter, public :: HAMSX_OK = 0, HAMSX_NK = 200
   integer(c_int), parameter, public :: HAMSX_PREC_FP64 = 0, HAMSX_PREC_MIXED = 1
+  integer(c_int), parameter, public :: HAMSX_SOLVER_LU = 0, HAMSX_SOLVER_GMRES = 1
 
   public :: hamsx_version, hamsx_last_error, hamsx_device_count, hamsx_create, hamsx_destroy, &
             hamsx_set_precision, hamsx_panel_geometry, hamsx_wave_properties, hamsx_set_mesh, &
             hamsx_set_frequency, hamsx_set_waves, hamsx_assemble, hamsx_factor, hamsx_solve_radiation, &
             hamsx_solve_diffraction, hamsx_forces, hamsx_solve_frequency, hamsx_submit_frequency, &
             hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
+            hamsx_set_solver, &
             hamsx_stage_times
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
@@ -160,6 +162,13 @@ module hamsx_mod
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx, out, ipiv
       integer(c_int), value :: which
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_solver(ctx, solver, tol, restart, maxiter) bind(C, name="hamsx_set_solver") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: solver, restart, maxiter
+      real(c_double), value :: tol
       integer(c_int) :: rc
     end function
     function hamsx_refine_stats(ctx, iters, resid) bind(C, name="hamsx_refine_stats") result(rc)

@@ -20,6 +20,8 @@ HAMSX_ENOCONV = -5
 HAMSX_NK = 200
 PREC_FP64 = 0
 PREC_MIXED = 1
+SOLVER_LU = 0
+SOLVER_GMRES = 1
 
 #: every symbol include/hamsx.h declares, with (restype, argtypes)
 _dp = C.POINTER(C.c_double)
@@ -47,6 +49,7 @@ SIGNATURES = {
     "hamsx_query": (C.c_int, [_vp]),
     "hamsx_collect": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
+    "hamsx_set_solver": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),

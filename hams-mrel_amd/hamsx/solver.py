@@ -73,6 +73,14 @@ class FrequencySolver:
         _lib.check(self._lib.hamsx_set_precision(self._h, mode), "hamsx_set_precision")
         self.precision = precision
 
+    def set_solver(self, solver: str = "lu", tol: float = 1.0e-4, restart: int = 25, maxiter: int = 100):
+        """'lu' (ZGETRF/ZGETRS) or 'gmres' (restarted GMRES with the per-body
+        block-diagonal LU preconditioner; defaults are the reference's
+        IterativeSolvers.f90:56-58)."""
+        code = {"lu": _lib.SOLVER_LU, "gmres": _lib.SOLVER_GMRES}[solver]
+        _lib.check(self._lib.hamsx_set_solver(self._h, code, tol, restart, maxiter), "hamsx_set_solver")
+        self.solver = solver
+
     def set_waves(self, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1):
         b = np.ascontiguousarray(headings_rad, dtype=np.float64)
         self.nbeta = len(b)

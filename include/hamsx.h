@@ -139,8 +139,21 @@ int hamsx_collect(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
  * out[nelem*nelem] (complex); ipiv[nelem] (1-based) for which = 2, else NULL. */
 int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
 
-/* Refinement statistics of the last mixed-precision solve: iterations used
- * and the final max-norm relative residual. */
+/* Linear solver for the radiation and diffraction systems.  HAMSX_SOLVER_LU
+ * (default): ZGETRF/ZGETRS as above.  HAMSX_SOLVER_GMRES: restarted GMRES
+ * with the per-body block-diagonal LU preconditioner, x0 = 0, stopping at
+ * relative residual tol or maxiter iterations (the reference's ISOLV = 2,
+ * ZITER_SOLVE_MULTI with SETUP_BLOCK_PRECONDITIONER,
+ * src/IterativeSolvers.f90:56-58 (restart 25, tol 1e-4, maxiter 100),
+ * :190-587, called from src/AssbMatxMulti.f90:283-289, :349-357); the
+ * operator is the assembled matrix itself.  FP64 precision mode only. */
+#define HAMSX_SOLVER_LU 0
+#define HAMSX_SOLVER_GMRES 1
+int hamsx_set_solver(hamsx_ctx *ctx, int solver, double tol, int restart, int maxiter);
+
+/* Statistics of the last solve: mixed precision -- refinement iterations
+ * and the final max-norm relative residual; GMRES -- Arnoldi iterations and
+ * the largest final relative residual 2-norm over the right-hand sides. */
 int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);
 
 /* Measurement hook (no counterpart in the reference, which has no timers):

@@ -215,3 +215,28 @@ def test_submitted_solves_in_flight_equal_blocking_solves(mesh_small):
     for r, g in zip(ref, got):
         for a, b in zip(r, g):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("depth", [-1.0, 35.0])
+def test_gmres_matches_lu(depth):
+    """Restarted GMRES with the block-diagonal preconditioner converges to the
+    LU solution: at a tight tolerance to 1e-8 on every coefficient, at the
+    reference's own tolerance (1e-4) as loosely as the reference's GMRES
+    integration test allows (test/test_integration_gmres.f90:69,97)."""
+    m = small_pair(na=28, nz=7, nr=3)
+    heads = np.deg2rad([0.0, 45.0])
+    say(f"[gpu] gmres vs lu n={m.nelem} depth={depth}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        lu = s.solve_frequency(1.1, depth)
+        s.set_solver("gmres", tol=1e-11, restart=25, maxiter=400)
+        tight = s.solve_frequency(1.1, depth)
+        it, res = s.refine_stats()
+        s.set_solver("gmres")
+        loose = s.solve_frequency(1.1, depth)
+        it2, res2 = s.refine_stats()
+    say(f"    tight: {it} iterations, residual {res:.2e}; reference tolerance: {it2} iterations, residual {res2:.2e}")
+    assert res < 1e-11 and res2 < 1e-4
+    for a, b, c in zip(tight[:3], lu[:3], loose[:3]):
+        assert rel(a, b) < 1e-8
+        assert rel(c, b) < 1e-3
