@@ -85,6 +85,29 @@ def cylinder(radius: float, draft: float, na: int, nz: int, nr: int, x0: float =
     return _pack(xyz, panels)
 
 
+def waterplane_disk(radius: float, na: int, nr: int, x0: float = 0.0, y0: float = 0.0) -> PanelMesh:
+    """Lid of a vertical cylinder for irregular-frequency removal: the
+    waterplane disk at z = 0 in nr rings of na panels (triangles at the
+    centre).  Only the panel centroids are used (as field points)."""
+    th = 2.0 * np.pi * np.arange(na) / na
+    xyz, rings = [], []
+    for m in range(nr):
+        r = radius * (nr - m) / nr
+        rings.append(list(range(len(xyz), len(xyz) + na)))
+        for a in range(na):
+            xyz.append((x0 + r * math.cos(th[a]), y0 + r * math.sin(th[a]), 0.0))
+    centre = len(xyz)
+    xyz.append((x0, y0, 0.0))
+    panels = []
+    for m in range(nr - 1):
+        for a in range(na):
+            b = (a + 1) % na
+            panels.append([rings[m][a], rings[m][b], rings[m + 1][b], rings[m + 1][a]])
+    for a in range(na):
+        panels.append([rings[-1][a], rings[-1][(a + 1) % na], centre])
+    return _pack(np.array(xyz), panels)
+
+
 def box(length: float, beam: float, draft: float, nl: int, nb: int, nt: int, x0: float = 0.0,
         y0: float = 0.0) -> PanelMesh:
     """Rectangular barge L x B x T: four walls and a bottom of uniform quads."""

@@ -1353,12 +1353,16 @@ void orc_panel_geometry(int nnode, const double *xyz, int nelem, const int *ncn,
  * (BodyIntgrMulti.f90:48-77), ASSB_LEFT_MULTI (AssbMatxMulti.f90:53-105); the
  * single-layer entry S is RBC_MULTI_RIGHT's DUM (BodyIntgrMulti.f90:95-100).
  * ====================================================================== */
-static void pair_entry(int i, int j, int nelem, int nnode, const double *xyz, const int *ncn,
-                       const int *ncon, const double *cen, const double *area, const double *pnsz,
-                       const double *dxyz, double v, double h, const double *wvn, int nk, cplx *a_ij,
-                       cplx *s_ij)
+/* The entries of field point (xi, yi, zi) against hull source panel j: the
+ * pair body of CALGREEN_MULTI / CALGREEN_MULTI_IRR (hull rows :123-180 and lid
+ * rows :260-316 of CalGreenFuncMulti.f90) with BODINT_LEFT_IRR_MULTI and
+ * RBC_IRR_MULTI's DUM (BodyIntgr_irrMulti.f90:48-127), without the 2*pi of
+ * the diagonal. */
+static void field_entry(double xi, double yi, double zi, int j, int nelem, int nnode, const double *xyz,
+                        const int *ncn, const int *ncon, const double *cen, const double *area,
+                        const double *pnsz, const double *dxyz, double v, double h, const double *wvn, int nk,
+                        cplx *t_ij, cplx *s_ij)
 {
-    double xi = cen[i], yi = cen[i + nelem], zi = cen[i + 2 * nelem];
     double xj = cen[j], yj = cen[j + nelem], zj = cen[j + 2 * nelem];
     double dist = sqrt(ipow(xi - xj, 2) + ipow(yi - yj, 2) + ipow(zi - zj, 2));
     int flag = (dist <= 50.0 * pnsz[j]) ? 1 : 0;
@@ -1377,14 +1381,23 @@ static void pair_entry(int i, int j, int nelem, int nnode, const double *xyz, co
     else orc_fingreen3d(xj, xi, yj, yi, zj, zi, v, wvn, nk, h, flag, g);
     cplx g0 = g[0] + I * g[1], g1 = g[2] + I * g[3], g2 = g[4] + I * g[5], g3 = g[6] + I * g[7];
     double ear = area[j], n1 = dxyz[j], n2 = dxyz[j + nelem], n3 = dxyz[j + 2 * nelem];
-    cplx t;
     if (flag) {
-        t = (dij[0] + g1 * ear) * n1 + (dij[1] + g2 * ear) * n2 + (dij[2] + g3 * ear) * n3;
+        *t_ij = (dij[0] + g1 * ear) * n1 + (dij[1] + g2 * ear) * n2 + (dij[2] + g3 * ear) * n3;
         *s_ij = sij + g0 * ear;
     } else {
-        t = (g1 * n1 + g2 * n2 + g3 * n3) * ear;
+        *t_ij = (g1 * n1 + g2 * n2 + g3 * n3) * ear;
         *s_ij = g0 * ear;
     }
+}
+
+static void pair_entry(int i, int j, int nelem, int nnode, const double *xyz, const int *ncn,
+                       const int *ncon, const double *cen, const double *area, const double *pnsz,
+                       const double *dxyz, double v, double h, const double *wvn, int nk, cplx *a_ij,
+                       cplx *s_ij)
+{
+    cplx t;
+    field_entry(cen[i], cen[i + nelem], cen[i + 2 * nelem], j, nelem, nnode, xyz, ncn, ncon, cen, area, pnsz,
+                dxyz, v, h, wvn, nk, &t, s_ij);
     *a_ij = (i == j) ? (2.0 * PI_C + t) : t;
 }
 
@@ -1441,11 +1454,11 @@ void orc_assemble_rows(int nrows, const int *rows, int nelem, int nnode, const d
  * RBC_MULTI_RIGHT (BodyIntgrMulti.f90:83-127); ASSB_DBC_MULTI (:187-251)
  * with DBC_MULTI_RIGHT (BodyIntgrMulti.f90:134-173).  NSYS = 1.
  * ====================================================================== */
-void orc_rhs(int nelem, const double *S, const double *cen, const double *dxyz, int nbody,
-             const int *body_start, int nbeta, const double *beta, double v, double w1, double wk,
-             double h, double amp, double xw0, double xw1, double *rhs)
+/* rows [0, nrows) of the right-hand sides from S (nrows x nelem, column-major) */
+static void rhs_rows(int nrows, int nelem, const double *S, const double *cen, const double *dxyz, int nbody,
+                     const int *body_start, int nbeta, const double *beta, double v, double w1, double wk,
+                     double h, double amp, double xw0, double xw1, double *rhs)
 {
-    int nr = 6 * nbody + nbeta;
     cplx *dum2 = (cplx *)malloc(sizeof(cplx) * (size_t)nelem * (nbeta > 0 ? nbeta : 1));
     for (int b = 0; b < nbeta; b++)
         for (int j = 0; j < nelem; j++) {
@@ -1454,16 +1467,16 @@ void orc_rhs(int nelem, const double *S, const double *cen, const double *dxyz, 
             cplx px = dpo[0] + I * dpo[1], py = dpo[2] + I * dpo[3], pz = dpo[4] + I * dpo[5];
             dum2[j + (size_t)nelem * b] = px * dxyz[j] + py * dxyz[j + nelem] + pz * dxyz[j + 2 * nelem];
         }
-    for (int i = 0; i < nelem; i++) {
+    for (int i = 0; i < nrows; i++) {
         for (int b = 0; b < nbody; b++) {
             cplx acc[6] = {0, 0, 0, 0, 0, 0};
             for (int j = body_start[b]; j < body_start[b + 1]; j++) {
-                size_t o = 2 * ((size_t)i + (size_t)nelem * j);
+                size_t o = 2 * ((size_t)i + (size_t)nrows * j);
                 cplx s = S[o] + I * S[o + 1];
                 for (int m = 0; m < 6; m++) acc[m] = acc[m] + s * dxyz[j + nelem * m];
             }
             for (int m = 0; m < 6; m++) {
-                size_t o = 2 * ((size_t)i + (size_t)nelem * (6 * b + m));
+                size_t o = 2 * ((size_t)i + (size_t)nrows * (6 * b + m));
                 rhs[o] = creal(acc[m]);
                 rhs[o + 1] = cimag(acc[m]);
             }
@@ -1471,17 +1484,23 @@ void orc_rhs(int nelem, const double *S, const double *cen, const double *dxyz, 
         for (int b = 0; b < nbeta; b++) {
             cplx acc = 0;
             for (int j = 0; j < nelem; j++) {
-                size_t o = 2 * ((size_t)i + (size_t)nelem * j);
+                size_t o = 2 * ((size_t)i + (size_t)nrows * j);
                 cplx s = S[o] + I * S[o + 1];
                 acc = acc + (-(s * dum2[j + (size_t)nelem * b]));
             }
-            size_t o = 2 * ((size_t)i + (size_t)nelem * (6 * nbody + b));
+            size_t o = 2 * ((size_t)i + (size_t)nrows * (6 * nbody + b));
             rhs[o] = creal(acc);
             rhs[o + 1] = cimag(acc);
         }
     }
-    (void)nr;
     free(dum2);
+}
+
+void orc_rhs(int nelem, const double *S, const double *cen, const double *dxyz, int nbody,
+             const int *body_start, int nbeta, const double *beta, double v, double w1, double wk,
+             double h, double amp, double xw0, double xw1, double *rhs)
+{
+    rhs_rows(nelem, nelem, S, cen, dxyz, nbody, body_start, nbeta, beta, v, w1, wk, h, amp, xw0, xw1, rhs);
 }
 
 /* ========================================================================
@@ -1629,5 +1648,72 @@ int orc_solve_frequency(int nelem, int nnode, const double *xyz, const int *ncn,
     orc_forces(nelem, X, cen, area, dxyz, nbody, body_start, nbeta, beta, v, w1, wk, h, amp, xw0, xw1, rho,
                isol, amas, bdmp, exfc);
     free(A); free(S); free(X); free(ipiv);
+    return info;
+}
+
+/* ========================================================================
+ * Irregular-frequency removal (IRSP = 1): the nlid lid centroids are extra
+ * field points.  A and S grow to (N + L) x N (lid rows: ASSB_LEFT_IRR_MULTI
+ * AssbMatx_irrMulti.f90:85-106, no 2*pi), the right-hand sides to N + L rows
+ * (ASSB_RBC_IRR_MULTI :191-319, ASSB_DBC_IRR_MULTI :327-464), and the solved
+ * system is C = A^T A, D = A^T B (plain transposes, :154-171, :300-316,
+ * :447-461), followed by ZGETRF/ZGETRS (:175-178, DIFFRACTION_SOLVER_IRR_MULTI
+ * :529-585).  NSYS = 1.
+ * ====================================================================== */
+int orc_solve_frequency_irr(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                            const double *cen, const double *area, const double *pnsz, const double *dxyz,
+                            int nbody, const int *body_start, int nlid, const double *lid_cen, double w1, double h,
+                            int nbeta, const double *beta, double amp, double xw0, double xw1, double rho, int isol,
+                            double *amas, double *bdmp, double *exfc, int nthreads)
+{
+    const int nk = 200;
+    double wvn[200] = {0};
+    double v = w1 * w1 / G_C, wk;
+    if (h <= 0.0) wk = v;
+    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    const int n = nelem, nf = nelem + nlid, nr = 6 * nbody + nbeta;
+    double *A = (double *)malloc(sizeof(double) * 2 * (size_t)nf * n);
+    double *S = (double *)malloc(sizeof(double) * 2 * (size_t)nf * n);
+    double *B = (double *)malloc(sizeof(double) * 2 * (size_t)nf * nr);
+    double *C = (double *)malloc(sizeof(double) * 2 * (size_t)n * n);
+    double *X = (double *)malloc(sizeof(double) * 2 * (size_t)n * nr);
+    int *ipiv = (int *)malloc(sizeof(int) * n);
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = 0; i < nf; i++)
+        for (int j = 0; j < n; j++) {
+            cplx a, sv;
+            if (i < n) pair_entry(i, j, n, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn, nk, &a, &sv);
+            else {
+                const int l = i - n;
+                field_entry(lid_cen[l], lid_cen[l + nlid], lid_cen[l + 2 * nlid], j, n, nnode, xyz, ncn, ncon, cen,
+                            area, pnsz, dxyz, v, h, wvn, nk, &a, &sv);
+            }
+            size_t o = 2 * ((size_t)i + (size_t)nf * j);
+            A[o] = creal(a); A[o + 1] = cimag(a);
+            S[o] = creal(sv); S[o + 1] = cimag(sv);
+        }
+    rhs_rows(nf, n, S, cen, dxyz, nbody, body_start, nbeta, beta, v, w1, wk, h, amp, xw0, xw1, B);
+    const cplx *Ac = (const cplx *)A, *Bc = (const cplx *)B;
+    cplx *Cc = (cplx *)C, *Xc = (cplx *)X;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < n; i++) {
+            cplx sum = 0;
+            for (int k = 0; k < nf; k++) sum = sum + Ac[k + (size_t)nf * i] * Ac[k + (size_t)nf * j];
+            Cc[i + (size_t)n * j] = sum;
+        }
+    }
+    for (int q = 0; q < nr; q++)
+        for (int i = 0; i < n; i++) {
+            cplx sum = 0;
+            for (int k = 0; k < nf; k++) sum = sum + Bc[k + (size_t)nf * q] * Ac[k + (size_t)nf * i];
+            Xc[i + (size_t)n * q] = sum;
+        }
+    int info = orc_zgetrf(n, C, ipiv);
+    orc_zgetrs(n, C, ipiv, nr, X);
+    orc_forces(n, X, cen, area, dxyz, nbody, body_start, nbeta, beta, v, w1, wk, h, amp, xw0, xw1, rho, isol,
+               amas, bdmp, exfc);
+    free(A); free(S); free(B); free(C); free(X); free(ipiv);
     return info;
 }

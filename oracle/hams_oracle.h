@@ -102,6 +102,16 @@ int orc_solve_frequency(int nelem, int nnode, const double *node_xyz, const int 
                         double rho, int isol, double *amas, double *bdmp, double *exfc,
                         int nthreads);
 
+/* IRSP = 1: nlid lid centroids lid_cen[nlid*3] (column-major) as extra field
+ * points; C = A^T A, D = A^T B, LU.  Outputs as orc_solve_frequency. */
+int orc_solve_frequency_irr(int nelem, int nnode, const double *node_xyz, const int *ncn,
+                            const int *ncon, const double *cen, const double *area,
+                            const double *pnsz, const double *dxyz, int nbody,
+                            const int *body_start, int nlid, const double *lid_cen, double w1,
+                            double h, int nbeta, const double *beta, double amp, double xw0,
+                            double xw1, double rho, int isol, double *amas, double *bdmp,
+                            double *exfc, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

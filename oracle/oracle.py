@@ -78,6 +78,10 @@ def _load(path):
         o.orc_zgetrs.argtypes = [C.c_int, _dp, _ip, C.c_int, _dp]
         o.orc_forces.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _d, _d, _d, _d, _d, _d,
                                  _d, _d, C.c_int, _dp, _dp, _dp]
+        o.orc_solve_frequency_irr.restype = C.c_int
+        o.orc_solve_frequency_irr.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip,
+                                              C.c_int, _dp, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp,
+                                              _dp, C.c_int]
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
@@ -188,4 +192,20 @@ def solve_frequency(m, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0)
                                      _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
                                      _p(m.body_start, C.c_int), w1, h, len(beta), _p(beta), amp, xw[0], xw[1], rho,
                                      isol, _p(amas), _p(bdmp), _p(exfc), nthreads)
+    return amas, bdmp, exfc[:, :len(beta)], info
+
+
+def solve_frequency_irr(m, lid_cen, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1,
+                        nthreads: int = 8, fma: bool = False):
+    """Irregular-frequency removal: lid_cen (L x 3) are the lid centroids."""
+    nd = 6 * m.nbody
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    lc = np.asfortranarray(lid_cen, dtype=np.float64)
+    amas = np.zeros((nd, nd), order="F")
+    bdmp = np.zeros((nd, nd), order="F")
+    exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
+    info = lib(fma).orc_solve_frequency_irr(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+                                         _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
+                                         _p(m.body_start, C.c_int), lc.shape[0], _p(lc), w1, h, len(beta), _p(beta),
+                                         amp, xw[0], xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), nthreads)
     return amas, bdmp, exfc[:, :len(beta)], info

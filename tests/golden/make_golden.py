@@ -13,6 +13,9 @@ Outputs (small .npz files, committed):
   incident.npz    DINP / VINP
   config1.npz     test/data/config1: input meshes and deck values, and the 18
                   expected_output tables (OAMASS1-6, ODAMPING1-6, OEXFOR1-6)
+  example5.npz    examples/example5_single_body (irregular-frequency removal):
+                  hull and waterplane meshes, the deck values and the 18 tables
+                  of Output/Hams_format
 """
 import ctypes as C
 import os
@@ -182,6 +185,20 @@ def main():
         headings_deg=np.array(ctl.headings_deg), xr=ctl.xr, lcs=ctl.lcs, isol=ctl.isol,
         **{f"xyz{b}": m.xyz for b, m in enumerate(meshes)}, **{f"ncn{b}": m.ncn for b, m in enumerate(meshes)},
         **{f"ncon{b}": m.ncon for b, m in enumerate(meshes)}, **tabs)
+    # ---- example 5: one body with a waterplane lid (If_remove_irr_freq = 1) ----
+    ex5 = "/root/reference/examples/example5_single_body"
+    hull = read_pnl(os.path.join(ex5, "Input", "HullMesh.pnl"))
+    lid = read_pnl(os.path.join(ex5, "Input", "WaterPlaneMesh.pnl"))
+    tabs5 = {}
+    for kind in ("OAMASS", "ODAMPING", "OEXFOR"):
+        for md in range(1, 7):
+            tabs5[f"{kind}{md}"] = read_hams_block_file(os.path.join(ex5, "Output", "Hams_format", f"{kind}{md}.txt"))
+    # ControlFile.in: depth 70, 10 frequencies 0.05 + 0.01 k rad/s (INFT 3),
+    # 36 headings 10 k deg, reference point 0, Wave_diffrac_solution 1
+    np.savez_compressed(
+        os.path.join(HERE, "example5.npz"), depth=70.0, omegas=0.05 + 0.01 * np.arange(10),
+        headings_deg=10.0 * np.arange(36), xr=np.zeros(3), isol=1, hull_xyz=hull.xyz, hull_ncn=hull.ncn,
+        hull_ncon=hull.ncon, lid_xyz=lid.xyz, lid_ncn=lid.ncn, lid_ncon=lid.ncon, **tabs5)
     print("golden fixtures written to", HERE)
 
 
