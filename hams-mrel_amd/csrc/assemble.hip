@@ -22,6 +22,7 @@ __device__ __forceinline__ int near_flag(double xi, double yi, double zi, double
   return dist <= 50.0 * pn ? 1 : 0;
 }
 
+// n: the table stride (field points)
 __device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int js, int jf) {
   const double* t = f.tab;
   const size_t mn = (size_t)kModes * n;
@@ -34,20 +35,20 @@ __device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int j
 // the per-mode scalars, and thread (0, 0) N_0.
 __global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
 #pragma clang fp contract(off)
-  const int panel = blockIdx.x * 256 + threadIdx.x, mode = blockIdx.y, n = m.n;
+  const int panel = blockIdx.x * 256 + threadIdx.x, mode = blockIdx.y, n = m.nf;
   if (panel >= n) return;
   double* t = f.tab;
   const size_t mn = (size_t)kModes * n;
   double* p = t + kTabHead;
   double cz, sz, ct, st;
-  mode_panel(mode, f.wvn, f.h, m.cz[panel], cz, sz, ct, st);
+  mode_panel(mode, f.wvn, f.h, m.fz[panel], cz, sz, ct, st);
   const size_t o = (size_t)mode * n + panel;
   p[o] = cz;
   p[mn + o] = sz;
   p[2 * mn + o] = ct;
   p[3 * mn + o] = st;
   if (mode == 0) {
-    const double z = m.cz[panel];
+    const double z = m.fz[panel];
     p[4 * mn + panel] = cosh(f.wk * (z + f.h));
     p[4 * mn + n + panel] = sinh(f.wk * (z + f.h));
   }
@@ -66,9 +67,9 @@ __global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
 template <bool DEEP, int REG>
 __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, int i, int j, double2* __restrict__ A,
                                            double2* __restrict__ S, int* flags) {
-  const int n = m.n;
+  const int n = m.n, nf = m.nf;
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
-  const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+  const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
   const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
   double sij = 0.0, dij[3] = {0.0, 0.0, 0.0};
   if (flag) d_newman(m.frame + (size_t)kFrameLen * j, m.ncn[j], xi, yi, zi, sij, dij);
@@ -78,7 +79,7 @@ __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, i
   if (DEEP) {
     d_infgreen(p, f.v, flag, g);
   } else {
-    const FreqParams fp = freq_params(f, n, j, i);
+    const FreqParams fp = freq_params(f, nf, j, i);
     d_fingreen_r<REG>(p, fp, flag, g, &ovf);
   }
   if (ovf) atomicOr(flags, 1);
@@ -91,8 +92,10 @@ __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, i
     t = (g[1] * n1 + g[2] * n2 + g[3] * n3) * ear;
     s = g[0] * ear;
   }
-  if (i == j) t.r = 2.0 * kPi + t.r;  // AMAT(IEL,IEL) = 2*PI before the sum (AssbMatxMulti.f90:72-74)
-  const size_t o = (size_t)i + (size_t)n * j;
+  // AMAT(IEL,IEL) = 2*PI before the sum (AssbMatxMulti.f90:72-74); lid rows have
+  // none (AssbMatx_irrMulti.f90:87-89)
+  if (i == j) t.r = 2.0 * kPi + t.r;
+  const size_t o = (size_t)i + (size_t)nf * j;
   A[o] = make_double2(t.r, t.i);
   S[o] = make_double2(s.r, s.i);
 }
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2*
   const int lane = threadIdx.x & 63;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
   const int i = blockIdx.x * 64 + lane;
-  if (j >= m.n || i >= m.n) return;
+  if (j >= m.n || i >= m.nf) return;
   pair_entry<DEEP, kRegAny>(m, f, i, j, A, S, flags);
 }
 
@@ -123,7 +126,7 @@ constexpr unsigned kNoPair = 0xFFFFFFFFu;
 
 __device__ __forceinline__ int pair_key(const MeshDev& m, double h, int i, int j) {
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
-  const double xi = m.cx[i], yi = m.cy[i], zi = m.cz[i];
+  const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
   const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
   PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
   return 2 * fin_subkey(p.rr, h) + flag;
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(256) void k_bin_count(MeshDev m, double h, int* __r
   hist[threadIdx.x] = 0;
   __syncthreads();
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (i < m.n && j < m.n) atomicAdd(&hist[pair_key(m, h, i, j)], 1);
+  if (i < m.nf && j < m.n) atomicAdd(&hist[pair_key(m, h, i, j)], 1);
   __syncthreads();
   if (hist[threadIdx.x]) atomicAdd(&count[threadIdx.x], hist[threadIdx.x]);
 }
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshDev m, double h, int* _
   hist[threadIdx.x] = 0;
   __syncthreads();
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const bool live = i < m.n && j < m.n;
+  const bool live = i < m.nf && j < m.n;
   int key = 0, rank = 0;
   if (live) {
     key = pair_key(m, h, i, j);
@@ -247,13 +250,14 @@ __global__ void k_dum2(MeshDev m, FreqDev f, WaveDev w, double2* dum2) {
 // panels (all in one body): 6 radiation columns of that body (n_1..n_6,
 // ASSB_RBC_MULTI) and nbeta diffraction columns (-dum2, ASSB_DBC_MULTI).
 constexpr int kBetaGroup = 8;
-__global__ __launch_bounds__(64) void k_rhs_partial(int n, const double* __restrict__ nrm, const double2* __restrict__ S,
-                                                    const double2* __restrict__ dum2, int nbeta, const int* chunk_j0,
-                                                    const int* chunk_j1, double2* __restrict__ partial) {
+__global__ __launch_bounds__(64) void k_rhs_partial(int n, int nf, const double* __restrict__ nrm,
+                                                    const double2* __restrict__ S, const double2* __restrict__ dum2,
+                                                    int nbeta, const int* chunk_j0, const int* chunk_j1,
+                                                    double2* __restrict__ partial) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   const int c = blockIdx.y;
   const int g0 = blockIdx.z * kBetaGroup;
-  if (i >= n) return;
+  if (i >= nf) return;
   const int j0 = chunk_j0[c], j1 = chunk_j1[c];
   const int nslot = 6 + nbeta;
   cd rad[6], dif[kBetaGroup];
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(64) void k_rhs_partial(int n, const double* __restr
   for (int k = 0; k < kBetaGroup; k++) dif[k] = mk(0.0);
   const int ng = min(kBetaGroup, nbeta - g0);
   for (int j = j0; j < j1; j++) {
-    double2 sv = S[(size_t)i + (size_t)n * j];
+    double2 sv = S[(size_t)i + (size_t)nf * j];
     cd s{sv.x, sv.y};
     if (blockIdx.z == 0)
       for (int k = 0; k < 6; k++) rad[k] = rad[k] + s * nrm[(size_t)k * n + j];
@@ -270,14 +274,14 @@ __global__ __launch_bounds__(64) void k_rhs_partial(int n, const double* __restr
       dif[b] = dif[b] - s * cd{dv.x, dv.y};
     }
   }
-  double2* out = partial + (size_t)c * nslot * n;
+  double2* out = partial + (size_t)c * nslot * nf;
   if (blockIdx.z == 0)
-    for (int k = 0; k < 6; k++) out[(size_t)k * n + i] = make_double2(rad[k].r, rad[k].i);
-  for (int b = 0; b < ng; b++) out[(size_t)(6 + g0 + b) * n + i] = make_double2(dif[b].r, dif[b].i);
+    for (int k = 0; k < 6; k++) out[(size_t)k * nf + i] = make_double2(rad[k].r, rad[k].i);
+  for (int b = 0; b < ng; b++) out[(size_t)(6 + g0 + b) * nf + i] = make_double2(dif[b].r, dif[b].i);
 }
 
 // Sum the chunk partials in chunk order: X(:, 6b+m) over the chunks of body b,
-// X(:, 6nb+beta) over all chunks.
+// X(:, 6nb+beta) over all chunks (n = rows = leading dimension).
 __global__ void k_rhs_reduce(int n, int nbody, int nbeta, int nchunk, const int* chunk_body,
                              const double2* __restrict__ partial, double2* __restrict__ X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,7 +368,7 @@ __global__ void k_clear_flag(int* flags) { flags[0] = 0; }
 namespace launch {
 
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags) {
-  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
   if (f.h < 0.0)
     HX_LAUNCH(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
@@ -373,16 +377,16 @@ void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, dou
 }
 
 void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f) {
-  HX_LAUNCH(k_mode_tables, dim3((m.n + 255) / 256, kModes), dim3(256), 0, s, m, f);
+  HX_LAUNCH(k_mode_tables, dim3((m.nf + 255) / 256, kModes), dim3(256), 0, s, m, f);
 }
 
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count) {
-  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_bin_count, grid, dim3(256), 0, s, m, h, count);
 }
 
 void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list) {
-  dim3 grid((m.n + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_bin_scatter, grid, dim3(256), 0, s, m, h, cursor, list);
 }
 
@@ -416,10 +420,10 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
   const int n = m.n;
   if (w.nbeta > 0) HX_LAUNCH(k_dum2, dim3((n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, dum2);
   int ngroups = w.nbeta > 0 ? (w.nbeta + kBetaGroup - 1) / kBetaGroup : 1;
-  HX_LAUNCH(k_rhs_partial, dim3((n + 63) / 64, nchunk, ngroups), dim3(64), 0, s, n, m.nrm, S, dum2,
-                     w.nbeta, chunk_j0_d, chunk_j1_d, partial);
-  HX_LAUNCH(k_rhs_reduce, dim3((n + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, n, nbody, w.nbeta,
-                     nchunk, chunk_body_d, partial, X);
+  HX_LAUNCH(k_rhs_partial, dim3((m.nf + 63) / 64, nchunk, ngroups), dim3(64), 0, s, n, m.nf, m.nrm, S, dum2,
+            w.nbeta, chunk_j0_d, chunk_j1_d, partial);
+  HX_LAUNCH(k_rhs_reduce, dim3((m.nf + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, m.nf, nbody, w.nbeta,
+            nchunk, chunk_body_d, partial, X);
 }
 
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,

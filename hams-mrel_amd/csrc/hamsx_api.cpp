@@ -19,6 +19,8 @@
 #include <complex>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "hamsx_internal.hpp"
 
 namespace hamsx {
@@ -210,8 +212,12 @@ struct hamsx_ctx {
   int solver = HAMSX_SOLVER_LU;
   int gm_restart = 25, gm_maxiter = 100;
   double gm_tol = 1.0e-4;
-  // mesh
-  int n = 0, nbody = 0, nchunk = 0;
+  // mesh; with a lid (irregular-frequency removal) the nf = n + nlid field
+  // points are the hull centroids followed by the lid centroids
+  int n = 0, nbody = 0, nchunk = 0, nlid = 0, nf = 0;
+  DevBuf<double> fx, fy, fz;
+  DevBuf<double2> Cm, Xe;  // with a lid: C = A^T A (n x n) and the extended right-hand sides (nf x nrhs)
+  rocblas_handle blas = nullptr;
   std::vector<int> body_start;
   DevBuf<double> cx, cy, cz, area, pnsz, nrm, frame;
   DevBuf<int> ncn, body, body_start_d, chunk_body, chunk_j0, chunk_j1;
@@ -238,12 +244,13 @@ struct hamsx_ctx {
   DevBuf<int> pPerm, pIpiv;
   std::vector<size_t> pOff;
   DevBuf<int> bin_cnt;
-  int bin_n = 0;
+  int bin_n = 0, bin_nf = 0;
   double bin_h = 0.0;
   size_t reg_begin[5] = {0}, reg_end[5] = {0};
   DevBuf<double> out, norms;
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   bool pending = false;  // a submitted frequency solve not yet collected
+  bool have_C = false;   // C = A^T A is current (lid only)
   int last_iters = 0;
   double last_resid = 0.0;
   // stage timing (measurement hook)
@@ -258,7 +265,7 @@ struct hamsx_ctx {
   int stage_cnt[HAMSX_NSTAGE] = {0};
 
   hamsx::MeshDev mesh() const {
-    return hamsx::MeshDev{n, cx.p, cy.p, cz.p, area.p, pnsz.p, nrm.p, ncn.p, frame.p, body.p};
+    return hamsx::MeshDev{n, nf, fx.p, fy.p, fz.p, cx.p, cy.p, cz.p, area.p, pnsz.p, nrm.p, ncn.p, frame.p, body.p};
   }
   hamsx::WaveDev waves() const { return hamsx::WaveDev{nbeta, beta.p, amp, xw0, xw1, rho, isol}; }
   int nrhs() const { return 6 * nbody + nbeta; }
@@ -345,7 +352,7 @@ bool force_grid_assembly() {
 int ensure_bins(hamsx_ctx* c) {
   using namespace hamsx;
   const double h = c->freq.h;
-  if (c->bin_n == c->n && c->bin_h == h) return 0;
+  if (c->bin_n == c->n && c->bin_nf == c->nf && c->bin_h == h) return 0;
   c->bin_n = 0;
   int rc;
   if ((rc = c->bin_cnt.ensure(kFinBins))) return rc;
@@ -377,6 +384,7 @@ int ensure_bins(hamsx_ctx* c) {
   if ((rc = check_launch())) return rc;
   HX_TRY(hipStreamSynchronize(c->stream));  // start[] is read by the copy above
   c->bin_n = c->n;
+  c->bin_nf = c->nf;
   c->bin_h = h;
   return 0;
 }
@@ -386,7 +394,8 @@ int ensure_work(hamsx_ctx* c) {
   int rc;
   if ((rc = c->X.ensure(n * nr)) || (rc = c->Xtmp.ensure(n * nr)) ||
       (rc = c->dum2.ensure(n * std::max(1, c->nbeta))) ||
-      (rc = c->partial.ensure((size_t)c->nchunk * (6 + c->nbeta) * n)) || (rc = c->out.ensure(c->out_len() + 8)))
+      (rc = c->partial.ensure((size_t)c->nchunk * (6 + c->nbeta) * std::max<size_t>(n, c->nf))) ||
+      (rc = c->out.ensure(c->out_len() + 8)))
     return rc;
   if (c->prec == HAMSX_PREC_MIXED) {
     if ((rc = c->B.ensure(n * nr)) || (rc = c->R.ensure(n * nr)) || (rc = c->D.ensure(n * nr)) ||
@@ -397,17 +406,58 @@ int ensure_work(hamsx_ctx* c) {
   return 0;
 }
 
+int blas_handle(hamsx_ctx* c) {
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return fail(HAMSX_EDEVICE, "rocblas_create_handle failed");
+  }
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+    return fail(HAMSX_EDEVICE, "rocblas_set_stream failed");
+  return 0;
+}
+
 int do_rhs(hamsx_ctx* c) {
   if (!c->have_S) return fail(HAMSX_ESTATE, "right-hand sides need hamsx_assemble first");
   if (!c->have_waves) return fail(HAMSX_ESTATE, "right-hand sides need hamsx_set_waves first");
   int rc = ensure_work(c);
   if (rc) return rc;
   StageTimer tm(c, 1);
+  const bool lid = c->nlid > 0;
+  if (lid && (rc = c->Xe.ensure((size_t)c->nf * c->nrhs()))) return rc;
   hamsx::launch::rhs(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start.data(), c->body_start_d.p,
                      c->S.p, c->dum2.p, c->partial.p, c->nchunk, c->chunk_body.p, c->chunk_j0.p, c->chunk_j1.p,
-                     c->X.p);
-  return check_launch();
+                     lid ? c->Xe.p : c->X.p);
+  if ((rc = check_launch())) return rc;
+  if (lid) {  // D = A^T B, plain transpose (ASSB_RBC/DBC_IRR_MULTI, AssbMatx_irrMulti.f90:300-316, :447-461)
+    if ((rc = blas_handle(c))) return rc;
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    if (rocblas_zgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->n, c->nrhs(), c->nf, &one,
+                      reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf,
+                      reinterpret_cast<const rocblas_double_complex*>(c->Xe.p), c->nf, &zero,
+                      reinterpret_cast<rocblas_double_complex*>(c->X.p), c->n) != rocblas_status_success)
+      return fail(HAMSX_EDEVICE, "rocblas_zgemm (A^T B) failed");
+  }
+  return 0;
 }
+
+// With a lid: C = A^T A (ASSB_LEFT_IRR_MULTI, AssbMatx_irrMulti.f90:154-171),
+// the matrix the LU, the refinement and GMRES then work on.
+int ensure_normal_matrix(hamsx_ctx* c) {
+  if (c->nlid == 0 || c->have_C) return 0;
+  int rc;
+  if ((rc = c->Cm.ensure((size_t)c->n * c->n)) || (rc = blas_handle(c))) return rc;
+  const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+  if (rocblas_zgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->n, c->n, c->nf, &one,
+                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf,
+                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf, &zero,
+                    reinterpret_cast<rocblas_double_complex*>(c->Cm.p), c->n) != rocblas_status_success)
+    return fail(HAMSX_EDEVICE, "rocblas_zgemm (A^T A) failed");
+  c->have_C = true;
+  return 0;
+}
+
+// the square matrix the solvers work on: A, or C = A^T A with a lid
+double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
 
 int do_factor(hamsx_ctx* c) {
   if (c->have_LU) return 0;
@@ -421,12 +471,14 @@ int do_factor(hamsx_ctx* c) {
       (rc = c->ludiag.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB)))
     return rc;
   StageTimer tm(c, 2);
+  if ((rc = ensure_normal_matrix(c))) return rc;
+  double2* M = system_matrix(c);
   if (c->prec == HAMSX_PREC_FP64) {
-    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
+    hamsx::launch::lu_factor(c->stream, M, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
                              c->ludiag.p);
-    c->have_A = false;  // factored in place
+    if (c->nlid == 0) c->have_A = false;  // factored in place (with a lid, C is factored and A kept)
   } else {
-    hamsx::launch::to_c64(c->stream, c->A.p, c->Af.p, (size_t)n * n);
+    hamsx::launch::to_c64(c->stream, M, c->Af.p, (size_t)n * n);
     hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
                                  c->flags.p + 1, c->ludiag.p);
   }
@@ -466,7 +518,7 @@ int gmres_precond_setup(hamsx_ctx* c) {
   for (int b = 0; b < nb; b++) {
     const int r0 = c->body_start[b], sz = c->body_start[b + 1] - r0;
     double2* blk = c->pLU.p + c->pOff[b];
-    hamsx::launch::block_copy(c->stream, c->A.p, c->n, r0, sz, blk);
+    hamsx::launch::block_copy(c->stream, system_matrix(c), c->n, r0, sz, blk);
     hamsx::launch::lu_factor(c->stream, blk, sz, c->pIpiv.p, c->pPerm.p + r0, c->widx.p, c->wcnt.p,
                              c->flags.p + 1, c->ludiag.p);
   }
@@ -510,6 +562,8 @@ int do_solve_gmres(hamsx_ctx* c) {
   size_t maxb = 0;
   for (int b = 0; b < c->nbody; b++) maxb = std::max(maxb, (size_t)(c->body_start[b + 1] - c->body_start[b]));
   if ((rc = c->gP.ensure(maxb * nr)) || (rc = c->gPt.ensure(maxb * nr))) return rc;
+  if ((rc = ensure_normal_matrix(c))) return rc;
+  const double2* Aop = system_matrix(c);
   if ((rc = gmres_precond_setup(c))) return rc;
   // B = right-hand sides, X = 0
   HX_TRY(hipMemcpyAsync(c->gB.p, c->X.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
@@ -530,7 +584,7 @@ int do_solve_gmres(hamsx_ctx* c) {
     if (iter == 0)
       HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
     else
-      hamsx::launch::matvec(c->stream, c->A.p, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+      hamsx::launch::matvec(c->stream, Aop, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
     hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
     HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
     HX_TRY(hipStreamSynchronize(c->stream));
@@ -555,7 +609,7 @@ int do_solve_gmres(hamsx_ctx* c) {
       iter++;
       double2* Vj = c->gV.p + blk * j;
       gmres_precond_apply(c, Vj, c->gZ.p, nr);
-      hamsx::launch::matvec(c->stream, c->A.p, n, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, c->mvw.p);
+      hamsx::launch::matvec(c->stream, Aop, n, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, c->mvw.p);
       double2* d1 = c->gDots.p;
       double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
       hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1);
@@ -632,7 +686,7 @@ int do_solve_gmres(hamsx_ctx* c) {
     HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above are reused next cycle
   }
   // final true residual for the statistics
-  hamsx::launch::matvec(c->stream, c->A.p, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+  hamsx::launch::matvec(c->stream, Aop, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
   hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
   HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
@@ -660,17 +714,17 @@ int do_solve(hamsx_ctx* c) {
   const int n = c->n, nr = c->nrhs();
   StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
-    hamsx::launch::lu_solve(c->stream, c->A.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
   } else {
     const size_t cnt = (size_t)n * nr;
     HX_TRY(hipMemcpyAsync(c->B.p, c->X.p, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
     hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
     for (int it = 0; it < c->refine_iters; it++) {
-      hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p);
+      hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p);
       hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr);
       hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt);
     }
-    hamsx::launch::residual(c->stream, c->A.p, n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p);
+    hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p);
     c->last_iters = c->refine_iters;
   }
   c->have_X = true;
@@ -794,6 +848,10 @@ int hamsx_destroy(hamsx_ctx* c) {
                   &c->perm, &c->widx, &c->wcnt, &c->flags, &c->bin_cnt})
     b->release();
   c->plist.release();
+  for (auto* b : {&c->fx, &c->fy, &c->fz}) b->release();
+  c->Cm.release();
+  c->Xe.release();
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   for (auto* b : {&c->gV, &c->gW, &c->gZ, &c->gU, &c->gB, &c->gP, &c->gPt, &c->gDots, &c->pLU}) b->release();
   c->gNorm.release();
   c->gScale.release();
@@ -1005,7 +1063,39 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   HX_TRY(hipMemcpy(c->chunk_body.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
   HX_TRY(hipMemcpy(c->chunk_j0.p, c0.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
   HX_TRY(hipMemcpy(c->chunk_j1.p, c1.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
-  c->have_A = c->have_S = c->have_LU = c->have_X = false;
+  c->nlid = 0;
+  c->nf = nelem;
+  if ((rc = c->fx.ensure(n)) || (rc = c->fy.ensure(n)) || (rc = c->fz.ensure(n))) return rc;
+  HX_TRY(hipMemcpy(c->fx.p, cen, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->fy.p, cen + n, sizeof(double) * n, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->fz.p, cen + 2 * n, sizeof(double) * n, hipMemcpyHostToDevice));
+  c->have_A = c->have_S = c->have_LU = c->have_X = c->have_C = false;
+  return 0;
+}
+
+int hamsx_set_lid(hamsx_ctx* c, int nlid, const double* lid_cen) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_lid needs hamsx_set_mesh first");
+  if (nlid < 0 || (nlid > 0 && !lid_cen)) return fail(HAMSX_EARG, "bad lid arguments");
+  if (c->n + nlid > 0xFFFF) return fail(HAMSX_EARG, "hull + lid panels exceed 65535");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  const size_t n = c->n, nf = n + nlid;
+  std::vector<double> f(3 * nf);
+  for (int d = 0; d < 3; d++) {
+    HX_TRY(hipMemcpy(f.data() + d * nf, (d == 0 ? c->cx.p : d == 1 ? c->cy.p : c->cz.p), sizeof(double) * n,
+                     hipMemcpyDeviceToHost));
+    for (int l = 0; l < nlid; l++) f[d * nf + n + l] = lid_cen[l + (size_t)nlid * d];
+  }
+  if ((rc = c->fx.ensure(nf)) || (rc = c->fy.ensure(nf)) || (rc = c->fz.ensure(nf))) return rc;
+  HX_TRY(hipMemcpy(c->fx.p, f.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->fy.p, f.data() + nf, sizeof(double) * nf, hipMemcpyHostToDevice));
+  HX_TRY(hipMemcpy(c->fz.p, f.data() + 2 * nf, sizeof(double) * nf, hipMemcpyHostToDevice));
+  c->nlid = nlid;
+  c->nf = (int)nf;
+  c->have_A = c->have_S = c->have_LU = c->have_X = c->have_C = false;
   return 0;
 }
 
@@ -1057,9 +1147,11 @@ int hamsx_assemble(hamsx_ctx* c) {
   int rc = set_device(c);
   if (rc) return rc;
   const bool finite = c->freq.h >= 0.0;
-  const bool binned = finite && c->n <= 0xFFFF && !force_grid_assembly();
+  const bool binned = finite && c->nf <= 0xFFFF && !force_grid_assembly();
+  if ((rc = c->A.ensure((size_t)c->nf * c->n)) || (rc = c->S.ensure((size_t)c->nf * c->n))) return rc;
   if (binned && (rc = ensure_bins(c))) return rc;
-  if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->n)))) return rc;
+  if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->nf)))) return rc;
+  c->have_C = false;
   c->freq.tab = finite ? c->modetab.p : nullptr;
   {
     StageTimer tm(c, 0);
@@ -1174,6 +1266,15 @@ int hamsx_get_matrix(hamsx_ctx* c, int which, double* out, int* ipiv) {
   const size_t nn = (size_t)c->n * c->n;
   if (which == 0 || which == 2) {
     if (which == 0 && !c->have_A) return fail(HAMSX_ESTATE, "A is not available (not assembled, or factored in place)");
+    if (which == 0 && c->nlid > 0) {  // with a lid: C = A^T A, the matrix that is factored
+      if (c->have_LU && c->prec == HAMSX_PREC_FP64)
+        return fail(HAMSX_ESTATE, "C is factored in place; read it before hamsx_factor");
+      int rc2 = ensure_normal_matrix(c);
+      if (rc2) return rc2;
+      HX_TRY(hipMemcpyAsync(out, c->Cm.p, sizeof(double2) * nn, hipMemcpyDeviceToHost, c->stream));
+      HX_TRY(hipStreamSynchronize(c->stream));
+      return 0;
+    }
     if (which == 2 && !c->have_LU) return fail(HAMSX_ESTATE, "no factorisation available");
     if (which == 2 && c->prec == HAMSX_PREC_MIXED) {
       std::vector<float2> tmp(nn);

@@ -22,7 +22,9 @@ constexpr int kSelRows = 256; // candidate rows per tournament block (one per th
 
 // Device-resident, frequency-invariant mesh (structure of arrays).
 struct MeshDev {
-  int n;
+  int n;                // hull panels (sources; field points 0..n-1)
+  int nf;               // field points: the n hull centroids, then the lid centroids (irregular-frequency removal)
+  const double *fx, *fy, *fz;  // field-point coordinates, nf each
   const double* cx;
   const double* cy;
   const double* cz;
@@ -83,6 +85,7 @@ void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int regi
                      size_t count, double2* A, double2* S, int* flags);
 void clear_flags(hipStream_t s, int* flags);
 // right-hand sides X(n, 6*nbody + nbeta) = S * generator columns
+// (rows: the m.nf field points; S and X have leading dimension m.nf)
 void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_h,
          const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
          const int* chunk_body_d, const int* chunk_j0_d, const int* chunk_j1_d, double2* X);

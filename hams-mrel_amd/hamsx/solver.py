@@ -73,6 +73,15 @@ class FrequencySolver:
         _lib.check(self._lib.hamsx_set_precision(self._h, mode), "hamsx_set_precision")
         self.precision = precision
 
+    def set_lid(self, lid_cen):
+        """Irregular-frequency removal: lid_cen (L x 3) are the lid (waterplane)
+        panel centroids; None or an empty array removes the lid."""
+        if lid_cen is None or len(lid_cen) == 0:
+            _lib.check(self._lib.hamsx_set_lid(self._h, 0, None), "hamsx_set_lid")
+            return
+        lc = np.asfortranarray(lid_cen, dtype=np.float64)
+        _lib.check(self._lib.hamsx_set_lid(self._h, lc.shape[0], _lib.ptr(lc)), "hamsx_set_lid")
+
     def set_solver(self, solver: str = "lu", tol: float = 1.0e-4, restart: int = 25, maxiter: int = 100):
         """'lu' (ZGETRF/ZGETRS) or 'gmres' (restarted GMRES with the per-body
         block-diagonal LU preconditioner; defaults are the reference's

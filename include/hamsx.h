@@ -80,6 +80,16 @@ int hamsx_set_mesh(hamsx_ctx *ctx, int nelem, int nnode, const double *node_xyz,
                    const int *ncn, const int *ncon, const double *cen, const double *area,
                    const double *pnsz, const double *dxyz, int nbody, const int *body_start);
 
+/* Irregular-frequency removal (IRSP = 1): lid_cen[nlid*3] (column-major) are
+ * the centroids of the waterplane (lid) panels of all bodies, used as extra
+ * field points against the hull sources.  The solve then runs on the normal
+ * equations C = A^T A, D = A^T B of the (N + nlid) x N system (plain
+ * transposes), as CALGREEN_MULTI_IRR + ASSB_LEFT/RBC/DBC_IRR_MULTI +
+ * RADIATION/DIFFRACTION_SOLVER_IRR_MULTI (src/CalGreenFuncMulti.f90:189-318,
+ * src/AssbMatx_irrMulti.f90:53-585).  nlid = 0 removes the lid; a new
+ * hamsx_set_mesh also removes it.  hamsx_get_matrix(which = 0) then returns C. */
+int hamsx_set_lid(hamsx_ctx *ctx, int nlid, const double *lid_cen);
+
 /* ---- per-frequency state -------------------------------------------------
  * The scalars CalWaveProperts leaves in WaveDyn_mod (V, WK, W1, H, WVN). */
 int hamsx_set_frequency(hamsx_ctx *ctx, double v, double wk, double w1, double h,

@@ -240,3 +240,65 @@ def test_gmres_matches_lu(depth):
     for a, b, c in zip(tight[:3], lu[:3], loose[:3]):
         assert rel(a, b) < 1e-8
         assert rel(c, b) < 1e-3
+
+
+def lid_centroids(lid):
+    from hamsx.mesh import panel_geometry
+    cen, _, _, _ = panel_geometry(lid, np.zeros(3))
+    return cen
+
+
+@pytest.mark.parametrize("depth,w", [(-1.0, 1.6), (30.0, 1.2)])
+def test_irregular_frequency_removal_matches_oracle(depth, w):
+    """Hull + waterplane lid (IRSP = 1): the GPU normal-equation solve against
+    the oracle's restatement of the reference's IRR path."""
+    from hamsx.mesh import waterplane_disk
+    hull = cylinder(4.0, 6.0, 24, 6, 3)
+    m = combine([hull], [np.zeros(3)])
+    lid = lid_centroids(waterplane_disk(4.0, 24, 3))
+    heads = np.deg2rad([0.0, 60.0])
+    say(f"[gpu] irr n={m.nelem} lid={len(lid)} depth={depth} w={w}")
+    with FrequencySolver(m) as s:
+        s.set_lid(lid)
+        s.set_waves(heads)
+        am, bd, ex, info = s.solve_frequency(w, depth)
+        s.set_lid(None)
+        am0, _, _, _ = s.solve_frequency(w, depth)
+    ao, bo, eo, io = oracle.solve_frequency_irr(m, lid, w, depth, heads)
+    assert info == 0 and io == 0
+    errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+    floor = (0.0, 0.0, 0.0)
+    if depth > 0:
+        af, bf, ef, _ = oracle.solve_frequency_irr(m, lid, w, depth, heads, fma=True)
+        floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+    say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e}   floor {floor}; "
+        f"without lid {rel(am0, ao):.2e}")
+    for e, f in zip(errs, floor):
+        assert e <= max(1e-8, 4 * f)
+    assert rel(am0, ao) > 100 * errs[0]   # the lid changes the answer
+
+
+def test_irregular_frequency_example5_on_gpu():
+    """The reference's example 5 (234 hull + 383 lid panels, h = 70 m) at two
+    of its frequencies: GPU against the oracle."""
+    import os
+    from conftest import GOLDEN
+    from hamsx.deck import PanelMesh
+    z = np.load(os.path.join(GOLDEN, "example5.npz"))
+    hull = PanelMesh(z["hull_xyz"], z["hull_ncn"], z["hull_ncon"])
+    lid = lid_centroids(PanelMesh(z["lid_xyz"], z["lid_ncn"], z["lid_ncon"]))
+    m = combine([hull], [z["xr"]])
+    heads = np.deg2rad([0.0, 90.0])
+    say(f"[gpu] example5 irr n={m.nelem} lid={len(lid)}")
+    with FrequencySolver(m) as s:
+        s.set_lid(lid)
+        s.set_waves(heads)
+        for w in (float(z["omegas"][0]), float(z["omegas"][-1])):
+            am, bd, ex, info = s.solve_frequency(w, float(z["depth"]))
+            ao, bo, eo, _ = oracle.solve_frequency_irr(m, lid, w, float(z["depth"]), heads)
+            af, bf, ef, _ = oracle.solve_frequency_irr(m, lid, w, float(z["depth"]), heads, fma=True)
+            errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+            floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+            say(f"    w={w:.3f} rel err {errs} floor {floor}")
+            for e, f in zip(errs, floor):
+                assert e <= max(1e-8, 4 * f)
