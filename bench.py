@@ -77,22 +77,21 @@ def region_pairs(mesh, depth):
 
 
 def roofline(case, mesh, ms, precision):
-    """Roofline of the kernel with the largest device time per solve."""
+    """Roofline of the single kernel with the largest device time per solve.
+
+    Candidates are kernels launched once per solve: the four finite-depth
+    region kernels, or the deep-water grid assembly kernel.  The LU and the
+    triangular solves are chains of hundreds of short launches; they are
+    reported separately (roofline_lu)."""
     n = mesh.nelem
-    cands = [k for k in REGIONS + ("factor", "solve") if ms.get(k, 0.0) > 0.0]
+    if case.depth < 0:
+        nbytes = 32.0 * n * n
+        t = ms["assemble"]
+        return {"kernel": "k_assemble<deep> (Havelock-fit Green function + Newman + A/S entries, one thread per "
+                          "pair)", "bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "launch_ms": t,
+                "pairs": n * n, "algorithmic_bytes": nbytes, "achieved": None, "frac": None, "traffic": None}
+    cands = [k for k in REGIONS if ms.get(k, 0.0) > 0.0]
     dom = max(cands, key=lambda k: ms[k])
-    if dom == "factor":
-        flops = 8.0 * n ** 3 / 3.0
-        peak = PEAK_FP64_TFLOPS if precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
-        return {"kernel": "LU factorisation (tournament-pivoted panels + MFMA trailing updates)", "bound": "mfma",
-                "achieved": flops / (ms[dom] * 1e-3) / 1e12, "peak": peak, "unit": "TFLOP/s",
-                "frac": flops / (ms[dom] * 1e-3) / 1e12 / peak, "traffic": None, "launch_ms": ms[dom],
-                "work": "8 N^3 / 3 real flops"}
-    if dom == "solve":
-        nbytes = 16.0 * n * n
-        return {"kernel": "triangular solves", "bound": "hbm", "achieved": nbytes / (ms[dom] * 1e-3) / 1e9,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": nbytes / (ms[dom] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                "traffic": None, "launch_ms": ms[dom], "work": "16 N^2 bytes (the factors read once)"}
     pairs = region_pairs(mesh, case.depth)[dom]
     cnt = REGION_COUNTERS.get(case.name, {}).get(dom)
     out = {"kernel": f"k_assemble_list<{dom[4:].upper()}> (finite-depth Green function region + Newman + A/S "
