@@ -20,6 +20,10 @@
 
 #include "hamsx_internal.hpp"
 
+// wave-wide maximum from the ROCm device library (DPP reduction, result in
+// every lane)
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_max_f64(double);
+
 namespace hamsx {
 
 template <typename R>
@@ -64,14 +68,24 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // [row0 + b*kSelRows, ...).  Level > 0: block b merges candidate sets
 // [b*kMerge, b*kMerge + kMerge) of the previous level.  Each block runs
 // partial pivoting on its candidates' panel entries (pivot = largest
-// |re| + |im| as IZAMAX, ties to the first candidate), each candidate's NB
-// entries in registers: the argmax is a wave shuffle reduction plus one LDS
-// exchange, the pivot row is broadcast through LDS, and a retired pivot row
-// keeps its L entries (left of its step) and U entries.  Outputs the chosen
+// |re| + |im| as IZAMAX, ties to the first candidate).  Outputs the chosen
 // rows in pivot order; the final level (diag != nullptr) also writes the
 // factored NB x NB diagonal block L\U in pivot order and ZGETRF's INFO.
+//
+// Each candidate's NB entries live in registers as a rotating window: at
+// step c, slot k holds column c + k (k < NB - c) and the slots behind them
+// hold the multipliers of steps 0..c-1.  Every step rotates the window by
+// one and appends its multiplier, so the pivot column is always slot 0 and
+// the step loop stays rolled.  One barrier per step: each wave finds its own
+// largest |re| + |im| (a DPP wave reduction) and its first lane holding it
+// (a ballot), and that lane publishes its value, row number and window
+// (retired slots zeroed) to LDS; after the barrier every thread picks the
+// winner among the four waves and reads the pivot row from LDS.  Buffers
+// alternate between steps, so a step's writes never meet the previous
+// step's reads.
 // ---------------------------------------------------------------------------
 constexpr int kMerge = kSelRows / kPanelNB;
+static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
 template <typename R>
 __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
                                                 int ncols, int row0, const int* __restrict__ idx_in,
@@ -80,9 +94,9 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
                                                 int* __restrict__ info) {
   __shared__ int sidx[kSelRows];
   __shared__ int off[kMerge + 1];
-  __shared__ double redv[2][4];
-  __shared__ int redi[2][4];
-  __shared__ double prr[2][kPanelNB], pri[2][kPanelNB], rcr[2], rci[2];
+  __shared__ double wmax[2][4];
+  __shared__ int wrow[2][4];
+  __shared__ double2 wwin[2][4][kPanelNB];  // each wave's candidate pivot window
   __shared__ int sel[kPanelNB];
   const int t = threadIdx.x, b = blockIdx.x;
   int mc;
@@ -112,7 +126,7 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
   double vr[kPanelNB], vi[kPanelNB];  // computed in FP64 for either factor type
   {
     // unconditional loads at clamped, in-bounds addresses (all issued before
-    // the first use), masked afterwards
+    // the first use), masked afterwards; columns >= ncols are zero
     const size_t g = (size_t)sidx[active ? t : 0];
     typename CT<R>::c2 v[kPanelNB];
 #pragma unroll
@@ -126,73 +140,75 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
   }
   const int nsel = min(ncols, mc);
   int mystep = -1, zero_at = 0;
+  const int wave = t >> 6, lane = t & 63;
+#pragma unroll 1
+  for (int c = 0; c < nsel; c++) {
+    const int buf = c & 1;
+    const double m = active ? fabs(vr[0]) + fabs(vi[0]) : -1.0;
+    const double mw = __ockl_wfred_max_f64(m);
+    // ties go to the lowest row; with NaN in every lane no lane matches and
+    // lane 0 answers, so each wave always publishes a valid row
+    const unsigned long long hit = __ballot(m == mw);
+    const int first = hit ? __builtin_ctzll(hit) : 0;
+    if (lane == first) {
+      const int live = kPanelNB - c;  // slots holding columns c..NB-1
 #pragma unroll
-  for (int c = 0; c < kPanelNB; c++) {
-    if (c < nsel) {
-      const int buf = c & 1;
-      double m = active ? fabs(vr[c]) + fabs(vi[c]) : -1.0;
-      int bi = active ? t : 0x7fffffff;
-      for (int o = 32; o > 0; o >>= 1) {
-        const double om = __shfl_xor(m, o);
-        const int oi = __shfl_xor(bi, o);
-        if (om > m || (om == m && oi < bi)) {
-          m = om;
-          bi = oi;
-        }
-      }
-      if ((t & 63) == 0) {
-        redv[buf][t >> 6] = m;
-        redi[buf][t >> 6] = bi;
-      }
-      __syncthreads();
-      double best = redv[buf][0];
-      int p = redi[buf][0];
-      for (int w = 1; w < 4; w++)
-        if (redv[buf][w] > best || (redv[buf][w] == best && redi[buf][w] < p)) {
-          best = redv[buf][w];
-          p = redi[buf][w];
-        }
-      const bool nonzero = best > 0.0;
-      if (t == p) {
-#pragma unroll
-        for (int q = 0; q < kPanelNB; q++)
-          if (q >= c) {
-            prr[buf][q] = vr[q];
-            pri[buf][q] = vi[q];
-          }
-        double a = 0.0, bb = 0.0;
-        if (nonzero) crecip(vr[c], vi[c], a, bb);
-        rcr[buf] = a;
-        rci[buf] = bb;
-        sel[c] = t;
-        mystep = c;
-        active = false;
-      }
-      __syncthreads();
-      if (active && nonzero) {
-        const double ir = rcr[buf], ii = rci[buf];
-        const double lr = vr[c] * ir - vi[c] * ii, li = vr[c] * ii + vi[c] * ir;
-        vr[c] = lr;
-        vi[c] = li;
-#pragma unroll
-        for (int q = 0; q < kPanelNB; q++)
-          if (q > c && q < ncols) {
-            const double ur = prr[buf][q], ui = pri[buf][q];
-            vr[q] -= lr * ur - li * ui;
-            vi[q] -= lr * ui + li * ur;
-          }
-      }
-      if (!nonzero && zero_at == 0) zero_at = c + 1;
+      for (int k = 0; k < kPanelNB; k++)
+        wwin[buf][wave][k] = k < live ? make_double2(vr[k], vi[k]) : make_double2(0.0, 0.0);
+      wmax[buf][wave] = mw;
+      wrow[buf][wave] = t;
     }
+    __syncthreads();
+    double best = wmax[buf][0];
+    int w = 0;
+    for (int q = 1; q < 4; q++)
+      if (wmax[buf][q] > best) {  // equal maxima keep the lower wave, i.e. the lower row
+        best = wmax[buf][q];
+        w = q;
+      }
+    const int p = wrow[buf][w];
+    const bool nonzero = best > 0.0;
+    if (t == p) {
+      sel[c] = t;
+      mystep = c;
+      active = false;
+    }
+    if (active) {
+      double lr = vr[0], li = vi[0];
+      if (nonzero) {
+        const double2 pv = wwin[buf][w][0];
+        double ir = 0.0, ii = 0.0;
+        crecip(pv.x, pv.y, ir, ii);
+        lr = vr[0] * ir - vi[0] * ii;
+        li = vr[0] * ii + vi[0] * ir;
+#pragma unroll
+        for (int k = 1; k < kPanelNB; k++) {
+          const double2 u = wwin[buf][w][k];
+          vr[k] -= lr * u.x - li * u.y;
+          vi[k] -= lr * u.y + li * u.x;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kPanelNB - 1; k++) {
+        vr[k] = vr[k + 1];
+        vi[k] = vi[k + 1];
+      }
+      vr[kPanelNB - 1] = lr;
+      vi[kPanelNB - 1] = li;
+    }
+    if (!nonzero && zero_at == 0) zero_at = c + 1;
   }
   __syncthreads();
   if (t < nsel) idx_out[b * kPanelNB + t] = sidx[sel[t]];
   if (t == 0) cnt_out[b] = nsel;
   if (diag != nullptr) {
+    // the pivot row of step s holds column (k + s) mod NB in slot k
     if (mystep >= 0)
 #pragma unroll
-      for (int q = 0; q < kPanelNB; q++)
-        if (q < ncols) diag[mystep + kPanelNB * q] = {(R)vr[q], (R)vi[q]};
+      for (int k = 0; k < kPanelNB; k++) {
+        const int q = (k + mystep) & (kPanelNB - 1);
+        if (q < ncols) diag[mystep + kPanelNB * q] = {(R)vr[k], (R)vi[k]};
+      }
     if (t == 0 && zero_at && info[0] == 0) info[0] = col0 + zero_at;
   }
 }
