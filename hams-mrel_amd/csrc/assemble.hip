@@ -62,27 +62,11 @@ __global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
   }
 }
 
-// One pair: the Green function (deep water, or finite depth in region REG),
-// the Newman integrals when FLAG = 1, and the entries A(i,j), S(i,j).
-template <bool DEEP, int REG>
-__device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, int i, int j, double2* __restrict__ A,
-                                           double2* __restrict__ S, int* flags) {
+// A(i,j) and S(i,j) from the Green function g and, when FLAG = 1, the
+// Newman integrals of source panel j (BODINT_MULTI_LEFT / RBC_MULTI_RIGHT).
+__device__ __forceinline__ void store_entry(const MeshDev& m, int i, int j, int flag, double sij, const double dij[3],
+                                            const cd g[4], double2* __restrict__ A, double2* __restrict__ S) {
   const int n = m.n, nf = m.nf;
-  const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
-  const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
-  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
-  double sij = 0.0, dij[3] = {0.0, 0.0, 0.0};
-  if (flag) d_newman(m.frame + (size_t)kFrameLen * j, m.ncn[j], xi, yi, zi, sij, dij);
-  PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
-  cd g[4];
-  int ovf = 0;
-  if (DEEP) {
-    d_infgreen(p, f.v, flag, g);
-  } else {
-    const FreqParams fp = freq_params(f, nf, j, i);
-    d_fingreen_r<REG>(p, fp, flag, g, &ovf);
-  }
-  if (ovf) atomicOr(flags, 1);
   const double ear = m.area[j], n1 = m.nrm[j], n2 = m.nrm[n + j], n3 = m.nrm[2 * n + j];
   cd t, s;
   if (flag) {
@@ -98,6 +82,53 @@ __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, i
   const size_t o = (size_t)i + (size_t)nf * j;
   A[o] = make_double2(t.r, t.i);
   S[o] = make_double2(s.r, s.i);
+}
+
+// One pair: the Green function (deep water, or finite depth in region REG),
+// the Newman integrals when FLAG = 1, and the entries A(i,j), S(i,j).
+template <bool DEEP, int REG>
+__device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, int i, int j, double2* __restrict__ A,
+                                           double2* __restrict__ S, int* flags) {
+  const int nf = m.nf;
+  const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
+  const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
+  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
+  double sij = 0.0, dij[3] = {0.0, 0.0, 0.0};
+  if (flag) d_newman(m.frame + (size_t)kFrameLen * j, m.ncn[j], xi, yi, zi, sij, dij);
+  PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
+  cd g[4];
+  int ovf = 0;
+  if (DEEP) {
+    d_infgreen(p, f.v, flag, g);
+  } else {
+    const FreqParams fp = freq_params(f, nf, j, i);
+    d_fingreen_r<REG>(p, fp, flag, g, &ovf);
+  }
+  if (ovf) atomicOr(flags, 1);
+  store_entry(m, i, j, flag, sij, dij, g, A, S);
+}
+
+// Both orientations of hull pair {a, b} in region REG (PIDCOCK, EIGENE or
+// EIGEN): (field a, source b) and (field b, source a), the Bessel functions
+// of their common R evaluated once.  Each entry equals what pair_entry
+// gives for it.  Hull field point k is hull centroid k.
+template <int REG>
+__device__ __forceinline__ void pair_entry2(const MeshDev& m, const FreqDev& f, int a, int b,
+                                            double2* __restrict__ A, double2* __restrict__ S) {
+  const int nf = m.nf;
+  const double xa = m.fx[a], ya = m.fy[a], za = m.fz[a], xb = m.fx[b], yb = m.fy[b], zb = m.fz[b];
+  const double sxa = m.cx[a], sya = m.cy[a], sza = m.cz[a], sxb = m.cx[b], syb = m.cy[b], szb = m.cz[b];
+  const int flagA = near_flag(xa, ya, za, sxb, syb, szb, m.pnsz[b]);
+  const int flagB = near_flag(xb, yb, zb, sxa, sya, sza, m.pnsz[a]);
+  double sA = 0.0, dA[3] = {0.0, 0.0, 0.0}, sB = 0.0, dB[3] = {0.0, 0.0, 0.0};
+  if (flagA) d_newman(m.frame + (size_t)kFrameLen * b, m.ncn[b], xa, ya, za, sA, dA);
+  if (flagB) d_newman(m.frame + (size_t)kFrameLen * a, m.ncn[a], xb, yb, zb, sB, dB);
+  const PairGeom pA = pair_geom(sxb, xa, syb, ya, szb, za), pB = pair_geom(sxa, xb, sya, yb, sza, zb);
+  const FreqParams fpA = freq_params(f, nf, b, a), fpB = freq_params(f, nf, a, b);
+  cd gA[4], gB[4];
+  d_fingreen2<REG>(pA, pB, fpA, fpB, flagA, flagB, gA, gB);
+  store_entry(m, a, b, flagA, sA, dA, gA, A, S);
+  store_entry(m, b, a, flagB, sB, dB, gB, A, S);
 }
 
 // Grid form: a 256-thread block is four wave64s; each wave owns one source
@@ -117,48 +148,61 @@ __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2*
 
 // ---------------------------------------------------------------------------
 // Finite depth, binned form.  The pairs are counting-sorted once per
-// (mesh, h) by key = 2 * fin_subkey + FLAG into a list of packed (j << 16 | i)
-// indices, each bin padded to a multiple of 64 with kNoPair, so every wave
-// runs one region with one series length and one FLAG.  One kernel per
-// region walks its contiguous slice of the list.
+// (mesh, h) by the keys of hamsx_internal.hpp into a list of packed
+// (j << 16 | i) indices -- for a hull pair entry i = a < j = b -- each bin
+// padded to a multiple of 64 with kNoPair, so every wave runs one region
+// with one series length and one FLAG pattern.  Per region one kernel walks
+// the single entries and one the hull-pair entries.
 // ---------------------------------------------------------------------------
 constexpr unsigned kNoPair = 0xFFFFFFFFu;
 
+// key of grid position (field i, source j), or -1 when the pair is listed
+// under its hull partner (j, i)
 __device__ __forceinline__ int pair_key(const MeshDev& m, double h, int i, int j) {
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
   const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
   const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
   PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
-  return 2 * fin_subkey(p.rr, h) + flag;
+  const int sub = fin_subkey(p.rr, h);
+  if (sub >= kSubPidcock && i < m.n && i != j) {
+    if (i > j) return -1;
+    const int flag_ba = near_flag(m.fx[j], m.fy[j], m.fz[j], m.cx[i], m.cy[i], m.cz[i], m.pnsz[i]);
+    return kPairKey0 + 4 * (sub - kSubPidcock) + 2 * flag + flag_ba;
+  }
+  return 2 * sub + flag;
 }
 
 __global__ __launch_bounds__(256) void k_bin_count(MeshDev m, double h, int* __restrict__ count) {
   __shared__ int hist[kFinBins];
-  hist[threadIdx.x] = 0;
+  for (int k = threadIdx.x; k < kFinBins; k += 256) hist[k] = 0;
   __syncthreads();
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
-  if (i < m.nf && j < m.n) atomicAdd(&hist[pair_key(m, h, i, j)], 1);
+  if (i < m.nf && j < m.n) {
+    const int key = pair_key(m, h, i, j);
+    if (key >= 0) atomicAdd(&hist[key], 1);
+  }
   __syncthreads();
-  if (hist[threadIdx.x]) atomicAdd(&count[threadIdx.x], hist[threadIdx.x]);
+  for (int k = threadIdx.x; k < kFinBins; k += 256)
+    if (hist[k]) atomicAdd(&count[k], hist[k]);
 }
 
 // cursor[] starts at each bin's (padded) offset and ends at offset + count.
 __global__ __launch_bounds__(256) void k_bin_scatter(MeshDev m, double h, int* __restrict__ cursor,
                                                      unsigned* __restrict__ list) {
   __shared__ int hist[kFinBins], base[kFinBins];
-  hist[threadIdx.x] = 0;
+  for (int k = threadIdx.x; k < kFinBins; k += 256) hist[k] = 0;
   __syncthreads();
   const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const bool live = i < m.nf && j < m.n;
-  int key = 0, rank = 0;
-  if (live) {
+  int key = -1, rank = 0;
+  if (i < m.nf && j < m.n) {
     key = pair_key(m, h, i, j);
-    rank = atomicAdd(&hist[key], 1);
+    if (key >= 0) rank = atomicAdd(&hist[key], 1);
   }
   __syncthreads();
-  if (hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], hist[threadIdx.x]);
+  for (int k = threadIdx.x; k < kFinBins; k += 256)
+    if (hist[k]) base[k] = atomicAdd(&cursor[k], hist[k]);
   __syncthreads();
-  if (live) list[base[key] + rank] = ((unsigned)j << 16) | (unsigned)i;
+  if (key >= 0) list[base[key] + rank] = ((unsigned)j << 16) | (unsigned)i;
 }
 
 // EIGENE's register-resident epsilon tables take ~200 VGPRs: asking for two
@@ -173,6 +217,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REG == kReg
   const unsigned e = list[t];
   if (e == kNoPair) return;
   pair_entry<false, REG>(m, f, (int)(e & 0xFFFFu), (int)(e >> 16), A, S, flags);
+}
+
+// Hull-pair entries: both orientations per thread.
+template <int REG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REG == kRegEigene || REG == kRegPidcock ? 2 : 1))) void k_assemble_pairs(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
+                                                        int count, double2* __restrict__ A,
+                                                        double2* __restrict__ S) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const unsigned e = list[t];
+  if (e == kNoPair) return;
+  pair_entry2<REG>(m, f, (int)(e & 0xFFFFu), (int)(e >> 16), A, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,10 +448,26 @@ void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigne
 
 void clear_flags(hipStream_t s, int* flags) { HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags); }
 
-void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, const unsigned* list,
-                     size_t count, double2* A, double2* S, int* flags) {
+void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, bool paired,
+                     const unsigned* list, size_t count, double2* A, double2* S, int* flags) {
   if (count == 0) return;
   const dim3 grid((unsigned)((count + 255) / 256));
+  if (paired) {
+    switch (region) {
+      case kRegPidcock:
+        HX_LAUNCH(k_assemble_pairs<kRegPidcock>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S);
+        break;
+      case kRegEigene:
+        HX_LAUNCH(k_assemble_pairs<kRegEigene>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S);
+        break;
+      case kRegEigen:
+        HX_LAUNCH(k_assemble_pairs<kRegEigen>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S);
+        break;
+      default:
+        break;  // LINTON pairs are always single entries
+    }
+    return;
+  }
   switch (region) {
     case kRegLinton:
       HX_LAUNCH(k_assemble_list<kRegLinton>, grid, dim3(256), 0, s, m, f, list, (int)count, A, S, flags);

@@ -501,30 +501,39 @@ __device__ __forceinline__ double wave_nm0(double wk, double h) {
   return h / 2.0 * (1.0 + sinh(2.0 * wk * h) / (2.0 * wk * h));
 }
 
-// Propagating mode (EIGEN :300-313, EIGENE :433-446, PIDCOCK :598-606)
-__device__ inline void d_wave_mode(double r, const FreqParams& fp, bool pidcock, cd fg[3]) {
+// Propagating mode (EIGEN :300-313, EIGENE :433-446, PIDCOCK :598-606),
+// from J0, J1, Y0, Y1 of k R (which do not depend on the orientation)
+__device__ __forceinline__ void wave_mode_from(double j0, double j1, double y0, double y1, const FreqParams& fp,
+                                               bool pidcock, cd fg[3]) {
   double wk = fp.wk;
   double nm0 = fp.nm0;
   double chf = fp.chz[fp.js], chp = fp.chz[fp.jf], shf = fp.shz[fp.js];
   double p0 = kPi / nm0 * chf * chp;
   double dp0 = pidcock ? wk * kPi / nm0 * shf * chp : wk * (kPi / nm0 * shf * chp);
-  double j0, j1, y0, y1;
-  d_jy01b(wk * r, j0, j1, y0, y1);
   fg[0] = cd{-p0 * y0, p0 * j0};
   fg[1] = cd{wk * p0 * y1, -(wk * p0) * j1};
   fg[2] = cd{-dp0 * y0, dp0 * j0};
 }
+__device__ inline void d_wave_mode(double r, const FreqParams& fp, bool pidcock, cd fg[3]) {
+  double j0, j1, y0, y1;
+  d_jy01b(fp.wk * r, j0, j1, y0, y1);
+  wave_mode_from(j0, j1, y0, y1, fp, pidcock, fg);
+}
 
-// Evanescent term m (0-based mode index) of EIGEN / EIGENE (:321-337)
-__device__ __forceinline__ void d_eigen_term(double r, int mode, const FreqParams& fp, double rm[3]) {
+// Evanescent term m (0-based mode index) of EIGEN / EIGENE (:321-337), from
+// K0, K1 of k_m R
+__device__ __forceinline__ void eigen_term_from(double k0, double k1, int mode, const FreqParams& fp, double rm[3]) {
   const double km = fp.wvn[mode], pm = fp.pm[mode];
-  double k0, k1;
-  d_ik01a(km * r, k0, k1);
   const size_t row = (size_t)mode * fp.n;
   const double ckzf = fp.cz[row + fp.js], ckzp = fp.cz[row + fp.jf], skzf = fp.sz[row + fp.js];
   rm[0] = pm * k0 * ckzf * ckzp;
   rm[1] = -(pm * km * k1 * ckzf * ckzp);
   rm[2] = -(pm * km * k0 * skzf * ckzp);
+}
+__device__ __forceinline__ void d_eigen_term(double r, int mode, const FreqParams& fp, double rm[3]) {
+  double k0, k1;
+  d_ik01a(fp.wvn[mode] * r, k0, k1);
+  eigen_term_from(k0, k1, mode, fp, rm);
 }
 
 // Rankine-term removal shared by EIGEN, EIGENE and PIDCOCK
@@ -629,6 +638,25 @@ __device__ inline void d_image_sums(double r, double zf, double zp, double h, do
   out[2] = (-s1z + s2z) / (h * h);
 }
 
+// One PIDCOCK series term (:611-640) added to the partial sums, from K0, K1
+// of k_m R and of u_m R
+__device__ __forceinline__ void pidcock_term(double k0, double k1, double k0t, double k1t, int mode,
+                                             const FreqParams& fp, double& g, double& gr, double& gz) {
+  const double h = fp.h;
+  const size_t row = (size_t)mode * fp.n;
+  double umt = fp.ut[mode], um = fp.wvn[mode];
+  double ac = fp.ac[mode];
+  const double cf = fp.cz[row + fp.js], cp = fp.cz[row + fp.jf], sf = fp.sz[row + fp.js];
+  const double tcf = fp.ct[row + fp.js], tcp = fp.ct[row + fp.jf], tsf = fp.st[row + fp.js];
+  double am = ac * cf * cp;
+  double amt = tcf * tcp / h;
+  double dam = -(um * ac * sf * cp);
+  double damt = -(umt * tsf * tcp / h);
+  g = g + (am * k0 - amt * k0t);
+  gr = gr + (-(um * am * k1) + umt * amt * k1t);
+  gz = gz + (dam * k0 - damt * k0t);
+}
+
 // PIDCOCK: 0.0005 <= R/h < 0.05 (:558-660).  Partial sums go to a per-lane
 // array once; the three epsilon tables then run in registers.
 __device__ __forceinline__ void d_pidcock(double r, double zf, double zp, const FreqParams& fp, cd fg[3]) {
@@ -648,21 +676,10 @@ __device__ __forceinline__ void d_pidcock(double r, double zf, double zp, const 
   ser[0][0] = ser[1][0] = ser[2][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     const int mode = i - 1;
-    const size_t row = (size_t)mode * fp.n;
-    double umt = fp.ut[mode], um = fp.wvn[mode];
     double k0, k1, k0t, k1t;
-    d_ik01a(um * r, k0, k1);
-    d_ik01a(umt * r, k0t, k1t);
-    double ac = fp.ac[mode];
-    const double cf = fp.cz[row + fp.js], cp = fp.cz[row + fp.jf], sf = fp.sz[row + fp.js];
-    const double tcf = fp.ct[row + fp.js], tcp = fp.ct[row + fp.jf], tsf = fp.st[row + fp.js];
-    double am = ac * cf * cp;
-    double amt = tcf * tcp / h;
-    double dam = -(um * ac * sf * cp);
-    double damt = -(umt * tsf * tcp / h);
-    g = g + (am * k0 - amt * k0t);
-    gr = gr + (-(um * am * k1) + umt * amt * k1t);
-    gz = gz + (dam * k0 - damt * k0t);
+    d_ik01a(fp.wvn[mode] * r, k0, k1);
+    d_ik01a(fp.ut[mode] * r, k0t, k1t);
+    pidcock_term(k0, k1, k0t, k1t, mode, fp, g, gr, gz);
     ser[0][i - 1] = g;
     ser[1][i - 1] = gr;
     ser[2][i - 1] = gz;
@@ -681,6 +698,150 @@ __device__ __forceinline__ void d_pidcock(double r, double zf, double zp, const 
   fg[1].r = fg[1].r + res[1] + im[1];
   fg[2].r = fg[2].r + res[2] + im[2];
   d_remove_rankine(r, zf, zp, fg);
+}
+
+// ---- Both orientations of one hull-panel pair ----
+// For panels a and b, the pairs (source b, field a) -- orientation A -- and
+// (source a, field b) -- orientation B -- have the same R, so PIDCOCK,
+// EIGENE and EIGEN share every Bessel value between them: J0, J1, Y0, Y1 of
+// k R and K0, K1 of k_m R (and of u_m R).  These routines evaluate those once
+// per mode and form each orientation's terms, partial sums, epsilon tables
+// and Rankine corrections with the expressions of the one-orientation
+// routines above, so each orientation's result is the one those give, bit
+// for bit.  zs / zd are orientation A's source and field z (orientation B
+// swaps them); fpA and fpB differ only in their table columns (js, jf).
+
+// EIGEN, both orientations: each keeps the reference's own break test (:330).
+__device__ inline void d_eigen2(double r, double zs, double zd, const FreqParams& fpA, const FreqParams& fpB, cd fgA[3],
+                                cd fgB[3]) {
+  double j0, j1, y0, y1;
+  d_jy01b(fpA.wk * r, j0, j1, y0, y1);
+  wave_mode_from(j0, j1, y0, y1, fpA, false, fgA);
+  wave_mode_from(j0, j1, y0, y1, fpB, false, fgB);
+  bool goA = true, goB = true;
+  for (int i = 2; i <= 10; i++) {
+    double k0, k1, rm[3];
+    d_ik01a(fpA.wvn[i - 1] * r, k0, k1);
+    if (goA) {
+      eigen_term_from(k0, k1, i - 1, fpA, rm);
+      fgA[0].r += rm[0];
+      fgA[1].r += rm[1];
+      fgA[2].r += rm[2];
+      if (fabs(rm[0]) < HX_F(1.0e-6) && fabs(rm[1]) < HX_F(1.0e-6) && fabs(rm[2]) < HX_F(1.0e-6)) goA = false;
+    }
+    if (goB) {
+      eigen_term_from(k0, k1, i - 1, fpB, rm);
+      fgB[0].r += rm[0];
+      fgB[1].r += rm[1];
+      fgB[2].r += rm[2];
+      if (fabs(rm[0]) < HX_F(1.0e-6) && fabs(rm[1]) < HX_F(1.0e-6) && fabs(rm[2]) < HX_F(1.0e-6)) goB = false;
+    }
+    if (!goA && !goB) break;
+  }
+  d_remove_rankine(r, zs, zd, fgA);
+  d_remove_rankine(r, zd, zs, fgB);
+}
+
+// EIGENE, both orientations: six partial-sum series, six epsilon tables.
+__device__ __forceinline__ void d_eigene2(double r, double zs, double zd, const FreqParams& fpA,
+                                          const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
+  double ser[6][kEigeneMax];
+  int nt;
+  {
+#pragma clang fp contract(off)
+    nt = (int)(HX_F(-88.89) * fabs(r / fpA.h) + HX_F(54.45));
+  }
+  if (nt > kEigeneMax) nt = kEigeneMax;
+  double j0, j1, y0, y1;
+  d_jy01b(fpA.wk * r, j0, j1, y0, y1);
+  wave_mode_from(j0, j1, y0, y1, fpA, false, fgA);
+  wave_mode_from(j0, j1, y0, y1, fpB, false, fgB);
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 6; q++) ser[q][0] = 0.0;
+  for (int i = 2; i <= nt; i++) {
+    double k0, k1, rm[3];
+    d_ik01a(fpA.wvn[i - 1] * r, k0, k1);
+    eigen_term_from(k0, k1, i - 1, fpA, rm);
+    g[0] = g[0] + rm[0];
+    g[1] = g[1] + rm[1];
+    g[2] = g[2] + rm[2];
+    eigen_term_from(k0, k1, i - 1, fpB, rm);
+    g[3] = g[3] + rm[0];
+    g[4] = g[4] + rm[1];
+    g[5] = g[5] + rm[2];
+#pragma unroll
+    for (int q = 0; q < 6; q++) ser[q][i - 1] = g[q];
+  }
+  double res[6];
+#pragma unroll 1
+  for (int q = 0; q < 6; q++) {
+    double sv[kEigeneMax], av[kEigeneMax];
+#pragma unroll
+    for (int i = 0; i < kEigeneMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
+    res[q] = d_limes_reg<kEigeneMax>(nt - 1, sv, av);
+  }
+  for (int k = 0; k < 3; k++) {
+    fgA[k].r += res[k];
+    fgB[k].r += res[3 + k];
+  }
+  d_remove_rankine(r, zs, zd, fgA);
+  d_remove_rankine(r, zd, zs, fgB);
+}
+
+// PIDCOCK, both orientations: K0, K1 of k_m R and u_m R once per mode.
+__device__ __forceinline__ void d_pidcock2(double r, double zs, double zd, const FreqParams& fpA,
+                                           const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
+  double ser[6][kPidcockMax];
+  const double ga = 0.5772156649015328;
+  double h = fpA.h;
+  int nt;
+  {
+#pragma clang fp contract(off)
+    nt = (int)(HX_F(-1010.10) * fabs(r / h) + HX_F(100.50));
+  }
+  if (nt > kPidcockMax) nt = kPidcockMax;
+  double j0, j1, y0, y1;
+  d_jy01b(fpA.wk * r, j0, j1, y0, y1);
+  wave_mode_from(j0, j1, y0, y1, fpA, true, fgA);
+  wave_mode_from(j0, j1, y0, y1, fpB, true, fgB);
+  const double lg = 2.0 / h * (ga + log(r / 4.0 / h)), rg = 2.0 / (r * h);
+  fgA[0].r += lg;
+  fgA[1].r += rg;
+  fgB[0].r += lg;
+  fgB[1].r += rg;
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 6; q++) ser[q][0] = 0.0;
+  for (int i = 2; i <= nt; i++) {
+    const int mode = i - 1;
+    double k0, k1, k0t, k1t;
+    d_ik01a(fpA.wvn[mode] * r, k0, k1);
+    d_ik01a(fpA.ut[mode] * r, k0t, k1t);
+    pidcock_term(k0, k1, k0t, k1t, mode, fpA, g[0], g[1], g[2]);
+    pidcock_term(k0, k1, k0t, k1t, mode, fpB, g[3], g[4], g[5]);
+#pragma unroll
+    for (int q = 0; q < 6; q++) ser[q][i - 1] = g[q];
+  }
+  double res[6];
+#pragma unroll 1
+  for (int q = 0; q < 6; q++) {
+    double sv[kPidcockMax], av[kPidcockMax];
+#pragma unroll
+    for (int i = 0; i < kPidcockMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
+    res[q] = 4.0 * d_limes_reg<kPidcockMax>(nt - 1, sv, av);
+  }
+  double im[3];
+  d_image_sums(r, zs, zd, h, im);
+  fgA[0].r = fgA[0].r + res[0] + im[0];
+  fgA[1].r = fgA[1].r + res[1] + im[1];
+  fgA[2].r = fgA[2].r + res[2] + im[2];
+  d_image_sums(r, zd, zs, h, im);
+  fgB[0].r = fgB[0].r + res[3] + im[0];
+  fgB[1].r = fgB[1].r + res[4] + im[1];
+  fgB[2].r = fgB[2].r + res[5] + im[2];
+  d_remove_rankine(r, zs, zd, fgA);
+  d_remove_rankine(r, zd, zs, fgB);
 }
 
 // ---- LINTON (R/h < 0.0005), FinGreen3D.f90:712-861 and its helpers ----
@@ -955,12 +1116,36 @@ __device__ __forceinline__ int fin_subkey(double rr, double h) {
       return kSubEigen;
   }
 }
-// FINGREEN3D (:81-155) with FINGREEN3D_R's region split (:183-191).  REG
-// fixes the region at compile time for pairs binned by fin_region; kRegAny
-// decides per pair.
+// FINGREEN3D (:81-155): the Green function and its gradient from the
+// region routine's fg (value, d/dR, d/dz) ...
+__device__ __forceinline__ void green_from_fg(const PairGeom& p, const cd fg[3], cd g[4]) {
+  if (p.rr > HX_F(1.0e-6)) {
+    g[1] = fg[1] * p.xpq / p.rr;
+    g[2] = fg[1] * p.ypq / p.rr;
+  }
+  g[0] = fg[0];
+  g[3] = fg[2];
+}
+// ... plus the Rankine terms of the image (and of the source when FLAG = 0)
+__device__ __forceinline__ void green_rankine(const PairGeom& p, double sgn, int flag, cd g[4]) {
+  double r13 = p.r1 * p.r1 * p.r1, r3 = p.r * p.r * p.r;
+  g[0].r += sgn * (1.0 / p.r1);
+  g[1].r -= sgn * (p.xpq / r13);
+  g[2].r -= sgn * (p.ypq / r13);
+  g[3].r -= sgn * (p.zpq / r13);
+  if (flag == 0) {
+    g[0].r += 1.0 / p.r;
+    g[1].r -= p.xpq / r3;
+    g[2].r -= p.ypq / r3;
+    g[3].r -= (p.zsrc - p.zfld) / r3;
+  }
+}
+
+// FINGREEN3D with FINGREEN3D_R's region split (:183-191).  REG fixes the
+// region at compile time for pairs binned by fin_region; kRegAny decides per
+// pair.
 template <int REG>
 __device__ inline void d_fingreen_r(const PairGeom& p, const FreqParams& fp, int flag, cd g[4], int* ovf) {
-  double r13 = p.r1 * p.r1 * p.r1, r3 = p.r * p.r * p.r;
   for (int k = 0; k < 4; k++) g[k] = mk(0.0);
   double sgn = 1.0;
   if (fabs(fp.v) < HX_F(1.0e-8)) {
@@ -977,23 +1162,36 @@ __device__ inline void d_fingreen_r(const PairGeom& p, const FreqParams& fp, int
       d_eigene(p.rr, p.zsrc, p.zfld, fp, fg);
     else
       d_eigen(p.rr, p.zsrc, p.zfld, fp, fg);
-    if (p.rr > HX_F(1.0e-6)) {
-      g[1] = fg[1] * p.xpq / p.rr;
-      g[2] = fg[1] * p.ypq / p.rr;
-    }
-    g[0] = fg[0];
-    g[3] = fg[2];
+    green_from_fg(p, fg, g);
   }
-  g[0].r += sgn * (1.0 / p.r1);
-  g[1].r -= sgn * (p.xpq / r13);
-  g[2].r -= sgn * (p.ypq / r13);
-  g[3].r -= sgn * (p.zpq / r13);
-  if (flag == 0) {
-    g[0].r += 1.0 / p.r;
-    g[1].r -= p.xpq / r3;
-    g[2].r -= p.ypq / r3;
-    g[3].r -= (p.zsrc - p.zfld) / r3;
+  green_rankine(p, sgn, flag, g);
+}
+
+// Both orientations of a hull-panel pair in region REG (PIDCOCK, EIGENE or
+// EIGEN): pA / pB and fpA / fpB as for d_eigene2.  Each orientation's g is
+// what d_fingreen_r<REG> gives for it.
+template <int REG>
+__device__ inline void d_fingreen2(const PairGeom& pA, const PairGeom& pB, const FreqParams& fpA,
+                                   const FreqParams& fpB, int flagA, int flagB, cd gA[4], cd gB[4]) {
+  static_assert(REG == kRegPidcock || REG == kRegEigene || REG == kRegEigen, "paired regions");
+  for (int k = 0; k < 4; k++) gA[k] = gB[k] = mk(0.0);
+  double sgn = 1.0;
+  if (fabs(fpA.v) < HX_F(1.0e-8)) {
+  } else if (fabs(fpA.v + 1.0) < HX_F(1.0e-8)) {
+    sgn = -1.0;
+  } else {
+    cd fgA[3], fgB[3];
+    if (REG == kRegPidcock)
+      d_pidcock2(pA.rr, pA.zsrc, pA.zfld, fpA, fpB, fgA, fgB);
+    else if (REG == kRegEigene)
+      d_eigene2(pA.rr, pA.zsrc, pA.zfld, fpA, fpB, fgA, fgB);
+    else
+      d_eigen2(pA.rr, pA.zsrc, pA.zfld, fpA, fpB, fgA, fgB);
+    green_from_fg(pA, fgA, gA);
+    green_from_fg(pB, fgB, gB);
   }
+  green_rankine(pA, sgn, flagA, gA);
+  green_rankine(pB, sgn, flagB, gB);
 }
 
 __device__ inline void d_fingreen(const PairGeom& p, const FreqParams& fp, int flag, cd g[4], int* ovf) {

@@ -246,7 +246,9 @@ struct hamsx_ctx {
   DevBuf<int> bin_cnt;
   int bin_n = 0, bin_nf = 0;
   double bin_h = 0.0;
-  size_t reg_begin[5] = {0}, reg_end[5] = {0};
+  // list slices per region: single entries [reg_begin, reg_end), hull-pair
+  // entries [pair_begin, pair_end)
+  size_t reg_begin[5] = {0}, reg_end[5] = {0}, pair_begin[5] = {0}, pair_end[5] = {0};
   DevBuf<double> out, norms;
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   bool pending = false;  // a submitted frequency solve not yet collected
@@ -363,16 +365,19 @@ int ensure_bins(hamsx_ctx* c) {
   HX_TRY(hipMemcpyAsync(cnt.data(), c->bin_cnt.p, sizeof(int) * kFinBins, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
   size_t off = 0;
-  for (int r = 0; r < 5; r++) c->reg_begin[r] = c->reg_end[r] = 0;
-  bool seen[5] = {false, false, false, false, false};
+  for (int r = 0; r < 5; r++) c->reg_begin[r] = c->reg_end[r] = c->pair_begin[r] = c->pair_end[r] = 0;
+  bool seen[2][5] = {};
   for (int k = 0; k < kFinBins; k++) {
     start[k] = (int)off;
     const size_t padded = ((size_t)cnt[k] + 63) / 64 * 64;
     if (cnt[k] > 0) {
       const int r = fin_key_region(k);
-      if (!seen[r]) c->reg_begin[r] = off;
-      seen[r] = true;
-      c->reg_end[r] = off + padded;
+      const bool pr = fin_key_paired(k);
+      size_t* b = pr ? c->pair_begin : c->reg_begin;
+      size_t* e = pr ? c->pair_end : c->reg_end;
+      if (!seen[pr][r]) b[r] = off;
+      seen[pr][r] = true;
+      e[r] = off + padded;
     }
     off += padded;
   }
@@ -1161,8 +1166,10 @@ int hamsx_assemble(hamsx_ctx* c) {
       // most expensive regions first, so the cheap ones fill the tail
       for (int r : {hamsx::kRegLinton, hamsx::kRegPidcock, hamsx::kRegEigene, hamsx::kRegEigen}) {
         StageTimer tr(c, 4 + r);
-        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, c->plist.p + c->reg_begin[r],
+        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, false, c->plist.p + c->reg_begin[r],
                                        c->reg_end[r] - c->reg_begin[r], c->A.p, c->S.p, c->flags.p);
+        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, true, c->plist.p + c->pair_begin[r],
+                                       c->pair_end[r] - c->pair_begin[r], c->A.p, c->S.p, c->flags.p);
       }
     } else
       hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);

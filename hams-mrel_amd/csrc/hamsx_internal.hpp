@@ -37,17 +37,26 @@ struct MeshDev {
 };
 
 // Regions of the finite-depth Green function by R/h (FinGreen3D.f90:183-191)
-// and the key the binned assembly sorts pairs by: key = 2 * subkey + FLAG,
-// subkey ascending by region -- LINTON 0; PIDCOCK 1 + (99 - NT) for NT in
-// 49..99; EIGENE 52 + (50 - NT) for NT in 10..50; EIGEN 93.  The key depends
-// on the geometry and h only.
+// and the keys the binned assembly sorts pairs by.  subkey ascends by
+// region: LINTON 0; PIDCOCK 1 + (99 - NT) for NT in 49..99; EIGENE
+// 52 + (50 - NT) for NT in 10..50; EIGEN 93.
+//   * single pairs (field i, source j), one orientation per list entry:
+//     key = 2 * subkey + FLAG -- every LINTON pair and every pair whose
+//     field point is a lid point;
+//   * hull pairs {a < b} outside LINTON, both orientations per list entry:
+//     key = kPairKey0 + 4 * (subkey - 1) + 2 * FLAG(a, b) + FLAG(b, a),
+//     FLAG(i, j) being the flag of field i and source j.
+// Keys depend on the geometry and h only.
 enum FinRegion { kRegAny = 0, kRegLinton = 1, kRegPidcock = 2, kRegEigene = 3, kRegEigen = 4 };
 constexpr int kSubPidcock = 1, kSubEigene = 52, kSubEigen = 93;
-constexpr int kFinBins = 256;
-static_assert(2 * kSubEigen + 2 <= kFinBins, "key space");
+constexpr int kPairKey0 = 2 * kSubEigen + 2;
+constexpr int kFinBins = kPairKey0 + 4 * kSubEigen;  // 560
+inline constexpr int fin_sub_region(int sub) {
+  return sub < kSubPidcock ? kRegLinton : sub < kSubEigene ? kRegPidcock : sub < kSubEigen ? kRegEigene : kRegEigen;
+}
+inline constexpr bool fin_key_paired(int key) { return key >= kPairKey0; }
 inline constexpr int fin_key_region(int key) {
-  return key / 2 < kSubPidcock ? kRegLinton : key / 2 < kSubEigene ? kRegPidcock
-                                            : key / 2 < kSubEigen ? kRegEigene : kRegEigen;
+  return fin_sub_region(key < kPairKey0 ? key / 2 : kSubPidcock + (key - kPairKey0) / 4);
 }
 
 struct FreqDev {
@@ -80,9 +89,10 @@ void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f);
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count);
 void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list);
 // one region's kernel (1 LINTON, 2 PIDCOCK, 3 EIGENE, 4 EIGEN) over its slice
-// of the sorted pair list
-void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, const unsigned* list,
-                     size_t count, double2* A, double2* S, int* flags);
+// of the sorted pair list: single pairs, or (paired) both orientations of
+// hull pairs
+void assemble_region(hipStream_t s, const MeshDev& m, const FreqDev& f, int region, bool paired,
+                     const unsigned* list, size_t count, double2* A, double2* S, int* flags);
 void clear_flags(hipStream_t s, int* flags);
 // right-hand sides X(n, 6*nbody + nbeta) = S * generator columns
 // (rows: the m.nf field points; S and X have leading dimension m.nf)

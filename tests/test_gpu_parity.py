@@ -84,6 +84,29 @@ def test_binned_assembly_equals_grid(mesh_small, depth, w, monkeypatch):
     np.testing.assert_array_equal(out["0"][1], out["1"][1])
 
 
+@pytest.mark.parametrize("depth,w", [(30.0, 1.2)])
+def test_binned_assembly_equals_grid_with_lid(depth, w, monkeypatch):
+    """With lid field points the binned assembly mixes single entries (lid
+    rows, LINTON pairs) with hull-pair entries (both orientations per
+    thread); C = A^T A over all field rows and the hull block of S still
+    match the unbinned kernel bit for bit."""
+    from hamsx.mesh import waterplane_disk
+    hull = cylinder(4.0, 6.0, 24, 6, 3)
+    m = combine([hull], [np.zeros(3)])
+    lid = lid_centroids(waterplane_disk(4.0, 24, 3))
+    say(f"[gpu] binned vs grid with lid n={m.nelem} lid={len(lid)} depth={depth} w={w}")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("HAMSX_ASSEMBLE_GRID", mode)
+        with FrequencySolver(m) as s:
+            s.set_lid(lid)
+            s.calwaveproperts(w, depth)
+            s.assemble()
+            out[mode] = (s.get_matrix("A"), s.get_matrix("S"))
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    np.testing.assert_array_equal(out["0"][1], out["1"][1])
+
+
 def test_lu_backward_error():
     m = small_pair(na=40, nz=8, nr=4)   # 2 x 480 panels: three tournament levels
     say(f"[gpu] LU n={m.nelem}")
