@@ -521,13 +521,17 @@ __device__ inline void d_wave_mode(double r, const FreqParams& fp, bool pidcock,
 }
 
 // Evanescent term m (0-based mode index) of EIGEN / EIGENE (:321-337), from
-// K0, K1 of k_m R
+// K0, K1 of k_m R.  The product cos k_m(z+h) cos k_m(zeta+h) is formed first:
+// it is then the same for both orientations of a pair, bit for bit, so the
+// value and d/dR series of the two orientations coincide (the reference's
+// left-to-right product differs from it by rounding only).
 __device__ __forceinline__ void eigen_term_from(double k0, double k1, int mode, const FreqParams& fp, double rm[3]) {
   const double km = fp.wvn[mode], pm = fp.pm[mode];
   const size_t row = (size_t)mode * fp.n;
   const double ckzf = fp.cz[row + fp.js], ckzp = fp.cz[row + fp.jf], skzf = fp.sz[row + fp.js];
-  rm[0] = pm * k0 * ckzf * ckzp;
-  rm[1] = -(pm * km * k1 * ckzf * ckzp);
+  const double cc = ckzf * ckzp;
+  rm[0] = pm * k0 * cc;
+  rm[1] = -(pm * km * k1 * cc);
   rm[2] = -(pm * km * k0 * skzf * ckzp);
 }
 __device__ __forceinline__ void d_eigen_term(double r, int mode, const FreqParams& fp, double rm[3]) {
@@ -639,7 +643,9 @@ __device__ inline void d_image_sums(double r, double zf, double zp, double h, do
 }
 
 // One PIDCOCK series term (:611-640) added to the partial sums, from K0, K1
-// of k_m R and of u_m R
+// of k_m R and of u_m R.  As in eigen_term_from the cosine product is formed
+// first (AC (cos cos) rather than (AC cos) cos), so the value and d/dR
+// series do not depend on the orientation.
 __device__ __forceinline__ void pidcock_term(double k0, double k1, double k0t, double k1t, int mode,
                                              const FreqParams& fp, double& g, double& gr, double& gz) {
   const double h = fp.h;
@@ -648,7 +654,7 @@ __device__ __forceinline__ void pidcock_term(double k0, double k1, double k0t, d
   double ac = fp.ac[mode];
   const double cf = fp.cz[row + fp.js], cp = fp.cz[row + fp.jf], sf = fp.sz[row + fp.js];
   const double tcf = fp.ct[row + fp.js], tcp = fp.ct[row + fp.jf], tsf = fp.st[row + fp.js];
-  double am = ac * cf * cp;
+  double am = ac * (cf * cp);
   double amt = tcf * tcp / h;
   double dam = -(um * ac * sf * cp);
   double damt = -(umt * tsf * tcp / h);
@@ -742,10 +748,11 @@ __device__ inline void d_eigen2(double r, double zs, double zd, const FreqParams
   d_remove_rankine(r, zd, zs, fgB);
 }
 
-// EIGENE, both orientations: six partial-sum series, six epsilon tables.
+// EIGENE, both orientations: the value and d/dR series are shared (see
+// eigen_term_from), so four partial-sum series and four epsilon tables.
 __device__ __forceinline__ void d_eigene2(double r, double zs, double zd, const FreqParams& fpA,
                                           const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
-  double ser[6][kEigeneMax];
+  double ser[4][kEigeneMax];
   int nt;
   {
 #pragma clang fp contract(off)
@@ -756,9 +763,9 @@ __device__ __forceinline__ void d_eigene2(double r, double zs, double zd, const 
   d_jy01b(fpA.wk * r, j0, j1, y0, y1);
   wave_mode_from(j0, j1, y0, y1, fpA, false, fgA);
   wave_mode_from(j0, j1, y0, y1, fpB, false, fgB);
-  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double g[4] = {0.0, 0.0, 0.0, 0.0};  // value, d/dR, d/dz (A), d/dz (B)
 #pragma unroll
-  for (int q = 0; q < 6; q++) ser[q][0] = 0.0;
+  for (int q = 0; q < 4; q++) ser[q][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     double k0, k1, rm[3];
     d_ik01a(fpA.wvn[i - 1] * r, k0, k1);
@@ -767,32 +774,33 @@ __device__ __forceinline__ void d_eigene2(double r, double zs, double zd, const 
     g[1] = g[1] + rm[1];
     g[2] = g[2] + rm[2];
     eigen_term_from(k0, k1, i - 1, fpB, rm);
-    g[3] = g[3] + rm[0];
-    g[4] = g[4] + rm[1];
-    g[5] = g[5] + rm[2];
+    g[3] = g[3] + rm[2];
 #pragma unroll
-    for (int q = 0; q < 6; q++) ser[q][i - 1] = g[q];
+    for (int q = 0; q < 4; q++) ser[q][i - 1] = g[q];
   }
-  double res[6];
+  double res[4];
 #pragma unroll 1
-  for (int q = 0; q < 6; q++) {
+  for (int q = 0; q < 4; q++) {
     double sv[kEigeneMax], av[kEigeneMax];
 #pragma unroll
     for (int i = 0; i < kEigeneMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
     res[q] = d_limes_reg<kEigeneMax>(nt - 1, sv, av);
   }
-  for (int k = 0; k < 3; k++) {
-    fgA[k].r += res[k];
-    fgB[k].r += res[3 + k];
-  }
+  fgA[0].r += res[0];
+  fgA[1].r += res[1];
+  fgA[2].r += res[2];
+  fgB[0].r += res[0];
+  fgB[1].r += res[1];
+  fgB[2].r += res[3];
   d_remove_rankine(r, zs, zd, fgA);
   d_remove_rankine(r, zd, zs, fgB);
 }
 
-// PIDCOCK, both orientations: K0, K1 of k_m R and u_m R once per mode.
+// PIDCOCK, both orientations: K0, K1 of k_m R and u_m R once per mode; the
+// value and d/dR series are shared (see pidcock_term), four epsilon tables.
 __device__ __forceinline__ void d_pidcock2(double r, double zs, double zd, const FreqParams& fpA,
                                            const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
-  double ser[6][kPidcockMax];
+  double ser[4][kPidcockMax];
   const double ga = 0.5772156649015328;
   double h = fpA.h;
   int nt;
@@ -810,22 +818,23 @@ __device__ __forceinline__ void d_pidcock2(double r, double zs, double zd, const
   fgA[1].r += rg;
   fgB[0].r += lg;
   fgB[1].r += rg;
-  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double g[4] = {0.0, 0.0, 0.0, 0.0};  // value, d/dR, d/dz (A), d/dz (B)
+  double gb0 = 0.0, gb1 = 0.0;         // orientation B's copies of the shared sums
 #pragma unroll
-  for (int q = 0; q < 6; q++) ser[q][0] = 0.0;
+  for (int q = 0; q < 4; q++) ser[q][0] = 0.0;
   for (int i = 2; i <= nt; i++) {
     const int mode = i - 1;
     double k0, k1, k0t, k1t;
     d_ik01a(fpA.wvn[mode] * r, k0, k1);
     d_ik01a(fpA.ut[mode] * r, k0t, k1t);
     pidcock_term(k0, k1, k0t, k1t, mode, fpA, g[0], g[1], g[2]);
-    pidcock_term(k0, k1, k0t, k1t, mode, fpB, g[3], g[4], g[5]);
+    pidcock_term(k0, k1, k0t, k1t, mode, fpB, gb0, gb1, g[3]);
 #pragma unroll
-    for (int q = 0; q < 6; q++) ser[q][i - 1] = g[q];
+    for (int q = 0; q < 4; q++) ser[q][i - 1] = g[q];
   }
-  double res[6];
+  double res[4];
 #pragma unroll 1
-  for (int q = 0; q < 6; q++) {
+  for (int q = 0; q < 4; q++) {
     double sv[kPidcockMax], av[kPidcockMax];
 #pragma unroll
     for (int i = 0; i < kPidcockMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
@@ -837,9 +846,9 @@ __device__ __forceinline__ void d_pidcock2(double r, double zs, double zd, const
   fgA[1].r = fgA[1].r + res[1] + im[1];
   fgA[2].r = fgA[2].r + res[2] + im[2];
   d_image_sums(r, zd, zs, h, im);
-  fgB[0].r = fgB[0].r + res[3] + im[0];
-  fgB[1].r = fgB[1].r + res[4] + im[1];
-  fgB[2].r = fgB[2].r + res[5] + im[2];
+  fgB[0].r = fgB[0].r + res[0] + im[0];
+  fgB[1].r = fgB[1].r + res[1] + im[1];
+  fgB[2].r = fgB[2].r + res[3] + im[2];
   d_remove_rankine(r, zs, zd, fgA);
   d_remove_rankine(r, zd, zs, fgB);
 }
