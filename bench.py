@@ -40,10 +40,12 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector and FP64 matrix (dense)
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# FP64 work and HBM traffic per panel pair of each finite-depth region
-# kernel, measured with rocprofv3 counters on the named workload (summary and
-# commands: profiles/pmc_assembly_two_cylinders_fp64.md).  flops = lane-level
-# FP64 operations issued (FMA = 2, hardware transcendental = 1):
+# FP64 work and HBM traffic per panel pair (one A entry and one S entry) of
+# each finite-depth region, summed over the region's kernels (hull-pair
+# kernel + single-entry kernel), measured with rocprofv3 counters on the
+# named workload (summary and commands:
+# profiles/pmc_assembly_two_cylinders_fp64.md).  flops = lane-level FP64
+# operations issued (FMA = 2, hardware transcendental = 1):
 # SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64 x 64 x the active-lane fraction
 # SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU).  Traffic = FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors that make a 256 MiB device copy read 256 MiB
@@ -51,10 +53,10 @@ PEAK_HBM_GBS = 8000.0
 # (A and S entries written once).
 REGION_COUNTERS = {
     "cylinders-2x1984-h50": {
-        "asm_linton": {"flops_per_pair": 571424.7, "traffic_per_pair": (37738365 + 53079083) / 57728},
-        "asm_pidcock": {"flops_per_pair": 326377.2, "traffic_per_pair": (10817558282 + 15217550549) / 1079424},
-        "asm_eigene": {"flops_per_pair": 119803.3, "traffic_per_pair": (8720059880 + 7938387605) / 6735360},
-        "asm_eigen": {"flops_per_pair": 7985.8, "traffic_per_pair": (39528137 + 251920384) / 7872512},
+        "asm_linton": {"flops_per_pair": 571390.4, "traffic_per_pair": (38278104 + 54102251) / 57728},
+        "asm_pidcock": {"flops_per_pair": 187615.2, "traffic_per_pair": (7758628724 + 10256639040) / 1079424},
+        "asm_eigene": {"flops_per_pair": 68951.6, "traffic_per_pair": (6289391727 + 5676591616) / 6735360},
+        "asm_eigen": {"flops_per_pair": 4247.2, "traffic_per_pair": (30479918 + 253196160) / 7872512},
     },
 }
 REGIONS = ("asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
@@ -94,8 +96,10 @@ def roofline(case, mesh, ms, precision):
     dom = max(cands, key=lambda k: ms[k])
     pairs = region_pairs(mesh, case.depth)[dom]
     cnt = REGION_COUNTERS.get(case.name, {}).get(dom)
-    out = {"kernel": f"k_assemble_list<{dom[4:].upper()}> (finite-depth Green function region + Newman + A/S "
-                     "entries, one thread per pair)",
+    out = {"kernel": f"k_assemble_pairs<{dom[4:].upper()}> (finite-depth Green function region + Newman + A/S "
+                     "entries; both orientations of a hull pair per thread)" if dom != "asm_linton" else
+                     "k_assemble_list<LINTON> (finite-depth Green function region + Newman + A/S entries, one "
+                     "thread per pair)",
            "bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS, "launch_ms": ms[dom], "pairs": pairs,
            "algorithmic_bytes": 32.0 * pairs}
     if cnt:
