@@ -207,7 +207,8 @@ struct hamsx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int prec = HAMSX_PREC_FP64;
-  int refine_iters = 3;
+  int refine_iters = 4;       // most correction steps of the mixed path
+  double refine_tol = 1.0e-10; // stop when max_q |R_q|_max / |X_q|_max <= this
   // linear solver: HAMSX_SOLVER_LU or HAMSX_SOLVER_GMRES (restart, tolerance, iteration cap)
   int solver = HAMSX_SOLVER_LU;
   int gm_restart = 25, gm_maxiter = 100;
@@ -721,16 +722,25 @@ int do_solve(hamsx_ctx* c) {
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
   } else {
+    // complex64 solve, then FP64 residual / complex64 correction steps until
+    // the residual test passes; the test runs on the device and the steps
+    // after convergence are launched as no-ops, so nothing waits on the host
     const size_t cnt = (size_t)n * nr;
+    int* state = c->flags.p + 2;
+    hamsx::launch::refine_reset(c->stream, state);
     HX_TRY(hipMemcpyAsync(c->B.p, c->X.p, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
     hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
     for (int it = 0; it < c->refine_iters; it++) {
-      hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p);
-      hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr);
-      hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt);
+      hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p,
+                              state);
+      hamsx::launch::refine_check(c->stream, c->norms.p, nr, c->refine_tol, state);
+      hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr, state);
+      hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt, state);
     }
-    hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p);
-    c->last_iters = c->refine_iters;
+    // residual of the final iterate (a no-op when the loop converged: the
+    // norms then hold the residual that passed the test)
+    hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p,
+                            state);
   }
   c->have_X = true;
   return check_launch();
@@ -759,7 +769,7 @@ int enqueue_forces(hamsx_ctx* c) {
   }
   if ((rc = check_launch())) return rc;
   HX_TRY(hipMemcpyAsync(hout, c->out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
-  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0)
     HX_TRY(hipMemcpyAsync(hout + len + 2, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
                           c->stream));
@@ -776,10 +786,11 @@ int complete_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int*
   if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
   if (bdmp) std::memcpy(bdmp, hout + (size_t)nd * nd, sizeof(double) * nd * nd);
   if (exfc) std::memcpy(exfc, hout + 2 * (size_t)nd * nd, sizeof(double) * 2 * nd * c->nbeta);
-  int fl[2];
+  int fl[4];  // quadrature overflow, INFO, refinement converged, correction steps
   std::memcpy(fl, hout + len, sizeof fl);
   if (info_out) *info_out = fl[1];
-  if (c->prec == HAMSX_PREC_MIXED) {
+  bool noconv = false;
+  if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0 && c->solver == HAMSX_SOLVER_LU) {
     const double* nm = hout + len + 2;
     double worst = 0.0;
     for (int q = 0; q < c->nrhs(); q++) {
@@ -787,8 +798,13 @@ int complete_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int*
       worst = std::max(worst, xn > 0 ? rn / xn : rn);
     }
     c->last_resid = worst;
+    c->last_iters = fl[3];
+    noconv = !fl[2] && c->refine_tol > 0.0 && worst > c->refine_tol;
   }
   if (fl[0] & 1) return fail(HAMSX_EDEVICE, "adaptive quadrature stack exhausted in the finite-depth kernel");
+  if (noconv)
+    return fail(HAMSX_ENOCONV, "iterative refinement: residual %.3e above %.3e after %d steps", c->last_resid,
+                c->refine_tol, c->last_iters);
   return 0;
 }
 
@@ -1318,6 +1334,16 @@ int hamsx_set_solver(hamsx_ctx* c, int solver, double tol, int restart, int maxi
     c->gm_maxiter = maxiter;
   }
   c->solver = solver;
+  c->have_X = false;
+  return 0;
+}
+
+int hamsx_set_refinement(hamsx_ctx* c, int max_steps, double tol) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (max_steps < 0 || !(tol >= 0.0)) return fail(HAMSX_EARG, "max_steps must be >= 0 and tol >= 0");
+  c->refine_iters = max_steps;
+  c->refine_tol = tol;
   c->have_X = false;
   return 0;
 }

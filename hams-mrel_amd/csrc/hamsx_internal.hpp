@@ -112,15 +112,22 @@ void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w,
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
                    double2* diag);
-void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
+// skip (may be null, here and below): a device flag that turns every
+// launch into a no-op when set (refinement after convergence)
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
+                  const int* skip = nullptr);
 // Y = beta B + alpha A X (B may be null), A read once; P: matvec_workspace(n, nrhs) complex128
 size_t matvec_workspace(int n, int nrhs);
 void matvec(hipStream_t s, const double2* A, int n, const double2* X, int nrhs, const double2* B, double alpha,
-            double beta, double2* Y, double2* P);
+            double beta, double2* Y, double2* P, const int* skip = nullptr);
 // R = B - A X with per-column max norms of R (norms[0:nrhs]) and X (norms[nrhs:2 nrhs])
 void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
-              double* norms, double2* P);
-void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count);
+              double* norms, double2* P, const int* skip = nullptr);
+void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count, const int* skip = nullptr);
+// refinement state[0] = converged, state[1] = correction steps: reset, then
+// after each residual test max_q |R_q|_max / |X_q|_max <= tol
+void refine_reset(hipStream_t s, int* state);
+void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state);
 // restarted GMRES helpers (krylov.hip); blocks are n x nrhs, V holds nv of them
 void block_dots(hipStream_t s, const double2* V, int nv, const double2* W, int n, int nrhs, double2* dots);
 void block_update(hipStream_t s, const double2* V, int nv, double2* W, int n, int nrhs, const double2* h,

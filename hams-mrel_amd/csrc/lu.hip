@@ -69,7 +69,7 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // [b*kMerge, b*kMerge + kMerge) of the previous level.  Each block runs
 // partial pivoting on its candidates' panel entries (pivot = largest
 // |re| + |im| as IZAMAX, ties to the first candidate).  Outputs the chosen
-// rows in pivot order; the final level (diag != nullptr) also writes the
+// rows in pivot order; the final level (FINAL, diag set) also writes the
 // factored NB x NB diagonal block L\U in pivot order and ZGETRF's INFO.
 //
 // Each candidate's NB entries live in registers as a rotating window: at
@@ -85,18 +85,19 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // step's reads.
 // ---------------------------------------------------------------------------
 constexpr int kMerge = kSelRows / kPanelNB;
+constexpr int kSelWaves = kSelRows / 64;
 static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
-template <typename R>
-__global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
+template <typename R, bool FINAL>
+__global__ __launch_bounds__(kSelRows) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
                                                 int ncols, int row0, const int* __restrict__ idx_in,
                                                 const int* __restrict__ cnt_in, int nsets_in, int* __restrict__ idx_out,
                                                 int* __restrict__ cnt_out, typename CT<R>::c2* __restrict__ diag,
                                                 int* __restrict__ info) {
   __shared__ int sidx[kSelRows];
   __shared__ int off[kMerge + 1];
-  __shared__ double wmax[2][4];
-  __shared__ int wrow[2][4];
-  __shared__ double2 wwin[2][4][kPanelNB];  // each wave's candidate pivot window
+  __shared__ double wmax[2][kSelWaves];
+  __shared__ int wrow[2][kSelWaves];
+  __shared__ double2 wwin[2][kSelWaves][kPanelNB];  // each wave's candidate pivot window
   __shared__ int sel[kPanelNB];
   const int t = threadIdx.x, b = blockIdx.x;
   int mc;
@@ -151,17 +152,20 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
     const unsigned long long hit = __ballot(m == mw);
     const int first = hit ? __builtin_ctzll(hit) : 0;
     if (lane == first) {
+      // The final level keeps the multipliers in the retired slots, so it
+      // publishes them as zeros (the update then leaves them unchanged);
+      // earlier levels only need the pivot order and publish the raw window.
       const int live = kPanelNB - c;  // slots holding columns c..NB-1
 #pragma unroll
       for (int k = 0; k < kPanelNB; k++)
-        wwin[buf][wave][k] = k < live ? make_double2(vr[k], vi[k]) : make_double2(0.0, 0.0);
+        wwin[buf][wave][k] = (!FINAL || k < live) ? make_double2(vr[k], vi[k]) : make_double2(0.0, 0.0);
       wmax[buf][wave] = mw;
       wrow[buf][wave] = t;
     }
     __syncthreads();
     double best = wmax[buf][0];
     int w = 0;
-    for (int q = 1; q < 4; q++)
+    for (int q = 1; q < kSelWaves; q++)
       if (wmax[buf][q] > best) {  // equal maxima keep the lower wave, i.e. the lower row
         best = wmax[buf][q];
         w = q;
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(256) void k_select(const typename CT<R>::c2* __rest
   __syncthreads();
   if (t < nsel) idx_out[b * kPanelNB + t] = sidx[sel[t]];
   if (t == 0) cnt_out[b] = nsel;
-  if (diag != nullptr) {
+  if (FINAL) {
     // the pivot row of step s holds column (k + s) mod NB in slot k
     if (mystep >= 0)
 #pragma unroll
@@ -486,12 +490,20 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
     const int m = n - col0;
     int nsets = (m + kSelRows - 1) / kSelRows, cur = 0;
-    HX_LAUNCH(k_select<R>, dim3(nsets), dim3(256), 0, s, A, n, n, col0, nb, col0, (const int*)nullptr,
-              (const int*)nullptr, 0, idx[cur], cnt[cur], nsets == 1 ? diag : (c2*)nullptr, info);
+    if (nsets == 1)
+      HX_LAUNCH((k_select<R, true>), dim3(nsets), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0,
+                (const int*)nullptr, (const int*)nullptr, 0, idx[cur], cnt[cur], diag, info);
+    else
+      HX_LAUNCH((k_select<R, false>), dim3(nsets), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0,
+                (const int*)nullptr, (const int*)nullptr, 0, idx[cur], cnt[cur], (c2*)nullptr, info);
     while (nsets > 1) {
       const int nout = (nsets + kMerge - 1) / kMerge;
-      HX_LAUNCH(k_select<R>, dim3(nout), dim3(256), 0, s, A, n, n, col0, nb, col0, idx[cur], cnt[cur], nsets,
-                idx[1 - cur], cnt[1 - cur], nout == 1 ? diag : (c2*)nullptr, info);
+      if (nout == 1)
+        HX_LAUNCH((k_select<R, true>), dim3(nout), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0, idx[cur],
+                  cnt[cur], nsets, idx[1 - cur], cnt[1 - cur], diag, info);
+      else
+        HX_LAUNCH((k_select<R, false>), dim3(nout), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0, idx[cur],
+                  cnt[cur], nsets, idx[1 - cur], cnt[1 - cur], (c2*)nullptr, info);
       cur = 1 - cur;
       nsets = nout;
     }
@@ -527,7 +539,9 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
 constexpr int kTrsmRhs = 8;
 template <typename R, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
-                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs) {
+                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
   __shared__ double dr[kPanelNB], di[kPanelNB];                      // 1 / T(c, c) (upper)
   __shared__ double yr[kPanelNB][kTrsmRhs], yi[kPanelNB][kTrsmRhs];  // solved block rows
@@ -626,32 +640,45 @@ __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __
 }
 
 __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
-                              int n, int nrhs) {
+                              int n, int nrhs, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (i < n && q < nrhs) dst[(size_t)i + (size_t)n * q] = src[(size_t)perm[i] + (size_t)n * q];
 }
 
+__global__ void k_copy_unless(const double2* __restrict__ src, double2* __restrict__ dst, size_t count,
+                              const int* __restrict__ skip) {
+  if (*skip) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// skip (may be null): a device flag; when set, every launch returns at once
+// (the refinement loop of the mixed path after convergence).
 template <typename R>
 static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const int* perm, double2* X,
-                       double2* Xtmp, int nrhs) {
+                       double2* Xtmp, int nrhs, const int* skip = nullptr) {
   if (nrhs <= 0) return;
   const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
   // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
   // backward pass: off-diagonal rows updated in X, solved rows into T.
-  HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, perm, X, Xtmp, n, nrhs);
+  HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, perm, X, Xtmp, n, nrhs, skip);
   for (int k0 = 0; k0 < n; k0 += kPanelNB) {
     const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
     const int rest = n - k0 - kb;
     const int g = rest > 0 ? (rest + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block<R, true>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, Xtmp, X, nrhs);
+    HX_LAUNCH((k_trsm_block<R, true>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, Xtmp, X, nrhs, skip);
   }
   for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
     const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
     const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block<R, false>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, X, Xtmp, nrhs);
+    HX_LAUNCH((k_trsm_block<R, false>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, X, Xtmp, nrhs, skip);
   }
-  (void)hipMemcpyAsync(X, Xtmp, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
+  if (skip)
+    HX_LAUNCH(k_copy_unless, dim3(1024), dim3(256), 0, s, Xtmp, X, (size_t)n * nrhs, skip);
+  else
+    (void)hipMemcpyAsync(X, Xtmp, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -672,7 +699,8 @@ __global__ void k_to_c64(const double2* __restrict__ A, float2* __restrict__ Af,
 constexpr int kMvRhs = 16, kMvChunk = 256;
 __global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restrict__ A, int n,
                                                         const double2* __restrict__ X, int nrhs,
-                                                        double2* __restrict__ P) {
+                                                        double2* __restrict__ P, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double2 red[3][64][kMvRhs];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane, ic = min(i, n - 1);
@@ -711,7 +739,9 @@ __global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restric
 }
 
 __global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n, int nrhs,
-                                const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y) {
+                                const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y,
+                                const int* __restrict__ skip) {
+  if (skip && *skip) return;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * nrhs;
   if (e >= tot) return;
   double sr = 0.0, si = 0.0;
@@ -728,7 +758,9 @@ __global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n
   Y[e] = make_double2(yr, yi);
 }
 
-__global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms) {
+__global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms,
+                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double red[256];
   const int q = blockIdx.x, t = threadIdx.x;
   double m = 0.0;
@@ -745,12 +777,35 @@ __global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double*
   if (t == 0) norms[q] = red[0];
 }
 
-__global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, size_t count) {
+__global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, size_t count,
+                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < count; i += (size_t)gridDim.x * blockDim.x) {
     X[i].x += D[i].x;
     X[i].y += D[i].y;
   }
+}
+
+// Refinement bookkeeping, after each residual: unless already converged,
+// set the converged flag when max over columns of |R|_max / |X|_max <= tol
+// (a column with X = 0 compares |R|_max itself), else count the correction
+// step that follows.
+__global__ void k_refine_check(const double* __restrict__ norms, int nrhs, double tol, int* __restrict__ state) {
+  if (threadIdx.x != 0 || state[0]) return;
+  double worst = 0.0;
+  for (int q = 0; q < nrhs; q++) {
+    const double rn = norms[q], xn = norms[nrhs + q];
+    worst = fmax(worst, xn > 0.0 ? rn / xn : rn);
+  }
+  if (worst <= tol)
+    state[0] = 1;
+  else
+    state[1] += 1;  // a correction step follows
+}
+__global__ void k_refine_reset(int* __restrict__ state) {
+  state[0] = 0;
+  state[1] = 0;
 }
 
 namespace launch {
@@ -768,29 +823,34 @@ void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* w
                    double2* diag) {
   lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag));
 }
-void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
-  lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs);
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
+                  const int* skip) {
+  lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs, skip);
+}
+void refine_reset(hipStream_t s, int* state) { HX_LAUNCH(k_refine_reset, dim3(1), dim3(1), 0, s, state); }
+void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state) {
+  HX_LAUNCH(k_refine_check, dim3(1), dim3(64), 0, s, norms, nrhs, tol, state);
 }
 size_t matvec_workspace(int n, int nrhs) {
   return (size_t)((n + kMvChunk - 1) / kMvChunk) * n * nrhs;
 }
 void matvec(hipStream_t s, const double2* A, int n, const double2* X, int nrhs, const double2* B, double alpha,
-            double beta, double2* Y, double2* P) {
+            double beta, double2* Y, double2* P, const int* skip) {
   const int nchunk = (n + kMvChunk - 1) / kMvChunk;
   HX_LAUNCH(k_matvec_partial, dim3((n + 63) / 64, nchunk, (nrhs + kMvRhs - 1) / kMvRhs), dim3(256), 0, s, A, n, X,
-            nrhs, P);
+            nrhs, P, skip);
   const size_t tot = (size_t)n * nrhs;
   HX_LAUNCH(k_matvec_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, P, nchunk, n, nrhs, B, alpha,
-            beta, Y);
+            beta, Y, skip);
 }
 void residual(hipStream_t s, const double2* A, int n, const double2* X, const double2* B, double2* R, int nrhs,
-              double* norms, double2* P) {
-  matvec(s, A, n, X, nrhs, B, -1.0, 1.0, R, P);
-  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms);
-  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs);
+              double* norms, double2* P, const int* skip) {
+  matvec(s, A, n, X, nrhs, B, -1.0, 1.0, R, P, skip);
+  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, R, n, nrhs, norms, skip);
+  HX_LAUNCH(k_colmax, dim3(nrhs), dim3(256), 0, s, X, n, nrhs, norms + nrhs, skip);
 }
-void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count) {
-  HX_LAUNCH(k_axpy, dim3(1024), dim3(256), 0, s, X, D, count);
+void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count, const int* skip) {
+  HX_LAUNCH(k_axpy, dim3(1024), dim3(256), 0, s, X, D, count, skip);
 }
 }  // namespace launch
 }  // namespace hamsx

@@ -33,7 +33,7 @@ module hamsx_mod
             hamsx_set_frequency, hamsx_set_waves, hamsx_assemble, hamsx_factor, hamsx_solve_radiation, &
             hamsx_solve_diffraction, hamsx_forces, hamsx_solve_frequency, hamsx_submit_frequency, &
             hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
-            hamsx_set_solver, hamsx_set_lid, &
+            hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, &
             hamsx_stage_times
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
@@ -175,6 +175,13 @@ module hamsx_mod
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: solver, restart, maxiter
+      real(c_double), value :: tol
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_refinement(ctx, max_steps, tol) bind(C, name="hamsx_set_refinement") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: max_steps
       real(c_double), value :: tol
       integer(c_int) :: rc
     end function

@@ -51,6 +51,7 @@ SIGNATURES = {
     "hamsx_collect": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
     "hamsx_set_solver": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "hamsx_set_refinement": (C.c_int, [_vp, C.c_int, C.c_double]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),

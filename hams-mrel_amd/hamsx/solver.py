@@ -172,6 +172,11 @@ class FrequencySolver:
                    "hamsx_get_matrix")
         return (out, ipiv) if code == 2 else out
 
+    def set_refinement(self, max_steps: int = 4, tol: float = 1.0e-10):
+        """Mixed precision: at most max_steps FP64-residual / complex64
+        correction steps, stopping once max |R|/|X| per column <= tol."""
+        _lib.check(self._lib.hamsx_set_refinement(self._h, int(max_steps), float(tol)), "hamsx_set_refinement")
+
     def refine_stats(self):
         it, res = C.c_int(), C.c_double()
         _lib.check(self._lib.hamsx_refine_stats(self._h, C.byref(it), C.byref(res)), "hamsx_refine_stats")

@@ -161,7 +161,16 @@ int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
 #define HAMSX_SOLVER_GMRES 1
 int hamsx_set_solver(hamsx_ctx *ctx, int solver, double tol, int restart, int maxiter);
 
-/* Statistics of the last solve: mixed precision -- refinement iterations
+/* Mixed precision (HAMSX_PREC_MIXED) iterative refinement: after the
+ * complex64 LU solve, FP64 residual + complex64 correction steps run until
+ * max over the right-hand sides of |R|_max / |X|_max <= tol, at most
+ * max_steps of them (defaults 4 and 1e-10).  The test runs on the device
+ * (no host wait); hamsx_forces / hamsx_collect return HAMSX_ENOCONV when it
+ * was not met.  tol = 0 runs exactly max_steps steps.  No counterpart in the reference, which has no
+ * mixed-precision path. */
+int hamsx_set_refinement(hamsx_ctx *ctx, int max_steps, double tol);
+
+/* Statistics of the last solve: mixed precision -- correction steps taken
  * and the final max-norm relative residual; GMRES -- Arnoldi iterations and
  * the largest final relative residual 2-norm over the right-hand sides. */
 int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);

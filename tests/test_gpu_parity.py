@@ -193,6 +193,36 @@ def test_mixed_precision_within_1e6(mesh_small, depth):
         assert e <= max(1e-6, 4 * f)
 
 
+def test_mixed_refinement_stops_on_device(mesh_small):
+    """Mixed precision: the device-side residual test stops the correction
+    steps once max |R|/|X| <= tol; a fixed four-step run and a run allowed no
+    correction bracket it (the latter reports HAMSX_ENOCONV)."""
+    from hamsx._lib import HAMSX_ENOCONV, HamsxError
+    m = mesh_small
+    heads = np.deg2rad([0.0, 60.0])
+    w, depth = 1.1, -1.0
+    say(f"[gpu] mixed refinement n={m.nelem}")
+    with FrequencySolver(m, precision="mixed") as s:
+        s.set_waves(heads)
+        auto = s.solve_frequency(w, depth)
+        it, res = s.refine_stats()
+        s.set_refinement(4, 0.0)
+        full = s.solve_frequency(w, depth)
+        it4, res4 = s.refine_stats()
+        s.set_refinement(0, 1e-10)
+        with pytest.raises(HamsxError) as e:
+            s.solve_frequency(w, depth)
+        it0, res0 = s.refine_stats()
+    ao, bo, eo, _ = oracle.solve_frequency(m, w, depth, heads)
+    say(f"    tol 1e-10: {it} steps, residual {res:.2e}; 4 fixed steps: residual {res4:.2e}; "
+        f"no step: residual {res0:.2e}")
+    assert e.value.code == HAMSX_ENOCONV and it0 == 0 and res0 > 1e-10
+    assert 1 <= it <= 4 and res <= 1e-10 and it4 == 4 and res4 <= 1e-12
+    for got in (auto, full):
+        for g, o in zip(got[:3], (ao, bo, eo)):
+            assert rel(g, o) <= 1e-8
+
+
 def test_single_cylinder_config():
     case = configs.single_cylinder(na=24, nz=6, nr=3)
     m = combine(case.bodies, case.xr)
