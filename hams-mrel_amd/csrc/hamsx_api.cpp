@@ -22,6 +22,7 @@
 #include <rocblas/rocblas.h>
 
 #include "hamsx_internal.hpp"
+#include "hmatrix_host.hpp"
 
 namespace hamsx {
 bool debug_launches() {
@@ -213,6 +214,25 @@ struct hamsx_ctx {
   int solver = HAMSX_SOLVER_LU;
   int gm_restart = 25, gm_maxiter = 100;
   double gm_tol = 1.0e-4;
+  // H-matrix operator of the GMRES path (HMatrix.f90:56-63: leaf 32, eta 2,
+  // ACA tolerance 1e-4, rank <= 50); the partition is kept per mesh
+  bool hm_on = true;
+  double hm_eta = 2.0, hm_tol = 1.0e-4;
+  int hm_leaf = 32, hm_rank = 50;
+  bool hm_part = false, hm_built = false;
+  std::vector<double> hcen;  // hull centroids, n x 3
+  std::vector<int> hm_perm;
+  std::vector<hamsx::HNode> hm_tree;
+  std::vector<hamsx::HBlk> hm_blocks;
+  std::vector<int> hm_aca_idx;  // partition index of each ACA block
+  DevBuf<int> hmPerm, hmRank, hmLptr, hmLitem;
+  DevBuf<int4> hmAdesc, hmDdesc;
+  DevBuf<int2> hmLeaves;
+  DevBuf<size_t> hmUoff, hmVoff, hmRoff, hmToff, hmDoff;
+  DevBuf<unsigned char> hmUsed;
+  DevBuf<double2> hmU, hmVH, hmD, hmT, hmXp, hmYp;
+  int hm_nlr = 0, hm_ndense = 0, hm_nleaf = 0, hm_nrhs = 0;
+  double hm_mean_rank = 0.0, hm_ratio = 0.0;
   // mesh; with a lid (irregular-frequency removal) the nf = n + nlid field
   // points are the hull centroids followed by the lid centroids
   int n = 0, nbody = 0, nchunk = 0, nlid = 0, nf = 0;
@@ -556,6 +576,167 @@ void givens(cplx a, cplx b, double& cs, cplx& sn) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// H-matrix operator (HMATRIX_BUILD, HMatrix.f90:135-199): the partition of
+// the hull centroids once per mesh and parameters, then per frequency the
+// ACA of the admissible blocks of the operator on the device, the choice
+// dense / low-rank by the reference's storage test (rank (nr + nc) >= nr nc
+// stays dense, :731-737), the dense copies and the per-leaf block lists of
+// the product.
+//
+// ACA's stopping norm: the reference updates ||U VH||_F^2 with
+// sum_J Re(conj(U(J,l)) U(J,k)) Re(conj(VH(k,J)) VH(l,J)) over J = 1..NR
+// (:703-709), a product of real parts entry by entry that also reads VH past
+// its NC columns when NR > NC.  The kernel adds the exact cross terms
+// 2 Re(<u_l, u_k> <v_k, v_l>) instead, so its ranks follow the stated
+// criterion |u_k| |v_k| < tol ||U VH||_F.
+// ---------------------------------------------------------------------------
+int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
+  using hamsx::HBlk;
+  const int n = c->n;
+  int rc;
+  if (!c->hm_part) {
+    hamsx::hmatrix_partition(n, c->hcen.data(), c->hm_leaf, c->hm_eta, c->hm_perm, c->hm_tree, c->hm_blocks);
+    if ((rc = c->hmPerm.ensure(n))) return rc;
+    HX_TRY(hipMemcpyAsync(c->hmPerm.p, c->hm_perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    c->hm_aca_idx.clear();
+    for (size_t k = 0; k < c->hm_blocks.size(); k++)
+      if (c->hm_blocks[k].aca) c->hm_aca_idx.push_back((int)k);
+    // leaves in cluster order
+    std::vector<int2> leaves;
+    for (const auto& nd : c->hm_tree)
+      if (nd.left < 0) leaves.push_back(make_int2(nd.s, nd.e + 1));
+    std::sort(leaves.begin(), leaves.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
+    c->hm_nleaf = (int)leaves.size();
+    if ((rc = c->hmLeaves.ensure(leaves.size()))) return rc;
+    HX_TRY(hipMemcpyAsync(c->hmLeaves.p, leaves.data(), sizeof(int2) * leaves.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HX_TRY(hipStreamSynchronize(c->stream));
+    c->hm_part = true;
+  }
+  // ACA of the admissible blocks: worst-case storage for rank min(rank_max, nr, nc)
+  const int na = (int)c->hm_aca_idx.size();
+  std::vector<int4> adesc(na);
+  std::vector<size_t> uoff(na), voff(na), roff(na);
+  size_t us = 0, vs = 0, rsum = 0;
+  for (int a = 0; a < na; a++) {
+    const HBlk& b = c->hm_blocks[c->hm_aca_idx[a]];
+    const size_t km = (size_t)std::min(c->hm_rank, std::min(b.nr, b.nc));
+    adesc[a] = make_int4(b.rs, b.nr, b.cs, b.nc);
+    uoff[a] = us;
+    voff[a] = vs;
+    roff[a] = rsum;
+    us += (size_t)b.nr * km;
+    vs += km * (size_t)b.nc;
+    rsum += (size_t)b.nr;
+  }
+  if ((rc = c->hmAdesc.ensure(std::max(na, 1))) || (rc = c->hmUoff.ensure(std::max(na, 1))) ||
+      (rc = c->hmVoff.ensure(std::max(na, 1))) || (rc = c->hmRoff.ensure(std::max(na, 1))) ||
+      (rc = c->hmRank.ensure(std::max(na, 1))) || (rc = c->hmToff.ensure(std::max(na, 1))) ||
+      (rc = c->hmUsed.ensure(std::max<size_t>(rsum, 1))) || (rc = c->hmU.ensure(std::max<size_t>(us, 1))) ||
+      (rc = c->hmVH.ensure(std::max<size_t>(vs, 1))))
+    return rc;
+  if (na > 0) {
+    HX_TRY(hipMemcpyAsync(c->hmAdesc.p, adesc.data(), sizeof(int4) * na, hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmUoff.p, uoff.data(), sizeof(size_t) * na, hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmVoff.p, voff.data(), sizeof(size_t) * na, hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmRoff.p, roff.data(), sizeof(size_t) * na, hipMemcpyHostToDevice, c->stream));
+  }
+  hamsx::launch::hm_aca(c->stream, Aop, n, c->hmPerm.p, na, c->hmAdesc.p, c->hmUoff.p, c->hmVoff.p, c->hmRoff.p,
+                        c->hmUsed.p, c->hmU.p, c->hmVH.p, c->hmRank.p, c->hm_rank, c->hm_tol);
+  if ((rc = check_launch())) return rc;
+  std::vector<int> rank(na);
+  if (na > 0)
+    HX_TRY(hipMemcpyAsync(rank.data(), c->hmRank.p, sizeof(int) * na, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  // storage test, dense list, per-block kind (>= 0 dense index, < 0 ~low-rank index)
+  std::vector<int> kind(c->hm_blocks.size());
+  std::vector<int4> ddesc;
+  std::vector<size_t> doff, toff(na, 0);
+  size_t ds = 0, ts = 0, lr_words = 0;
+  long long rank_sum = 0;
+  int nlr = 0;
+  std::vector<int> aca_pos(c->hm_blocks.size(), -1);
+  for (int a = 0; a < na; a++) aca_pos[c->hm_aca_idx[a]] = a;
+  for (size_t k = 0; k < c->hm_blocks.size(); k++) {
+    const HBlk& b = c->hm_blocks[k];
+    const int a = aca_pos[k];
+    if (a >= 0 && rank[a] > 0 && (long long)rank[a] * (b.nr + b.nc) < (long long)b.nr * b.nc) {
+      kind[k] = ~a;
+      toff[a] = ts;
+      ts += (size_t)rank[a] * nrhs;
+      lr_words += (size_t)rank[a] * (b.nr + b.nc);
+      rank_sum += rank[a];
+      nlr++;
+      continue;
+    }
+    if (a >= 0) rank[a] = 0;  // kept dense: no product through its factors
+    kind[k] = (int)ddesc.size();
+    ddesc.push_back(make_int4(b.rs, b.nr, b.cs, b.nc));
+    doff.push_back(ds);
+    ds += (size_t)b.nr * b.nc;
+  }
+  const int nd = (int)ddesc.size();
+  if ((rc = c->hmDdesc.ensure(std::max(nd, 1))) || (rc = c->hmDoff.ensure(std::max(nd, 1))) ||
+      (rc = c->hmD.ensure(std::max<size_t>(ds, 1))) || (rc = c->hmT.ensure(std::max<size_t>(ts, 1))) ||
+      (rc = c->hmXp.ensure((size_t)n * nrhs)) || (rc = c->hmYp.ensure((size_t)n * nrhs)))
+    return rc;
+  if (na > 0) {
+    HX_TRY(hipMemcpyAsync(c->hmRank.p, rank.data(), sizeof(int) * na, hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmToff.p, toff.data(), sizeof(size_t) * na, hipMemcpyHostToDevice, c->stream));
+  }
+  if (nd > 0) {
+    HX_TRY(hipMemcpyAsync(c->hmDdesc.p, ddesc.data(), sizeof(int4) * nd, hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmDoff.p, doff.data(), sizeof(size_t) * nd, hipMemcpyHostToDevice, c->stream));
+  }
+  hamsx::launch::hm_dense_gather(c->stream, Aop, n, c->hmPerm.p, nd, c->hmDdesc.p, c->hmDoff.p, c->hmD.p);
+  // per-leaf lists, blocks in partition order
+  std::vector<int> lstart;
+  for (const auto& t : c->hm_tree)
+    if (t.left < 0) lstart.push_back(t.s);
+  std::sort(lstart.begin(), lstart.end());
+  std::vector<std::vector<int>> lists(c->hm_nleaf);
+  for (size_t k = 0; k < c->hm_blocks.size(); k++) {
+    const HBlk& b = c->hm_blocks[k];
+    int q = (int)(std::lower_bound(lstart.begin(), lstart.end(), b.rs) - lstart.begin());
+    for (; q < c->hm_nleaf && lstart[q] < b.rs + b.nr; q++) lists[q].push_back(kind[k]);
+  }
+  std::vector<int> lptr(c->hm_nleaf + 1, 0), litem;
+  for (int q = 0; q < c->hm_nleaf; q++) {
+    litem.insert(litem.end(), lists[q].begin(), lists[q].end());
+    lptr[q + 1] = (int)litem.size();
+  }
+  if ((rc = c->hmLptr.ensure(lptr.size())) || (rc = c->hmLitem.ensure(std::max<size_t>(litem.size(), 1)))) return rc;
+  HX_TRY(hipMemcpyAsync(c->hmLptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, c->stream));
+  if (!litem.empty())
+    HX_TRY(hipMemcpyAsync(c->hmLitem.p, litem.data(), sizeof(int) * litem.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
+  c->hm_nlr = nlr;
+  c->hm_ndense = nd;
+  c->hm_nrhs = nrhs;
+  c->hm_mean_rank = nlr > 0 ? (double)rank_sum / nlr : 0.0;
+  c->hm_ratio = (double)n * n / (double)std::max<size_t>(ds + lr_words, 1);
+  c->hm_built = true;
+  return check_launch();
+}
+
+hamsx::HmDev hm_dev(hamsx_ctx* c) {
+  return hamsx::HmDev{c->hmPerm.p,  (int)c->hm_aca_idx.size(), c->hm_nleaf, c->hmAdesc.p, c->hmRank.p,
+                      c->hmUoff.p,  c->hmVoff.p, c->hmToff.p, c->hmU.p,     c->hmVH.p,    c->hmT.p,
+                      c->hmDdesc.p, c->hmDoff.p, c->hmD.p,    c->hmLeaves.p, c->hmLptr.p, c->hmLitem.p};
+}
+
+// the GMRES operator: Y = beta B + alpha Op X, Op the H-matrix when built,
+// else the dense matrix
+void op_matvec(hamsx_ctx* c, const double2* Aop, const double2* X, int nrhs, const double2* B, double alpha,
+               double beta, double2* Y) {
+  if (c->hm_built)
+    hamsx::launch::hm_matvec(c->stream, hm_dev(c), c->n, X, nrhs, B, alpha, beta, Y, c->hmXp.p, c->hmYp.p);
+  else
+    hamsx::launch::matvec(c->stream, Aop, c->n, X, nrhs, B, alpha, beta, Y, c->mvw.p);
+}
+
 int do_solve_gmres(hamsx_ctx* c) {
   const int n = c->n, nr = c->nrhs(), m = c->gm_restart;
   const size_t blk = (size_t)n * nr;
@@ -571,6 +752,8 @@ int do_solve_gmres(hamsx_ctx* c) {
   if ((rc = ensure_normal_matrix(c))) return rc;
   const double2* Aop = system_matrix(c);
   if ((rc = gmres_precond_setup(c))) return rc;
+  c->hm_built = false;
+  if (c->hm_on && (rc = hm_build(c, Aop, nr))) return rc;
   // B = right-hand sides, X = 0
   HX_TRY(hipMemcpyAsync(c->gB.p, c->X.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
   HX_TRY(hipMemsetAsync(c->X.p, 0, sizeof(double2) * blk, c->stream));
@@ -590,7 +773,7 @@ int do_solve_gmres(hamsx_ctx* c) {
     if (iter == 0)
       HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
     else
-      hamsx::launch::matvec(c->stream, Aop, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+      op_matvec(c, Aop, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p);
     hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
     HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
     HX_TRY(hipStreamSynchronize(c->stream));
@@ -615,7 +798,7 @@ int do_solve_gmres(hamsx_ctx* c) {
       iter++;
       double2* Vj = c->gV.p + blk * j;
       gmres_precond_apply(c, Vj, c->gZ.p, nr);
-      hamsx::launch::matvec(c->stream, Aop, n, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, c->mvw.p);
+      op_matvec(c, Aop, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p);
       double2* d1 = c->gDots.p;
       double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
       hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1);
@@ -691,8 +874,8 @@ int do_solve_gmres(hamsx_ctx* c) {
     hamsx::launch::axpy_update(c->stream, c->X.p, c->gZ.p, blk);
     HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above are reused next cycle
   }
-  // final true residual for the statistics
-  hamsx::launch::matvec(c->stream, Aop, n, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p, c->mvw.p);
+  // final residual of the operator GMRES worked with, for the statistics
+  op_matvec(c, Aop, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p);
   hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
   HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
@@ -1062,6 +1245,8 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   }
   c->n = nelem;
   c->bin_n = 0;
+  c->hm_part = c->hm_built = false;
+  c->hcen.assign(cen, cen + 3 * n);
   c->nbody = nbody;
   c->nchunk = (int)cb.size();
   c->body_start.assign(body_start, body_start + nbody + 1);
@@ -1345,6 +1530,75 @@ int hamsx_set_refinement(hamsx_ctx* c, int max_steps, double tol) {
   c->refine_iters = max_steps;
   c->refine_tol = tol;
   c->have_X = false;
+  return 0;
+}
+
+int hamsx_set_hmatrix(hamsx_ctx* c, int enable, double eta, double aca_tol, int leaf, int rank_max) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (!(eta > 0.0) || !(aca_tol > 0.0) || leaf < 2 || rank_max < 1 || rank_max > 64)
+    return fail(HAMSX_EARG, "H-matrix: need eta > 0, aca_tol > 0, leaf >= 2, 1 <= rank_max <= 64");
+  c->hm_on = enable != 0;
+  c->hm_eta = eta;
+  c->hm_tol = aca_tol;
+  c->hm_leaf = leaf;
+  c->hm_rank = rank_max;
+  c->hm_part = c->hm_built = false;
+  c->have_X = false;
+  return 0;
+}
+
+int hamsx_hmatrix_stats(hamsx_ctx* c, int* nlowrank, int* ndense, double* mean_rank, double* compression) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (nlowrank) *nlowrank = c->hm_built ? c->hm_nlr : 0;
+  if (ndense) *ndense = c->hm_built ? c->hm_ndense : 0;
+  if (mean_rank) *mean_rank = c->hm_built ? c->hm_mean_rank : 0.0;
+  if (compression) *compression = c->hm_built ? c->hm_ratio : 0.0;
+  return 0;
+}
+
+int hamsx_hmatrix_matvec(hamsx_ctx* c, const double* x, int nrhs, double* y) {
+  if (!c || !x || !y || nrhs <= 0) return fail(HAMSX_EARG, "hamsx_hmatrix_matvec: bad arguments");
+  if (c->pending) return busy(c);
+  if (!c->have_A) return fail(HAMSX_ESTATE, "hamsx_hmatrix_matvec needs an assembled, unfactored matrix");
+  int rc = set_device(c);
+  if (rc || (rc = ensure_normal_matrix(c))) return rc;
+  const double2* Aop = system_matrix(c);
+  if (!c->hm_built || c->hm_nrhs != nrhs) {
+    c->hm_built = false;
+    if ((rc = hm_build(c, Aop, nrhs))) return rc;
+  }
+  const size_t cnt = (size_t)c->n * nrhs;
+  if ((rc = c->gU.ensure(cnt)) || (rc = c->gW.ensure(cnt))) return rc;
+  HX_TRY(hipMemcpyAsync(c->gU.p, x, sizeof(double2) * cnt, hipMemcpyHostToDevice, c->stream));
+  hamsx::launch::hm_matvec(c->stream, hm_dev(c), c->n, c->gU.p, nrhs, nullptr, 1.0, 0.0, c->gW.p, c->hmXp.p,
+                           c->hmYp.p);
+  HX_TRY(hipMemcpyAsync(y, c->gW.p, sizeof(double2) * cnt, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  return check_launch();
+}
+
+int hamsx_hmatrix_partition(int n, const double* cen, int leaf, double eta, int* perm, int max_blocks, int* blocks,
+                            int* nblocks) {
+  if (n <= 0 || !cen || !perm || !nblocks || leaf < 2 || !(eta > 0.0) || max_blocks < 0 ||
+      (max_blocks > 0 && !blocks))
+    return fail(HAMSX_EARG, "hamsx_hmatrix_partition: bad arguments");
+  std::vector<int> pv;
+  std::vector<hamsx::HNode> tree;
+  std::vector<hamsx::HBlk> bl;
+  hamsx::hmatrix_partition(n, cen, leaf, eta, pv, tree, bl);
+  std::copy(pv.begin(), pv.end(), perm);
+  *nblocks = (int)bl.size();
+  if ((int)bl.size() > max_blocks)
+    return fail(HAMSX_EARG, "hamsx_hmatrix_partition: %zu blocks, room for %d", bl.size(), max_blocks);
+  for (size_t k = 0; k < bl.size(); k++) {
+    int* o = blocks + 5 * k;
+    o[0] = bl[k].rs;
+    o[1] = bl[k].nr;
+    o[2] = bl[k].cs;
+    o[3] = bl[k].nc;
+    o[4] = bl[k].aca;
+  }
   return 0;
 }
 

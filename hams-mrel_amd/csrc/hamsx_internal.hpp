@@ -79,6 +79,26 @@ struct WaveDev {
   int isol;
 };
 
+// H-matrix operator on the device (hmatrix.hip): low-rank blocks
+// (ldesc/rank/uoff/voff/U/VH, T = per-block VH x scratch at toff), dense
+// blocks (ddesc/doff/D), and per leaf cluster of rows [leaves[k].x,
+// leaves[k].y) the blocks covering it, litem[lptr[k] .. lptr[k+1]) (item >= 0
+// dense block, ~item low-rank block).  perm: cluster position -> panel.
+struct HmDev {
+  const int* perm;
+  int nlr, nleaf;
+  const int4* ldesc;
+  const int* rank;
+  const size_t *uoff, *voff, *toff;
+  const double2 *U, *VH;
+  double2* T;
+  const int4* ddesc;
+  const size_t* doff;
+  const double2* D;
+  const int2* leaves;
+  const int *lptr, *litem;
+};
+
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
@@ -138,6 +158,15 @@ void block_combine(hipStream_t s, const double2* V, int m, int n, int nrhs, cons
 void rows_pack(hipStream_t s, const double2* X, int n, int r0, int nb, int nrhs, double2* P);
 void rows_unpack(hipStream_t s, const double2* P, int n, int r0, int nb, int nrhs, double2* X);
 void block_copy(hipStream_t s, const double2* A, int n, int r0, int nb, double2* B);
+// H-matrix (hmatrix.hip): ACA of nblk admissible blocks of A (lda = n), the
+// dense-block copies, and Y = beta B + alpha H X (Xp, Yp: n x nrhs scratch)
+void hm_aca(hipStream_t s, const double2* A, int lda, const int* perm, int nblk, const int4* desc, const size_t* uoff,
+            const size_t* voff, const size_t* roff, unsigned char* used, double2* U, double2* VH, int* rank,
+            int rank_max, double tol);
+void hm_dense_gather(hipStream_t s, const double2* A, int lda, const int* perm, int nblk, const int4* desc,
+                     const size_t* doff, double2* D);
+void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
+               double beta, double2* Y, double2* Xp, double2* Yp);
 }  // namespace launch
 
 }  // namespace hamsx

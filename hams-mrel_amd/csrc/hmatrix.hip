@@ -1,0 +1,336 @@
+// hmatrix.hip -- H-matrix operator for the GMRES path, gfx950: ACA
+// compression of the admissible blocks of the assembled matrix and the
+// H-matrix x block-of-vectors product (HMatrix.f90:556-820 of the reference:
+// ADD_LOWRANK_BLOCK, ADD_DENSE_BLOCK, HMATVEC).
+//
+// Layout (all blocks in cluster order, rows [rs, rs + nr), columns
+// [cs, cs + nc); perm maps a cluster position to a panel):
+//   * low-rank block: U (nr x rank, column l contiguous) and VH (rank x nc,
+//     row l contiguous), A_block ~= U VH;
+//   * dense block: D (nr x nc, column-major), a copy of A_block.
+// The product runs in three passes: T_b = VH_b x_b for every low-rank block;
+// then per leaf cluster of rows, y_i = sum over the blocks that cover row i,
+// in a fixed order, of D_b(i, :) x_b or U_b(i, :) T_b; then the scatter back
+// to panel order with y = beta B + alpha (H x).  Every output element is
+// summed by one thread in a fixed order, so the product is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "hamsx_internal.hpp"
+
+namespace hamsx {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double cabs2(double2 a) { return hypot(a.x, a.y); }
+__device__ __forceinline__ double2 cdivz(double2 a, double2 b) {  // a / b, Smith
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double q = b.y / b.x, d = b.x + b.y * q;
+    return make_double2((a.x + a.y * q) / d, (a.y - a.x * q) / d);
+  }
+  const double q = b.x / b.y, d = b.x * q + b.y;
+  return make_double2((a.x * q + a.y) / d, (a.y * q - a.x) / d);
+}
+
+// block-wide argmax of (v, i): largest v, lowest i on ties; every thread
+// gets the result
+__device__ __forceinline__ void block_argmax(double& v, int& i, double* sv, int* si) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(v, o);
+    const int oi = __shfl_xor(i, o);
+    if (ov > v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    sv[w] = v;
+    si[w] = i;
+  }
+  __syncthreads();
+  v = sv[0];
+  i = si[0];
+  for (int q = 1; q < 4; q++)
+    if (sv[q] > v || (sv[q] == v && si[q] < i)) {
+      v = sv[q];
+      i = si[q];
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sv) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sv[w] = v;
+  __syncthreads();
+  return (sv[0] + sv[1]) + (sv[2] + sv[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Partially pivoted ACA of one admissible block per 256-thread workgroup,
+// the steps of ADD_LOWRANK_BLOCK (HMatrix.f90:556-750): pivot row 0 first,
+// then the unused row with the largest |U(:, k-1)|; residual row, pivot
+// column = its largest entry; residual column divided by the pivot; stop
+// when |u_k| |v_k| < tol ||U VH||_F, after rank_max steps, on a zero pivot
+// or when every row is used.  ||U VH||_F^2 is updated with the exact cross
+// terms 2 Re(<u_l, u_k> <v_k, v_l>) of the rank-one sum (the reference's
+// update multiplies real parts entry by entry instead, see the note at the
+// host caller).  rank[b] receives the rank reached.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_aca(const double2* __restrict__ A, int lda, const int* __restrict__ perm,
+                                             const int4* __restrict__ desc, const size_t* __restrict__ uoff,
+                                             const size_t* __restrict__ voff, const size_t* __restrict__ roff,
+                                             unsigned char* __restrict__ used, double2* __restrict__ U,
+                                             double2* __restrict__ VH, int* __restrict__ rank, int rank_max,
+                                             double tol) {
+  __shared__ double sv[4];
+  __shared__ int si[4];
+  __shared__ double2 part[4][64];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int4 d = desc[b];
+  const int rs = d.x, nr = d.y, cs = d.z, nc = d.w;
+  const int kmax = min(rank_max, min(nr, nc));
+  double2* Ub = U + uoff[b];
+  double2* Vb = VH + voff[b];
+  unsigned char* ub = used + roff[b];
+  for (int i = t; i < nr; i += 256) ub[i] = 0;
+  double frob = 0.0;
+  int k = 0, pr = 0, rk = 0;
+  __syncthreads();
+  for (k = 0; k < kmax; k++) {
+    if (k > 0) {
+      double best = -1.0;
+      int bi = 0x7fffffff;
+      for (int i = t; i < nr; i += 256)
+        if (!ub[i]) {
+          const double v = cabs2(Ub[(size_t)(k - 1) * nr + i]);
+          if (v > best) {
+            best = v;
+            bi = i;
+          }
+        }
+      block_argmax(best, bi, sv, si);
+      if (best < 0.0) break;  // every row used
+      pr = bi;
+    }
+    __syncthreads();
+    if (t == 0) ub[pr] = 1;
+    // residual row -> VH(k, :), its largest entry and its squared norm
+    const size_t gi = (size_t)perm[rs + pr];
+    double rbest = -1.0, vv = 0.0;
+    int rj = 0x7fffffff;
+    for (int j = t; j < nc; j += 256) {
+      double2 v = A[gi + (size_t)lda * perm[cs + j]];
+      for (int l = 0; l < k; l++) {
+        const double2 p = cmul(Ub[(size_t)l * nr + pr], Vb[(size_t)l * nc + j]);
+        v.x -= p.x;
+        v.y -= p.y;
+      }
+      Vb[(size_t)k * nc + j] = v;
+      const double a = cabs2(v);
+      vv += v.x * v.x + v.y * v.y;
+      if (a > rbest) {
+        rbest = a;
+        rj = j;
+      }
+    }
+    block_argmax(rbest, rj, sv, si);  // its barriers publish the row
+    const int pc = rj;
+    const double2 piv = Vb[(size_t)k * nc + pc];
+    if (cabs2(piv) < 1.0e-30) break;
+    vv = block_sum(vv, sv);
+    // residual column / pivot -> U(:, k)
+    const size_t gj = (size_t)lda * perm[cs + pc];
+    double uu = 0.0;
+    for (int i = t; i < nr; i += 256) {
+      double2 v = A[(size_t)perm[rs + i] + gj];
+      for (int l = 0; l < k; l++) {
+        const double2 p = cmul(Ub[(size_t)l * nr + i], Vb[(size_t)l * nc + pc]);
+        v.x -= p.x;
+        v.y -= p.y;
+      }
+      v = cdivz(v, piv);
+      Ub[(size_t)k * nr + i] = v;
+      uu += v.x * v.x + v.y * v.y;
+    }
+    uu = block_sum(uu, sv);  // its barriers publish U(:, k)
+    rk = k + 1;
+    const double uvn = sqrt(uu) * sqrt(vv);
+    frob += uvn * uvn;
+    if (k > 0) {
+      // cross terms: thread (l, p) sums its quarter of <u_l, u_k> and of <v_k, v_l>
+      const int l = t & 63, p = t >> 6;
+      double2 su = make_double2(0.0, 0.0), svv = make_double2(0.0, 0.0);
+      if (l < k) {
+        const int i0 = p * nr / 4, i1 = (p + 1) * nr / 4;
+        for (int i = i0; i < i1; i++) {
+          const double2 a = Ub[(size_t)l * nr + i], c = Ub[(size_t)k * nr + i];
+          su.x += a.x * c.x + a.y * c.y;  // conj(a) c
+          su.y += a.x * c.y - a.y * c.x;
+        }
+        const int j0 = p * nc / 4, j1 = (p + 1) * nc / 4;
+        for (int j = j0; j < j1; j++) {
+          const double2 a = Vb[(size_t)k * nc + j], c = Vb[(size_t)l * nc + j];
+          svv.x += a.x * c.x + a.y * c.y;  // a conj(c)
+          svv.y += a.y * c.x - a.x * c.y;
+        }
+      }
+      part[p][l] = su;
+      __syncthreads();
+      double2 du = make_double2(0.0, 0.0);
+      if (p == 0 && l < k)
+        for (int q = 0; q < 4; q++) {
+          du.x += part[q][l].x;
+          du.y += part[q][l].y;
+        }
+      __syncthreads();
+      part[p][l] = svv;
+      __syncthreads();
+      double cross = 0.0;
+      if (p == 0 && l < k) {
+        double2 dv = make_double2(0.0, 0.0);
+        for (int q = 0; q < 4; q++) {
+          dv.x += part[q][l].x;
+          dv.y += part[q][l].y;
+        }
+        cross = du.x * dv.x - du.y * dv.y;  // Re(du dv)
+      }
+      frob += 2.0 * block_sum(cross, sv);
+    }
+    if (uvn < tol * sqrt(fmax(frob, 1.0e-30))) break;
+  }
+  if (t == 0) rank[b] = rk;
+}
+
+// dense blocks: D_b = A(perm rows, perm cols), one workgroup per block
+__global__ __launch_bounds__(256) void k_dense_gather(const double2* __restrict__ A, int lda,
+                                                     const int* __restrict__ perm, const int4* __restrict__ desc,
+                                                     const size_t* __restrict__ doff, double2* __restrict__ D) {
+  const int4 d = desc[blockIdx.x];
+  double2* Db = D + doff[blockIdx.x];
+  const size_t tot = (size_t)d.y * d.w;
+  for (size_t e = threadIdx.x; e < tot; e += 256) {
+    const int i = (int)(e % d.y), j = (int)(e / d.y);
+    Db[e] = A[(size_t)perm[d.x + i] + (size_t)lda * perm[d.z + j]];
+  }
+}
+
+// Xp(k, q) = X(perm[k], q)
+__global__ void k_perm_gather(const int* __restrict__ perm, const double2* __restrict__ X, int n, int nrhs,
+                              double2* __restrict__ Xp) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int k = (int)(e % n), q = (int)(e / n);
+  Xp[e] = X[(size_t)perm[k] + (size_t)n * q];
+}
+
+// T_b(l, q) = sum_j VH_b(l, j) Xp(cs + j, q), rank_b x nrhs (column-major);
+// workgroup (b, q-group of 4), thread (l = t / 4, q = q0 + t % 4)
+__global__ __launch_bounds__(256) void k_hm_vx(const int4* __restrict__ desc, const int* __restrict__ rank,
+                                               const size_t* __restrict__ voff, const size_t* __restrict__ toff,
+                                               const double2* __restrict__ VH, const double2* __restrict__ Xp,
+                                               int n, int nrhs, double2* __restrict__ T) {
+  const int b = blockIdx.x;
+  const int4 d = desc[b];
+  const int rk = rank[b];
+  const int l = threadIdx.x >> 2, q = blockIdx.y * 4 + (threadIdx.x & 3);
+  if (l >= rk || q >= nrhs) return;
+  const double2* v = VH + voff[b] + (size_t)l * d.w;
+  const double2* x = Xp + (size_t)d.z + (size_t)n * q;
+  double sr = 0.0, si = 0.0;
+  for (int j = 0; j < d.w; j++) {
+    const double2 a = v[j], c = x[j];
+    sr += a.x * c.x - a.y * c.y;
+    si += a.x * c.y + a.y * c.x;
+  }
+  T[toff[b] + (size_t)l + (size_t)rk * q] = make_double2(sr, si);
+}
+
+// Per leaf cluster of rows: thread (row r, rhs q) sums the blocks listed for
+// the leaf in order; item >= 0: dense block, item < 0: low-rank block ~item.
+// Workgroup (leaf, q-group of 8): 32 rows x 8 right-hand sides.
+__global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves, const int* __restrict__ lptr,
+                                                 const int* __restrict__ litem, const int4* __restrict__ ddesc,
+                                                 const size_t* __restrict__ doff, const double2* __restrict__ D,
+                                                 const int4* __restrict__ ldesc, const int* __restrict__ rank,
+                                                 const size_t* __restrict__ uoff, const size_t* __restrict__ toff,
+                                                 const double2* __restrict__ U, const double2* __restrict__ T,
+                                                 const double2* __restrict__ Xp, int n, int nrhs,
+                                                 double2* __restrict__ Yp) {
+  const int2 lf = leaves[blockIdx.x];
+  const int q = blockIdx.y * 8 + (threadIdx.x & 7);
+  if (q >= nrhs) return;
+  for (int row = lf.x + (threadIdx.x >> 3); row < lf.y; row += 32) {
+    double sr = 0.0, si = 0.0;
+    for (int it = lptr[blockIdx.x]; it < lptr[blockIdx.x + 1]; it++) {
+      const int item = litem[it];
+      if (item >= 0) {
+        const int4 d = ddesc[item];
+        const double2* a = D + doff[item] + (row - d.x);
+        const double2* x = Xp + (size_t)d.z + (size_t)n * q;
+        for (int j = 0; j < d.w; j++) {
+          const double2 av = a[(size_t)d.y * j], xv = x[j];
+          sr += av.x * xv.x - av.y * xv.y;
+          si += av.x * xv.y + av.y * xv.x;
+        }
+      } else {
+        const int lb = ~item;
+        const int4 d = ldesc[lb];
+        const int rk = rank[lb];
+        const double2* u = U + uoff[lb] + (row - d.x);
+        const double2* tv = T + toff[lb] + (size_t)rk * q;
+        for (int l = 0; l < rk; l++) {
+          const double2 av = u[(size_t)d.y * l], xv = tv[l];
+          sr += av.x * xv.x - av.y * xv.y;
+          si += av.x * xv.y + av.y * xv.x;
+        }
+      }
+    }
+    Yp[(size_t)row + (size_t)n * q] = make_double2(sr, si);
+  }
+}
+
+// Y(perm[k], q) = beta B(perm[k], q) + alpha Yp(k, q)   (B may be null)
+__global__ void k_perm_scatter(const int* __restrict__ perm, const double2* __restrict__ Yp, int n, int nrhs,
+                               const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int k = (int)(e % n), q = (int)(e / n);
+  const size_t o = (size_t)perm[k] + (size_t)n * q;
+  double yr = alpha * Yp[e].x, yi = alpha * Yp[e].y;
+  if (B) {
+    yr += beta * B[o].x;
+    yi += beta * B[o].y;
+  }
+  Y[o] = make_double2(yr, yi);
+}
+
+namespace launch {
+void hm_aca(hipStream_t s, const double2* A, int lda, const int* perm, int nblk, const int4* desc, const size_t* uoff,
+            const size_t* voff, const size_t* roff, unsigned char* used, double2* U, double2* VH, int* rank,
+            int rank_max, double tol) {
+  if (nblk <= 0) return;
+  HX_LAUNCH(k_aca, dim3(nblk), dim3(256), 0, s, A, lda, perm, desc, uoff, voff, roff, used, U, VH, rank, rank_max,
+            tol);
+}
+void hm_dense_gather(hipStream_t s, const double2* A, int lda, const int* perm, int nblk, const int4* desc,
+                     const size_t* doff, double2* D) {
+  if (nblk <= 0) return;
+  HX_LAUNCH(k_dense_gather, dim3(nblk), dim3(256), 0, s, A, lda, perm, desc, doff, D);
+}
+void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
+               double beta, double2* Y, double2* Xp, double2* Yp) {
+  const size_t tot = (size_t)n * nrhs;
+  const dim3 g1((unsigned)((tot + 255) / 256));
+  HX_LAUNCH(k_perm_gather, g1, dim3(256), 0, s, h.perm, X, n, nrhs, Xp);
+  if (h.nlr > 0)
+    HX_LAUNCH(k_hm_vx, dim3(h.nlr, (nrhs + 3) / 4), dim3(256), 0, s, h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n,
+              nrhs, h.T);
+  HX_LAUNCH(k_hm_leaf, dim3(h.nleaf, (nrhs + 7) / 8), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff,
+            h.D, h.ldesc, h.rank, h.uoff, h.toff, h.U, h.T, Xp, n, nrhs, Yp);
+  HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);
+}
+}  // namespace launch
+}  // namespace hamsx

@@ -52,6 +52,10 @@ SIGNATURES = {
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
     "hamsx_set_solver": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
     "hamsx_set_refinement": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "hamsx_set_hmatrix": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "hamsx_hmatrix_stats": (C.c_int, [_vp, _ip, _ip, _dp, _dp]),
+    "hamsx_hmatrix_matvec": (C.c_int, [_vp, _dp, C.c_int, _dp]),
+    "hamsx_hmatrix_partition": (C.c_int, [C.c_int, _dp, C.c_int, C.c_double, _ip, C.c_int, _ip, _ip]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),

@@ -172,6 +172,30 @@ class FrequencySolver:
                    "hamsx_get_matrix")
         return (out, ipiv) if code == 2 else out
 
+    def set_hmatrix(self, enable: bool = True, eta: float = 2.0, aca_tol: float = 1.0e-4, leaf: int = 32,
+                    rank_max: int = 50):
+        """H-matrix operator of the GMRES path (defaults: the reference's
+        HMatrix.f90 parameters); enable=False uses the dense matrix."""
+        _lib.check(self._lib.hamsx_set_hmatrix(self._h, int(bool(enable)), float(eta), float(aca_tol), int(leaf),
+                                                int(rank_max)), "hamsx_set_hmatrix")
+
+    def hmatrix_stats(self):
+        """(low-rank blocks, dense blocks, mean rank, N^2 / stored entries) of the last build."""
+        a, b, r, q = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+        _lib.check(self._lib.hamsx_hmatrix_stats(self._h, C.byref(a), C.byref(b), C.byref(r), C.byref(q)),
+                   "hamsx_hmatrix_stats")
+        return a.value, b.value, r.value, q.value
+
+    def hmatrix_matvec(self, x):
+        """H x with the H-matrix of the assembled (unfactored) operator."""
+        x = np.asfortranarray(x, dtype=np.complex128)
+        if x.ndim == 1:
+            x = x[:, None]
+        y = np.zeros_like(x, order="F")
+        _lib.check(self._lib.hamsx_hmatrix_matvec(self._h, _lib.ptr(x), x.shape[1], _lib.ptr(y)),
+                   "hamsx_hmatrix_matvec")
+        return y
+
     def set_refinement(self, max_steps: int = 4, tol: float = 1.0e-10):
         """Mixed precision: at most max_steps FP64-residual / complex64
         correction steps, stopping once max |R|/|X| per column <= tol."""
@@ -192,3 +216,19 @@ class FrequencySolver:
         _lib.check(self._lib.hamsx_stage_times(self._h, 1 if reset else 0, _lib.ptr(ms), _lib.ptr(cnt, C.c_int)),
                    "hamsx_stage_times")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_lib.STAGES)}
+
+
+def hmatrix_partition(cen, leaf: int = 32, eta: float = 2.0):
+    """The H-matrix partition of centroids cen (n x 3), computed by the
+    library on the host: (perm, blocks) with blocks rows (rs, nr, cs, nc, aca),
+    0-based, in creation order."""
+    lib = _lib.load()
+    c = np.asfortranarray(cen, dtype=np.float64)
+    n = c.shape[0]
+    perm = np.zeros(n, dtype=np.int32)
+    nb = C.c_int()
+    cap = 64 * n + 64
+    blocks = np.zeros(5 * cap, dtype=np.int32)
+    _lib.check(lib.hamsx_hmatrix_partition(n, _lib.ptr(c), int(leaf), float(eta), _lib.ptr(perm, C.c_int), cap,
+                                           _lib.ptr(blocks, C.c_int), C.byref(nb)), "hamsx_hmatrix_partition")
+    return perm, blocks[:5 * nb.value].reshape(nb.value, 5)

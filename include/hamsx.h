@@ -161,6 +161,35 @@ int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
 #define HAMSX_SOLVER_GMRES 1
 int hamsx_set_solver(hamsx_ctx *ctx, int solver, double tol, int restart, int maxiter);
 
+/* H-matrix operator of the GMRES path (the reference's HMATRIX_BUILD /
+ * HMATVEC, src/HMatrix.f90:135-820, built at src/AssbMatxMulti.f90:288,
+ * :356 for every ISOLV = 2 solve): cluster tree by geometric bisection with
+ * at most `leaf` panels per leaf, blocks admissible when the distance of the
+ * cluster centres exceeds eta times the larger diameter, admissible blocks
+ * compressed by partially pivoted ACA to tolerance aca_tol and rank at most
+ * rank_max (<= 64), the rest kept dense.  On by default with the reference's
+ * parameters (enable 1, eta 2, aca_tol 1e-4, leaf 32, rank_max 50); enable 0
+ * makes GMRES use the dense assembled matrix. */
+int hamsx_set_hmatrix(hamsx_ctx *ctx, int enable, double eta, double aca_tol, int leaf, int rank_max);
+
+/* After a GMRES solve with the H-matrix: low-rank and dense block counts,
+ * the mean rank of the low-rank blocks and N^2 over the stored entries. */
+int hamsx_hmatrix_stats(hamsx_ctx *ctx, int *nlowrank, int *ndense, double *mean_rank, double *compression);
+
+/* y = H x for nrhs columns (n x nrhs complex, host arrays) with the
+ * H-matrix of the current assembled (unfactored) operator, built if needed:
+ * the product GMRES uses, for checks against the dense matrix. */
+int hamsx_hmatrix_matvec(hamsx_ctx *ctx, const double *x, int nrhs, double *y);
+
+/* The partition alone, on the host (no device): cen n x 3 column-major;
+ * perm[k] = 0-based panel at cluster position k; blocks[5 k .. 5 k + 4] =
+ * row start, rows, column start, columns (cluster order, 0-based) and 1 for
+ * an ACA candidate / 0 for a dense block, in the reference's creation
+ * order.  *nblocks receives the count; HAMSX_EARG when it exceeds
+ * max_blocks. */
+int hamsx_hmatrix_partition(int n, const double *cen, int leaf, double eta, int *perm, int max_blocks, int *blocks,
+                            int *nblocks);
+
 /* Mixed precision (HAMSX_PREC_MIXED) iterative refinement: after the
  * complex64 LU solve, FP64 residual + complex64 correction steps run until
  * max over the right-hand sides of |R|_max / |X|_max <= tol, at most

@@ -281,6 +281,7 @@ def test_gmres_matches_lu(depth):
     say(f"[gpu] gmres vs lu n={m.nelem} depth={depth}")
     with FrequencySolver(m) as s:
         s.set_waves(heads)
+        s.set_hmatrix(False)  # dense operator: GMRES alone against the LU
         lu = s.solve_frequency(1.1, depth)
         s.set_solver("gmres", tol=1e-11, restart=25, maxiter=400)
         tight = s.solve_frequency(1.1, depth)
@@ -293,6 +294,64 @@ def test_gmres_matches_lu(depth):
     for a, b, c in zip(tight[:3], lu[:3], loose[:3]):
         assert rel(a, b) < 1e-8
         assert rel(c, b) < 1e-3
+
+
+@pytest.mark.parametrize("depth,tol", [(-1.0, 1e-6), (40.0, 1e-4)])
+def test_hmatrix_matvec_matches_oracle(depth, tol):
+    """The device H-matrix (ACA of the assembled matrix on the GPU) against
+    the oracle's restatement built from the same matrix, and against the
+    dense product."""
+    import hmatrix as ohm
+    m = small_pair()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((m.nelem, 3)) + 1j * rng.standard_normal((m.nelem, 3))
+    say(f"[gpu] hmatrix n={m.nelem} depth={depth} tol={tol}")
+    with FrequencySolver(m) as s:
+        s.calwaveproperts(1.2, depth)
+        s.assemble()
+        A = s.get_matrix("A")
+        s.set_hmatrix(True, 2.0, tol, 32, 50)
+        y = s.hmatrix_matvec(x)
+        nlr, nd, mr, ratio = s.hmatrix_stats()
+    H = ohm.build(A, m.cen, 32, 2.0, tol, 50)
+    yo = ohm.matvec(H, x)
+    lr = [b for b in H[1] if b[4] == "lr"]
+    omr = float(np.mean([b[5][0].shape[1] for b in lr])) if lr else 0.0
+    ed, eo = rel(y, A @ x), rel(y, yo)
+    say(f"    {nlr} low-rank (oracle {len(lr)}), {nd} dense, mean rank {mr:.2f} (oracle {omr:.2f}), "
+        f"compression {ratio:.2f}; vs dense {ed:.2e}, vs oracle H {eo:.2e}")
+    assert nlr == len(lr) and nlr > 0 and abs(mr - omr) <= 0.25
+    assert ed <= 50 * tol and eo <= 10 * tol
+
+
+@pytest.mark.parametrize("depth", [-1.0, 35.0])
+def test_gmres_hmatrix_matches_lu(depth):
+    """GMRES on the H-matrix operator: with a tight ACA tolerance it reaches
+    the LU coefficients to 1e-6; with the reference's parameters (ACA 1e-4,
+    GMRES 1e-4) it stays within the reference's own GMRES test bounds (added
+    mass 1e-3, damping 5e-2: test/test_integration_gmres.f90:69,97)."""
+    m = small_pair(na=28, nz=7, nr=3)
+    heads = np.deg2rad([0.0, 45.0])
+    say(f"[gpu] gmres + hmatrix vs lu n={m.nelem} depth={depth}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        lu = s.solve_frequency(1.1, depth)
+        s.set_solver("gmres", tol=1e-10, restart=25, maxiter=400)
+        s.set_hmatrix(True, 2.0, 1e-9, 32, 50)
+        tight = s.solve_frequency(1.1, depth)
+        st = s.hmatrix_stats()
+        s.set_solver("gmres")
+        s.set_hmatrix()
+        ref = s.solve_frequency(1.1, depth)
+        it, res = s.refine_stats()
+        st2 = s.hmatrix_stats()
+    et = [rel(a, b) for a, b in zip(tight[:3], lu[:3])]
+    er = [rel(a, b) for a, b in zip(ref[:3], lu[:3])]
+    say(f"    tight ACA: {st[0]} low-rank blocks, rank {st[2]:.1f}, err {et}; reference parameters: "
+        f"{st2[0]} low-rank, rank {st2[2]:.1f}, {it} iterations, residual {res:.1e}, err {er}")
+    assert st[0] > 0 and st2[0] > 0
+    assert max(et) < 1e-6
+    assert er[0] < 1e-3 and er[1] < 5e-2 and er[2] < 1e-3
 
 
 def lid_centroids(lid):
