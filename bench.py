@@ -121,7 +121,7 @@ def parse():
                     "stream each)")
     ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
     ap.add_argument("--config", default="two-cylinders",
-                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving"])
+                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving", "large-array"])
     ap.add_argument("--requests", type=int, default=40, help="serving: requests in the Poisson stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrival rate, requests/s")
     ap.add_argument("--seed", type=int, default=7, help="serving: seed of the request stream")
@@ -238,6 +238,72 @@ def serving(args):
     print(json.dumps(out), flush=True)
 
 
+def large_array(args):
+    """16-cylinder array with lids (about 30,000 hull panels, depth 80 m):
+    GMRES on the H-matrix operator (eta 2, ACA tolerance 1e-6, GMRES
+    tolerance 1e-8) per frequency, against the dense FP64 LU at the first
+    frequency.  One warm-up solve (pair binning, H-matrix partition) is
+    untimed; then `steps` frequencies spread over [0.2, 2.0] rad/s."""
+    from hamsx import configs
+    from hamsx.mesh import combine, panel_geometry
+    from hamsx.solver import FrequencySolver
+    case = configs.large_array()
+    mesh = combine(case.bodies, case.xr)
+    lid = np.concatenate([panel_geometry(lm, np.zeros(3))[0] for lm in case.meta["lids"]])
+    heads = np.deg2rad(case.headings_deg)
+    steps = args.steps if "--steps" in sys.argv else 2
+    omegas = [float(w) for w in np.linspace(case.omegas[0], case.omegas[-1], steps)]
+    print(f"[bench] {case.name}: N={mesh.nelem} hull panels, {len(lid)} lid points, depth {case.depth}, "
+          f"{steps} timed frequencies", file=sys.stderr, flush=True)
+    with FrequencySolver(mesh) as s:
+        s.set_lid(lid)
+        s.set_waves(heads)
+        s.set_solver("gmres", tol=1e-8, restart=25, maxiter=400)
+        s.set_hmatrix(True, 2.0, 1e-6, 32, 50)
+        t0 = time.perf_counter()
+        warm = s.solve_frequency(omegas[0], case.depth)
+        t_warm = time.perf_counter() - t0
+        print(f"[bench] warm-up solve {t_warm:.2f} s", file=sys.stderr, flush=True)
+        s.set_profiling(True)
+        s.stage_times(reset=True)
+        lat, iters = [], []
+        t0 = time.perf_counter()
+        for w in omegas:
+            t1 = time.perf_counter()
+            _, _, _, info = s.solve_frequency(w, case.depth)
+            lat.append(time.perf_counter() - t1)
+            iters.append(s.refine_stats())
+            if info != 0:
+                raise SystemExit(f"INFO={info} at w={w}")
+            print(f"[bench] w={w:.3f}: {lat[-1]:.2f} s, {iters[-1][0]} GMRES iterations", file=sys.stderr,
+                  flush=True)
+        elapsed = time.perf_counter() - t0
+        stages = s.stage_times(reset=True)
+        s.set_profiling(False)
+        nlr, nd, mrank, ratio = s.hmatrix_stats()
+        s.set_solver("lu")
+        t1 = time.perf_counter()
+        lu = s.solve_frequency(omegas[0], case.depth)
+        t_lu = time.perf_counter() - t1
+    errs = [float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(warm[:3], lu[:3])]
+    ms = {k: v[0] / max(1, v[1]) for k, v in stages.items()}
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": METRIC, "value": steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic cylinder array with lids (no reference files read)",
+        "config": {"workload": case.name, "panels": mesh.nelem, "lid_points": int(len(lid)), "depth_m": case.depth,
+                   "solver": "gmres(25) tol 1e-8 + block-LU preconditioner on the H-matrix (eta 2, ACA 1e-6)",
+                   "frequencies": omegas},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "stage_ms_per_solve": ms,
+        "gmres_iterations": [int(i[0]) for i in iters], "gmres_residual": [float(i[1]) for i in iters],
+        "hmatrix": {"low_rank_blocks": nlr, "dense_blocks": nd, "mean_rank": mrank, "compression": ratio},
+        "vs_dense_lu": {"amas": errs[0], "bdmp": errs[1], "exfc": errs[2], "lu_solve_s": t_lu},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -249,6 +315,9 @@ def main():
         sys.exit(2)
     if args.config == "serving":
         serving(args)
+        return
+    if args.config == "large-array":
+        large_array(args)
         return
     from hamsx.mesh import combine
     from hamsx.solver import FrequencySolver

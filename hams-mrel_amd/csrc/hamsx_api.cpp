@@ -262,7 +262,9 @@ struct hamsx_ctx {
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
   DevBuf<double> gNorm, gScale;
-  DevBuf<int> pPerm, pIpiv;
+  DevBuf<int> pPerm, pIpiv, pSz;
+  DevBuf<size_t> pLuOff, pXoff;  // per body: factor offset, packed right-hand-side offset
+  int pMax = 0;
   std::vector<size_t> pOff;
   DevBuf<int> bin_cnt;
   int bin_n = 0, bin_nf = 0;
@@ -548,17 +550,27 @@ int gmres_precond_setup(hamsx_ctx* c) {
     hamsx::launch::lu_factor(c->stream, blk, sz, c->pIpiv.p, c->pPerm.p + r0, c->widx.p, c->wcnt.p,
                              c->flags.p + 1, c->ludiag.p);
   }
+  // descriptors of the batched block solves
+  const int nr = c->nrhs();
+  std::vector<int> sz(nb);
+  std::vector<size_t> xo(nb);
+  for (int b = 0; b < nb; b++) {
+    sz[b] = c->body_start[b + 1] - c->body_start[b];
+    xo[b] = (size_t)nr * c->body_start[b];
+  }
+  c->pMax = (int)maxb;
+  if ((rc = c->pSz.ensure(nb)) || (rc = c->pLuOff.ensure(nb)) || (rc = c->pXoff.ensure(nb))) return rc;
+  HX_TRY(hipMemcpyAsync(c->pSz.p, sz.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+  HX_TRY(hipMemcpyAsync(c->pLuOff.p, c->pOff.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, c->stream));
+  HX_TRY(hipMemcpyAsync(c->pXoff.p, xo.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
   return check_launch();
 }
 
-// Z = M^-1 V (n x nrhs), body by body
+// Z = M^-1 V (n x nrhs): all bodies' block solves in one batched pass
 void gmres_precond_apply(hamsx_ctx* c, const double2* V, double2* Z, int nrhs) {
-  for (int b = 0; b < c->nbody; b++) {
-    const int r0 = c->body_start[b], sz = c->body_start[b + 1] - r0;
-    hamsx::launch::rows_pack(c->stream, V, c->n, r0, sz, nrhs, c->gP.p);
-    hamsx::launch::lu_solve(c->stream, c->pLU.p + c->pOff[b], sz, c->pPerm.p + r0, c->gP.p, c->gPt.p, nrhs);
-    hamsx::launch::rows_unpack(c->stream, c->gP.p, c->n, r0, sz, nrhs, Z);
-  }
+  hamsx::launch::lu_solve_systems(c->stream, c->nbody, c->pMax, c->pLU.p, c->pLuOff.p, c->pSz.p, c->pXoff.p,
+                                  c->body_start_d.p, c->body.p, c->pPerm.p, c->n, V, Z, c->gP.p, c->gPt.p, nrhs);
 }
 
 void givens(cplx a, cplx b, double& cs, cplx& sn) {
@@ -746,9 +758,7 @@ int do_solve_gmres(hamsx_ctx* c) {
       (rc = c->gNorm.ensure(nr)) || (rc = c->gScale.ensure(nr)) ||
       (rc = c->mvw.ensure(hamsx::launch::matvec_workspace(n, nr))))
     return rc;
-  size_t maxb = 0;
-  for (int b = 0; b < c->nbody; b++) maxb = std::max(maxb, (size_t)(c->body_start[b + 1] - c->body_start[b]));
-  if ((rc = c->gP.ensure(maxb * nr)) || (rc = c->gPt.ensure(maxb * nr))) return rc;
+  if ((rc = c->gP.ensure(blk)) || (rc = c->gPt.ensure(blk))) return rc;
   if ((rc = ensure_normal_matrix(c))) return rc;
   const double2* Aop = system_matrix(c);
   if ((rc = gmres_precond_setup(c))) return rc;

@@ -226,31 +226,68 @@ __global__ void k_perm_gather(const int* __restrict__ perm, const double2* __res
   Xp[e] = X[(size_t)perm[k] + (size_t)n * q];
 }
 
+// The two product passes stage chunks of kHmJ rows of the right-hand sides
+// (kHmQ columns per workgroup, more columns through grid.y) in LDS and
+// keep each thread's share of the columns in registers, so every stored
+// H entry is read once per kHmQ columns.
+constexpr int kHmJ = 16, kHmQ = 128;
+
+__device__ __forceinline__ void stage_rows(double2 (&xs)[kHmJ][kHmQ], const double2* __restrict__ src, size_t ld,
+                                           int jn, int nq) {
+  for (int e = threadIdx.x; e < kHmJ * kHmQ; e += 256) {
+    const int jj = e / kHmQ, qq = e % kHmQ;
+    xs[jj][qq] = (jj < jn && qq < nq) ? src[(size_t)jj + ld * qq] : make_double2(0.0, 0.0);
+  }
+}
+
 // T_b(l, q) = sum_j VH_b(l, j) Xp(cs + j, q), rank_b x nrhs (column-major);
-// workgroup (b, q-group of 4), thread (l = t / 4, q = q0 + t % 4)
+// workgroup (b, column group of kHmQ): thread (l = t / 4, column lane t % 4)
+// holds columns q0 + lane + 4k
 __global__ __launch_bounds__(256) void k_hm_vx(const int4* __restrict__ desc, const int* __restrict__ rank,
                                                const size_t* __restrict__ voff, const size_t* __restrict__ toff,
                                                const double2* __restrict__ VH, const double2* __restrict__ Xp,
                                                int n, int nrhs, double2* __restrict__ T) {
+  __shared__ double2 xs[kHmJ][kHmQ];
   const int b = blockIdx.x;
   const int4 d = desc[b];
   const int rk = rank[b];
-  const int l = threadIdx.x >> 2, q = blockIdx.y * 4 + (threadIdx.x & 3);
-  if (l >= rk || q >= nrhs) return;
-  const double2* v = VH + voff[b] + (size_t)l * d.w;
-  const double2* x = Xp + (size_t)d.z + (size_t)n * q;
-  double sr = 0.0, si = 0.0;
-  for (int j = 0; j < d.w; j++) {
-    const double2 a = v[j], c = x[j];
-    sr += a.x * c.x - a.y * c.y;
-    si += a.x * c.y + a.y * c.x;
+  if (rk == 0) return;
+  const int q0 = blockIdx.y * kHmQ, nq = min(kHmQ, nrhs - q0);
+  const int l = threadIdx.x >> 2, ql = threadIdx.x & 3;
+  constexpr int kPer = kHmQ / 4;
+  double ar[kPer], ai[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) ar[k] = ai[k] = 0.0;
+  const double2* v = VH + voff[b] + (size_t)(l < rk ? l : 0) * d.w;
+  for (int j0 = 0; j0 < d.w; j0 += kHmJ) {
+    const int jn = min(kHmJ, d.w - j0);
+    __syncthreads();
+    stage_rows(xs, Xp + (size_t)d.z + j0 + (size_t)n * q0, (size_t)n, jn, nq);
+    __syncthreads();
+    if (l < rk)
+      for (int jj = 0; jj < jn; jj++) {
+        const double2 a = v[j0 + jj];
+#pragma unroll
+        for (int k = 0; k < kPer; k++) {
+          const double2 c = xs[jj][ql + 4 * k];
+          ar[k] += a.x * c.x - a.y * c.y;
+          ai[k] += a.x * c.y + a.y * c.x;
+        }
+      }
   }
-  T[toff[b] + (size_t)l + (size_t)rk * q] = make_double2(sr, si);
+  if (l < rk)
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int q = ql + 4 * k;
+      if (q < nq) T[toff[b] + (size_t)l + (size_t)rk * (q0 + q)] = make_double2(ar[k], ai[k]);
+    }
 }
 
-// Per leaf cluster of rows: thread (row r, rhs q) sums the blocks listed for
-// the leaf in order; item >= 0: dense block, item < 0: low-rank block ~item.
-// Workgroup (leaf, q-group of 8): 32 rows x 8 right-hand sides.
+// Per leaf cluster of rows: thread (row lane t / 8, column lane t % 8) sums,
+// for its row and columns q0 + lane + 8k, the blocks listed for the leaf in
+// order; item >= 0: dense block, item < 0: low-rank block ~item.
+// Workgroup (leaf, column group of kHmQ); leaves of more than 32 rows are
+// done 32 rows at a time.
 __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves, const int* __restrict__ lptr,
                                                  const int* __restrict__ litem, const int4* __restrict__ ddesc,
                                                  const size_t* __restrict__ doff, const double2* __restrict__ D,
@@ -259,36 +296,62 @@ __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves
                                                  const double2* __restrict__ U, const double2* __restrict__ T,
                                                  const double2* __restrict__ Xp, int n, int nrhs,
                                                  double2* __restrict__ Yp) {
+  __shared__ double2 xs[kHmJ][kHmQ];
   const int2 lf = leaves[blockIdx.x];
-  const int q = blockIdx.y * 8 + (threadIdx.x & 7);
-  if (q >= nrhs) return;
-  for (int row = lf.x + (threadIdx.x >> 3); row < lf.y; row += 32) {
-    double sr = 0.0, si = 0.0;
+  const int q0 = blockIdx.y * kHmQ, nq = min(kHmQ, nrhs - q0);
+  const int rl = threadIdx.x >> 3, ql = threadIdx.x & 7;
+  constexpr int kPer = kHmQ / 8;
+  for (int r0 = lf.x; r0 < lf.y; r0 += 32) {
+    const int row = r0 + rl;
+    const bool live = row < lf.y;
+    double ar[kPer], ai[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) ar[k] = ai[k] = 0.0;
     for (int it = lptr[blockIdx.x]; it < lptr[blockIdx.x + 1]; it++) {
       const int item = litem[it];
+      const double2* a;     // this row of the block's left factor, column stride nr
+      const double2* src;   // right factor rows, leading dimension ld
+      size_t ld;
+      int nr, kk;
       if (item >= 0) {
         const int4 d = ddesc[item];
-        const double2* a = D + doff[item] + (row - d.x);
-        const double2* x = Xp + (size_t)d.z + (size_t)n * q;
-        for (int j = 0; j < d.w; j++) {
-          const double2 av = a[(size_t)d.y * j], xv = x[j];
-          sr += av.x * xv.x - av.y * xv.y;
-          si += av.x * xv.y + av.y * xv.x;
-        }
+        nr = d.y;
+        kk = d.w;
+        a = D + doff[item] + (live ? row - d.x : 0);
+        src = Xp + d.z;
+        ld = (size_t)n;
       } else {
         const int lb = ~item;
         const int4 d = ldesc[lb];
-        const int rk = rank[lb];
-        const double2* u = U + uoff[lb] + (row - d.x);
-        const double2* tv = T + toff[lb] + (size_t)rk * q;
-        for (int l = 0; l < rk; l++) {
-          const double2 av = u[(size_t)d.y * l], xv = tv[l];
-          sr += av.x * xv.x - av.y * xv.y;
-          si += av.x * xv.y + av.y * xv.x;
-        }
+        nr = d.y;
+        kk = rank[lb];
+        a = U + uoff[lb] + (live ? row - d.x : 0);
+        src = T + toff[lb];
+        ld = (size_t)kk;
+      }
+      for (int j0 = 0; j0 < kk; j0 += kHmJ) {
+        const int jn = min(kHmJ, kk - j0);
+        __syncthreads();
+        stage_rows(xs, src + j0 + ld * q0, ld, jn, nq);
+        __syncthreads();
+        if (live)
+          for (int jj = 0; jj < jn; jj++) {
+            const double2 av = a[(size_t)nr * (j0 + jj)];
+#pragma unroll
+            for (int k = 0; k < kPer; k++) {
+              const double2 c = xs[jj][ql + 8 * k];
+              ar[k] += av.x * c.x - av.y * c.y;
+              ai[k] += av.x * c.y + av.y * c.x;
+            }
+          }
       }
     }
-    Yp[(size_t)row + (size_t)n * q] = make_double2(sr, si);
+    if (live)
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const int q = ql + 8 * k;
+        if (q < nq) Yp[(size_t)row + (size_t)n * (q0 + q)] = make_double2(ar[k], ai[k]);
+      }
   }
 }
 
@@ -325,11 +388,11 @@ void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs,
   const size_t tot = (size_t)n * nrhs;
   const dim3 g1((unsigned)((tot + 255) / 256));
   HX_LAUNCH(k_perm_gather, g1, dim3(256), 0, s, h.perm, X, n, nrhs, Xp);
+  const unsigned qg = (unsigned)((nrhs + kHmQ - 1) / kHmQ);
   if (h.nlr > 0)
-    HX_LAUNCH(k_hm_vx, dim3(h.nlr, (nrhs + 3) / 4), dim3(256), 0, s, h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n,
-              nrhs, h.T);
-  HX_LAUNCH(k_hm_leaf, dim3(h.nleaf, (nrhs + 7) / 8), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff,
-            h.D, h.ldesc, h.rank, h.uoff, h.toff, h.U, h.T, Xp, n, nrhs, Yp);
+    HX_LAUNCH(k_hm_vx, dim3(h.nlr, qg), dim3(256), 0, s, h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+  HX_LAUNCH(k_hm_leaf, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, h.D, h.ldesc,
+            h.rank, h.uoff, h.toff, h.U, h.T, Xp, n, nrhs, Yp);
   HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);
 }
 }  // namespace launch

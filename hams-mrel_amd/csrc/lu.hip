@@ -538,10 +538,9 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
 // ---------------------------------------------------------------------------
 constexpr int kTrsmRhs = 8;
 template <typename R, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
-                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
-                                                    const int* __restrict__ skip) {
-  if (skip && *skip) return;
+__device__ __forceinline__ void trsm_block_body(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
+                                                double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
+                                                int bx) {
   __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
   __shared__ double dr[kPanelNB], di[kPanelNB];                      // 1 / T(c, c) (upper)
   __shared__ double yr[kPanelNB][kTrsmRhs], yi[kPanelNB][kTrsmRhs];  // solved block rows
@@ -605,11 +604,11 @@ __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __
       yr[r][ql] = live ? vr : 0.0;
       yi[r][ql] = live ? vi : 0.0;
     }
-    if (live && blockIdx.x == 0) Y[(size_t)(k0 + r) + (size_t)n * (q0 + ql)] = make_double2(vr, vi);
+    if (live && bx == 0) Y[(size_t)(k0 + r) + (size_t)n * (q0 + ql)] = make_double2(vr, vi);
   }
   __syncthreads();
   // update the other rows: below the block (LOWER) or above it (upper)
-  const int i = LOWER ? k0 + kb + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
+  const int i = LOWER ? k0 + kb + bx * 256 + t : bx * 256 + t;
   if (LOWER ? i >= n : i >= k0) return;
   double ar[kTrsmRhs], ai[kTrsmRhs];
 #pragma unroll
@@ -637,6 +636,32 @@ __global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __
       v.y -= ai[q];
       X[(size_t)i + (size_t)n * (q0 + q)] = v;
     }
+}
+
+template <typename R, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
+                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  trsm_block_body<R, LOWER>(LU, n, k0, kb, X, Y, nrhs, blockIdx.x);
+}
+
+// The same step for a batch of independent systems (blockIdx.z = system z
+// of order nz[z], factors at LU + lu_off[z], right-hand sides at
+// X + x_off[z] with leading dimension nz[z]).  The grid is sized for the
+// largest system; systems that have no block at k0 return, and blocks past
+// a system's rows return inside the step.
+template <typename R, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_block_batched(const typename CT<R>::c2* __restrict__ LU,
+                                                            const size_t* __restrict__ lu_off,
+                                                            const int* __restrict__ nz,
+                                                            const size_t* __restrict__ x_off, int k0,
+                                                            double2* __restrict__ X, double2* __restrict__ Y,
+                                                            int nrhs) {
+  const int z = blockIdx.z, n = nz[z];
+  if (k0 >= n) return;
+  trsm_block_body<R, LOWER>(LU + lu_off[z], n, k0, min(kPanelNB, n - k0), X + x_off[z], Y + x_off[z], nrhs,
+                            blockIdx.x);
 }
 
 __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
@@ -787,6 +812,32 @@ __global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, s
   }
 }
 
+// Batched systems stored one after another: rows [r0[z], r0[z+1]) of an
+// n x nrhs block X belong to system z and are packed as an nz x nrhs block at
+// nrhs * r0[z] (body[i] = the system of row i).
+__global__ void k_pack_systems(const double2* __restrict__ X, int n, int nrhs, const int* __restrict__ body,
+                               const int* __restrict__ r0, double2* __restrict__ P) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int i = (int)(e % n), q = (int)(e / n), z = body[i], a = r0[z], sz = r0[z + 1] - a;
+  P[(size_t)nrhs * a + (size_t)(i - a) + (size_t)sz * q] = X[e];
+}
+__global__ void k_gather_systems(const double2* __restrict__ P, int n, int nrhs, const int* __restrict__ body,
+                                 const int* __restrict__ r0, const int* __restrict__ perm, double2* __restrict__ Pt) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int i = (int)(e % n), q = (int)(e / n), z = body[i], a = r0[z], sz = r0[z + 1] - a;
+  const size_t base = (size_t)nrhs * a + (size_t)sz * q;
+  Pt[base + (i - a)] = P[base + perm[i]];  // perm: each system's own row permutation, local indices
+}
+__global__ void k_unpack_systems(const double2* __restrict__ P, int n, int nrhs, const int* __restrict__ body,
+                                 const int* __restrict__ r0, double2* __restrict__ X) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * nrhs) return;
+  const int i = (int)(e % n), q = (int)(e / n), z = body[i], a = r0[z], sz = r0[z + 1] - a;
+  X[e] = P[(size_t)nrhs * a + (size_t)(i - a) + (size_t)sz * q];
+}
+
 // Refinement bookkeeping, after each residual: unless already converged,
 // set the converged flag when max over columns of |R|_max / |X|_max <= tol
 // (a column with X = 0 compares |R|_max itself), else count the correction
@@ -826,6 +877,28 @@ void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* w
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
                   const int* skip) {
   lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs, skip);
+}
+void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
+                      const size_t* x_off, const int* r0, const int* body, const int* perm, int n, const double2* V,
+                      double2* Z, double2* P, double2* Pt, int nrhs) {
+  const size_t tot = (size_t)n * nrhs;
+  const dim3 g1((unsigned)((tot + 255) / 256));
+  const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
+  HX_LAUNCH(k_pack_systems, g1, dim3(256), 0, s, V, n, nrhs, body, r0, P);
+  HX_LAUNCH(k_gather_systems, g1, dim3(256), 0, s, P, n, nrhs, body, r0, perm, Pt);
+  // forward pass: rows updated in Pt, solved rows into P; backward: updated in P, solved into Pt
+  for (int k0 = 0; k0 < nmax; k0 += kPanelNB) {
+    const int rest = nmax - k0 - min(kPanelNB, nmax - k0);
+    const int g = rest > 0 ? (rest + 255) / 256 : 1;
+    HX_LAUNCH((k_trsm_block_batched<double, true>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz, x_off,
+              k0, Pt, P, nrhs);
+  }
+  for (int k0 = ((nmax - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
+    const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
+    HX_LAUNCH((k_trsm_block_batched<double, false>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz, x_off,
+              k0, P, Pt, nrhs);
+  }
+  HX_LAUNCH(k_unpack_systems, g1, dim3(256), 0, s, Pt, n, nrhs, body, r0, Z);
 }
 void refine_reset(hipStream_t s, int* state) { HX_LAUNCH(k_refine_reset, dim3(1), dim3(1), 0, s, state); }
 void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state) {

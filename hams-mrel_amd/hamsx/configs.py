@@ -4,11 +4,12 @@ orientation (see :mod:`hamsx.mesh`); the reference point of each body is its
 waterline centre."""
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from .mesh import box_for_panels, cylinder, cylinder_for_panels
+from .mesh import box_for_panels, cylinder, cylinder_for_panels, waterplane_disk
 
 
 @dataclass
@@ -60,6 +61,24 @@ def five_cylinders(seed=2024, total=10000, nfreq=100) -> Case:
     xr = [np.array([x, 0.0, 0.0]) for x in xs]
     return Case(f"cylinders-5x2000-h60-seed{seed}", bodies, xr, 60.0, np.linspace(0.1, 3.0, nfreq), [0.0],
                 meta={"seed": seed})
+
+
+def large_array(panels_each=1875, nfreq=50) -> Case:
+    """16 cylinders R = 4 m, draft 8 m on a 4 x 4 grid, 30 m spacing
+    (about 30,000 hull panels), depth 80 m, 50 frequencies in [0.2, 2.0]
+    rad/s, heading 0.  Each cylinder carries a waterplane lid for
+    irregular-frequency removal, with panels about twice the hull's size
+    (meta["lids"])."""
+    bodies, xr, lids = [], [], []
+    for k in range(16):
+        x0, y0 = 30.0 * (k % 4) - 45.0, 30.0 * (k // 4) - 45.0
+        b = cylinder_for_panels(4.0, 8.0, panels_each, x0, y0)
+        bodies.append(b)
+        xr.append(np.array([x0, y0, 0.0]))
+        na = max(8, int(round(2 * math.pi * 4.0 / (2.0 * math.sqrt(4.0 * 8.0 * 2 * math.pi / panels_each)))))
+        lids.append(waterplane_disk(4.0, na, max(1, na // 6), x0, y0))
+    return Case(f"array-16x{panels_each}-lid-h80", bodies, xr, 80.0, np.linspace(0.2, 2.0, nfreq), [0.0],
+                meta={"lids": lids})
 
 
 def random_request(rng: np.random.Generator, pmin=500, pmax=12000) -> Case:
