@@ -33,7 +33,8 @@ module hamsx_mod
             hamsx_set_frequency, hamsx_set_waves, hamsx_assemble, hamsx_factor, hamsx_solve_radiation, &
             hamsx_solve_diffraction, hamsx_forces, hamsx_solve_frequency, hamsx_submit_frequency, &
             hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
-            hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, &
+            hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, hamsx_set_hmatrix, &
+            hamsx_hmatrix_stats, hamsx_hmatrix_matvec, hamsx_hmatrix_partition, &
             hamsx_stage_times
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
@@ -176,6 +177,39 @@ module hamsx_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: solver, restart, maxiter
       real(c_double), value :: tol
+      integer(c_int) :: rc
+    end function
+    function hamsx_set_hmatrix(ctx, enable, eta, aca_tol, leaf, rank_max) bind(C, name="hamsx_set_hmatrix") &
+        result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable, leaf, rank_max
+      real(c_double), value :: eta, aca_tol
+      integer(c_int) :: rc
+    end function
+    function hamsx_hmatrix_stats(ctx, nlowrank, ndense, mean_rank, compression) &
+        bind(C, name="hamsx_hmatrix_stats") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: nlowrank, ndense
+      real(c_double), intent(out) :: mean_rank, compression
+      integer(c_int) :: rc
+    end function
+    function hamsx_hmatrix_matvec(ctx, x, nrhs, y) bind(C, name="hamsx_hmatrix_matvec") result(rc)
+      import :: c_ptr, c_int, c_double_complex
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: x(*)
+      integer(c_int), value :: nrhs
+      complex(c_double_complex), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+    function hamsx_hmatrix_partition(n, cen, leaf, eta, perm, max_blocks, blocks, nblocks) &
+        bind(C, name="hamsx_hmatrix_partition") result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: n, leaf, max_blocks
+      real(c_double), intent(in) :: cen(*)
+      real(c_double), value :: eta
+      integer(c_int), intent(out) :: perm(*), blocks(*), nblocks
       integer(c_int) :: rc
     end function
     function hamsx_set_refinement(ctx, max_steps, tol) bind(C, name="hamsx_set_refinement") result(rc)
