@@ -225,7 +225,8 @@ struct hamsx_ctx {
   std::vector<hamsx::HNode> hm_tree;
   std::vector<hamsx::HBlk> hm_blocks;
   std::vector<int> hm_aca_idx;  // partition index of each ACA block
-  DevBuf<int> hmPerm, hmRank, hmLptr, hmLitem;
+  DevBuf<int> hmPerm, hmRank, hmLptr, hmLitem, hmVxList;
+  int hm_vx_start[5] = {0, 0, 0, 0, 0};
   DevBuf<int4> hmAdesc, hmDdesc;
   DevBuf<int2> hmLeaves;
   DevBuf<size_t> hmUoff, hmVoff, hmRoff, hmToff, hmDoff;
@@ -688,6 +689,18 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
     doff.push_back(ds);
     ds += (size_t)b.nr * b.nc;
   }
+  // low-rank blocks bucketed by rank for the T = VH x pass
+  std::vector<int> vx;
+  for (int g = 0; g < 4; g++) {
+    c->hm_vx_start[g] = (int)vx.size();
+    const int lo = g == 0 ? 1 : (4 << g) + 1, hi = 8 << g;
+    for (int a = 0; a < na; a++)
+      if (rank[a] >= lo && rank[a] <= hi) vx.push_back(a);
+  }
+  c->hm_vx_start[4] = (int)vx.size();
+  if ((rc = c->hmVxList.ensure(std::max<size_t>(vx.size(), 1)))) return rc;
+  if (!vx.empty())
+    HX_TRY(hipMemcpyAsync(c->hmVxList.p, vx.data(), sizeof(int) * vx.size(), hipMemcpyHostToDevice, c->stream));
   const int nd = (int)ddesc.size();
   if ((rc = c->hmDdesc.ensure(std::max(nd, 1))) || (rc = c->hmDoff.ensure(std::max(nd, 1))) ||
       (rc = c->hmD.ensure(std::max<size_t>(ds, 1))) || (rc = c->hmT.ensure(std::max<size_t>(ts, 1))) ||
@@ -734,9 +747,27 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
 }
 
 hamsx::HmDev hm_dev(hamsx_ctx* c) {
-  return hamsx::HmDev{c->hmPerm.p,  (int)c->hm_aca_idx.size(), c->hm_nleaf, c->hmAdesc.p, c->hmRank.p,
-                      c->hmUoff.p,  c->hmVoff.p, c->hmToff.p, c->hmU.p,     c->hmVH.p,    c->hmT.p,
-                      c->hmDdesc.p, c->hmDoff.p, c->hmD.p,    c->hmLeaves.p, c->hmLptr.p, c->hmLitem.p};
+  hamsx::HmDev h{};
+  h.perm = c->hmPerm.p;
+  h.nlr = (int)c->hm_aca_idx.size();
+  h.nleaf = c->hm_nleaf;
+  h.vx_list = c->hmVxList.p;
+  for (int g = 0; g < 5; g++) h.vx_start[g] = c->hm_vx_start[g];
+  h.ldesc = c->hmAdesc.p;
+  h.rank = c->hmRank.p;
+  h.uoff = c->hmUoff.p;
+  h.voff = c->hmVoff.p;
+  h.toff = c->hmToff.p;
+  h.U = c->hmU.p;
+  h.VH = c->hmVH.p;
+  h.T = c->hmT.p;
+  h.ddesc = c->hmDdesc.p;
+  h.doff = c->hmDoff.p;
+  h.D = c->hmD.p;
+  h.leaves = c->hmLeaves.p;
+  h.lptr = c->hmLptr.p;
+  h.litem = c->hmLitem.p;
+  return h;
 }
 
 // the GMRES operator: Y = beta B + alpha Op X, Op the H-matrix when built,

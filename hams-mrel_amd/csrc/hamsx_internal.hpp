@@ -87,6 +87,10 @@ struct WaveDev {
 struct HmDev {
   const int* perm;
   int nlr, nleaf;
+  // low-rank blocks with rank > 0 grouped by rank (<= 8, 16, 32, 64):
+  // group g is vx_list[vx_start[g] .. vx_start[g+1])
+  const int* vx_list;
+  int vx_start[5];
   const int4* ldesc;
   const int* rank;
   const size_t *uoff, *voff, *toff;

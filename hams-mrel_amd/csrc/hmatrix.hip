@@ -240,46 +240,52 @@ __device__ __forceinline__ void stage_rows(double2 (&xs)[kHmJ][kHmQ], const doub
   }
 }
 
-// T_b(l, q) = sum_j VH_b(l, j) Xp(cs + j, q), rank_b x nrhs (column-major);
-// workgroup (b, column group of kHmQ): thread (l = t / 4, column lane t % 4)
-// holds columns q0 + lane + 4k
-__global__ __launch_bounds__(256) void k_hm_vx(const int4* __restrict__ desc, const int* __restrict__ rank,
-                                               const size_t* __restrict__ voff, const size_t* __restrict__ toff,
-                                               const double2* __restrict__ VH, const double2* __restrict__ Xp,
-                                               int n, int nrhs, double2* __restrict__ T) {
+// T_b(l, q) = sum_j VH_b(l, j) Xp(cs + j, q), rank_b x nrhs (column-major),
+// for the low-rank blocks list[0 .. count) whose ranks are at most 2 KPER.
+// Workgroup (block, column group of kHmQ): thread (column q = t % 128,
+// parity p = t / 128) holds rows l = p + 2k, k < KPER; VH and the
+// right-hand-side rows go through LDS in chunks of kHmJ columns.
+template <int KPER>
+__global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, const int4* __restrict__ desc,
+                                               const int* __restrict__ rank, const size_t* __restrict__ voff,
+                                               const size_t* __restrict__ toff, const double2* __restrict__ VH,
+                                               const double2* __restrict__ Xp, int n, int nrhs,
+                                               double2* __restrict__ T) {
   __shared__ double2 xs[kHmJ][kHmQ];
-  const int b = blockIdx.x;
+  __shared__ double2 vs[2 * KPER][kHmJ];
+  const int b = list[blockIdx.x];
   const int4 d = desc[b];
   const int rk = rank[b];
-  if (rk == 0) return;
   const int q0 = blockIdx.y * kHmQ, nq = min(kHmQ, nrhs - q0);
-  const int l = threadIdx.x >> 2, ql = threadIdx.x & 3;
-  constexpr int kPer = kHmQ / 4;
-  double ar[kPer], ai[kPer];
+  const int q = threadIdx.x & (kHmQ - 1), p = threadIdx.x >> 7;
+  double ar[KPER], ai[KPER];
 #pragma unroll
-  for (int k = 0; k < kPer; k++) ar[k] = ai[k] = 0.0;
-  const double2* v = VH + voff[b] + (size_t)(l < rk ? l : 0) * d.w;
+  for (int k = 0; k < KPER; k++) ar[k] = ai[k] = 0.0;
+  const double2* v = VH + voff[b];
   for (int j0 = 0; j0 < d.w; j0 += kHmJ) {
     const int jn = min(kHmJ, d.w - j0);
     __syncthreads();
     stage_rows(xs, Xp + (size_t)d.z + j0 + (size_t)n * q0, (size_t)n, jn, nq);
+    for (int e = threadIdx.x; e < 2 * KPER * kHmJ; e += 256) {
+      const int l = e / kHmJ, jj = e % kHmJ;
+      vs[l][jj] = (l < rk && jj < jn) ? v[(size_t)l * d.w + j0 + jj] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
-    if (l < rk)
-      for (int jj = 0; jj < jn; jj++) {
-        const double2 a = v[j0 + jj];
+    for (int jj = 0; jj < jn; jj++) {
+      const double2 c = xs[jj][q];
 #pragma unroll
-        for (int k = 0; k < kPer; k++) {
-          const double2 c = xs[jj][ql + 4 * k];
-          ar[k] += a.x * c.x - a.y * c.y;
-          ai[k] += a.x * c.y + a.y * c.x;
-        }
+      for (int k = 0; k < KPER; k++) {
+        const double2 a = vs[p + 2 * k][jj];
+        ar[k] += a.x * c.x - a.y * c.y;
+        ai[k] += a.x * c.y + a.y * c.x;
       }
+    }
   }
-  if (l < rk)
+  if (q < nq)
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int q = ql + 4 * k;
-      if (q < nq) T[toff[b] + (size_t)l + (size_t)rk * (q0 + q)] = make_double2(ar[k], ai[k]);
+    for (int k = 0; k < KPER; k++) {
+      const int l = p + 2 * k;
+      if (l < rk) T[toff[b] + (size_t)l + (size_t)rk * (q0 + q)] = make_double2(ar[k], ai[k]);
     }
 }
 
@@ -389,8 +395,23 @@ void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs,
   const dim3 g1((unsigned)((tot + 255) / 256));
   HX_LAUNCH(k_perm_gather, g1, dim3(256), 0, s, h.perm, X, n, nrhs, Xp);
   const unsigned qg = (unsigned)((nrhs + kHmQ - 1) / kHmQ);
-  if (h.nlr > 0)
-    HX_LAUNCH(k_hm_vx, dim3(h.nlr, qg), dim3(256), 0, s, h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+  // low-rank blocks bucketed by rank: <= 8, <= 16, <= 32, <= 64
+  const int* lists[4] = {h.vx_list, h.vx_list + h.vx_start[1], h.vx_list + h.vx_start[2], h.vx_list + h.vx_start[3]};
+  for (int g = 0; g < 4; g++) {
+    const int cnt = h.vx_start[g + 1] - h.vx_start[g];
+    if (cnt <= 0) continue;
+    const dim3 grid(cnt, qg);
+    if (g == 0)
+      HX_LAUNCH(k_hm_vx<4>, grid, dim3(256), 0, s, lists[0], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+    else if (g == 1)
+      HX_LAUNCH(k_hm_vx<8>, grid, dim3(256), 0, s, lists[1], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+    else if (g == 2)
+      HX_LAUNCH(k_hm_vx<16>, grid, dim3(256), 0, s, lists[2], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs,
+                h.T);
+    else
+      HX_LAUNCH(k_hm_vx<32>, grid, dim3(256), 0, s, lists[3], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs,
+                h.T);
+  }
   HX_LAUNCH(k_hm_leaf, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, h.D, h.ldesc,
             h.rank, h.uoff, h.toff, h.U, h.T, Xp, n, nrhs, Yp);
   HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);
