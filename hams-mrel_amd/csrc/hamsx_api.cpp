@@ -470,17 +470,20 @@ int do_rhs(hamsx_ctx* c) {
 }
 
 // With a lid: C = A^T A (ASSB_LEFT_IRR_MULTI, AssbMatx_irrMulti.f90:154-171),
-// the matrix the LU, the refinement and GMRES then work on.
+// the matrix the LU, the refinement and GMRES then work on.  The plain
+// transpose makes C complex symmetric, so only its lower triangle is
+// computed (zsyrk, half the flops of the full product) and then mirrored.
 int ensure_normal_matrix(hamsx_ctx* c) {
   if (c->nlid == 0 || c->have_C) return 0;
   int rc;
   if ((rc = c->Cm.ensure((size_t)c->n * c->n)) || (rc = blas_handle(c))) return rc;
   const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-  if (rocblas_zgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->n, c->n, c->nf, &one,
-                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf,
+  if (rocblas_zsyrk(c->blas, rocblas_fill_lower, rocblas_operation_transpose, c->n, c->nf, &one,
                     reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf, &zero,
                     reinterpret_cast<rocblas_double_complex*>(c->Cm.p), c->n) != rocblas_status_success)
-    return fail(HAMSX_EDEVICE, "rocblas_zgemm (A^T A) failed");
+    return fail(HAMSX_EDEVICE, "rocblas_zsyrk (A^T A) failed");
+  hamsx::launch::mirror_lower(c->stream, c->Cm.p, c->n);
+  if ((rc = check_launch())) return rc;
   c->have_C = true;
   return 0;
 }

@@ -134,6 +134,8 @@ void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w,
             const double2* X, double* out);
 // single-precision copy of A for the mixed path, and the helpers of refinement
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
+// C(r, c) = C(c, r) above the diagonal (n x n): completes a lower-triangle syrk
+void mirror_lower(hipStream_t s, double2* C, int n);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
                    double2* diag);
 // skip (may be null, here and below): a device flag that turns every

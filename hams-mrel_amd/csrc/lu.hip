@@ -706,6 +706,27 @@ static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const
     (void)hipMemcpyAsync(X, Xtmp, sizeof(double2) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s);
 }
 
+// Complex-symmetric completion: C(r, c) = C(c, r) for r < c, from the lower
+// triangle a syrk left (n x n, column-major).  One 32 x 32 tile of the upper
+// triangle per block, read from its mirror tile through LDS so both the
+// reads and the writes run down columns.
+__global__ __launch_bounds__(256) void k_mirror_lower(double2* __restrict__ C, int n) {
+  const int bi = blockIdx.x, bj = blockIdx.y;  // destination tile: rows bi, columns bj
+  if (bi > bj) return;
+  __shared__ double2 t[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // source tile: rows bj*32.., columns bi*32..
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bj * 32 + tx, c = bi * 32 + k;
+    if (r < n && c < n) t[k][tx] = C[(size_t)c * n + r];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bi * 32 + tx, c = bj * 32 + k;
+    if (r < c && c < n) C[(size_t)c * n + r] = t[tx][k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Mixed precision helpers: FP32 copy of A, FP64 residual R = B - A X with
 // per-column max norms, and X += D.
@@ -866,6 +887,11 @@ void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx
 }
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
   lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
+}
+void mirror_lower(hipStream_t s, double2* C, int n) {
+  if (n <= 1) return;
+  const unsigned t = (unsigned)((n + 31) / 32);
+  HX_LAUNCH(k_mirror_lower, dim3(t, t), dim3(256), 0, s, C, n);
 }
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
   HX_LAUNCH(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
