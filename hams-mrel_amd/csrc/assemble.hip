@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
 // Newman integrals of source panel j (BODINT_MULTI_LEFT / RBC_MULTI_RIGHT).
 __device__ __forceinline__ void store_entry(const MeshDev& m, int i, int j, int flag, double sij, const double dij[3],
                                             const cd g[4], double2* __restrict__ A, double2* __restrict__ S) {
-  const int n = m.n, nf = m.nf;
+  const int n = m.n, ld = m.nf - m.row0;
   const double ear = m.area[j], n1 = m.nrm[j], n2 = m.nrm[n + j], n3 = m.nrm[2 * n + j];
   cd t, s;
   if (flag) {
@@ -79,7 +79,7 @@ __device__ __forceinline__ void store_entry(const MeshDev& m, int i, int j, int 
   // AMAT(IEL,IEL) = 2*PI before the sum (AssbMatxMulti.f90:72-74); lid rows have
   // none (AssbMatx_irrMulti.f90:87-89)
   if (i == j) t.r = 2.0 * kPi + t.r;
-  const size_t o = (size_t)i + (size_t)nf * j;
+  const size_t o = (size_t)(i - m.row0) + (size_t)ld * j;
   A[o] = make_double2(t.r, t.i);
   S[o] = make_double2(s.r, s.i);
 }
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2*
                                                   double2* __restrict__ S, int* flags) {
   const int lane = threadIdx.x & 63;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
-  const int i = blockIdx.x * 64 + lane;
+  const int i = m.row0 + blockIdx.x * 64 + lane;
   if (j >= m.n || i >= m.nf) return;
   pair_entry<DEEP, kRegAny>(m, f, i, j, A, S, flags);
 }
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_bin_count(MeshDev m, double h, int* __r
   __shared__ int hist[kFinBins];
   for (int k = threadIdx.x; k < kFinBins; k += 256) hist[k] = 0;
   __syncthreads();
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = m.row0 + blockIdx.x * 64 + (threadIdx.x & 63);
   if (i < m.nf && j < m.n) {
     const int key = pair_key(m, h, i, j);
     if (key >= 0) atomicAdd(&hist[key], 1);
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshDev m, double h, int* _
   __shared__ int hist[kFinBins], base[kFinBins];
   for (int k = threadIdx.x; k < kFinBins; k += 256) hist[k] = 0;
   __syncthreads();
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6), i = m.row0 + blockIdx.x * 64 + (threadIdx.x & 63);
   int key = -1, rank = 0;
   if (i < m.nf && j < m.n) {
     key = pair_key(m, h, i, j);
@@ -419,12 +419,112 @@ __global__ __launch_bounds__(256) void k_forces(MeshDev m, FreqDev f, WaveDev w,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Field points (CalPotentialMulti, PressureElevationMulti.f90:55-275, NSYS = 1).
+// The assembly kernels above fill, for P field points, M = [S_fp | T_fp]
+// (P x 2N): the single-layer entries DUM(1) and the double-layer entries
+// DUM(2) of every (field point, hull panel) pair.  With Z = [W; -X] (2N x
+// nrhs) -- W(j, 6b+m) = n_m(j) on body b's panels (FACTOR_NORMAL, :420-440),
+// W(j, 6nb+k) = -DINCP(j, k), X the solved potentials MXPOT -- the sum over
+// JEL of TERM1 - TERM2 (:151-200) is the product M Z.
+// ---------------------------------------------------------------------------
+__global__ void k_field_weights(MeshDev m, int nbody, const double2* __restrict__ dincp,
+                                const double2* __restrict__ X, double2* __restrict__ Z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y, n = m.n, nd = 6 * nbody;
+  if (j >= n) return;
+  double2 w;
+  if (col < nd) {
+    w = make_double2(m.body[j] == col / 6 ? m.nrm[(size_t)(col % 6) * n + j] : 0.0, 0.0);
+  } else {
+    const double2 d = dincp[(size_t)(col - nd) * n + j];
+    w = make_double2(-d.x, -d.y);
+  }
+  const size_t o = (size_t)col * 2 * n;
+  const double2 x = X[(size_t)col * n + j];
+  Z[o + j] = w;
+  Z[o + n + j] = make_double2(-x.x, -x.y);
+}
+
+// The free-term test (:200-249): onb[p] = 1 when field point p lies on a hull
+// panel -- within MAX(1E-6, 1E-3 PNSZ) of its plane, measured from its first
+// vertex, and inside its polygon: the edge products (edge x (x - vertex)) . n
+// all >= -TOLA or all <= TOLA, TOLA = 1E-3 PNSZ^2.  One thread per (panel,
+// point); vtx[(3q + d) n + j] is coordinate d of vertex q of panel j; onb
+// starts at zero.
+__global__ __launch_bounds__(256) void k_on_body(MeshDev m, const double* __restrict__ vtx, int nfp,
+                                                 const double* __restrict__ px, const double* __restrict__ py,
+                                                 const double* __restrict__ pz, int* __restrict__ onb) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * 256 + threadIdx.x, p = blockIdx.y;
+  if (j >= m.n || p >= nfp) return;
+  const int n = m.n, nc = m.ncn[j];
+  const double x0 = px[p], x1 = py[p], x2 = pz[p];
+  const double e1 = m.nrm[j], e2 = m.nrm[n + j], e3 = m.nrm[2 * n + j], pn = m.pnsz[j];
+  const double tolp = fmax(1.0e-6, 1.0e-3 * pn), tola = 1.0e-3 * (pn * pn);
+  const double dpln = (x0 - vtx[j]) * e1 + (x1 - vtx[(size_t)n + j]) * e2 + (x2 - vtx[2 * (size_t)n + j]) * e3;
+  if (fabs(dpln) > tolp) return;
+  double smin = 1.0e30, smax = -1.0e30;
+  for (int q = 0; q < nc; q++) {
+    const int r = q + 1 < nc ? q + 1 : 0;
+    const double a0 = vtx[(size_t)(3 * q) * n + j], a1 = vtx[(size_t)(3 * q + 1) * n + j],
+                 a2 = vtx[(size_t)(3 * q + 2) * n + j];
+    const double ev0 = vtx[(size_t)(3 * r) * n + j] - a0, ev1 = vtx[(size_t)(3 * r + 1) * n + j] - a1,
+                 ev2 = vtx[(size_t)(3 * r + 2) * n + j] - a2;
+    const double wv0 = x0 - a0, wv1 = x1 - a1, wv2 = x2 - a2;
+    const double cx = ev1 * wv2 - ev2 * wv1, cy = ev2 * wv0 - ev0 * wv2, cz = ev0 * wv1 - ev1 * wv0;
+    const double sv = cx * e1 + cy * e2 + cz * e3;
+    smin = fmin(smin, sv);
+    smax = fmax(smax, sv);
+  }
+  if (smin >= -tola || smax <= tola) atomicOr(&onb[p], 1);
+}
+
+__global__ void k_fill_int(int* __restrict__ a, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) a[i] = 0;
+}
+
+// The factor of CalPressureMulti (i w rho) or CalElevationMulti (i w / g),
+// rho and 1 at the zero- and infinite-frequency limits (:303-348).
+__device__ __forceinline__ cd field_factor(cd v, int kind, double z, bool limit, double w1, double rho) {
+  if (kind == 3) kind = fabs(z) > HX_F(1.0e-6) ? 1 : 2;  // the output drivers' choice (:446, :495)
+  if (kind == 1) return limit ? cd{rho * v.r, rho * v.i} : cd{-((w1 * rho) * v.i), (w1 * rho) * v.r};
+  if (kind == 2) return limit ? v : cd{-((w1 / kG) * v.i), (w1 / kG) * v.r};
+  return v;
+}
+
+// POT = (M Z)(p, col) / SLD, SLD = 2 pi on a panel and 4 pi in the fluid
+// (:251-258), plus VINP for a diffraction column when ISOL = 1 (:260-271),
+// times the factor of `kind`; out[col * ld + p] for the chunk's points, and
+// inc[k * ld + p] (may be null) the incident-wave value of heading k
+// (CalPotentialIncMulti with its pressure/elevation factors, :277-300,
+// :516-556).
+__global__ void k_field_finish(int nfp, const double* __restrict__ px, const double* __restrict__ py,
+                               const double* __restrict__ pz, const int* __restrict__ onb, FreqDev f, WaveDev w,
+                               int nd, int kind, const double2* __restrict__ P, size_t ld, double2* __restrict__ out,
+                               double2* __restrict__ inc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+  if (p >= nfp) return;
+  const bool limit = fabs(f.v) < HX_F(1.0e-8) || fabs(f.v + 1.0) < HX_F(1.0e-8);
+  const double sld = onb[p] ? 2.0 * kPi : 4.0 * kPi;
+  const double2 pv = P[(size_t)col * nfp + p];
+  cd v{pv.x / sld, pv.y / sld};
+  cd f0 = mk(0.0);
+  if (col >= nd && w.isol == 1 && !limit) f0 = d_vinp(px[p], py[p], pz[p], f, w, w.beta[col - nd]);
+  v = field_factor(v + f0, kind, pz[p], limit, f.w1, w.rho);
+  out[(size_t)col * ld + p] = make_double2(v.r, v.i);
+  if (inc && col >= nd) {
+    const cd fi = field_factor(f0, kind, pz[p], limit, f.w1, w.rho);
+    inc[(size_t)(col - nd) * ld + p] = make_double2(fi.r, fi.i);
+  }
+}
+
 __global__ void k_clear_flag(int* flags) { flags[0] = 0; }
 
 namespace launch {
 
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags) {
-  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf - m.row0 + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_clear_flag, dim3(1), dim3(1), 0, s, flags);
   if (f.h < 0.0)
     HX_LAUNCH(k_assemble<true>, grid, dim3(256), 0, s, m, f, A, S, flags);
@@ -437,12 +537,12 @@ void mode_tables(hipStream_t s, const MeshDev& m, const FreqDev& f) {
 }
 
 void bin_count(hipStream_t s, const MeshDev& m, double h, int* count) {
-  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf - m.row0 + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_bin_count, grid, dim3(256), 0, s, m, h, count);
 }
 
 void bin_scatter(hipStream_t s, const MeshDev& m, double h, int* cursor, unsigned* list) {
-  dim3 grid((m.nf + 63) / 64, (m.n + 3) / 4);
+  dim3 grid((m.nf - m.row0 + 63) / 64, (m.n + 3) / 4);
   HX_LAUNCH(k_bin_scatter, grid, dim3(256), 0, s, m, h, cursor, list);
 }
 
@@ -496,6 +596,28 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
             w.nbeta, chunk_j0_d, chunk_j1_d, partial);
   HX_LAUNCH(k_rhs_reduce, dim3((m.nf + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, m.nf, nbody, w.nbeta,
             nchunk, chunk_body_d, partial, X);
+}
+
+void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out) {
+  if (w.nbeta > 0) HX_LAUNCH(k_dum2, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, out);
+}
+
+void field_weights(hipStream_t s, const MeshDev& m, int nbody, int nbeta, const double2* dincp, const double2* X,
+                   double2* Z) {
+  HX_LAUNCH(k_field_weights, dim3((m.n + 255) / 256, 6 * nbody + nbeta), dim3(256), 0, s, m, nbody, dincp, X, Z);
+}
+
+void on_body(hipStream_t s, const MeshDev& m, const double* vtx, int nfp, const double* px, const double* py,
+             const double* pz, int* onb) {
+  HX_LAUNCH(k_fill_int, dim3((nfp + 255) / 256), dim3(256), 0, s, onb, nfp);
+  HX_LAUNCH(k_on_body, dim3((m.n + 255) / 256, nfp), dim3(256), 0, s, m, vtx, nfp, px, py, pz, onb);
+}
+
+void field_finish(hipStream_t s, int nfp, const double* px, const double* py, const double* pz, const int* onb,
+                  const FreqDev& f, const WaveDev& w, int nd, int kind, const double2* P, size_t ld, double2* out,
+                  double2* inc) {
+  HX_LAUNCH(k_field_finish, dim3((nfp + 255) / 256, nd + w.nbeta), dim3(256), 0, s, nfp, px, py, pz, onb, f, w, nd,
+            kind, P, ld, out, inc);
 }
 
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,

@@ -182,6 +182,10 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   int ensure(size_t count) {
     if (count <= n && p) return 0;
     if (p) (void)hipFree(p);
@@ -200,6 +204,20 @@ struct DevBuf {
     p = nullptr;
     n = 0;
   }
+};
+
+// Finite-depth pair bins: the pairs (field row i >= row0, source j) of one
+// mesh view counting-sorted by the keys of hamsx_internal.hpp, valid for
+// (n, nf, h, key); key names the field rows (0: the system's rows).
+struct PairBins {
+  DevBuf<unsigned> plist;
+  DevBuf<int> cnt;
+  int n = 0, nf = 0;
+  double h = 0.0;
+  unsigned long long key = 0;
+  // list slices per region: single entries [reg_begin, reg_end), hull-pair
+  // entries [pair_begin, pair_end)
+  size_t reg_begin[5] = {0}, reg_end[5] = {0}, pair_begin[5] = {0}, pair_end[5] = {0};
 };
 
 }  // namespace
@@ -257,8 +275,17 @@ struct hamsx_ctx {
   DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag, mvw;
   DevBuf<float2> Af;
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
-  // finite-depth pair bins, valid for bin_n panels at depth bin_h
-  DevBuf<unsigned> plist;
+  // finite-depth pair bins of the system's rows
+  PairBins bins;
+  // field points (CalPotentialMulti): nfp points (SoA); vtx: panel vertices,
+  // vtx[(3q + d) n + j]; the field view ffx/ffy/ffz holds the hull centroids
+  // and then one chunk of field points; field_gen names the point set
+  int nfp = 0;
+  unsigned long long field_gen = 0;
+  DevBuf<double> fpx, fpy, fpz, vtx, ffx, ffy, ffz, fmodetab;
+  PairBins fbins;
+  DevBuf<double2> fM, fZ, fP, fOut, fInc, fDincp;
+  DevBuf<int> fOnb;
   DevBuf<double> modetab;  // per-frequency finite-depth mode tables
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
@@ -267,12 +294,6 @@ struct hamsx_ctx {
   DevBuf<size_t> pLuOff, pXoff;  // per body: factor offset, packed right-hand-side offset
   int pMax = 0;
   std::vector<size_t> pOff;
-  DevBuf<int> bin_cnt;
-  int bin_n = 0, bin_nf = 0;
-  double bin_h = 0.0;
-  // list slices per region: single entries [reg_begin, reg_end), hull-pair
-  // entries [pair_begin, pair_end)
-  size_t reg_begin[5] = {0}, reg_end[5] = {0}, pair_begin[5] = {0}, pair_end[5] = {0};
   DevBuf<double> out, norms;
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   bool pending = false;  // a submitted frequency solve not yet collected
@@ -327,8 +348,8 @@ struct StageTimer {
   hamsx_ctx* c;
   int stage;
   hipEvent_t a = nullptr;
-  StageTimer(hamsx_ctx* ctx, int st) : c(ctx), stage(st) {
-    if (!c->profiling) return;
+  StageTimer(hamsx_ctx* ctx, int st) : c(ctx), stage(st) {  // st < 0: not timed
+    if (!c->profiling || st < 0) return;
     a = take_event(c);
     if (a) (void)hipEventRecord(a, c->stream);
   }
@@ -373,23 +394,24 @@ bool force_grid_assembly() {
   return e && *e && *e != '0';
 }
 
-// Counting sort of the N^2 pairs by (region, NT, FLAG) for the current depth;
-// done once per (mesh, h) and reused by every frequency at that depth.
-int ensure_bins(hamsx_ctx* c) {
+// Counting sort of the pairs of mesh view m by (region, NT, FLAG) for the
+// current depth; done once per (mesh, rows, h) and reused by every frequency
+// at that depth.
+int ensure_bins(hamsx_ctx* c, PairBins& B, const hamsx::MeshDev& m, unsigned long long key) {
   using namespace hamsx;
   const double h = c->freq.h;
-  if (c->bin_n == c->n && c->bin_nf == c->nf && c->bin_h == h) return 0;
-  c->bin_n = 0;
+  if (B.n == m.n && B.nf == m.nf && B.h == h && B.key == key) return 0;
+  B.n = 0;
   int rc;
-  if ((rc = c->bin_cnt.ensure(kFinBins))) return rc;
-  HX_TRY(hipMemsetAsync(c->bin_cnt.p, 0, sizeof(int) * kFinBins, c->stream));
-  launch::bin_count(c->stream, c->mesh(), h, c->bin_cnt.p);
+  if ((rc = B.cnt.ensure(kFinBins))) return rc;
+  HX_TRY(hipMemsetAsync(B.cnt.p, 0, sizeof(int) * kFinBins, c->stream));
+  launch::bin_count(c->stream, m, h, B.cnt.p);
   if ((rc = check_launch())) return rc;
   std::vector<int> cnt(kFinBins), start(kFinBins);
-  HX_TRY(hipMemcpyAsync(cnt.data(), c->bin_cnt.p, sizeof(int) * kFinBins, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(cnt.data(), B.cnt.p, sizeof(int) * kFinBins, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
   size_t off = 0;
-  for (int r = 0; r < 5; r++) c->reg_begin[r] = c->reg_end[r] = c->pair_begin[r] = c->pair_end[r] = 0;
+  for (int r = 0; r < 5; r++) B.reg_begin[r] = B.reg_end[r] = B.pair_begin[r] = B.pair_end[r] = 0;
   bool seen[2][5] = {};
   for (int k = 0; k < kFinBins; k++) {
     start[k] = (int)off;
@@ -397,8 +419,8 @@ int ensure_bins(hamsx_ctx* c) {
     if (cnt[k] > 0) {
       const int r = fin_key_region(k);
       const bool pr = fin_key_paired(k);
-      size_t* b = pr ? c->pair_begin : c->reg_begin;
-      size_t* e = pr ? c->pair_end : c->reg_end;
+      size_t* b = pr ? B.pair_begin : B.reg_begin;
+      size_t* e = pr ? B.pair_end : B.reg_end;
       if (!seen[pr][r]) b[r] = off;
       seen[pr][r] = true;
       e[r] = off + padded;
@@ -406,16 +428,31 @@ int ensure_bins(hamsx_ctx* c) {
     off += padded;
   }
   if (off > (size_t)INT32_MAX) return fail(HAMSX_EARG, "too many panel pairs for the binned assembly");
-  if ((rc = c->plist.ensure(std::max<size_t>(off, 1)))) return rc;
-  HX_TRY(hipMemsetAsync(c->plist.p, 0xFF, sizeof(unsigned) * off, c->stream));
-  HX_TRY(hipMemcpyAsync(c->bin_cnt.p, start.data(), sizeof(int) * kFinBins, hipMemcpyHostToDevice, c->stream));
-  launch::bin_scatter(c->stream, c->mesh(), h, c->bin_cnt.p, c->plist.p);
+  if ((rc = B.plist.ensure(std::max<size_t>(off, 1)))) return rc;
+  HX_TRY(hipMemsetAsync(B.plist.p, 0xFF, sizeof(unsigned) * off, c->stream));
+  HX_TRY(hipMemcpyAsync(B.cnt.p, start.data(), sizeof(int) * kFinBins, hipMemcpyHostToDevice, c->stream));
+  launch::bin_scatter(c->stream, m, h, B.cnt.p, B.plist.p);
   if ((rc = check_launch())) return rc;
   HX_TRY(hipStreamSynchronize(c->stream));  // start[] is read by the copy above
-  c->bin_n = c->n;
-  c->bin_nf = c->nf;
-  c->bin_h = h;
+  B.n = m.n;
+  B.nf = m.nf;
+  B.h = h;
+  B.key = key;
   return 0;
+}
+
+// The binned assembly of mesh view m into A and S (leading dimension
+// m.nf - m.row0), regions in decreasing cost so the cheap ones fill the tail.
+void assemble_binned(hamsx_ctx* c, const PairBins& B, const hamsx::MeshDev& m, const hamsx::FreqDev& f, double2* A,
+                     double2* S, int* ovf, bool timed) {
+  hamsx::launch::clear_flags(c->stream, ovf);
+  for (int r : {hamsx::kRegLinton, hamsx::kRegPidcock, hamsx::kRegEigene, hamsx::kRegEigen}) {
+    StageTimer tr(c, timed ? 4 + r : -1);
+    hamsx::launch::assemble_region(c->stream, m, f, r, false, B.plist.p + B.reg_begin[r], B.reg_end[r] - B.reg_begin[r],
+                                   A, S, ovf);
+    hamsx::launch::assemble_region(c->stream, m, f, r, true, B.plist.p + B.pair_begin[r],
+                                   B.pair_end[r] - B.pair_begin[r], A, S, ovf);
+  }
 }
 
 int ensure_work(hamsx_ctx* c) {
@@ -1041,6 +1078,21 @@ int finish_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int* i
   return complete_forces(c, amas, bdmp, exfc, info_out);
 }
 
+// Field points per pass: the 16-bit row index of the pair list bounds the
+// binned form to n + F <= 65535, and M = [S | T] (F x 2n complex) is kept
+// under 4 GiB.  HAMSX_FIELD_CHUNK (diagnostic) lowers the count further, to
+// exercise the pass loop on small cases.
+int field_chunk(const hamsx_ctx* c, bool binned) {
+  size_t cap = ((size_t)4 << 30) / (32 * (size_t)c->n);
+  if (binned) cap = std::min(cap, (size_t)(0xFFFF - c->n));
+  cap = std::min<size_t>(cap, 32768);
+  if (const char* e = std::getenv("HAMSX_FIELD_CHUNK")) {
+    const long v = std::atol(e);
+    if (v > 0) cap = std::min<size_t>(cap, (size_t)v);
+  }
+  return (int)std::max<size_t>(1, std::min<size_t>(cap, (size_t)c->nfp));
+}
+
 int busy(hamsx_ctx*) {
   return fail(HAMSX_ESTATE, "a submitted frequency solve is pending on this context: call hamsx_collect first");
 }
@@ -1071,13 +1123,13 @@ int hamsx_create(hamsx_ctx** out, int device) {
     delete c;
     return fail(HAMSX_EDEVICE, "stream creation failed on device %d", device);
   }
-  int rc = c->flags.ensure(4);
+  int rc = c->flags.ensure(5);
   if (rc == 0) rc = c->wvn.ensure(hamsx::kNK);
   if (rc) {
     hamsx_destroy(c);
     return rc;
   }
-  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 4, c->stream) != hipSuccess) {
+  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 5, c->stream) != hipSuccess) {
     hamsx_destroy(c);
     return fail(HAMSX_EDEVICE, "flag initialisation failed");
   }
@@ -1093,9 +1145,8 @@ int hamsx_destroy(hamsx_ctx* c) {
                   &c->norms})
     b->release();
   for (auto* b : {&c->ncn, &c->body, &c->body_start_d, &c->chunk_body, &c->chunk_j0, &c->chunk_j1, &c->ipiv,
-                  &c->perm, &c->widx, &c->wcnt, &c->flags, &c->bin_cnt})
+                  &c->perm, &c->widx, &c->wcnt, &c->flags})
     b->release();
-  c->plist.release();
   for (auto* b : {&c->fx, &c->fy, &c->fz}) b->release();
   c->Cm.release();
   c->Xe.release();
@@ -1278,6 +1329,12 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
     }
     newman_frame(nc, xv, frame.data() + e * kFrameLen);
   }
+  std::vector<double> vtx(n * 12, 0.0);
+  for (size_t e = 0; e < n; e++)
+    for (int q = 0; q < ncn[e]; q++) {
+      const int nd = ncon[e + n * q] - 1;
+      for (int d = 0; d < 3; d++) vtx[(size_t)(3 * q + d) * n + e] = xyz[nd + (size_t)nnode * d];
+    }
   std::vector<int> cb, c0, c1;
   for (int b = 0; b < nbody; b++) {
     for (int j = body_start[b]; j < body_start[b + 1]; j++) body[j] = b;
@@ -1288,7 +1345,7 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
     }
   }
   c->n = nelem;
-  c->bin_n = 0;
+  c->bins.n = 0;
   c->hm_part = c->hm_built = false;
   c->hcen.assign(cen, cen + 3 * n);
   c->nbody = nbody;
@@ -1313,6 +1370,10 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   HX_TRY(hipMemcpy(c->chunk_body.p, cb.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
   HX_TRY(hipMemcpy(c->chunk_j0.p, c0.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
   HX_TRY(hipMemcpy(c->chunk_j1.p, c1.data(), sizeof(int) * cb.size(), hipMemcpyHostToDevice));
+  if ((rc = c->vtx.ensure(n * 12))) return rc;
+  HX_TRY(hipMemcpy(c->vtx.p, vtx.data(), sizeof(double) * n * 12, hipMemcpyHostToDevice));
+  c->nfp = 0;
+  c->field_gen++;
   c->nlid = 0;
   c->nf = nelem;
   if ((rc = c->fx.ensure(n)) || (rc = c->fy.ensure(n)) || (rc = c->fz.ensure(n))) return rc;
@@ -1399,24 +1460,16 @@ int hamsx_assemble(hamsx_ctx* c) {
   const bool finite = c->freq.h >= 0.0;
   const bool binned = finite && c->nf <= 0xFFFF && !force_grid_assembly();
   if ((rc = c->A.ensure((size_t)c->nf * c->n)) || (rc = c->S.ensure((size_t)c->nf * c->n))) return rc;
-  if (binned && (rc = ensure_bins(c))) return rc;
+  if (binned && (rc = ensure_bins(c, c->bins, c->mesh(), 0))) return rc;
   if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->nf)))) return rc;
   c->have_C = false;
   c->freq.tab = finite ? c->modetab.p : nullptr;
   {
     StageTimer tm(c, 0);
     if (finite) hamsx::launch::mode_tables(c->stream, c->mesh(), c->freq);
-    if (binned) {
-      hamsx::launch::clear_flags(c->stream, c->flags.p);
-      // most expensive regions first, so the cheap ones fill the tail
-      for (int r : {hamsx::kRegLinton, hamsx::kRegPidcock, hamsx::kRegEigene, hamsx::kRegEigen}) {
-        StageTimer tr(c, 4 + r);
-        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, false, c->plist.p + c->reg_begin[r],
-                                       c->reg_end[r] - c->reg_begin[r], c->A.p, c->S.p, c->flags.p);
-        hamsx::launch::assemble_region(c->stream, c->mesh(), c->freq, r, true, c->plist.p + c->pair_begin[r],
-                                       c->pair_end[r] - c->pair_begin[r], c->A.p, c->S.p, c->flags.p);
-      }
-    } else
+    if (binned)
+      assemble_binned(c, c->bins, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p, true);
+    else
       hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
   }
   if ((rc = check_launch())) return rc;
@@ -1508,6 +1561,104 @@ int hamsx_collect(hamsx_ctx* c, double* amas, double* bdmp, double* exfc) {
   int info = 0;
   if ((rc = complete_forces(c, amas, bdmp, exfc, &info))) return rc;
   return info;
+}
+
+int hamsx_set_field_points(hamsx_ctx* c, int nfp, const double* xfp) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_field_points needs hamsx_set_mesh first");
+  if (nfp < 0 || (nfp > 0 && !xfp)) return fail(HAMSX_EARG, "bad field-point arguments");
+  for (size_t k = 0; k < 3 * (size_t)nfp; k++)
+    if (!std::isfinite(xfp[k])) return fail(HAMSX_EARG, "field point %zu is not finite", k % nfp + 1);
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  if (nfp > 0) {
+    if ((rc = c->fpx.ensure(nfp)) || (rc = c->fpy.ensure(nfp)) || (rc = c->fpz.ensure(nfp))) return rc;
+    HX_TRY(hipMemcpy(c->fpx.p, xfp, sizeof(double) * nfp, hipMemcpyHostToDevice));
+    HX_TRY(hipMemcpy(c->fpy.p, xfp + nfp, sizeof(double) * nfp, hipMemcpyHostToDevice));
+    HX_TRY(hipMemcpy(c->fpz.p, xfp + 2 * (size_t)nfp, sizeof(double) * nfp, hipMemcpyHostToDevice));
+  }
+  c->nfp = nfp;
+  c->field_gen++;
+  return 0;
+}
+
+int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident, int* onbody) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (kind < HAMSX_FIELD_POTENTIAL || kind > HAMSX_FIELD_AUTO) return fail(HAMSX_EARG, "unknown field kind %d", kind);
+  if (c->nfp == 0) return fail(HAMSX_ESTATE, "no field points: call hamsx_set_field_points first");
+  int rc = set_device(c);
+  if (rc || (rc = do_solve(c))) return rc;
+  const int n = c->n, nfp = c->nfp, nd = 6 * c->nbody, nr = c->nrhs(), nbeta = c->nbeta;
+  const bool finite = c->freq.h >= 0.0;
+  const bool binned = finite && n < 0xFFFF && !force_grid_assembly();
+  const int F = field_chunk(c, binned);
+  const bool want_inc = incident && nbeta > 0;
+  if ((rc = c->fDincp.ensure((size_t)n * std::max(1, nbeta))) || (rc = c->fZ.ensure((size_t)2 * n * nr)) ||
+      (rc = c->fM.ensure((size_t)2 * n * F)) || (rc = c->fP.ensure((size_t)F * nr)) ||
+      (rc = c->fOut.ensure((size_t)nfp * nr)) || (rc = c->fOnb.ensure(nfp)) || (rc = c->ffx.ensure(n + F)) ||
+      (rc = c->ffy.ensure(n + F)) || (rc = c->ffz.ensure(n + F)) ||
+      (want_inc && (rc = c->fInc.ensure((size_t)nfp * nbeta))) ||
+      (finite && (rc = c->fmodetab.ensure(hamsx::mode_tab_len(n + F)))) || (rc = blas_handle(c)))
+    return rc;
+  const hamsx::MeshDev m = c->mesh();
+  hamsx::launch::dincp(c->stream, m, c->freq, c->waves(), c->fDincp.p);
+  hamsx::launch::field_weights(c->stream, m, c->nbody, nbeta, c->fDincp.p, c->X.p, c->fZ.p);
+  if ((rc = check_launch())) return rc;
+  HX_TRY(hipMemcpyAsync(c->ffx.p, c->cx.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HX_TRY(hipMemcpyAsync(c->ffy.p, c->cy.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  HX_TRY(hipMemcpyAsync(c->ffz.p, c->cz.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  hamsx::FreqDev ff = c->freq;
+  ff.tab = finite ? c->fmodetab.p : nullptr;
+  int* ovf = c->flags.p + 4;
+  const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+  for (int p0 = 0, chunk = 0; p0 < nfp; p0 += F, chunk++) {
+    const int nc = std::min(F, nfp - p0);
+    HX_TRY(hipMemcpyAsync(c->ffx.p + n, c->fpx.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->ffy.p + n, c->fpy.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->ffz.p + n, c->fpz.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
+    // the field view: sources are the hull panels, rows n .. n + nc - 1 the chunk's points
+    hamsx::MeshDev fm = m;
+    fm.fx = c->ffx.p;
+    fm.fy = c->ffy.p;
+    fm.fz = c->ffz.p;
+    fm.nf = n + nc;
+    fm.row0 = n;
+    double2* S = c->fM.p;
+    double2* T = c->fM.p + (size_t)nc * n;
+    if (finite) hamsx::launch::mode_tables(c->stream, fm, ff);
+    if (binned) {
+      if ((rc = ensure_bins(c, c->fbins, fm, (c->field_gen << 20) | (unsigned long long)chunk))) return rc;
+      assemble_binned(c, c->fbins, fm, ff, T, S, ovf, false);
+    } else {
+      hamsx::launch::assemble(c->stream, fm, ff, T, S, ovf);
+    }
+    hamsx::launch::on_body(c->stream, m, c->vtx.p, nc, c->fpx.p + p0, c->fpy.p + p0, c->fpz.p + p0, c->fOnb.p + p0);
+    if ((rc = check_launch())) return rc;
+    if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nc, nr, 2 * n, &one,
+                      reinterpret_cast<const rocblas_double_complex*>(c->fM.p), nc,
+                      reinterpret_cast<const rocblas_double_complex*>(c->fZ.p), 2 * n, &zero,
+                      reinterpret_cast<rocblas_double_complex*>(c->fP.p), nc) != rocblas_status_success)
+      return fail(HAMSX_EDEVICE, "rocblas_zgemm (field points) failed");
+    hamsx::launch::field_finish(c->stream, nc, c->fpx.p + p0, c->fpy.p + p0, c->fpz.p + p0, c->fOnb.p + p0, c->freq,
+                                c->waves(), nd, kind, c->fP.p, (size_t)nfp, c->fOut.p + p0,
+                                want_inc ? c->fInc.p + p0 : nullptr);
+    if ((rc = check_launch())) return rc;
+    int ov = 0;
+    HX_TRY(hipMemcpyAsync(&ov, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HX_TRY(hipStreamSynchronize(c->stream));
+    if (ov & 1) return fail(HAMSX_EDEVICE, "adaptive quadrature stack exhausted in the finite-depth kernel");
+  }
+  if (values)
+    HX_TRY(hipMemcpyAsync(values, c->fOut.p, sizeof(double2) * (size_t)nfp * nr, hipMemcpyDeviceToHost, c->stream));
+  if (want_inc)
+    HX_TRY(hipMemcpyAsync(incident, c->fInc.p, sizeof(double2) * (size_t)nfp * nbeta, hipMemcpyDeviceToHost,
+                          c->stream));
+  if (onbody) HX_TRY(hipMemcpyAsync(onbody, c->fOnb.p, sizeof(int) * nfp, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int hamsx_get_matrix(hamsx_ctx* c, int which, double* out, int* ipiv) {

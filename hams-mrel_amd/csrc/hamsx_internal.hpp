@@ -34,6 +34,8 @@ struct MeshDev {
   const int* ncn;
   const double* frame;  // Newman local frame, kFrameLen doubles per panel
   const int* body;      // body index of each panel
+  int row0;             // first field row assembled: rows i >= row0 are stored at i - row0, leading
+                        // dimension nf - row0 (0 for the system; n for field-point rows)
 };
 
 // Regions of the finite-depth Green function by R/h (FinGreen3D.f90:183-191)
@@ -133,6 +135,17 @@ void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2*
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
             const double2* X, double* out);
 // single-precision copy of A for the mixed path, and the helpers of refinement
+// Field points: DINCP(j, k) = grad(phi_I) . n at each hull panel (n x nbeta);
+// Z = [W; -X] (2n x nrhs); the free-term test onb[nfp]; the finished values
+// out[col * ld + p] (and inc[k * ld + p] when inc is not null).
+void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out);
+void field_weights(hipStream_t s, const MeshDev& m, int nbody, int nbeta, const double2* dincp, const double2* X,
+                   double2* Z);
+void on_body(hipStream_t s, const MeshDev& m, const double* vtx, int nfp, const double* px, const double* py,
+             const double* pz, int* onb);
+void field_finish(hipStream_t s, int nfp, const double* px, const double* py, const double* pz, const int* onb,
+                  const FreqDev& f, const WaveDev& w, int nd, int kind, const double2* P, size_t ld, double2* out,
+                  double2* inc);
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 // C(r, c) = C(c, r) above the diagonal (n x n): completes a lower-triangle syrk
 void mirror_lower(hipStream_t s, double2* C, int n);

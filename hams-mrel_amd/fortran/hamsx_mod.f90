@@ -27,6 +27,8 @@ module hamsx_mod
   integer(c_int), parameter, public :: HAMSX_OK = 0, HAMSX_NK = 200
   integer(c_int), parameter, public :: HAMSX_PREC_FP64 = 0, HAMSX_PREC_MIXED = 1
   integer(c_int), parameter, public :: HAMSX_SOLVER_LU = 0, HAMSX_SOLVER_GMRES = 1
+  integer(c_int), parameter, public :: HAMSX_FIELD_POTENTIAL = 0, HAMSX_FIELD_PRESSURE = 1, &
+                                       HAMSX_FIELD_ELEVATION = 2, HAMSX_FIELD_AUTO = 3
 
   public :: hamsx_version, hamsx_last_error, hamsx_device_count, hamsx_create, hamsx_destroy, &
             hamsx_set_precision, hamsx_panel_geometry, hamsx_wave_properties, hamsx_set_mesh, &
@@ -35,7 +37,7 @@ module hamsx_mod
             hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
             hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, hamsx_set_hmatrix, &
             hamsx_hmatrix_stats, hamsx_hmatrix_matvec, hamsx_hmatrix_partition, &
-            hamsx_stage_times
+            hamsx_stage_times, hamsx_set_field_points, hamsx_field_values
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
             PANEL_GEOMETRY_GPU, hamsx_error_message
@@ -170,6 +172,22 @@ module hamsx_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: nlid
       real(c_double), intent(in) :: lid_cen(*)
+      integer(c_int) :: rc
+    end function
+    ! XFP(NFP,3) of ReadControlFile; pass any array with nfp = 0 to remove the points
+    function hamsx_set_field_points(ctx, nfp, xfp) bind(C, name="hamsx_set_field_points") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nfp
+      real(c_double), intent(in) :: xfp(*)
+      integer(c_int) :: rc
+    end function
+    ! values(NFP, 6*NBODY+NBETA); incident(NFP, NBETA) and onbody(NFP) may be c_null_ptr
+    function hamsx_field_values(ctx, kind, values, incident, onbody) bind(C, name="hamsx_field_values") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      type(c_ptr), value :: values, incident, onbody
       integer(c_int) :: rc
     end function
     function hamsx_set_solver(ctx, solver, tol, restart, maxiter) bind(C, name="hamsx_set_solver") result(rc)

@@ -22,6 +22,10 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 SOLVER_LU = 0
 SOLVER_GMRES = 1
+FIELD_POTENTIAL = 0
+FIELD_PRESSURE = 1
+FIELD_ELEVATION = 2
+FIELD_AUTO = 3
 
 #: every symbol include/hamsx.h declares, with (restype, argtypes)
 _dp = C.POINTER(C.c_double)
@@ -57,6 +61,8 @@ SIGNATURES = {
     "hamsx_hmatrix_matvec": (C.c_int, [_vp, _dp, C.c_int, _dp]),
     "hamsx_hmatrix_partition": (C.c_int, [C.c_int, _dp, C.c_int, C.c_double, _ip, C.c_int, _ip, _ip]),
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
+    "hamsx_set_field_points": (C.c_int, [_vp, C.c_int, _dp]),
+    "hamsx_field_values": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),
 }

@@ -64,6 +64,7 @@ class ControlDeck:
     irsp: int
     nthread: int
     isolv: int = 1
+    xfp: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))  # field points (NFP x 3)
 
 
 def read_control(path: str) -> ControlDeck:
@@ -108,15 +109,26 @@ def read_control(path: str) -> ControlDeck:
     isol = _fixed_int(nxt(), 26, 16)
     irsp = _fixed_int(nxt(), 23, 16)
     nthread = _fixed_int(nxt(), 23, 16)
+    rest = list(it)
+    k = 0
     isolv = 1
-    try:
-        line = nxt()
-        if "Solver_type" in line:
-            isolv = _fixed_int(line, 26, 16) or 1
-    except StopIteration:
-        pass
+    # optional Solver_type line; when absent the reference steps back (backspace)
+    if rest and "Solver_type" in rest[0]:
+        try:
+            isolv = _fixed_int(rest[0], 26, 16)
+        except ValueError:
+            isolv = 1
+        if isolv not in (1, 2):
+            isolv = 1
+        k = 1
+    # field points (:193-203): two lines, NFP in (27x,i16), then NFP list-directed x y z records
+    xfp = np.zeros((0, 3))
+    if k + 2 < len(rest):
+        nfp = _fixed_int(rest[k + 2], 27, 16)
+        if nfp > 0:
+            xfp = np.array([[_ffloat(t) for t in rest[k + 3 + i].split()[:3]] for i in range(nfp)])
     return ControlDeck(h, sybo, inft, ouft, wvnb, nbody, lcs, heads, xr, refl, isol, irsp,
-                       nthread, isolv)
+                       nthread, isolv, xfp)
 
 
 @dataclass

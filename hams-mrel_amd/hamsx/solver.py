@@ -38,6 +38,7 @@ class FrequencySolver:
         self._h = h
         self.set_precision(precision)
         self.nbeta = 0
+        self.nfp = 0
         self.set_mesh(mesh)
 
     def set_mesh(self, mesh: Combined):
@@ -48,6 +49,7 @@ class FrequencySolver:
             _lib.ptr(m.cen), _lib.ptr(m.area), _lib.ptr(m.pnsz), _lib.ptr(m.dxyz), m.nbody,
             _lib.ptr(m.body_start, C.c_int)), "hamsx_set_mesh")
         self.mesh = mesh
+        self.nfp = 0
 
     # -- lifecycle -------------------------------------------------------------
     def close(self):
@@ -81,6 +83,37 @@ class FrequencySolver:
             return
         lc = np.asfortranarray(lid_cen, dtype=np.float64)
         _lib.check(self._lib.hamsx_set_lid(self._h, lc.shape[0], _lib.ptr(lc)), "hamsx_set_lid")
+
+    def set_field_points(self, xfp):
+        """Field points (P x 3) for pressure / elevation output (the XFP array
+        of ControlFile.in); None or an empty array removes them."""
+        if xfp is None or len(xfp) == 0:
+            _lib.check(self._lib.hamsx_set_field_points(self._h, 0, None), "hamsx_set_field_points")
+            self.nfp = 0
+            return
+        xf = np.asfortranarray(xfp, dtype=np.float64).reshape(-1, 3)
+        _lib.check(self._lib.hamsx_set_field_points(self._h, xf.shape[0], _lib.ptr(xf)), "hamsx_set_field_points")
+        self.nfp = xf.shape[0]
+
+    def field_values(self, kind: str = "auto", incident: bool = False):
+        """Values at the field points for every mode of the current frequency
+        (CalPotentialMulti / CalPressureMulti / CalElevationMulti): kind
+        'potential', 'pressure', 'elevation' or 'auto' (pressure below the
+        free surface, elevation on it).  Returns (values P x (6 nbody +
+        nbeta), onbody P bool) and, with incident=True, the incident-wave
+        values P x nbeta as a third item."""
+        code = {"potential": _lib.FIELD_POTENTIAL, "pressure": _lib.FIELD_PRESSURE,
+                "elevation": _lib.FIELD_ELEVATION, "auto": _lib.FIELD_AUTO}[kind]
+        nfp, nr = self.nfp, 6 * self.mesh.nbody + self.nbeta
+        vals = np.zeros((nfp, nr), dtype=np.complex128, order="F")
+        inc = np.zeros((nfp, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        ob = np.zeros(nfp, dtype=np.int32)
+        _lib.check(self._lib.hamsx_field_values(self._h, code, _lib.ptr(vals),
+                                                _lib.ptr(inc) if incident else None, _lib.ptr(ob, C.c_int)),
+                   "hamsx_field_values")
+        if incident:
+            return vals, ob.astype(bool), inc[:, :self.nbeta]
+        return vals, ob.astype(bool)
 
     def set_solver(self, solver: str = "lu", tol: float = 1.0e-4, restart: int = 25, maxiter: int = 100):
         """'lu' (ZGETRF/ZGETRS) or 'gmres' (restarted GMRES with the per-body

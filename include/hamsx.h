@@ -204,6 +204,39 @@ int hamsx_set_refinement(hamsx_ctx *ctx, int max_steps, double tol);
  * the largest final relative residual 2-norm over the right-hand sides. */
 int hamsx_refine_stats(hamsx_ctx *ctx, int *iters, double *resid);
 
+/* ---- field points ----------------------------------------------------------
+ * Pressure and free-surface elevation away from the hull panels:
+ * CalPotentialMulti / CalPressureMulti / CalElevationMulti
+ * (src/PressureElevationMulti.f90:55-348) for every point and every mode, as
+ * the output drivers OutputPressureElevation_RadiationMulti / _DiffractionMulti
+ * / _IncidenceMulti (:407-556) call them point by point.
+ *
+ * hamsx_set_field_points: xfp[nfp*3] column-major, the XFP(NFP,3) array that
+ * ReadControlFile reads (src/InputFiles.f90:193-203).  nfp = 0 removes them;
+ * a new hamsx_set_mesh also does. */
+int hamsx_set_field_points(hamsx_ctx *ctx, int nfp, const double *xfp);
+
+/* kinds for hamsx_field_values */
+#define HAMSX_FIELD_POTENTIAL 0  /* CalPotentialMulti: POT */
+#define HAMSX_FIELD_PRESSURE 1   /* CalPressureMulti: i w rho POT */
+#define HAMSX_FIELD_ELEVATION 2  /* CalElevationMulti: i w / g POT */
+#define HAMSX_FIELD_AUTO 3       /* the output drivers' choice per point: pressure
+                                    where |z| > 1e-6, elevation on z = 0 */
+
+/* Values of the current frequency's potentials (solved here if the solve has
+ * not run yet) at the field points:
+ *   values[nfp*(6nb + nbeta)] complex, column-major (point fastest): column
+ *   6b + m is radiation mode m + 1 of body b (FACTOR_NORMAL on body b's panels,
+ *   PHIDOF = b + 1), column 6nb + k the diffraction potential of heading k,
+ *   including the incident potential when ISOL = 1;
+ *   incident[nfp*nbeta] complex (may be NULL): the incident wave alone
+ *   (CalPotentialIncMulti: VINP when ISOL = 1, else 0), same kind;
+ *   onbody[nfp] (may be NULL): 1 where the point lies on a hull panel (free
+ *   term 2 pi), 0 in the fluid (4 pi).
+ * The reference's WAMIT scaling (WamitNondimensMulti, :352-402) is left to
+ * the caller's writer.  NSYS = 1. */
+int hamsx_field_values(hamsx_ctx *ctx, int kind, double *values, double *incident, int *onbody);
+
 /* Measurement hook (no counterpart in the reference, which has no timers):
  * with profiling on, every stage is bracketed by HIP events on the context's
  * stream.  hamsx_stage_times returns the device milliseconds accumulated per

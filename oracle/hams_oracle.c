@@ -1717,3 +1717,147 @@ int orc_solve_frequency_irr(int nelem, int nnode, const double *xyz, const int *
     free(A); free(S); free(B); free(C); free(X); free(ipiv);
     return info;
 }
+
+/* ========================================================================
+ * The potentials of one frequency: the assembly, right-hand sides, ZGETRF and
+ * ZGETRS of orc_solve_frequency, returning MXPOT (phi[i + nelem*col],
+ * col < 6 nbody radiation, 6 nbody + b diffraction for heading beta[b]).
+ * ====================================================================== */
+int orc_solve_potentials(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                         const double *cen, const double *area, const double *pnsz, const double *dxyz,
+                         int nbody, const int *body_start, double w1, double h, int nbeta,
+                         const double *beta, double amp, double xw0, double xw1, double *phi, int nthreads)
+{
+    const int nk = 200;
+    double wvn[200] = {0};
+    double v = w1 * w1 / G_C, wk;
+    if (h <= 0.0) wk = v;
+    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    size_t nn = (size_t)nelem * nelem;
+    int nr = 6 * nbody + nbeta;
+    double *A = (double *)malloc(sizeof(double) * 2 * nn);
+    double *S = (double *)malloc(sizeof(double) * 2 * nn);
+    int *ipiv = (int *)malloc(sizeof(int) * nelem);
+    orc_assemble(nelem, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn, nk, A, S, nthreads);
+    orc_rhs(nelem, S, cen, dxyz, nbody, body_start, nbeta, beta, v, w1, wk, h, amp, xw0, xw1, phi);
+    int info = orc_zgetrf(nelem, A, ipiv);
+    orc_zgetrs(nelem, A, ipiv, nr, phi);
+    free(A); free(S); free(ipiv);
+    return info;
+}
+
+/* ========================================================================
+ * Field points: CalPotentialMulti (PressureElevationMulti.f90:55-275),
+ * CalPressureMulti (:303-324) and CalElevationMulti (:329-348), NSYS = 1
+ * (SX = SY = 1, XP = XET).
+ * ====================================================================== */
+/* The free-term test (:200-249): 1 when xp lies on a hull panel -- within
+ * MAX(1E-6, 1E-3 PNSZ) of its plane (measured from its first vertex) and
+ * inside its polygon, the edge cross products (edge x (xp - vertex)) . n all
+ * >= -TOLA or all <= TOLA, TOLA = 1E-3 PNSZ^2. */
+int orc_on_body(const double *xp, int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                const double *pnsz, const double *dxyz)
+{
+    for (int j = 0; j < nelem; j++) {
+        const double e1 = dxyz[j], e2 = dxyz[j + nelem], e3 = dxyz[j + 2 * nelem];
+        const double tolp = fmax(1.0e-6, 1.0e-3 * pnsz[j]);
+        const double tola = 1.0e-3 * ipow(pnsz[j], 2);
+        const int k1 = ncon[j] - 1;
+        const double dpln = (xp[0] - xyz[k1]) * e1 + (xp[1] - xyz[k1 + nnode]) * e2 + (xp[2] - xyz[k1 + 2 * nnode]) * e3;
+        if (fabs(dpln) > tolp) continue;
+        double smin = 1.0e30, smax = -1.0e30;
+        for (int k = 0; k < ncn[j]; k++) {
+            const int ka = ncon[j + nelem * k] - 1;
+            const int kb = ncon[j + nelem * (k + 1 < ncn[j] ? k + 1 : 0)] - 1;
+            double ev[3], wv[3];
+            for (int d = 0; d < 3; d++) {
+                ev[d] = xyz[kb + nnode * d] - xyz[ka + nnode * d];
+                wv[d] = xp[d] - xyz[ka + nnode * d];
+            }
+            const double cx = ev[1] * wv[2] - ev[2] * wv[1];
+            const double cy = ev[2] * wv[0] - ev[0] * wv[2];
+            const double cz = ev[0] * wv[1] - ev[1] * wv[0];
+            const double s = cx * e1 + cy * e2 + cz * e3;
+            smin = fmin(smin, s);
+            smax = fmax(smax, s);
+        }
+        if (smin >= -tola || smax <= tola) return 1;
+    }
+    return 0;
+}
+
+/* kind 0: potential (CalPotentialMulti); 1: pressure, i w rho POT
+ * (CalPressureMulti); 2: elevation, i w / g POT (CalElevationMulti); at the
+ * zero- and infinite-frequency limits rho POT and POT.  For every field
+ * point p and mode column col: col = 6 b + (MD - 1) is radiation mode MD with
+ * FACTOR_NORMAL = 1 on body b's panels (the output driver's PHIDOF = b + 1,
+ * :421-440), col = 6 nbody + k diffraction for heading beta[k] (with the
+ * incident potential VINP when isol = 1).  out[p + nfp*col] complex. */
+void orc_field_values(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                      const double *cen, const double *area, const double *pnsz, const double *dxyz,
+                      int nbody, const int *body_start, double w1, double h, int nbeta, const double *beta,
+                      double amp, double xw0, double xw1, double rho, int isol, const double *phi, int nfp,
+                      const double *xfp, int kind, double *out, int *onbody, int nthreads)
+{
+    const int nk = 200;
+    double wvn[200] = {0};
+    double v = w1 * w1 / G_C, wk;
+    if (h <= 0.0) wk = v;
+    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    const int nd = 6 * nbody, nr = nd + nbeta;
+    const int limit = fabs(v) < F1EM8 || fabs(v + 1.0) < F1EM8;
+    int *bodyof = (int *)malloc(sizeof(int) * (size_t)nelem);
+    for (int b = 0; b < nbody; b++)
+        for (int j = body_start[b]; j < body_start[b + 1]; j++) bodyof[j] = b;
+    /* DINCP(JEL) = grad(phi_I)(XT) . ENT at the source centroid (:131-150) */
+    cplx *dincp = (cplx *)malloc(sizeof(cplx) * (size_t)nelem * (nbeta > 0 ? nbeta : 1));
+    for (int b = 0; b < nbeta; b++)
+        for (int j = 0; j < nelem; j++) {
+            double dpo[6];
+            orc_dinp(cen[j], cen[j + nelem], cen[j + 2 * nelem], xw0, xw1, beta[b], v, w1, wk, h, amp, dpo);
+            cplx px = dpo[0] + I * dpo[1], py = dpo[2] + I * dpo[3], pz = dpo[4] + I * dpo[5];
+            dincp[j + (size_t)nelem * b] = px * dxyz[j] + py * dxyz[j + nelem] + pz * dxyz[j + 2 * nelem];
+        }
+    const cplx *ph = (const cplx *)phi;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int p = 0; p < nfp; p++) {
+        const double xp[3] = {xfp[p], xfp[p + nfp], xfp[p + 2 * nfp]};
+        cplx *pot = (cplx *)calloc((size_t)nr, sizeof(cplx));
+        for (int j = 0; j < nelem; j++) {
+            cplx dum2, dum1;  /* DUM(2): the double-layer entry, DUM(1): the single-layer entry */
+            field_entry(xp[0], xp[1], xp[2], j, nelem, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn,
+                        nk, &dum2, &dum1);
+            for (int col = 0; col < nd; col++) {
+                const int md = col % 6;
+                const double fn = bodyof[j] == col / 6 ? 1.0 : 0.0;
+                const cplx term1 = dum1 * dxyz[j + nelem * md] * fn;
+                const cplx term2 = dum2 * ph[j + (size_t)nelem * col];
+                pot[col] = pot[col] + (term1 - term2);
+            }
+            for (int b = 0; b < nbeta; b++) {
+                const cplx term1 = -dum1 * dincp[j + (size_t)nelem * b];
+                const cplx term2 = dum2 * ph[j + (size_t)nelem * (nd + b)];
+                pot[nd + b] = pot[nd + b] + (term1 - term2);
+            }
+        }
+        const int ob = orc_on_body(xp, nelem, nnode, xyz, ncn, ncon, pnsz, dxyz);
+        if (onbody) onbody[p] = ob;
+        const double sld = ob ? 2.0 * PI_C : 4.0 * PI_C;
+        for (int col = 0; col < nr; col++) {
+            cplx val = pot[col] / sld;
+            if (col >= nd && isol == 1 && !limit) {
+                double f0[2];
+                orc_vinp(xp[0], xp[1], xp[2], xw0, xw1, beta[col - nd], v, w1, wk, h, amp, f0);
+                val = val + (f0[0] + I * f0[1]);
+            }
+            if (kind == 1) val = limit ? rho * val : (I * w1 * rho) * val;
+            else if (kind == 2) val = limit ? val : (I * w1 / G_C) * val;
+            out[2 * ((size_t)p + (size_t)nfp * col)] = creal(val);
+            out[2 * ((size_t)p + (size_t)nfp * col) + 1] = cimag(val);
+        }
+        free(pot);
+    }
+    free(dincp);
+    free(bodyof);
+}

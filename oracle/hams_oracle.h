@@ -112,6 +112,28 @@ int orc_solve_frequency_irr(int nelem, int nnode, const double *node_xyz, const 
                             double xw1, double rho, int isol, double *amas, double *bdmp,
                             double *exfc, int nthreads);
 
+
+/* ---- potentials of one frequency: phi[i + nelem*col] (MXPOT); returns INFO ---- */
+int orc_solve_potentials(int nelem, int nnode, const double *node_xyz, const int *ncn,
+                         const int *ncon, const double *cen, const double *area,
+                         const double *pnsz, const double *dxyz, int nbody,
+                         const int *body_start, double w1, double h, int nbeta,
+                         const double *beta, double amp, double xw0, double xw1, double *phi,
+                         int nthreads);
+
+/* ---- field points: CalPotentialMulti / CalPressureMulti / CalElevationMulti
+ * (PressureElevationMulti.f90:55-348).  xfp[p + nfp*d]; kind 0 potential,
+ * 1 pressure, 2 elevation; out[p + nfp*col] complex for the 6 nbody radiation
+ * modes and the nbeta headings; onbody[p] (may be NULL) the free-term test. */
+int orc_on_body(const double *xp, int nelem, int nnode, const double *node_xyz, const int *ncn,
+                const int *ncon, const double *pnsz, const double *dxyz);
+void orc_field_values(int nelem, int nnode, const double *node_xyz, const int *ncn,
+                      const int *ncon, const double *cen, const double *area,
+                      const double *pnsz, const double *dxyz, int nbody, const int *body_start,
+                      double w1, double h, int nbeta, const double *beta, double amp, double xw0,
+                      double xw1, double rho, int isol, const double *phi, int nfp,
+                      const double *xfp, int kind, double *out, int *onbody, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

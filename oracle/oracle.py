@@ -82,6 +82,14 @@ def _load(path):
         o.orc_solve_frequency_irr.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip,
                                               C.c_int, _dp, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp,
                                               _dp, C.c_int]
+        o.orc_solve_potentials.restype = C.c_int
+        o.orc_solve_potentials.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
+                                           _d, C.c_int, _dp, _d, _d, _d, _dp, C.c_int]
+        o.orc_on_body.restype = C.c_int
+        o.orc_on_body.argtypes = [_dp, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp]
+        o.orc_field_values.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d, _d,
+                                       C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _ip,
+                                       C.c_int]
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
@@ -113,6 +121,7 @@ def ref():
         r.ref_fingreen3d.argtypes = [_d] * 7 + [_dp, C.c_int, _d, C.c_int, _dp]
         r.ref_set_mesh.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp]
         r.ref_sglint.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        r.ref_sglint2.argtypes = [_dp, C.c_int, _dp]
         r.ref_dinp.argtypes = [_d] * 11 + [_dp]
         r.ref_vinp.argtypes = [_d] * 11 + [_dp]
         r.ref_geometry.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
@@ -209,3 +218,35 @@ def solve_frequency_irr(m, lid_cen, w1: float, h: float, headings_rad, amp=1.0, 
                                          _p(m.body_start, C.c_int), lc.shape[0], _p(lc), w1, h, len(beta), _p(beta),
                                          amp, xw[0], xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), nthreads)
     return amas, bdmp, exfc[:, :len(beta)], info
+
+
+def solve_potentials(m, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), nthreads: int = 8,
+                     fma: bool = False):
+    """The solved potentials MXPOT (N x (6 nbody + nbeta)) of one frequency."""
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    nr = 6 * m.nbody + len(beta)
+    phi = np.zeros((m.nelem, nr), dtype=np.complex128, order="F")
+    info = lib(fma).orc_solve_potentials(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+                                      _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
+                                      _p(m.body_start, C.c_int), w1, h, len(beta), _p(beta), amp, xw[0], xw[1],
+                                      _p(phi), nthreads)
+    return phi, info
+
+
+def field_values(m, phi, xfp, w1: float, h: float, headings_rad, kind=0, amp=1.0, xw=(0.0, 0.0), rho=1000.0,
+                 isol=1, nthreads: int = 8, fma: bool = False):
+    """CalPotentialMulti / CalPressureMulti / CalElevationMulti (kind 0/1/2)
+    at the field points xfp (P x 3) for every mode column of phi.  Returns
+    (values P x (6 nbody + nbeta), onbody P)."""
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    xf = np.asfortranarray(xfp, dtype=np.float64).reshape(-1, 3)
+    nfp, nr = xf.shape[0], 6 * m.nbody + len(beta)
+    ph = np.asfortranarray(phi, dtype=np.complex128)
+    assert ph.shape == (m.nelem, nr)
+    out = np.zeros((nfp, nr), dtype=np.complex128, order="F")
+    ob = np.zeros(nfp, dtype=np.int32)
+    lib(fma).orc_field_values(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen),
+                              _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody, _p(m.body_start, C.c_int), w1, h,
+                              len(beta), _p(beta), amp, xw[0], xw[1], rho, isol, _p(ph), nfp, _p(xf), kind, _p(out),
+                              _p(ob, C.c_int), nthreads)
+    return out, ob.astype(bool)

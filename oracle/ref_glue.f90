@@ -215,6 +215,24 @@ contains
       dij = d
    end subroutine
 
+   ! Newman integral of source panel jel (1-based) at an arbitrary field point
+   ! xf: SGLINTBD_MULTI_TRI2 / _QUAD2 with IRR = 1 (hull panels), rk = (SIJ, DIJ)
+   subroutine ref_sglint2(xf, jel, rk) bind(C, name='ref_sglint2')
+      real(c_double), intent(in) :: xf(3)
+      integer(c_int), value :: jel
+      real(c_double), intent(out) :: rk(4)
+      real*8 :: x(3), r(4)
+      integer :: j
+      x = xf
+      j = jel
+      if (NCN_MULTI_COMB(j) .eq. 3) then
+         call SGLINTBD_MULTI_TRI2(x, j, r, 1)
+      else
+         call SGLINTBD_MULTI_QUAD2(x, j, r, 1)
+      end if
+      rk = r
+   end subroutine
+
    subroutine ref_dinp(x, y, z, xw_, yw_, beta_, vv, w1_, wk_, hh, amp_, dpo) bind(C, name='ref_dinp')
       real(c_double), value :: x, y, z, xw_, yw_, beta_, vv, w1_, wk_, hh, amp_
       complex(c_double_complex), intent(out) :: dpo(3)

@@ -16,6 +16,13 @@ Outputs (small .npz files, committed):
   example5.npz    examples/example5_single_body (irregular-frequency removal):
                   hull and waterplane meshes, the deck values and the 18 tables
                   of Output/Hams_format
+  newman2.npz     SGLINTBD_MULTI_TRI2/QUAD2 (field-point variants) on body 1 of
+                  test/data/config1 at points on, near and away from the panels
+  example2.npz    examples/example2_multi_bodies_wave_fields (10 bodies, 100 field
+                  points): meshes, deck values, and the shipped Wamit_format
+                  PressureElevation.6p / PressureElevationIncidence.6p values
+
+`python make_golden.py NAME ...` writes only the named fixtures.
 """
 import ctypes as C
 import os
@@ -36,6 +43,80 @@ d = C.c_double
 
 def p(a, t=C.c_double):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def newman2():
+    """Field-point Newman integrals on body 1 of config1."""
+    r = oracle.ref()
+    rng = np.random.default_rng(2468)
+    mesh = read_pnl(os.path.join(REF_DATA, "input", "HullMesh_1.pnl"))
+    ne, nn = mesh.nelem, mesh.nnode
+    xyz = np.asfortranarray(mesh.xyz)
+    ncn = mesh.ncn.astype(np.int32)
+    ncon = np.asfortranarray(mesh.ncon.astype(np.int32))
+    cen = np.zeros((ne, 3), order="F")
+    area, pnsz = np.zeros(ne), np.zeros(ne)
+    dxyz = np.zeros((ne, 6), order="F")
+    r.ref_geometry(nn, p(xyz), ne, p(ncn, C.c_int), p(ncon, C.c_int), p(np.zeros(3)), p(cen), p(area), p(pnsz),
+                   p(dxyz))
+    r.ref_set_mesh(nn, p(xyz), ne, p(ncn, C.c_int), p(ncon, C.c_int), p(cen))
+    rows = []
+    for j in list(range(0, ne, 5)) + [e for e in range(ne) if mesh.ncn[e] == 3][:20]:
+        nrm = dxyz[j, :3]
+        for kind in range(5):
+            if kind == 0:
+                xf = cen[j].copy()                                   # the panel's own centroid
+            elif kind == 1:                                          # in the plane, off the centroid
+                t = np.cross(nrm, [0.3, 0.5, 0.8])
+                xf = cen[j] + 0.3 * pnsz[j] * t / np.linalg.norm(t)
+            elif kind == 2:                                          # just off the plane
+                xf = cen[j] + rng.choice([-1.0, 1.0]) * 2e-3 * pnsz[j] * nrm
+            elif kind == 3:                                          # near, in the fluid
+                xf = cen[j] + rng.uniform(-1.0, 1.0, 3) * pnsz[j]
+            else:                                                    # far
+                xf = cen[j] + rng.uniform(-30.0, 30.0, 3) * pnsz[j]
+            rk = np.zeros(4)
+            r.ref_sglint2(p(np.ascontiguousarray(xf)), j + 1, p(rk))
+            rows.append([j, *xf, *rk])
+    np.savez_compressed(os.path.join(HERE, "newman2.npz"), xyz=mesh.xyz, ncn=mesh.ncn, ncon=mesh.ncon,
+                        rows=np.array(rows))
+
+
+def _nums14(s):
+    return [float(s[i:i + 14]) for i in range(0, len(s), 14) if s[i:i + 14].strip()]
+
+
+def example2():
+    """Field-point elevations of examples/example2_multi_bodies_wave_fields:
+    per period, the radiation block (NFP points x NBODY lines of 12 values,
+    OutputPressureElevation_RadiationMulti) then per heading the diffraction
+    lines (OutputPressureElevation_DiffractionMulti), and the incidence file
+    (OutputPressureElevation_IncidenceMulti: MOD, PHS, REL, IMG)."""
+    ex = "/root/reference/examples/example2_multi_bodies_wave_fields"
+    ctl = read_control(os.path.join(ex, "Input", "ControlFile.in"))
+    nb, nfp, nper, nbeta = ctl.nbody, len(ctl.xfp), len(ctl.wvnb), len(ctl.headings_deg)
+    bodies = [read_pnl(os.path.join(ex, "Input", f"HullMesh_{b + 1}.pnl"), ctl.lcs[b]) for b in range(nb)]
+    lines = open(os.path.join(ex, "Output", "Wamit_format", "PressureElevation.6p")).read().splitlines()
+    rad = np.zeros((nper, nfp, nb, 12))
+    dif = np.zeros((nper, nbeta, nfp, 2))
+    pos = 0
+    for k in range(nper):
+        for q in range(nfp):
+            for b in range(nb):
+                rad[k, q, b] = _nums14(lines[pos][24:])
+                pos += 1
+        for ib in range(nbeta):
+            for q in range(nfp):
+                t = lines[pos]
+                dif[k, ib, q] = [float(t[38:52]), float(t[52:66])]
+                pos += 1
+    assert pos == len(lines)
+    lines = open(os.path.join(ex, "Output", "Wamit_format", "PressureElevationIncidence.6p")).read().splitlines()
+    inc = np.array([_nums14(t[38:]) for t in lines]).reshape(nper, nbeta, nfp, 4)
+    np.savez_compressed(
+        os.path.join(HERE, "example2.npz"), depth=ctl.depth, periods=np.array(ctl.wvnb), inft=ctl.inft,
+        headings_deg=np.array(ctl.headings_deg), xr=ctl.xr, isol=ctl.isol, refl=ctl.refl, xfp=ctl.xfp,
+        xyz=np.stack([m.xyz for m in bodies]), ncn=bodies[0].ncn, ncon=bodies[0].ncon, rad=rad, dif=dif, inc=inc)
 
 
 def main():
@@ -199,8 +280,14 @@ def main():
         os.path.join(HERE, "example5.npz"), depth=70.0, omegas=0.05 + 0.01 * np.arange(10),
         headings_deg=10.0 * np.arange(36), xr=np.zeros(3), isol=1, hull_xyz=hull.xyz, hull_ncn=hull.ncn,
         hull_ncon=hull.ncon, lid_xyz=lid.xyz, lid_ncn=lid.ncn, lid_ncon=lid.ncon, **tabs5)
+    newman2()
+    example2()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            {"newman2": newman2, "example2": example2}[name]()
+    else:
+        main()

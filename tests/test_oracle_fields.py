@@ -1,0 +1,197 @@
+"""Pins the oracle's field-point restatement (CalPotentialMulti /
+CalElevationMulti, src/PressureElevationMulti.f90:55-348):
+
+* the field-point Newman integrals against the reference's own
+  SGLINTBD_MULTI_TRI2/QUAD2 (tests/golden/newman2.npz);
+* the free-term test against closed-form geometry;
+* the boundary-integral identity: at a panel centroid (free term 2 pi) the
+  field formula returns the solved potential itself, and just off the panel
+  (4 pi) it tends to it;
+* end to end against the reference's shipped example2 outputs: 100
+  free-surface points around 10 bodies, elevation per radiation mode and per
+  heading (Wamit_format/PressureElevation.6p, 7 printed digits) and the
+  incident elevation (PressureElevationIncidence.6p).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from hamsx.deck import PanelMesh
+from hamsx.mesh import cylinder
+from hamsx.wamit import nondim_field
+from test_oracle_config1 import combined_with_oracle_geometry
+
+o = oracle.lib()
+
+
+def p(a, t=C.c_double):
+    return C.cast(a.ctypes.data, C.POINTER(t))
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def test_field_point_newman_matches_reference():
+    """SIJ and DIJ at arbitrary field points (on the panel, in its plane,
+    2e-3 PNSZ off it, near and far) equal the reference's *2 variants."""
+    z = load("newman2.npz")
+    xyz, ncn, ncon = z["xyz"], z["ncn"], z["ncon"]
+    for row in z["rows"]:
+        j = int(row[0])
+        xf, ref = row[1:4].copy(), row[4:8]
+        xv = np.zeros(12)
+        for c in range(ncn[j]):
+            xv[3 * c:3 * c + 3] = xyz[ncon[j, c] - 1]
+        s, dv = C.c_double(), np.zeros(3)
+        o.orc_sglint(int(ncn[j]), p(xv), p(xf), C.byref(s), p(dv))
+        got = np.array([s.value, *dv])
+        assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref))), (j, xf)
+
+
+def _square_mesh():
+    """One unit square panel in the plane z = -1 (normal +z as ordered)."""
+    xyz = np.asfortranarray([[0.0, 0.0, -1.0], [1.0, 0.0, -1.0], [1.0, 1.0, -1.0], [0.0, 1.0, -1.0]])
+    return PanelMesh(xyz, np.array([4], dtype=np.int32), np.asfortranarray([[1, 2, 3, 4]], dtype=np.int32))
+
+
+@pytest.mark.parametrize("xp,expect", [
+    ((0.5, 0.5, -1.0), 1),          # centroid
+    ((0.05, 0.9, -1.0), 1),         # inside, near a corner
+    ((1.0, 0.5, -1.0), 1),          # on an edge
+    ((1.2, 0.5, -1.0), 0),          # in the plane, outside the polygon
+    ((0.5, 0.5, -1.0005), 1),       # within TOLP = 1e-3 PNSZ of the plane
+    ((0.5, 0.5, -1.01), 0),         # beyond it
+])
+def test_on_body_test_geometry(xp, expect):
+    m = combined_with_oracle_geometry([_square_mesh()], [np.zeros(3)])
+    x = np.array(xp, dtype=np.float64)
+    got = o.orc_on_body(p(x), m.nelem, m.nnode, p(m.xyz), p(m.ncn, C.c_int), p(m.ncon, C.c_int), p(m.pnsz),
+                        p(m.dxyz))
+    assert got == expect
+
+
+@pytest.fixture(scope="module")
+def pair():
+    bodies = [cylinder(1.0, 2.0, 12, 4, 3), cylinder(1.0, 2.0, 12, 4, 3, x0=4.0)]
+    return combined_with_oracle_geometry(bodies, [np.zeros(3), np.array([4.0, 0.0, 0.0])])
+
+
+@pytest.mark.parametrize("depth,w", [(-1.0, 0.8), (-1.0, 1.7), (6.0, 0.8), (6.0, 1.7)])
+def test_field_formula_reproduces_the_solution_on_the_body(pair, depth, w):
+    """Row i of the solved system is 2 pi phi_i + sum_j T_ij phi_j = B_i, so
+    the field formula at centroid i (on the body: SLD = 2 pi) must give phi_i
+    back: TERM1 (the body's normals, FACTOR_NORMAL, or -DINCP), TERM2 and SLD
+    all enter.  With ISOL = 1 the diffraction columns add VINP."""
+    m = pair
+    heads = np.deg2rad([0.0, 60.0])
+    phi, info = oracle.solve_potentials(m, w, depth, heads)
+    assert info == 0
+    xfp = np.asarray(m.cen)
+    v, wk, _ = oracle.wave(w, depth)
+    for isol in (1, 2):
+        val, ob = oracle.field_values(m, phi, xfp, w, depth, heads, kind=0, isol=isol)
+        assert ob.all()
+        ref = phi.copy()
+        if isol == 1:
+            for b, be in enumerate(heads):
+                for i in range(m.nelem):
+                    f = np.zeros(2)
+                    o.orc_vinp(xfp[i, 0], xfp[i, 1], xfp[i, 2], 0.0, 0.0, be, v, w, wk, depth, 1.0, p(f))
+                    ref[i, 12 + b] += complex(f[0], f[1])
+        err = np.max(np.abs(val - ref)) / np.max(np.abs(ref))
+        assert err < 1e-12, (isol, err)
+
+
+@pytest.mark.parametrize("depth,w", [(-1.0, 0.8), (-1.0, 1.7)])
+def test_field_formula_is_continuous_off_the_body(pair, depth, w):
+    """Moved off the centroids along the fluid-side normal by eps PNSZ (free
+    term 4 pi), the values tend to the solution as eps -> 0: the panel's own
+    solid angle supplies the other 2 pi.  Deep water only: the reference's
+    FINGREEN3D is not continuous at RR -> 0 (at h = 6, w = 1.7, z = -1.75 its
+    FLAG = 1 value is 0.576 + 0.641i at RR <= 1e-8 and -0.167 + 0.641i from
+    RR = 1e-6 on), so in finite depth the on-panel value and the limit from
+    the fluid differ by that jump times the panel area."""
+    m = pair
+    heads = np.deg2rad([0.0, 60.0])
+    phi, _ = oracle.solve_potentials(m, w, depth, heads)
+    xfp = np.asarray(m.cen)
+    scale = np.max(np.abs(phi), axis=0)
+    scale = np.maximum(scale, np.max(scale[:12]))   # radiation modes: one scale (yaw is ~0 on a cylinder)
+    errs = []
+    for eps in (1e-2, 2e-3):
+        xo = xfp - eps * m.pnsz[:, None] * np.asarray(m.dxyz)[:, :3]
+        val, ob = oracle.field_values(m, phi, xo, w, depth, heads, kind=0, isol=2)
+        assert not ob.any()
+        errs.append(float(np.max(np.abs(val - phi) / scale)))
+    assert errs[1] < 0.5 * errs[0] and errs[1] < 1e-2, errs
+
+
+@pytest.fixture(scope="module")
+def example2():
+    z = load("example2.npz")
+    bodies = [PanelMesh(np.asfortranarray(z["xyz"][b]), z["ncn"], np.asfortranarray(z["ncon"]))
+              for b in range(len(z["xr"]))]
+    m = combined_with_oracle_geometry(bodies, z["xr"])
+    return z, m
+
+
+def rad_table(vals, nb, nfp):
+    out = np.zeros((nfp, nb, 12))
+    for b in range(nb):
+        for md in range(6):
+            out[:, b, 2 * md] = vals[:, 6 * b + md].real
+            out[:, b, 2 * md + 1] = vals[:, 6 * b + md].imag
+    return out
+
+
+def check_example2(z, k, vals, inc, rad_tol=5e-5, dif_tol=1e-6):
+    """vals: AUTO-kind values (here elevations: every point is on z = 0) for
+    period k; inc: the incident elevations (nfp x nbeta) or None."""
+    nb, nfp = len(z["xr"]), len(z["xfp"])
+    w = 2 * np.pi / z["periods"][k]
+    nv = nondim_field(vals, z["xfp"][:, 2], nb, w, refl=float(z["refl"]))
+    got = rad_table(nv, nb, nfp)
+    ref = z["rad"][k]
+    for b in range(nb):
+        e = np.linalg.norm(got[:, b] - ref[:, b]) / np.linalg.norm(ref[:, b])
+        assert e <= rad_tol, ("radiation", k, b, e)
+    gd = np.stack([nv[:, 6 * nb:].real, nv[:, 6 * nb:].imag], axis=-1).transpose(1, 0, 2)
+    e = np.linalg.norm(gd - z["dif"][k]) / np.linalg.norm(z["dif"][k])
+    assert e <= dif_tol, ("diffraction", k, e)
+    if inc is not None:
+        iv = nondim_field(np.concatenate([np.zeros((nfp, 6 * nb)), inc], axis=1), z["xfp"][:, 2], nb, w)[:, 6 * nb:]
+        ref = z["inc"][k]            # MOD, PHS, REL, IMG per (heading, point)
+        gi = np.stack([iv.real, iv.imag], axis=-1).transpose(1, 0, 2)
+        assert np.max(np.abs(gi - ref[..., 2:4])) <= 1e-6
+        assert np.max(np.abs(np.abs(iv.T) - ref[..., 0])) <= 1e-6
+
+
+def test_oracle_reproduces_example2_field_elevations(example2):
+    """Period 1 s (the 0.5 s period runs on the GPU path): the oracle's
+    solve and field formula reproduce the shipped free-surface elevations.
+    Radiation per body <= 5e-5 and diffraction <= 1e-6 (Frobenius,
+    relative): the shipped values were written by the reference's default
+    build (FP32 storage of the Green-function values), which moves the small
+    radiated elevations (|value| <= 0.14) by about 1e-5 relative; the
+    diffraction elevations agree to the printed digits."""
+    z, m = example2
+    k = 1
+    w = 2 * np.pi / z["periods"][k]
+    heads = np.deg2rad(z["headings_deg"])
+    phi, info = oracle.solve_potentials(m, w, float(z["depth"]), heads, nthreads=8)
+    assert info == 0
+    vals, ob = oracle.field_values(m, phi, z["xfp"], w, float(z["depth"]), heads, kind=2, isol=int(z["isol"]))
+    assert not ob.any()
+    v, wk, _ = oracle.wave(w, float(z["depth"]))
+    inc = np.zeros((len(z["xfp"]), len(heads)), dtype=np.complex128)
+    for b, be in enumerate(heads):
+        for q, x in enumerate(z["xfp"]):
+            f = np.zeros(2)
+            o.orc_vinp(x[0], x[1], x[2], 0.0, 0.0, be, v, w, wk, float(z["depth"]), 1.0, p(f))
+            inc[q, b] = 1j * w / oracle.G * complex(f[0], f[1])
+    check_example2(z, k, vals, inc)
