@@ -121,7 +121,8 @@ def parse():
                     "stream each)")
     ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
     ap.add_argument("--config", default="two-cylinders",
-                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving", "large-array"])
+                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving", "large-array",
+                             "wave-field"])
     ap.add_argument("--requests", type=int, default=40, help="serving: requests in the Poisson stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrival rate, requests/s")
     ap.add_argument("--seed", type=int, default=7, help="serving: seed of the request stream")
@@ -304,6 +305,90 @@ def large_array(args):
     print(json.dumps(out), flush=True)
 
 
+def wave_field_points():
+    """A 64 x 64 free-surface grid on [-120, 120]^2 m around the two
+    cylinders (points inside a hull waterline dropped) and a vertical line of
+    32 pressure points between them, z in [-45, -1] m."""
+    g = np.linspace(-120.0, 120.0, 64)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    fs = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)])
+    keep = (np.hypot(fs[:, 0] + 20.0, fs[:, 1]) > 5.5) & (np.hypot(fs[:, 0] - 20.0, fs[:, 1]) > 5.5)
+    line = np.column_stack([np.zeros(32), np.full(32, 3.0), np.linspace(-45.0, -1.0, 32)])
+    return np.vstack([fs[keep], line])
+
+
+def wave_field(args):
+    """Two cylinders (the default workload) with wave-field output: each step
+    is one frequency solve followed by the field values of every mode at
+    every point (pressure below the free surface, elevation on it), copied to
+    the host, on one context."""
+    from hamsx import configs
+    from hamsx.mesh import combine
+    from hamsx.solver import FrequencySolver
+    case = configs.two_cylinders()
+    mesh = combine(case.bodies, case.xr)
+    pts = wave_field_points()
+    heads = np.deg2rad(case.headings_deg)
+    omegas = [float(w) for w in case.omegas]
+    steps = args.steps
+    print(f"[bench] {case.name} + wave field: N={mesh.nelem} panels, {len(pts)} field points, {steps} frequencies",
+          file=sys.stderr, flush=True)
+    with FrequencySolver(mesh) as s:
+        s.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        s.set_field_points(pts)
+        for k in range(max(1, args.warmup)):
+            s.solve_frequency(omegas[k % len(omegas)], case.depth)
+            s.field_values("auto")
+        s.set_profiling(True)
+        s.stage_times(reset=True)
+        lat = []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            t1 = time.perf_counter()
+            _, _, _, info = s.solve_frequency(omegas[k % len(omegas)], case.depth)
+            vals, onbody = s.field_values("auto")
+            lat.append(time.perf_counter() - t1)
+            if info != 0:
+                raise SystemExit(f"zero pivot in solve {k}: INFO={info}")
+        elapsed = time.perf_counter() - t0
+        stages = s.stage_times(reset=True)
+    ms = {k: v[0] / max(1, v[1]) for k, v in stages.items()}
+    lat_ms = np.array(lat) * 1e3
+    pairs = len(pts) * mesh.nelem
+    out = {
+        "metric": METRIC, "value": steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic seeded cylinder meshes and field grid (no reference files read)",
+        "config": {"workload": case.name + "+wave-field-4k", "panels": mesh.nelem, "field_points": int(len(pts)),
+                   "depth_m": case.depth, "frequencies": len(omegas), "headings_deg": case.headings_deg,
+                   "in_flight": 1},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "stage_ms_per_solve": ms,
+        "field": {"ms_per_solve": ms["field"], "pairs": pairs,
+                  "pairs_per_s": pairs / (ms["field"] * 1e-3) if ms["field"] > 0 else None},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        base = cpu_baseline(case, mesh, args.cpu_rows, threads)
+        # the oracle's CalPotentialMulti on a strided sample of the points (any potentials will do:
+        # the cost is the pair loop), scaled to all points
+        w = float(case.omegas[len(case.omegas) // 2])
+        sample = pts[np.linspace(0, len(pts) - 1, 128).astype(int)]
+        phi = np.ones((mesh.nelem, 6 * mesh.nbody + len(heads)), dtype=np.complex128, order="F")
+        t0 = time.perf_counter()
+        oracle.field_values(mesh, phi, sample, w, case.depth, heads, kind=2, nthreads=threads)
+        t_field = (time.perf_counter() - t0) * len(pts) / len(sample)
+        per = base["seconds_per_solve"] + t_field
+        base.update({"value": 1.0 / per, "seconds_per_solve": per,
+                     "sample": base["sample"] + f"; + oracle field values of 128 of {len(pts)} points, scaled = "
+                                                f"{t_field:.1f} s"})
+        out["cpu_baseline"] = base
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -318,6 +403,9 @@ def main():
         return
     if args.config == "large-array":
         large_array(args)
+        return
+    if args.config == "wave-field":
+        wave_field(args)
         return
     from hamsx.mesh import combine
     from hamsx.solver import FrequencySolver

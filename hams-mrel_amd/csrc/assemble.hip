@@ -519,6 +519,151 @@ __global__ void k_field_finish(int nfp, const double* __restrict__ px, const dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// Motions (SolveMotionMulti, SolveMotionMulti.f90:46-150): one workgroup per
+// heading solves
+//   [-w^2 (M + A) + i w (B + B_lin + V) + C + K] x = -Im(F) - i Re(F)
+// by partial-pivoting elimination (ZGESV: the pivot is the first maximum of
+// |re| + |im|), with V = 0 first and then, when the quadratic damping is not
+// negligible, V(i,j) = B_quad(i,j) w |x_j| until sum_j |dx_j| / |x_j| <= 1e-6.
+// coef: the force reductions (amas, bdmp: nd x nd; exfc: nd x nbeta complex);
+// bm: the block-diagonal body matrices MATX, BLNR, BQDR, CRS, KSTF (nd x nd
+// each); work: nd x nd complex per heading.  iters[b] = the steps taken, -1
+// when kMotionMaxIt did not converge.
+// ---------------------------------------------------------------------------
+constexpr int kMotionMaxDof = 192, kMotionMaxIt = 1000;
+
+__device__ __forceinline__ double cabs_d(double2 v) { return hypot(v.x, v.y); }
+__device__ __forceinline__ double2 cdiv_d(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double2 cmul_d(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void k_motion(int nd, double w1, const double* __restrict__ coef,
+                                                const double* __restrict__ bm, int quad, double2* __restrict__ work,
+                                                double2* __restrict__ dspl, int* __restrict__ iters) {
+  __shared__ double2 x[kMotionMaxDof], r[kMotionMaxDof];
+  __shared__ double red[256];
+  __shared__ int redi[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const size_t nn = (size_t)nd * nd;
+  double2* L = work + (size_t)b * nn;
+  const double *amas = coef, *bdmp = coef + nn;
+  const double2* exfc = reinterpret_cast<const double2*>(coef + 2 * nn);
+  const double *mass = bm, *blnr = bm + nn, *bqdr = bm + 2 * nn, *crs = bm + 3 * nn, *kstf = bm + 4 * nn;
+  int steps = 0;
+  for (int pass = 0;; pass++) {
+    for (size_t o = t; o < nn; o += 256) {
+      const int j = (int)(o / nd);
+      double damp = bdmp[o] + blnr[o];
+      if (pass > 0) damp = damp + bqdr[o] * w1 * cabs_d(x[j]);
+      const double re = -(w1 * w1 * (mass[o] + amas[o])) + crs[o] + kstf[o];
+      L[o] = make_double2(re, w1 * damp);
+    }
+    if (t < nd) {
+      const double2 e = exfc[(size_t)b * nd + t];
+      r[t] = make_double2(-e.y, -e.x);
+    }
+    __syncthreads();
+    // factorisation with row interchanges applied to r as well
+    for (int k = 0; k < nd; k++) {
+      double v = -1.0;
+      int idx = nd;
+      if (k + t < nd) {
+        const double2 c = L[(size_t)k * nd + k + t];
+        v = fabs(c.x) + fabs(c.y);
+        idx = k + t;
+      }
+      red[t] = v;
+      redi[t] = idx;
+      __syncthreads();
+      for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (t < s2) {
+          const double a = red[t], c = red[t + s2];
+          if (c > a || (c == a && redi[t + s2] < redi[t])) {
+            red[t] = c;
+            redi[t] = redi[t + s2];
+          }
+        }
+        __syncthreads();
+      }
+      const int p = redi[0];
+      const bool zero = red[0] == 0.0;
+      __syncthreads();
+      if (p != k) {
+        for (int c = t; c < nd; c += 256) {
+          const double2 u = L[(size_t)c * nd + k];
+          L[(size_t)c * nd + k] = L[(size_t)c * nd + p];
+          L[(size_t)c * nd + p] = u;
+        }
+        if (t == 0) {
+          const double2 u = r[k];
+          r[k] = r[p];
+          r[p] = u;
+        }
+      }
+      __syncthreads();
+      if (zero) continue;  // singular column: left as it is (ZGETRF reports INFO and goes on)
+      const double2 rp = cdiv_d(make_double2(1.0, 0.0), L[(size_t)k * nd + k]);
+      for (int q = k + 1 + t; q < nd; q += 256) L[(size_t)k * nd + q] = cmul_d(L[(size_t)k * nd + q], rp);
+      __syncthreads();
+      const int m = nd - k - 1;
+      for (int o = t; o < m * m; o += 256) {
+        const int q = k + 1 + o % m, c = k + 1 + o / m;
+        const double2 l = L[(size_t)k * nd + q], u = L[(size_t)c * nd + k], a = L[(size_t)c * nd + q];
+        const double2 lu = cmul_d(l, u);
+        L[(size_t)c * nd + q] = make_double2(a.x - lu.x, a.y - lu.y);
+      }
+      __syncthreads();
+    }
+    for (int k = 0; k < nd; k++) {  // L y = P r (unit lower)
+      if (t > k && t < nd) {
+        const double2 lu = cmul_d(L[(size_t)k * nd + t], r[k]);
+        r[t] = make_double2(r[t].x - lu.x, r[t].y - lu.y);
+      }
+      __syncthreads();
+    }
+    for (int k = nd - 1; k >= 0; k--) {  // U x = y
+      if (t == 0) r[k] = cdiv_d(r[k], L[(size_t)k * nd + k]);
+      __syncthreads();
+      if (t < k) {
+        const double2 lu = cmul_d(L[(size_t)k * nd + t], r[k]);
+        r[t] = make_double2(r[t].x - lu.x, r[t].y - lu.y);
+      }
+      __syncthreads();
+    }
+    if (pass == 0) {
+      if (t < nd) x[t] = r[t];
+      __syncthreads();
+      if (!quad) break;
+      continue;
+    }
+    if (t == 0) {
+      double rerr = 0.0;
+      for (int j = 0; j < nd; j++) {
+        const double a = cabs_d(r[j]);
+        if (a > 0.0) rerr += cabs_d(make_double2(r[j].x - x[j].x, r[j].y - x[j].y)) / a;
+      }
+      red[0] = rerr;
+    }
+    __syncthreads();
+    const double rerr = red[0];
+    if (t < nd) x[t] = r[t];
+    steps = pass;
+    __syncthreads();
+    if (rerr <= 1.0e-6) break;
+    if (pass >= kMotionMaxIt) {
+      steps = -1;
+      break;
+    }
+  }
+  if (t < nd) dspl[(size_t)b * nd + t] = x[t];
+  if (t == 0) iters[b] = steps;
+}
+
 __global__ void k_clear_flag(int* flags) { flags[0] = 0; }
 
 namespace launch {
@@ -618,6 +763,11 @@ void field_finish(hipStream_t s, int nfp, const double* px, const double* py, co
                   double2* inc) {
   HX_LAUNCH(k_field_finish, dim3((nfp + 255) / 256, nd + w.nbeta), dim3(256), 0, s, nfp, px, py, pz, onb, f, w, nd,
             kind, P, ld, out, inc);
+}
+
+void motions(hipStream_t s, int nd, int nbeta, double w1, const double* coef, const double* bm, int quad,
+             double2* work, double2* dspl, int* iters) {
+  if (nbeta > 0) HX_LAUNCH(k_motion, dim3(nbeta), dim3(256), 0, s, nd, w1, coef, bm, quad, work, dspl, iters);
 }
 
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,

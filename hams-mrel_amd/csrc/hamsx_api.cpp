@@ -286,6 +286,12 @@ struct hamsx_ctx {
   PairBins fbins;
   DevBuf<double2> fM, fZ, fP, fOut, fInc, fDincp;
   DevBuf<int> fOnb;
+  // motion equation: the block-diagonal body matrices MATX, BLNR, BQDR, CRS,
+  // KSTF (6nb x 6nb each) for bodymat_nbody bodies; motion_quad: max |BQDR| >= 1e-6
+  DevBuf<double> bmat;
+  int bodymat_nbody = 0, motion_quad = 0;
+  DevBuf<double2> mwork, mdspl;
+  DevBuf<int> miters;
   DevBuf<double> modetab;  // per-frequency finite-depth mode tables
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
@@ -1373,6 +1379,7 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   if ((rc = c->vtx.ensure(n * 12))) return rc;
   HX_TRY(hipMemcpy(c->vtx.p, vtx.data(), sizeof(double) * n * 12, hipMemcpyHostToDevice));
   c->nfp = 0;
+  c->bodymat_nbody = 0;
   c->field_gen++;
   c->nlid = 0;
   c->nf = nelem;
@@ -1604,6 +1611,7 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
       (finite && (rc = c->fmodetab.ensure(hamsx::mode_tab_len(n + F)))) || (rc = blas_handle(c)))
     return rc;
   const hamsx::MeshDev m = c->mesh();
+  StageTimer tm(c, 9);
   hamsx::launch::dincp(c->stream, m, c->freq, c->waves(), c->fDincp.p);
   hamsx::launch::field_weights(c->stream, m, c->nbody, nbeta, c->fDincp.p, c->X.p, c->fZ.p);
   if ((rc = check_launch())) return rc;
@@ -1658,6 +1666,65 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
                           c->stream));
   if (onbody) HX_TRY(hipMemcpyAsync(onbody, c->fOnb.p, sizeof(int) * nfp, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int hamsx_set_body_matrices(hamsx_ctx* c, const double* mass, const double* blnr, const double* bqdr,
+                            const double* crs, const double* kstf) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_body_matrices needs hamsx_set_mesh first");
+  if (!mass || !blnr || !bqdr || !crs || !kstf) return fail(HAMSX_EARG, "body matrices must not be NULL");
+  const int nb = c->nbody, nd = 6 * nb;
+  if (nd > 192) return fail(HAMSX_EARG, "the motion solve supports at most 32 bodies");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  const size_t nn = (size_t)nd * nd;
+  std::vector<double> h(5 * nn, 0.0);
+  const double* src[5] = {mass, blnr, bqdr, crs, kstf};
+  double qmax = 0.0;
+  // LOCALTOGLOBAL (SolveMotionMulti.f90:24-40): body b's 6 x 6 block on the diagonal
+  for (int q = 0; q < 5; q++)
+    for (int b = 0; b < nb; b++)
+      for (int j = 0; j < 6; j++)
+        for (int i = 0; i < 6; i++) {
+          const double v = src[q][36 * (size_t)b + i + 6 * j];
+          if (!std::isfinite(v)) return fail(HAMSX_EARG, "body matrix entry is not finite");
+          h[q * nn + (size_t)(6 * b + i) + (size_t)nd * (6 * b + j)] = v;
+          if (q == 2) qmax = std::max(qmax, std::fabs(v));
+        }
+  if ((rc = c->bmat.ensure(5 * nn))) return rc;
+  HX_TRY(hipMemcpy(c->bmat.p, h.data(), sizeof(double) * 5 * nn, hipMemcpyHostToDevice));
+  c->bodymat_nbody = nb;
+  c->motion_quad = qmax >= 1.0e-6 ? 1 : 0;  // DLANGE('M', BQDR) >= 1e-6 (:80, :89)
+  return 0;
+}
+
+int hamsx_motions(hamsx_ctx* c, double* dspl, int* iters) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (c->bodymat_nbody == 0 || c->bodymat_nbody != c->nbody)
+    return fail(HAMSX_ESTATE, "hamsx_motions needs hamsx_set_body_matrices for the current mesh");
+  int rc = set_device(c);
+  if (rc || (rc = do_solve(c))) return rc;
+  const int nd = 6 * c->nbody, nbeta = c->nbeta;
+  if (nbeta == 0) return 0;
+  if ((rc = c->mwork.ensure((size_t)nd * nd * nbeta)) || (rc = c->mdspl.ensure((size_t)nd * nbeta)) ||
+      (rc = c->miters.ensure(nbeta)))
+    return rc;
+  hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p, c->out.p);
+  hamsx::launch::motions(c->stream, nd, nbeta, c->freq.w1, c->out.p, c->bmat.p, c->motion_quad, c->mwork.p,
+                         c->mdspl.p, c->miters.p);
+  if ((rc = check_launch())) return rc;
+  std::vector<int> it(nbeta);
+  if (dspl)
+    HX_TRY(hipMemcpyAsync(dspl, c->mdspl.p, sizeof(double2) * (size_t)nd * nbeta, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(it.data(), c->miters.p, sizeof(int) * nbeta, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipStreamSynchronize(c->stream));
+  if (iters) std::copy(it.begin(), it.end(), iters);
+  for (int b = 0; b < nbeta; b++)
+    if (it[b] < 0) return fail(HAMSX_ENOCONV, "quadratic-damping iteration of heading %d did not converge", b + 1);
   return 0;
 }
 

@@ -146,6 +146,11 @@ void on_body(hipStream_t s, const MeshDev& m, const double* vtx, int nfp, const 
 void field_finish(hipStream_t s, int nfp, const double* px, const double* py, const double* pz, const int* onb,
                   const FreqDev& f, const WaveDev& w, int nd, int kind, const double2* P, size_t ld, double2* out,
                   double2* inc);
+// Motion amplitudes of every heading (SolveMotionMulti): coef = the force
+// reductions [amas | bdmp | exfc], bm = the five block-diagonal body
+// matrices, nd = 6 nbody <= 192; work nbeta x nd x nd.
+void motions(hipStream_t s, int nd, int nbeta, double w1, const double* coef, const double* bm, int quad,
+             double2* work, double2* dspl, int* iters);
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 // C(r, c) = C(c, r) above the diagonal (n x n): completes a lower-triangle syrk
 void mirror_lower(hipStream_t s, double2* C, int n);

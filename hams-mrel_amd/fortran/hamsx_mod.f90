@@ -37,7 +37,8 @@ module hamsx_mod
             hamsx_query, hamsx_collect, hamsx_get_matrix, hamsx_refine_stats, hamsx_set_profiling, &
             hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, hamsx_set_hmatrix, &
             hamsx_hmatrix_stats, hamsx_hmatrix_matvec, hamsx_hmatrix_partition, &
-            hamsx_stage_times, hamsx_set_field_points, hamsx_field_values
+            hamsx_stage_times, hamsx_set_field_points, hamsx_field_values, hamsx_set_body_matrices, &
+            hamsx_motions
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
             PANEL_GEOMETRY_GPU, hamsx_error_message
@@ -190,6 +191,20 @@ module hamsx_mod
       type(c_ptr), value :: values, incident, onbody
       integer(c_int) :: rc
     end function
+    ! each argument: NBODY blocks of 6 x 6, e.g. MATX_MULTI(B,:,:) copied to MASS(:,:,B)
+    function hamsx_set_body_matrices(ctx, mass, blnr, bqdr, crs, kstf) bind(C, name="hamsx_set_body_matrices") &
+        result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: mass(*), blnr(*), bqdr(*), crs(*), kstf(*)
+      integer(c_int) :: rc
+    end function
+    ! dspl(6*NBODY, NBETA) complex; iters(NBETA) may be c_null_ptr
+    function hamsx_motions(ctx, dspl, iters) bind(C, name="hamsx_motions") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, dspl, iters
+      integer(c_int) :: rc
+    end function
     function hamsx_set_solver(ctx, solver, tol, restart, maxiter) bind(C, name="hamsx_set_solver") result(rc)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -254,8 +269,8 @@ module hamsx_mod
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: reset
-      real(c_double), intent(out) :: ms(9)
-      integer(c_int), intent(out) :: count(9)
+      real(c_double), intent(out) :: ms(10)
+      integer(c_int), intent(out) :: count(10)
       integer(c_int) :: rc
     end function
     function c_strlen(s) bind(C, name="strlen") result(n)

@@ -63,10 +63,13 @@ SIGNATURES = {
     "hamsx_refine_stats": (C.c_int, [_vp, _ip, _dp]),
     "hamsx_set_field_points": (C.c_int, [_vp, C.c_int, _dp]),
     "hamsx_field_values": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip]),
+    "hamsx_set_body_matrices": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "hamsx_motions": (C.c_int, [_vp, _dp, _ip]),
     "hamsx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "hamsx_stage_times": (C.c_int, [_vp, C.c_int, _dp, _ip]),
 }
-STAGES = ("assemble", "rhs", "factor", "solve", "forces", "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
+STAGES = ("assemble", "rhs", "factor", "solve", "forces", "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen",
+          "field")
 
 _lib = None
 

@@ -149,6 +149,18 @@ class PanelMesh:
         return len(self.xyz)
 
 
+def place(mesh: PanelMesh, lcs) -> PanelMesh:
+    """The LCS transform of ReadBodyMeshMulti (``src/ReadPanelMeshMulti.f90:57-59``)
+    applied to a mesh read in local coordinates: rotation about z by lcs[3],
+    then translation by (lcs[0], lcs[1]); z is kept."""
+    c, s = math.cos(lcs[3]), math.sin(lcs[3])
+    xyz = np.array(mesh.xyz, dtype=np.float64)
+    x, y = xyz[:, 0].copy(), xyz[:, 1].copy()
+    xyz[:, 0] = 0.0 + lcs[0] + c * x - s * y
+    xyz[:, 1] = 0.0 + lcs[1] + s * x + c * y
+    return PanelMesh(xyz, mesh.ncn.copy(), mesh.ncon.copy(), mesh.isx, mesh.isy)
+
+
 def read_pnl(path: str, lcs=(0.0, 0.0, 0.0, 0.0)) -> PanelMesh:
     """HullMesh .pnl reader with the LCS transform of ReadBodyMeshMulti
     (``src/ReadPanelMeshMulti.f90:57-59``): rotation about z by lcs[3] then

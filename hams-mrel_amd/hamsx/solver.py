@@ -115,6 +115,30 @@ class FrequencySolver:
             return vals, ob.astype(bool), inc[:, :self.nbeta]
         return vals, ob.astype(bool)
 
+    def set_body_matrices(self, mass, blnr, bqdr, crs, kstf):
+        """The motion equation's body matrices (ReadHydroStaticMulti): each
+        argument is a list of nbody 6 x 6 arrays (rows as in Hydrostatic_*.in)."""
+        def blocks(ms):
+            if len(ms) != self.mesh.nbody:
+                raise ValueError(f"need {self.mesh.nbody} 6 x 6 matrices")
+            return np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=np.float64).T.ravel() for m in ms]))
+        arrs = [blocks(m) for m in (mass, blnr, bqdr, crs, kstf)]
+        _lib.check(self._lib.hamsx_set_body_matrices(self._h, *[_lib.ptr(a) for a in arrs]), "hamsx_set_body_matrices")
+
+    def set_hydrostatics(self, hydro):
+        """set_body_matrices from hamsx.deck.Hydrostatic records, one per body."""
+        self.set_body_matrices([h.mass for h in hydro], [h.blnr for h in hydro], [h.bqdr for h in hydro],
+                               [h.crs for h in hydro], [h.kstf for h in hydro])
+
+    def motions(self):
+        """SolveMotionMulti for every heading of the current frequency:
+        (DSPL 6 nbody x nbeta complex, quadratic-damping steps per heading)."""
+        nd = 6 * self.mesh.nbody
+        d = np.zeros((nd, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        it = np.zeros(max(1, self.nbeta), dtype=np.int32)
+        _lib.check(self._lib.hamsx_motions(self._h, _lib.ptr(d), _lib.ptr(it, C.c_int)), "hamsx_motions")
+        return d[:, :self.nbeta], it[:self.nbeta]
+
     def set_solver(self, solver: str = "lu", tol: float = 1.0e-4, restart: int = 25, maxiter: int = 100):
         """'lu' (ZGETRF/ZGETRS) or 'gmres' (restarted GMRES with the per-body
         block-diagonal LU preconditioner; defaults are the reference's

@@ -237,15 +237,37 @@ int hamsx_set_field_points(hamsx_ctx *ctx, int nfp, const double *xfp);
  * the caller's writer.  NSYS = 1. */
 int hamsx_field_values(hamsx_ctx *ctx, int kind, double *values, double *incident, int *onbody);
 
+/* ---- motions ----------------------------------------------------------------
+ * hamsx_set_body_matrices: the body matrices ReadHydroStaticMulti reads
+ * (src/HydroStaticMulti.f90:68-98) -- mass MATX, external linear damping
+ * BLNR, external quadratic damping BQDR, hydrostatic restoring CRS and
+ * external restoring KSTF -- each as nbody blocks of 6 x 6 (body b's block at
+ * [36 b], column-major), for the bodies of the current mesh (6 nbody <= 192).
+ * A new hamsx_set_mesh drops them. */
+int hamsx_set_body_matrices(hamsx_ctx *ctx, const double *mass, const double *blnr, const double *bqdr,
+                            const double *crs, const double *kstf);
+
+/* SolveMotionMulti (src/SolveMotionMulti.f90:46-150) for every heading of the
+ * current frequency (solved here if it has not been):
+ *   [-w^2 (MATX + AMAS) + i w (BDMP + BLNR + VDMP) + CRS + KSTF] DSPL = -Im(EXFC) - i Re(EXFC),
+ * first with VDMP = 0, then -- when max |BQDR| >= 1e-6 -- with VDMP(i,j) =
+ * BQDR(i,j) w |DSPL(j)| until sum_j |dDSPL_j| / |DSPL_j| <= 1e-6.
+ * dspl[6nb*nbeta] complex (DOF fastest): DSPL before the WAMIT scaling of
+ * Motion.4; iters[nbeta] (may be NULL): the quadratic-damping steps (0 when
+ * none ran).  Returns HAMSX_ENOCONV, with the last iterates in dspl, when a
+ * heading needs more than 1000 steps (the reference loops without a bound). */
+int hamsx_motions(hamsx_ctx *ctx, double *dspl, int *iters);
+
 /* Measurement hook (no counterpart in the reference, which has no timers):
  * with profiling on, every stage is bracketed by HIP events on the context's
  * stream.  hamsx_stage_times returns the device milliseconds accumulated per
  * stage since the last reset -- ms[0] assembly, ms[1] right-hand sides,
  * ms[2] factorisation, ms[3] solves (with refinement), ms[4] forces; within
  * the finite-depth assembly, ms[5..8] the region kernels LINTON, PIDCOCK,
- * EIGENE and EIGEN (R/h < 5e-4, < 0.05, < 0.5, >= 0.5) -- and the number of
- * launches of each in count[HAMSX_NSTAGE]. */
-#define HAMSX_NSTAGE 9
+ * EIGENE and EIGEN (R/h < 5e-4, < 0.05, < 0.5, >= 0.5); ms[9] the field-point
+ * values (hamsx_field_values) -- and the number of launches of each in
+ * count[HAMSX_NSTAGE]. */
+#define HAMSX_NSTAGE 10
 int hamsx_set_profiling(hamsx_ctx *ctx, int on);
 int hamsx_stage_times(hamsx_ctx *ctx, int reset, double *ms, int *count);
 

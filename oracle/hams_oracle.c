@@ -1861,3 +1861,93 @@ void orc_field_values(int nelem, int nnode, const double *xyz, const int *ncn, c
     free(dincp);
     free(bodyof);
 }
+
+/* ========================================================================
+ * Motions: SolveMotionMulti (SolveMotionMulti.f90:46-150) for one heading.
+ * nd = 6 nbody; amas, bdmp (nd x nd, column-major), exfc (nd complex) of the
+ * frequency and heading; the body matrices MATX/BLNR/BQDR/CRS/KSTF as
+ * nbody 6 x 6 blocks (body b's at [36 b], column-major) on the block
+ * diagonal (LOCALTOGLOBAL, :24-40).  dspl (nd complex) out.  Returns the
+ * quadratic-damping iterations taken (0 when max |BQDR| < 1e-6), or -1 when
+ * 1000 did not reach RERR <= 1e-6 (the reference loops without a bound).
+ * ====================================================================== */
+/* ZGESV: ZGETRF's partial pivoting (first maximum of |re| + |im|, IZAMAX)
+ * and ZGETRS's two triangular solves, right-looking and unblocked. */
+static int zgesv_small(int n, cplx *A, cplx *b)
+{
+    int info = 0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = -1.0;
+        for (int r = k; r < n; r++) {
+            const double v = fabs(creal(A[r + (size_t)n * k])) + fabs(cimag(A[r + (size_t)n * k]));
+            if (v > best) { best = v; p = r; }
+        }
+        if (best == 0.0) { if (!info) info = k + 1; continue; }
+        if (p != k) {
+            for (int c = 0; c < n; c++) {
+                cplx t = A[k + (size_t)n * c]; A[k + (size_t)n * c] = A[p + (size_t)n * c]; A[p + (size_t)n * c] = t;
+            }
+            cplx t = b[k]; b[k] = b[p]; b[p] = t;
+        }
+        const cplx rp = 1.0 / A[k + (size_t)n * k];
+        for (int r = k + 1; r < n; r++) A[r + (size_t)n * k] = A[r + (size_t)n * k] * rp;
+        for (int c = k + 1; c < n; c++) {
+            const cplx u = A[k + (size_t)n * c];
+            for (int r = k + 1; r < n; r++) A[r + (size_t)n * c] = A[r + (size_t)n * c] - A[r + (size_t)n * k] * u;
+        }
+    }
+    for (int k = 0; k < n; k++)                       /* L y = P b (unit lower) */
+        for (int r = k + 1; r < n; r++) b[r] = b[r] - A[r + (size_t)n * k] * b[k];
+    for (int k = n - 1; k >= 0; k--) {                /* U x = y */
+        b[k] = b[k] / A[k + (size_t)n * k];
+        for (int r = 0; r < k; r++) b[r] = b[r] - A[r + (size_t)n * k] * b[k];
+    }
+    return info;
+}
+
+static double block_diag(const double *blk, int nd, int i, int j)
+{
+    return (i / 6 == j / 6) ? blk[36 * (i / 6) + (i % 6) + 6 * (j % 6)] : 0.0;
+}
+
+int orc_motion(int nbody, double w1, const double *amas, const double *bdmp, const double *exfc,
+               const double *mass, const double *blnr, const double *bqdr, const double *crs,
+               const double *kstf, double *dspl)
+{
+    const int nd = 6 * nbody;
+    cplx *L = (cplx *)malloc(sizeof(cplx) * (size_t)nd * nd);
+    cplx *x = (cplx *)malloc(sizeof(cplx) * nd), *x1 = (cplx *)malloc(sizeof(cplx) * nd);
+    double norm = 0.0;                                 /* DLANGE('M', BQDR_FINAL) */
+    for (int i = 0; i < nd; i++)
+        for (int j = 0; j < nd; j++) norm = fmax(norm, fabs(block_diag(bqdr, nd, i, j)));
+    int iters = 0;
+    for (int pass = 0;; pass++) {
+        for (int j = 0; j < nd; j++)
+            for (int i = 0; i < nd; i++) {
+                const size_t o = i + (size_t)nd * j;
+                double damp = bdmp[o] + block_diag(blnr, nd, i, j);
+                if (pass > 0) damp = damp + block_diag(bqdr, nd, i, j) * w1 * cabs(x[j]);   /* VDMP */
+                const double re = -(w1 * w1 * (block_diag(mass, nd, i, j) + amas[o])) + block_diag(crs, nd, i, j)
+                                  + block_diag(kstf, nd, i, j);
+                L[o] = re + I * (w1 * damp);
+            }
+        cplx *rhs = pass == 0 ? x : x1;
+        for (int i = 0; i < nd; i++) rhs[i] = -exfc[2 * i + 1] - I * exfc[2 * i];
+        zgesv_small(nd, L, rhs);
+        if (pass == 0) {
+            if (norm < 1.0e-6) break;
+            continue;
+        }
+        double rerr = 0.0;
+        for (int j = 0; j < nd; j++)
+            if (cabs(x1[j]) > 0.0) rerr += cabs(x1[j] - x[j]) / cabs(x1[j]);
+        for (int j = 0; j < nd; j++) x[j] = x1[j];
+        iters = pass;
+        if (rerr <= 1.0e-6) break;
+        if (pass >= 1000) { iters = -1; break; }
+    }
+    for (int i = 0; i < nd; i++) { dspl[2 * i] = creal(x[i]); dspl[2 * i + 1] = cimag(x[i]); }
+    free(L); free(x); free(x1);
+    return iters;
+}

@@ -134,6 +134,14 @@ void orc_field_values(int nelem, int nnode, const double *node_xyz, const int *n
                       double xw1, double rho, int isol, const double *phi, int nfp,
                       const double *xfp, int kind, double *out, int *onbody, int nthreads);
 
+
+/* ---- motions: SolveMotionMulti (SolveMotionMulti.f90:46-150) for one heading;
+ * body matrices nbody x (6 x 6) column-major; returns the quadratic-damping
+ * iterations (0: none needed, -1: not converged in 1000). ---- */
+int orc_motion(int nbody, double w1, const double *amas, const double *bdmp, const double *exfc,
+               const double *mass, const double *blnr, const double *bqdr, const double *crs,
+               const double *kstf, double *dspl);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,8 @@ def _load(path):
         o.orc_field_values.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d, _d,
                                        C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _ip,
                                        C.c_int]
+        o.orc_motion.restype = C.c_int
+        o.orc_motion.argtypes = [C.c_int, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
@@ -250,3 +252,41 @@ def field_values(m, phi, xfp, w1: float, h: float, headings_rad, kind=0, amp=1.0
                               len(beta), _p(beta), amp, xw[0], xw[1], rho, isol, _p(ph), nfp, _p(xf), kind, _p(out),
                               _p(ob, C.c_int), nthreads)
     return out, ob.astype(bool)
+
+
+def _blocks(mats):
+    """nbody (6 x 6) matrices -> the nbody x 36 column-major block array."""
+    return np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=np.float64).T.ravel() for m in mats]))
+
+
+def motion(nbody, w1, amas, bdmp, exfc, mass, blnr, bqdr, crs, kstf, fma: bool = False):
+    """SolveMotionMulti for every heading column of exfc (6 nbody x nbeta):
+    each body matrix argument is a list of nbody 6 x 6 arrays.  Returns
+    (dspl 6 nbody x nbeta, iterations per heading)."""
+    nd = 6 * nbody
+    am = np.asfortranarray(amas, dtype=np.float64)
+    bd = np.asfortranarray(bdmp, dtype=np.float64)
+    blk = [_blocks(m) for m in (mass, blnr, bqdr, crs, kstf)]
+    ex = np.asarray(exfc, dtype=np.complex128).reshape(nd, -1)
+    out = np.zeros_like(ex)
+    its = []
+    for b in range(ex.shape[1]):
+        e = np.ascontiguousarray(ex[:, b])
+        d = np.zeros(nd, dtype=np.complex128)
+        its.append(lib(fma).orc_motion(nbody, w1, _p(am), _p(bd), _p(e), *[_p(x) for x in blk], _p(d)))
+        out[:, b] = d
+    return out, its
+
+
+def forces(m, phi, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1):
+    """RFORCE_MULTI / EFORCE_MULTI from the potentials phi (N x (6 nbody + nbeta))."""
+    v, wk, _ = wave(w1, h)
+    nd = 6 * m.nbody
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    ph = np.asfortranarray(phi, dtype=np.complex128)
+    amas = np.zeros((nd, nd), order="F")
+    bdmp = np.zeros((nd, nd), order="F")
+    exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
+    lib().orc_forces(m.nelem, _p(ph), _p(m.cen), _p(m.area), _p(m.dxyz), m.nbody, _p(m.body_start, C.c_int),
+                     len(beta), _p(beta), v, w1, wk, h, amp, xw[0], xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc))
+    return amas, bdmp, exfc[:, :len(beta)]

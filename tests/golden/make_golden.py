@@ -21,6 +21,11 @@ Outputs (small .npz files, committed):
   example2.npz    examples/example2_multi_bodies_wave_fields (10 bodies, 100 field
                   points): meshes, deck values, and the shipped Wamit_format
                   PressureElevation.6p / PressureElevationIncidence.6p values
+  motion.npz      the body matrices (Hydrostatic_*.in) and the shipped Motion.4
+                  table of example 2
+  example1.npz    examples/example1_multi_bodies (5 buoys, depth 0.9 m, 80
+                  frequencies): one hull mesh, the LCS, deck values, body matrices
+                  and the shipped Motion.4 table
 
 `python make_golden.py NAME ...` writes only the named fixtures.
 """
@@ -35,7 +40,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "hams-mrel_amd"), os.path.join(ROOT, "oracle")]
 
 import oracle  # noqa: E402
-from hamsx.deck import read_control, read_hams_block_file, read_pnl  # noqa: E402
+from hamsx.deck import read_control, read_hams_block_file, read_hydrostatic, read_pnl  # noqa: E402
 
 REF_DATA = "/root/reference/test/data/config1"
 d = C.c_double
@@ -117,6 +122,49 @@ def example2():
         os.path.join(HERE, "example2.npz"), depth=ctl.depth, periods=np.array(ctl.wvnb), inft=ctl.inft,
         headings_deg=np.array(ctl.headings_deg), xr=ctl.xr, isol=ctl.isol, refl=ctl.refl, xfp=ctl.xfp,
         xyz=np.stack([m.xyz for m in bodies]), ncn=bodies[0].ncn, ncon=bodies[0].ncon, rad=rad, dif=dif, inc=inc)
+
+
+def _motion4(path, nper, nbeta, nd):
+    """Motion.4 (SolveMotionMulti, (2ES14.6,I6,4ES14.6)): per frequency and
+    heading, rows MD = 1..nd of MAGN, PHS, NREL, NIMG."""
+    rows = [ln for ln in open(path).read().splitlines() if ln.strip()]
+    assert len(rows) == nper * nbeta * nd, (path, len(rows))
+    vals = np.array([[float(t[34:48]), float(t[48:62]), float(t[62:76]), float(t[76:90])] for t in rows])
+    mds = np.array([int(t[28:34]) for t in rows])
+    assert np.all(mds.reshape(nper, nbeta, nd) == np.arange(1, nd + 1))
+    return vals.reshape(nper, nbeta, nd, 4)
+
+
+def _hydro(paths):
+    hs = [read_hydrostatic(p) for p in paths]
+    return np.array([[h.mass, h.blnr, h.bqdr, h.crs, h.kstf] for h in hs])   # nbody x 5 x 6 x 6
+
+
+def motion():
+    ex2 = "/root/reference/examples/example2_multi_bodies_wave_fields"
+    c2 = read_control(os.path.join(ex2, "Input", "ControlFile.in"))
+    np.savez_compressed(
+        os.path.join(HERE, "motion.npz"),
+        ex2_hydro=_hydro([os.path.join(ex2, "Input", f"Hydrostatic_{b + 1}.in") for b in range(c2.nbody)]),
+        ex2_motion=_motion4(os.path.join(ex2, "Output", "Wamit_format", "Motion.4"), len(c2.wvnb),
+                            len(c2.headings_deg), 6 * c2.nbody))
+
+
+def example1():
+    ex = "/root/reference/examples/example1_multi_bodies"
+    ctl = read_control(os.path.join(ex, "Input", "ControlFile.in"))
+    nb = ctl.nbody
+    mesh = read_pnl(os.path.join(ex, "Input", "HullMesh_1.pnl"))
+    for b in range(1, nb):   # the five hull files are the same mesh placed by the LCS
+        other = read_pnl(os.path.join(ex, "Input", f"HullMesh_{b + 1}.pnl"))
+        assert np.array_equal(other.xyz, mesh.xyz) and np.array_equal(other.ncon, mesh.ncon)
+    np.savez_compressed(
+        os.path.join(HERE, "example1.npz"), depth=ctl.depth, omegas=np.array(ctl.wvnb), inft=ctl.inft,
+        headings_deg=np.array(ctl.headings_deg), xr=ctl.xr, lcs=ctl.lcs, isol=ctl.isol, refl=ctl.refl,
+        xyz=mesh.xyz, ncn=mesh.ncn, ncon=mesh.ncon,
+        hydro=_hydro([os.path.join(ex, "Input", f"Hydrostatic_{b + 1}.in") for b in range(nb)]),
+        motion=_motion4(os.path.join(ex, "Output", "Wamit_format", "Motion.4"), len(ctl.wvnb),
+                        len(ctl.headings_deg), 6 * nb))
 
 
 def main():
@@ -282,12 +330,14 @@ def main():
         hull_ncon=hull.ncon, lid_xyz=lid.xyz, lid_ncn=lid.ncn, lid_ncon=lid.ncon, **tabs5)
     newman2()
     example2()
+    motion()
+    example1()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         for name in sys.argv[1:]:
-            {"newman2": newman2, "example2": example2}[name]()
+            {"newman2": newman2, "example2": example2, "motion": motion, "example1": example1}[name]()
     else:
         main()
