@@ -1262,8 +1262,15 @@ int hamsx_panel_geometry(int nnode, const double* xyz, int nelem, const int* ncn
 
 int hamsx_wave_properties(int inft, double value, double h, double* v, double* wk, double* w1, double* wvn) {
   if (!v || !wk || !w1) return fail(HAMSX_EARG, "hamsx_wave_properties: NULL output");
-  if (!(value > 0.0)) return fail(HAMSX_EARG, "frequency value must be positive");
   if (wvn) std::fill(wvn, wvn + HAMSX_NK, 0.0);
+  if (inft == HAMSX_FREQ_LIMIT) {
+    // SYBO = 1 (ImplementSubs.f90:45-70): V = WK = W1 = 0 for the zero-frequency
+    // limit, -1 for the infinite-frequency limit; no dispersion roots
+    if (value != 0.0 && value != -1.0) return fail(HAMSX_EARG, "a frequency limit is 0 (zero) or -1 (infinite)");
+    *v = *wk = *w1 = value;
+    return 0;
+  }
+  if (!(value > 0.0)) return fail(HAMSX_EARG, "frequency value must be positive");
   double tmp[HAMSX_NK] = {0};
   double* roots = wvn ? wvn : tmp;
   const bool deep = h <= 0.0;
@@ -1471,9 +1478,10 @@ int hamsx_assemble(hamsx_ctx* c) {
   if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->nf)))) return rc;
   c->have_C = false;
   c->freq.tab = finite ? c->modetab.p : nullptr;
+  const bool limit = c->freq.v == 0.0 || c->freq.v == -1.0;
   {
     StageTimer tm(c, 0);
-    if (finite) hamsx::launch::mode_tables(c->stream, c->mesh(), c->freq);
+    if (finite && !limit) hamsx::launch::mode_tables(c->stream, c->mesh(), c->freq);
     if (binned)
       assemble_binned(c, c->bins, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p, true);
     else
@@ -1528,7 +1536,7 @@ int hamsx_solve_frequency(hamsx_ctx* c, double w1, double h, double* amas, doubl
   if (c->pending) return busy(c);
   if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_solve_frequency needs hamsx_set_waves first");
   double v, wk, w, wvn[HAMSX_NK];
-  int rc = hamsx_wave_properties(3, w1, h, &v, &wk, &w, wvn);
+  int rc = hamsx_wave_properties(w1 == 0.0 || w1 == -1.0 ? HAMSX_FREQ_LIMIT : 3, w1, h, &v, &wk, &w, wvn);
   if (rc || (rc = hamsx_set_frequency(c, v, wk, w, h, wvn, HAMSX_NK)) || (rc = hamsx_assemble(c))) return rc;
   if ((rc = do_solve(c))) return rc;
   int info = 0;
@@ -1541,7 +1549,7 @@ int hamsx_submit_frequency(hamsx_ctx* c, double w1, double h) {
   if (c->pending) return busy(c);
   if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_submit_frequency needs hamsx_set_waves first");
   double v, wk, w, wvn[HAMSX_NK];
-  int rc = hamsx_wave_properties(3, w1, h, &v, &wk, &w, wvn);
+  int rc = hamsx_wave_properties(w1 == 0.0 || w1 == -1.0 ? HAMSX_FREQ_LIMIT : 3, w1, h, &v, &wk, &w, wvn);
   if (rc || (rc = hamsx_set_frequency(c, v, wk, w, h, wvn, HAMSX_NK)) || (rc = hamsx_assemble(c))) return rc;
   if ((rc = do_solve(c)) || (rc = enqueue_forces(c))) return rc;
   c->pending = true;
@@ -1636,7 +1644,7 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
     fm.row0 = n;
     double2* S = c->fM.p;
     double2* T = c->fM.p + (size_t)nc * n;
-    if (finite) hamsx::launch::mode_tables(c->stream, fm, ff);
+    if (finite && c->freq.v != 0.0 && c->freq.v != -1.0) hamsx::launch::mode_tables(c->stream, fm, ff);
     if (binned) {
       if ((rc = ensure_bins(c, c->fbins, fm, (c->field_gen << 20) | (unsigned long long)chunk))) return rc;
       assemble_binned(c, c->fbins, fm, ff, T, S, ovf, false);

@@ -64,8 +64,12 @@ int hamsx_panel_geometry(int nnode, const double *node_xyz, int nelem, const int
  * number, 2 finite-depth wave number, 3 rad/s, 4 period, 5 wave length),
  * depth h (< 0: infinite).  Outputs v = w^2/g, wk, w1 and the NK dispersion
  * roots wvn[HAMSX_NK] (finite depth only; zeros otherwise).
+ * inft = HAMSX_FREQ_LIMIT gives the frequency limits of SYBO = 1
+ * (ImplementSubs.f90:45-70): value 0 -> v = wk = w1 = 0 (zero frequency),
+ * value -1 -> v = wk = w1 = -1 (infinite frequency), no roots.
  * Replaces CalWaveProperts, src/ImplementSubs.f90:40-151 with DISPERSION,
  * src/WavDynSubs.f90:40-127. */
+#define HAMSX_FREQ_LIMIT 0
 int hamsx_wave_properties(int inft, double value, double h, double *v, double *wk,
                           double *w1, double *wvn);
 
@@ -128,7 +132,11 @@ int hamsx_forces(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
 
 /* One whole frequency solve: wave properties of w1 (rad/s) at the mesh depth
  * h, assembly, factorisation, radiation + all diffraction solves and the
- * force reductions, with a single device synchronisation. */
+ * force reductions, with a single device synchronisation.  w1 = 0 and
+ * w1 = -1 solve the zero- and infinite-frequency limits (SYBO = 1): the
+ * Green functions' V = 0 / V = -1 branches; at the limits the reference
+ * reports the added mass only (bdmp and exfc are what its sums give: zero
+ * at w1 = 0). */
 int hamsx_solve_frequency(hamsx_ctx *ctx, double w1, double h, double *amas, double *bdmp,
                           double *exfc);
 

@@ -1348,6 +1348,17 @@ void orc_panel_geometry(int nnode, const double *xyz, int nelem, const int *ncn,
     }
 }
 
+/* CalWaveProperts for a frequency in rad/s (INFT = 3, ImplementSubs.f90:78-89);
+ * w1 = 0 and w1 = -1 are the SYBO = 1 limits (:45-70): V = WK = W1 = 0 or -1,
+ * no dispersion roots. */
+static void wave_setup(double w1, double h, int nk, double *v, double *wk, double *wvn)
+{
+    if (w1 == 0.0 || w1 == -1.0) { *v = *wk = w1; return; }
+    *v = w1 * w1 / G_C;
+    if (h <= 0.0) *wk = *v;
+    else { orc_dispersion(w1, h, nk, wvn); *wk = wvn[0]; }
+}
+
 /* ========================================================================
  * Assembly: CALGREEN_MULTI (CalGreenFuncMulti.f90:112-183), BODINT_MULTI_LEFT
  * (BodyIntgrMulti.f90:48-77), ASSB_LEFT_MULTI (AssbMatxMulti.f90:53-105); the
@@ -1632,9 +1643,8 @@ int orc_solve_frequency(int nelem, int nnode, const double *xyz, const int *ncn,
 {
     const int nk = 200;
     double wvn[200] = {0};
-    double v = w1 * w1 / G_C, wk;
-    if (h <= 0.0) wk = v;
-    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
     size_t nn = (size_t)nelem * nelem;
     int nr = 6 * nbody + nbeta;
     double *A = (double *)malloc(sizeof(double) * 2 * nn);
@@ -1668,9 +1678,8 @@ int orc_solve_frequency_irr(int nelem, int nnode, const double *xyz, const int *
 {
     const int nk = 200;
     double wvn[200] = {0};
-    double v = w1 * w1 / G_C, wk;
-    if (h <= 0.0) wk = v;
-    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
     const int n = nelem, nf = nelem + nlid, nr = 6 * nbody + nbeta;
     double *A = (double *)malloc(sizeof(double) * 2 * (size_t)nf * n);
     double *S = (double *)malloc(sizeof(double) * 2 * (size_t)nf * n);
@@ -1730,9 +1739,8 @@ int orc_solve_potentials(int nelem, int nnode, const double *xyz, const int *ncn
 {
     const int nk = 200;
     double wvn[200] = {0};
-    double v = w1 * w1 / G_C, wk;
-    if (h <= 0.0) wk = v;
-    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
     size_t nn = (size_t)nelem * nelem;
     int nr = 6 * nbody + nbeta;
     double *A = (double *)malloc(sizeof(double) * 2 * nn);
@@ -1801,9 +1809,8 @@ void orc_field_values(int nelem, int nnode, const double *xyz, const int *ncn, c
 {
     const int nk = 200;
     double wvn[200] = {0};
-    double v = w1 * w1 / G_C, wk;
-    if (h <= 0.0) wk = v;
-    else { orc_dispersion(w1, h, nk, wvn); wk = wvn[0]; }
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
     const int nd = 6 * nbody, nr = nd + nbeta;
     const int limit = fabs(v) < F1EM8 || fabs(v + 1.0) < F1EM8;
     int *bodyof = (int *)malloc(sizeof(int) * (size_t)nelem);

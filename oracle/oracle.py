@@ -135,9 +135,12 @@ def ref():
 # Convenience wrappers over combined meshes (hamsx.mesh.Combined layout)
 # ---------------------------------------------------------------------------
 def wave(w1: float, h: float):
-    """CalWaveProperts for a frequency in rad/s: v, wk, wvn."""
-    v = w1 * w1 / G
+    """CalWaveProperts for a frequency in rad/s: v, wk, wvn (w1 = 0 / -1: the
+    zero- / infinite-frequency limits, v = wk = w1)."""
     wvn = np.zeros(NK)
+    if w1 in (0.0, -1.0):
+        return w1, w1, wvn
+    v = w1 * w1 / G
     if h > 0:
         lib().orc_dispersion(w1, h, NK, _p(wvn))
         wk = wvn[0]

@@ -23,9 +23,12 @@ Outputs (small .npz files, committed):
                   PressureElevation.6p / PressureElevationIncidence.6p values
   motion.npz      the body matrices (Hydrostatic_*.in) and the shipped Motion.4
                   table of example 2
+  writers.npz     text of shipped output files (config1's HAMS tables, the first
+                  lines of example2's WAMIT files) for the writer round trips
   example1.npz    examples/example1_multi_bodies (5 buoys, depth 0.9 m, 80
-                  frequencies): one hull mesh, the LCS, deck values, body matrices
-                  and the shipped Motion.4 table
+                  frequencies and the two limits): one hull mesh, the LCS, deck
+                  values, body matrices, the shipped Motion.4 table and the
+                  OAMASS rows of the zero- and infinite-frequency limits
 
 `python make_golden.py NAME ...` writes only the named fixtures.
 """
@@ -164,7 +167,31 @@ def example1():
         xyz=mesh.xyz, ncn=mesh.ncn, ncon=mesh.ncon,
         hydro=_hydro([os.path.join(ex, "Input", f"Hydrostatic_{b + 1}.in") for b in range(nb)]),
         motion=_motion4(os.path.join(ex, "Output", "Wamit_format", "Motion.4"), len(ctl.wvnb),
-                        len(ctl.headings_deg), 6 * nb))
+                        len(ctl.headings_deg), 6 * nb),
+        # 0_inf_frequency_limits = 1: the first two "frequencies" of the tables are the zero- and
+        # infinite-frequency limits; OAMASS1..6 rows for them, (limit, body M, body N) order
+        oamass_limits=np.array([read_hams_block_file(os.path.join(ex, "Output", "Hams_format", f"OAMASS{md}.txt"))
+                                [:2 * nb * nb] for md in range(1, 7)]))
+
+
+def writers():
+    """Text of shipped output files, for the writer round trips: config1's
+    18 HAMS tables whole, and the first lines of example2's WAMIT files."""
+    cfg = os.path.join(REF_DATA, "expected_output")
+    ex2 = "/root/reference/examples/example2_multi_bodies_wave_fields/Output/Wamit_format"
+    out = {}
+    for kind in ("OAMASS", "ODAMPING", "OEXFOR"):
+        for md in range(1, 7):
+            out[f"config1_{kind}{md}"] = np.array(open(os.path.join(cfg, f"{kind}{md}.txt")).read().splitlines())
+    for name, count in (("AmssDamp.1", 360), ("ExcForce.3", 120), ("Motion.4", 120), ("Hydrostat.hst", 240),
+                        ("PressureElevationIncidence.6p", 200)):
+        lines = open(os.path.join(ex2, name)).read().splitlines()
+        out["ex2_" + name.replace(".", "_")] = np.array(lines[:count])
+    # .6p: the radiation lines of the first 10 points, and heading 0's diffraction lines (all points)
+    lines = open(os.path.join(ex2, "PressureElevation.6p")).read().splitlines()
+    out["ex2_6p_rad"] = np.array(lines[:100])
+    out["ex2_6p_dif"] = np.array(lines[1000:1100])
+    np.savez_compressed(os.path.join(HERE, "writers.npz"), **out)
 
 
 def main():
@@ -332,12 +359,14 @@ def main():
     example2()
     motion()
     example1()
+    writers()
     print("golden fixtures written to", HERE)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         for name in sys.argv[1:]:
-            {"newman2": newman2, "example2": example2, "motion": motion, "example1": example1}[name]()
+            {"newman2": newman2, "example2": example2, "motion": motion, "example1": example1,
+             "writers": writers}[name]()
     else:
         main()

@@ -135,3 +135,48 @@ def test_example2_motions_on_gpu(k):
     say(f"    steps {list(its)}  rel err {e:.3e}")
     assert all(i > 0 for i in its)
     assert e <= 5e-6
+
+
+@pytest.mark.parametrize("depth", [-1.0, 12.0])
+@pytest.mark.parametrize("w", [0.0, -1.0])
+def test_frequency_limits_match_oracle(depth, w):
+    """The zero- (w = 0) and infinite- (w = -1) frequency limits: only the
+    Rankine and image terms remain, so the GPU matches the oracle to
+    rounding.  At the infinite limit in finite depth the reference's
+    diffraction is NaN (DINP's cosh/sinh overflow at WK = 1e20, as example1's
+    shipped OEXFOR rows show), and so is ours."""
+    bodies = [cylinder(5.0, 8.0, 24, 6, 3, -12.0), cylinder(4.0, 6.0, 20, 6, 3, 12.0)]
+    m = combine(bodies, [np.array([-12.0, 0, -2.0]), np.array([12.0, 0, -2.0])])
+    say(f"[gpu] limit w={w} depth={depth}")
+    with FrequencySolver(m) as s:
+        s.set_waves([0.0])
+        amas, bdmp, exfc, info = s.solve_frequency(w, depth)
+    assert info == 0
+    ao, bo, eo, _ = oracle.solve_frequency(m, w, depth, [0.0])
+    assert np.max(np.abs(amas - ao)) <= 1e-12 * np.max(np.abs(ao))
+    assert np.max(np.abs(bdmp)) == 0.0 and np.max(np.abs(bo)) == 0.0
+    if w == -1.0 and depth > 0:
+        assert np.all(np.isnan(exfc)) and np.all(np.isnan(eo))
+    else:
+        assert np.max(np.abs(exfc)) == 0.0 and np.max(np.abs(eo)) == 0.0
+
+
+def test_example1_frequency_limits_on_gpu(example1):
+    """0_inf_frequency_limits = 1 in example1: the shipped OAMASS rows of the
+    two limits (25 body pairs each, 5 printed digits) against the GPU solve
+    at w = 0 and w = -1 (depth 0.9 m: the reference's V = 0 branch has no
+    sea bed).  Relative Frobenius per table <= 1e-4."""
+    z, m = example1
+    nb = len(z["xr"])
+    say(f"[gpu] example1 limits n={m.nelem}")
+    with FrequencySolver(m) as s:
+        s.set_waves(np.deg2rad(z["headings_deg"]), isol=int(z["isol"]))
+        res = [s.solve_frequency(w, float(z["depth"])) for w in (0.0, -1.0)]
+    for k, (amas, bdmp, exfc, info) in enumerate(res):
+        assert info == 0
+        for md in range(6):
+            ref = z["oamass_limits"][md][k * nb * nb:(k + 1) * nb * nb, 2:]
+            got = np.array([amas[6 * M + md, 6 * N:6 * N + 6] for M in range(nb) for N in range(nb)])
+            e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            say(f"    limit {k} OAMASS{md + 1}: rel err {e:.2e}")
+            assert e <= 1e-4
