@@ -277,6 +277,8 @@ struct hamsx_ctx {
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins of the system's rows
   PairBins bins;
+  // the LU's look-ahead stream and events (HAMSX_LU_LOOKAHEAD=0 factors on the one stream)
+  hamsx::LuAux lu_aux{nullptr, nullptr, nullptr};
   // field points (CalPotentialMulti): nfp points (SoA); vtx: panel vertices,
   // vtx[(3q + d) n + j]; the field view ffx/ffy/ffz holds the hull centroids
   // and then one chunk of field points; field_gen names the point set
@@ -531,6 +533,14 @@ int ensure_normal_matrix(hamsx_ctx* c) {
   return 0;
 }
 
+// The LU's look-ahead resources, or null when HAMSX_LU_LOOKAHEAD=0
+// (diagnostic: one stream, the same factors bit for bit).
+const hamsx::LuAux* lu_lookahead(hamsx_ctx* c) {
+  const char* e = std::getenv("HAMSX_LU_LOOKAHEAD");
+  if (e && *e == '0') return nullptr;
+  return c->lu_aux.s2 ? &c->lu_aux : nullptr;
+}
+
 // the square matrix the solvers work on: A, or C = A^T A with a lid
 double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
 
@@ -550,12 +560,12 @@ int do_factor(hamsx_ctx* c) {
   double2* M = system_matrix(c);
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_factor(c->stream, M, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
-                             c->ludiag.p);
+                             c->ludiag.p, lu_lookahead(c));
     if (c->nlid == 0) c->have_A = false;  // factored in place (with a lid, C is factored and A kept)
   } else {
     hamsx::launch::to_c64(c->stream, M, c->Af.p, (size_t)n * n);
     hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                                 c->flags.p + 1, c->ludiag.p);
+                                 c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
   }
   c->have_LU = true;
   return check_launch();
@@ -1129,6 +1139,13 @@ int hamsx_create(hamsx_ctx** out, int device) {
     delete c;
     return fail(HAMSX_EDEVICE, "stream creation failed on device %d", device);
   }
+  // LU look-ahead: a second stream (default priority) and its two events
+  if (hipStreamCreateWithFlags(&c->lu_aux.s2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->lu_aux.panel, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->lu_aux.update, hipEventDisableTiming) != hipSuccess) {
+    hamsx_destroy(c);
+    return fail(HAMSX_EDEVICE, "stream/event creation failed on device %d", device);
+  }
   int rc = c->flags.ensure(5);
   if (rc == 0) rc = c->wvn.ensure(hamsx::kNK);
   if (rc) {
@@ -1169,6 +1186,10 @@ int hamsx_destroy(hamsx_ctx* c) {
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->lu_aux.s2) (void)hipStreamSynchronize(c->lu_aux.s2);
+  if (c->lu_aux.panel) (void)hipEventDestroy(c->lu_aux.panel);
+  if (c->lu_aux.update) (void)hipEventDestroy(c->lu_aux.update);
+  if (c->lu_aux.s2) (void)hipStreamDestroy(c->lu_aux.s2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;

@@ -105,6 +105,13 @@ struct HmDev {
   const int *lptr, *litem;
 };
 
+// Look-ahead resources of the LU: a second stream (default priority) and two
+// events; null or s2 == null factors on the one stream.
+struct LuAux {
+  hipStream_t s2;
+  hipEvent_t panel, update;
+};
+
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
@@ -126,9 +133,11 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
          const int* body_start_d, const double2* S, double2* dum2, double2* partial, int nchunk,
          const int* chunk_body_d, const int* chunk_j0_d, const int* chunk_j1_d, double2* X);
 // LU with tournament pivoting; ipiv 1-based, perm = row permutation, info[0] = first zero pivot;
-// diag: kPanelNB x kPanelNB complex128 scratch for the factored diagonal block
+// diag: kPanelNB x kPanelNB complex128 scratch for the factored diagonal block.  With aux, each
+// step updates the next panel's columns first and the rest of the trailing matrix on aux->s2,
+// overlapping the next pivot selection; the factors are the same bit for bit.
 void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
-               double2* diag);
+               double2* diag, const LuAux* aux = nullptr);
 // X <- A^{-1} X using the factors, nrhs columns (leading dimension n)
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
 // AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
@@ -155,7 +164,7 @@ void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 // C(r, c) = C(c, r) above the diagonal (n x n): completes a lower-triangle syrk
 void mirror_lower(hipStream_t s, double2* C, int n);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
-                   double2* diag);
+                   double2* diag, const LuAux* aux = nullptr);
 // skip (may be null, here and below): a device flag that turns every
 // launch into a no-op when set (refinement after convergence)
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,

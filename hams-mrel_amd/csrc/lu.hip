@@ -365,8 +365,9 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
   }
 }
 
-// Trailing update C -= L U over the (n - r0) x (n - r0) block starting at
-// (r0, r0), K = kk <= 32.  MFMA 16x16x4 computes the transposed product
+// Trailing update C -= L U over rows [r0, n) and the column tiles tn0,
+// tn0 + 1, ... (64 columns each, from column r0) of the block starting at
+// (r0, r0), K = kk <= 32; ntiles = tiles_m x the number of column tiles.  MFMA 16x16x4 computes the transposed product
 // (U^T L^T) so that the 16 consecutive lanes of an accumulator row map to
 // 16 consecutive rows of column-major C.  Operand layout (both types): lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; the accumulator row held
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
 // XCDs works on a contiguous band of tiles (L21 rows stay in its L2).
 template <typename R>
 __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __restrict__ A, int lda, int n, int r0,
-                                                         int kk, int tiles_m, int ntiles) {
+                                                         int kk, int tiles_m, int ntiles, int tn0) {
   using v4 = typename CT<R>::v4;
   __shared__ R lr_[kPanelNB][64], li_[kPanelNB][64];  // L21 tile: [k][row]
   __shared__ R ur_[kPanelNB][64], ui_[kPanelNB][64];  // U12 tile: [k][col]
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
     if (rb >= ntiles) return;
     bid = rb;
   }
-  const int tm = bid % tiles_m, tn = bid / tiles_m;
+  const int tm = bid % tiles_m, tn = tn0 + bid / tiles_m;
   const int i0 = r0 + tm * 64, j0 = r0 + tn * 64;
   const int t = threadIdx.x;
   {
@@ -479,8 +480,10 @@ __global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info)
 
 template <typename R>
 static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
-                        int* info, typename CT<R>::c2* diag) {
+                        int* info, typename CT<R>::c2* diag, const LuAux* aux) {
   HX_LAUNCH(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
+  const bool la = aux && aux->s2;
+  bool pending = false;  // a deferred trailing update is running on aux->s2
   // workspace: two ping-pong candidate buffers of nsets0 sets
   const int nsets0 = (n + kSelRows - 1) / kSelRows;
   int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
@@ -508,6 +511,9 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
       nsets = nout;
     }
     HX_LAUNCH(k_pivots_to_swaps, dim3(1), dim3(64), 0, s, idx[cur], nb, col0, ipiv);
+    // the row swaps touch every column: the previous step's deferred update must be done
+    if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
+    pending = false;
     HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
     const int rest = n - col0 - nb;
     const int nrb = (rest + 255) / 256;
@@ -515,11 +521,27 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
               (const c2*)diag);
     if (rest > 0) {
       const int tiles = (rest + 63) / 64;
-      const int ntiles = tiles * tiles;
-      const int grid = ((ntiles + 7) / 8) * 8;
-      HX_LAUNCH(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles);
+      if (!la || tiles == 1) {
+        const int ntiles = tiles * tiles;
+        const int grid = ((ntiles + 7) / 8) * 8;
+        HX_LAUNCH(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles, 0);
+      } else {
+        // look-ahead: the first 64 trailing columns (the next panel) here, the
+        // rest on the second stream, overlapping the next pivot selection,
+        // which reads only the next panel's columns
+        HX_LAUNCH(k_trailing_update<R>, dim3(((tiles + 7) / 8) * 8), dim3(256), 0, s, A, n, n, col0 + nb, nb,
+                  tiles, tiles, 0);
+        (void)hipEventRecord(aux->panel, s);
+        (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
+        const int ntiles = tiles * (tiles - 1);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, n, col0 + nb, nb,
+                  tiles, ntiles, 1);
+        (void)hipEventRecord(aux->update, aux->s2);
+        pending = true;
+      }
     }
   }
+  if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -882,8 +904,8 @@ __global__ void k_refine_reset(int* __restrict__ state) {
 
 namespace launch {
 void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
-               double2* diag) {
-  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info, diag);
+               double2* diag, const LuAux* aux) {
+  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info, diag, aux);
 }
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
   lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
@@ -897,8 +919,8 @@ void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
   HX_LAUNCH(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);
 }
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
-                   double2* diag) {
-  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag));
+                   double2* diag, const LuAux* aux) {
+  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag), aux);
 }
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
                   const int* skip) {
