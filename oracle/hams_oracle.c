@@ -1369,13 +1369,30 @@ static void wave_setup(double w1, double h, int nk, double *v, double *wk, doubl
  * rows :260-316 of CalGreenFuncMulti.f90) with BODINT_LEFT_IRR_MULTI and
  * RBC_IRR_MULTI's DUM (BodyIntgr_irrMulti.f90:48-127), without the 2*pi of
  * the diagonal. */
+static void field_entry_flag(double xi, double yi, double zi, double gx, double gy, double gz, int j, int nelem,
+                             int nnode, const double *xyz, const int *ncn, const int *ncon, const double *cen,
+                             const double *area, const double *pnsz, const double *dxyz, double v, double h,
+                             const double *wvn, int nk, cplx *t_ij, cplx *s_ij);
+
 static void field_entry(double xi, double yi, double zi, int j, int nelem, int nnode, const double *xyz,
                         const int *ncn, const int *ncon, const double *cen, const double *area,
                         const double *pnsz, const double *dxyz, double v, double h, const double *wvn, int nk,
                         cplx *t_ij, cplx *s_ij)
 {
+    field_entry_flag(xi, yi, zi, xi, yi, zi, j, nelem, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn, nk,
+                     t_ij, s_ij);
+}
+
+/* As field_entry, with FLAG decided from the point (gx, gy, gz): the mirrored
+ * rows of the symmetric assembly keep the unmirrored distance
+ * (CalGreenFuncMulti.f90:134-139 before the IS loop). */
+static void field_entry_flag(double xi, double yi, double zi, double gx, double gy, double gz, int j, int nelem,
+                             int nnode, const double *xyz, const int *ncn, const int *ncon, const double *cen,
+                             const double *area, const double *pnsz, const double *dxyz, double v, double h,
+                             const double *wvn, int nk, cplx *t_ij, cplx *s_ij)
+{
     double xj = cen[j], yj = cen[j + nelem], zj = cen[j + 2 * nelem];
-    double dist = sqrt(ipow(xi - xj, 2) + ipow(yi - yj, 2) + ipow(zi - zj, 2));
+    double dist = sqrt(ipow(gx - xj, 2) + ipow(gy - yj, 2) + ipow(gz - zj, 2));
     int flag = (dist <= 50.0 * pnsz[j]) ? 1 : 0;
     double sij = 0.0, dij[3] = {0, 0, 0};
     if (flag) {
@@ -1957,4 +1974,181 @@ int orc_motion(int nbody, double w1, const double *amas, const double *bdmp, con
     for (int i = 0; i < nd; i++) { dspl[2 * i] = creal(x[i]); dspl[2 * i + 1] = cimag(x[i]); }
     free(L); free(x); free(x1);
     return iters;
+}
+
+/* ========================================================================
+ * Geometric symmetry, NSYS = 2 (one symmetry plane; ISX = 1: the mesh is the
+ * x >= 0 half, mirrored by x -> -x; ISY = 1: y -> -y).  CALGREEN_MULTI's IS
+ * loop (CalGreenFuncMulti.f90:141-177: field point mirrored for IS = 2, FLAG
+ * from the unmirrored distance :134-139), ASSB_LEFT_MULTI (AssbMatxMulti.f90
+ * :53-105), ASSB_RBC/DBC_MULTI with RBC/DBC_MULTI_RIGHT (:124-251,
+ * BodyIntgrMulti.f90:83-173), the two ZGETRF/ZGETRS and the MXPOT
+ * recombination (:257-383), RFORCE/EFORCE_MULTI's IP sums
+ * (PotentWavForceMulti.f90:45-226).  Sign tables of Const_mod
+ * (WavDynMods.f90:155-159), 1-based (IS, IP) = 1, 2:
+ *   SY(IS,IP) = +1 if IS == IP else -1; SX = 1; RXY(a,b) = -1 only at (2,2);
+ *   RX(IP,1) = +1, -1 for IP = 1, 2 and RX(IP,2) = 1; RY(IP,1) = 1 and
+ *   RY(IP,2) = +1, -1.
+ * The potentials of both halves go to phi2 (N x ncol x 2) when not NULL.
+ * ====================================================================== */
+static double sy_tab(int is, int ip) { return is == ip ? 1.0 : -1.0; }
+static double rxy_tab(int a, int b) { return (a == 2 && b == 2) ? -1.0 : 1.0; }
+
+int orc_solve_frequency_sym(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                            const double *cen, const double *area, const double *pnsz, const double *dxyz,
+                            int nbody, const int *body_start, int isx, double w1, double h, int nbeta,
+                            const double *beta, double amp, double xw0, double xw1, double rho, int isol,
+                            double *amas, double *bdmp, double *exfc, double *phi2, int nthreads)
+{
+    const int nk = 200, n = nelem, nd = 6 * nbody, nr = nd + nbeta;
+    double wvn[200] = {0};
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
+    /* reflection of a point for mirror index m (1: identity, 2: mirrored) */
+    #define REFX(m, x) ((isx && (m) == 2) ? -(x) : (x))
+    #define REFY(m, y) ((!isx && (m) == 2) ? -(y) : (y))
+    cplx *T = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2), *S = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2);
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int is = 1; is <= 2; is++) {
+                const double xi = REFX(is, cen[i]), yi = REFY(is, cen[i + n]), zi = cen[i + 2 * n];
+                cplx t, s;
+                field_entry_flag(xi, yi, zi, cen[i], cen[i + n], cen[i + 2 * n], j, n, nnode, xyz, ncn, ncon, cen, area,
+                                 pnsz, dxyz, v, h, wvn, nk, &t, &s);
+                T[i + (size_t)n * j + (size_t)n * n * (is - 1)] = t;
+                S[i + (size_t)n * j + (size_t)n * n * (is - 1)] = s;
+            }
+    /* AMAT(:,:,IP) */
+    cplx *A = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2);
+    for (int ip = 1; ip <= 2; ip++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                cplx a = (i == j) ? 2.0 * PI_C : 0.0;
+                for (int is = 1; is <= 2; is++) a = a + rxy_tab(is, ip) * T[i + (size_t)n * j + (size_t)n * n * (is - 1)];
+                A[i + (size_t)n * j + (size_t)n * n * (ip - 1)] = a;
+            }
+    /* right-hand sides, BRMAT / BDMAT (N x nr x 2) */
+    cplx *B = (cplx *)calloc((size_t)n * nr * 2, sizeof(cplx));
+    for (int i = 0; i < n; i++) {
+        for (int b = 0; b < nbody; b++) {
+            cplx bt[6][2] = {{0}};
+            for (int j = body_start[b]; j < body_start[b + 1]; j++)
+                for (int is = 1; is <= 2; is++) {
+                    const cplx dum = S[i + (size_t)n * j + (size_t)n * n * (is - 1)];
+                    for (int ip = 1; ip <= 2; ip++) {
+                        const double sy = sy_tab(is, ip);
+                        /* f(IS,IP,MD): ISX: MD 1,5,6 carry SY; ISY: MD 2,4,6 carry SY */
+                        for (int md = 0; md < 6; md++) {
+                            double f = 1.0;
+                            if (isx && (md == 0 || md == 4 || md == 5)) f = sy;
+                            if (!isx && (md == 1 || md == 3 || md == 5)) f = sy;
+                            bt[md][ip - 1] = bt[md][ip - 1] + dum * f * dxyz[j + (size_t)n * md];
+                        }
+                    }
+                }
+            for (int md = 0; md < 6; md++)
+                for (int ip = 1; ip <= 2; ip++) {
+                    cplx acc = 0;
+                    for (int is = 1; is <= 2; is++) acc = acc + rxy_tab(ip, is) * bt[md][is - 1];
+                    B[i + (size_t)n * (6 * b + md) + (size_t)n * nr * (ip - 1)] = acc;
+                }
+        }
+        for (int k = 0; k < nbeta; k++) {
+            cplx bt[2] = {0, 0};
+            for (int j = 0; j < n; j++)
+                for (int is = 1; is <= 2; is++) {
+                    const cplx dum1 = S[i + (size_t)n * j + (size_t)n * n * (is - 1)];
+                    for (int ip = 1; ip <= 2; ip++) {
+                        const double sy = sy_tab(is, ip);
+                        const double xq = isx ? sy * cen[j] : cen[j], yq = isx ? cen[j + n] : sy * cen[j + n];
+                        double dpo[6];
+                        orc_dinp(xq, yq, cen[j + 2 * n], xw0, xw1, beta[k], v, w1, wk, h, amp, dpo);
+                        const cplx px = dpo[0] + I * dpo[1], py = dpo[2] + I * dpo[3], pz = dpo[4] + I * dpo[5];
+                        const cplx dum2 = isx ? sy * px * dxyz[j] + py * dxyz[j + n] + pz * dxyz[j + 2 * n]
+                                              : px * dxyz[j] + sy * py * dxyz[j + n] + pz * dxyz[j + 2 * n];
+                        bt[ip - 1] = bt[ip - 1] - dum1 * dum2;
+                    }
+                }
+            for (int ip = 1; ip <= 2; ip++) {
+                cplx acc = 0;
+                for (int is = 1; is <= 2; is++) acc = acc + rxy_tab(ip, is) * bt[is - 1];
+                B[i + (size_t)n * (nd + k) + (size_t)n * nr * (ip - 1)] = acc;
+            }
+        }
+    }
+    free(T); free(S);
+    int info = 0;
+    int *ipiv = (int *)malloc(sizeof(int) * n);
+    for (int ip = 0; ip < 2; ip++) {
+        double *Ad = (double *)(A + (size_t)n * n * ip), *Bd = (double *)(B + (size_t)n * nr * ip);
+        int in = orc_zgetrf(n, Ad, ipiv);
+        if (in && !info) info = in;
+        orc_zgetrs(n, Ad, ipiv, nr, Bd);
+    }
+    free(ipiv); free(A);
+    /* MXPOT(:, col, IP) = sum_IS X(:, col, IS) RXY(IP, IS) / NSYS */
+    cplx *P = (cplx *)malloc(sizeof(cplx) * (size_t)n * nr * 2);
+    for (int ip = 1; ip <= 2; ip++)
+        for (size_t q = 0; q < (size_t)n * nr; q++) {
+            cplx acc = 0;
+            for (int is = 1; is <= 2; is++) acc = acc + B[q + (size_t)n * nr * (is - 1)] * rxy_tab(ip, is);
+            P[q + (size_t)n * nr * (ip - 1)] = acc / 2.0;
+        }
+    free(B);
+    /* forces: RFORCE_MULTI / EFORCE_MULTI with the IP sums */
+    for (int bm = 0; bm < nbody; bm++)
+        for (int bn = 0; bn < nbody; bn++) {
+            double am[6][6] = {{0}}, bd[6][6] = {{0}};
+            for (int md = 0; md < 6; md++)
+                for (int e = body_start[bm]; e < body_start[bm + 1]; e++)
+                    for (int ip = 1; ip <= 2; ip++) {
+                        const cplx p = P[e + (size_t)n * (6 * bn + md) + (size_t)n * nr * (ip - 1)];
+                        const cplx rphi = I * rho * p * area[e], iphi = I * rho * w1 * p * area[e];
+                        const double r1 = (isx && ip == 2) ? -1.0 : 1.0, r2 = (!isx && ip == 2) ? -1.0 : 1.0;
+                        const double sg[6] = {r1, r2, 1.0, r2, r1, r1 * r2};
+                        for (int k = 0; k < 6; k++) {
+                            am[k][md] = am[k][md] + cimag(rphi * sg[k] * dxyz[e + (size_t)n * k]);
+                            bd[k][md] = bd[k][md] - creal(iphi * sg[k] * dxyz[e + (size_t)n * k]);
+                        }
+                    }
+            for (int k = 0; k < 6; k++)
+                for (int md = 0; md < 6; md++) {
+                    amas[(6 * bm + k) + nd * (6 * bn + md)] = am[k][md];
+                    bdmp[(6 * bm + k) + nd * (6 * bn + md)] = bd[k][md];
+                }
+        }
+    for (int k = 0; k < nbeta; k++)
+        for (int bm = 0; bm < nbody; bm++) {
+            cplx f[6][2] = {{0}};
+            for (int e = body_start[bm]; e < body_start[bm + 1]; e++)
+                for (int ip = 1; ip <= 2; ip++) {
+                    const double r1 = (isx && ip == 2) ? -1.0 : 1.0, r2 = (!isx && ip == 2) ? -1.0 : 1.0;
+                    const cplx pd = P[e + (size_t)n * (nd + k) + (size_t)n * nr * (ip - 1)];
+                    cplx p;
+                    if (isol == 1) {
+                        double pi2[2];
+                        orc_vinp(r1 * cen[e], r2 * cen[e + n], cen[e + 2 * n], xw0, xw1, beta[k], v, w1, wk, h, amp, pi2);
+                        p = (pd + (pi2[0] + I * pi2[1])) * area[e];
+                    } else {
+                        p = pd * area[e];
+                    }
+                    const double sg[6] = {r1, r2, 1.0, r2, r1, r1 * r2};
+                    for (int q = 0; q < 6; q++) f[q][ip - 1] = f[q][ip - 1] + p * sg[q] * dxyz[e + (size_t)n * q];
+                }
+            for (int q = 0; q < 6; q++) {
+                cplx tot = 0;
+                for (int ip = 0; ip < 2; ip++) tot = tot + f[q][ip];
+                const cplx x = I * w1 * rho * tot;
+                const size_t o = 2 * ((size_t)(6 * bm + q) + (size_t)nd * k);
+                exfc[o] = creal(x);
+                exfc[o + 1] = cimag(x);
+            }
+        }
+    if (phi2) memcpy(phi2, P, sizeof(cplx) * (size_t)n * nr * 2);
+    free(P);
+    #undef REFX
+    #undef REFY
+    return info;
 }

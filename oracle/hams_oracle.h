@@ -142,6 +142,18 @@ int orc_motion(int nbody, double w1, const double *amas, const double *bdmp, con
                const double *mass, const double *blnr, const double *bqdr, const double *crs,
                const double *kstf, double *dspl);
 
+
+/* ---- NSYS = 2 (one symmetry plane: isx = 1 mirrors x, else y) on the half
+ * mesh; outputs as orc_solve_frequency; phi2 (may be NULL): N x (6nb+nbeta) x 2
+ * complex, the potentials of the mesh half and of its mirror. ---- */
+int orc_solve_frequency_sym(int nelem, int nnode, const double *node_xyz, const int *ncn,
+                            const int *ncon, const double *cen, const double *area,
+                            const double *pnsz, const double *dxyz, int nbody,
+                            const int *body_start, int isx, double w1, double h, int nbeta,
+                            const double *beta, double amp, double xw0, double xw1, double rho,
+                            int isol, double *amas, double *bdmp, double *exfc, double *phi2,
+                            int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,10 @@ def _load(path):
                                        C.c_int]
         o.orc_motion.restype = C.c_int
         o.orc_motion.argtypes = [C.c_int, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        o.orc_solve_frequency_sym.restype = C.c_int
+        o.orc_solve_frequency_sym.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip,
+                                              C.c_int, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp,
+                                              _dp, C.c_int]
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
@@ -293,3 +297,20 @@ def forces(m, phi, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rh
     lib().orc_forces(m.nelem, _p(ph), _p(m.cen), _p(m.area), _p(m.dxyz), m.nbody, _p(m.body_start, C.c_int),
                      len(beta), _p(beta), v, w1, wk, h, amp, xw[0], xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc))
     return amas, bdmp, exfc[:, :len(beta)]
+
+
+def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0,
+                        isol=1, nthreads: int = 8, potentials: bool = False):
+    """NSYS = 2 on the half mesh m (isx = 1: mirrored in x, else in y)."""
+    nd = 6 * m.nbody
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    amas = np.zeros((nd, nd), order="F")
+    bdmp = np.zeros((nd, nd), order="F")
+    exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
+    phi2 = np.zeros((m.nelem, nd + len(beta), 2), dtype=np.complex128, order="F")
+    info = lib().orc_solve_frequency_sym(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+                                         _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
+                                         _p(m.body_start, C.c_int), int(isx), w1, h, len(beta), _p(beta), amp, xw[0],
+                                         xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), _p(phi2), nthreads)
+    out = (amas, bdmp, exfc[:, :len(beta)], info)
+    return out + (phi2,) if potentials else out
