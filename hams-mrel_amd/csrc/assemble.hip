@@ -92,7 +92,7 @@ __device__ __forceinline__ void pair_entry(const MeshDev& m, const FreqDev& f, i
   const int nf = m.nf;
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
   const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
-  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
+  const int flag = near_flag(m.gx[i], m.gy[i], m.gz[i], xj, yj, zj, m.pnsz[j]);
   double sij = 0.0, dij[3] = {0.0, 0.0, 0.0};
   if (flag) d_newman(m.frame + (size_t)kFrameLen * j, m.ncn[j], xi, yi, zi, sij, dij);
   PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
@@ -161,7 +161,7 @@ constexpr unsigned kNoPair = 0xFFFFFFFFu;
 __device__ __forceinline__ int pair_key(const MeshDev& m, double h, int i, int j) {
   const double xj = m.cx[j], yj = m.cy[j], zj = m.cz[j];
   const double xi = m.fx[i], yi = m.fy[i], zi = m.fz[i];
-  const int flag = near_flag(xi, yi, zi, xj, yj, zj, m.pnsz[j]);
+  const int flag = near_flag(m.gx[i], m.gy[i], m.gz[i], xj, yj, zj, m.pnsz[j]);
   PairGeom p = pair_geom(xj, xi, yj, yi, zj, zi);
   const int sub = fin_subkey(p.rr, h);
   if (sub >= kSubPidcock && i < m.n && i != j) {
@@ -372,27 +372,36 @@ __global__ void k_rhs_reduce(int n, int nbody, int nbeta, int nchunk, const int*
 //             BDMP(6M+k, r) = -sum Re(i rho w phi DS n_k)
 //   r >= 6nb: EXFC(6M+k, beta) = i w rho sum (phi_D [+ phi_I]) DS n_k
 // ---------------------------------------------------------------------------
+// With a symmetry plane (mirror 0: x -> -x, 1: y -> -y) the mirror half's
+// potentials Xm add their terms with the RX / RY signs of the normals and
+// VINP at the mirrored centroids (the IP = 2 terms of RFORCE/EFORCE_MULTI).
 __global__ __launch_bounds__(256) void k_forces(MeshDev m, FreqDev f, WaveDev w, int nbody, const int* body_start,
-                                                const double2* __restrict__ X, double* __restrict__ out) {
+                                                const double2* __restrict__ X, const double2* __restrict__ Xm,
+                                                int mirror, double* __restrict__ out) {
   __shared__ double red[12][256];
   const int bm = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
   const int n = m.n, nd = 6 * nbody;
   const bool rad = r < nd;
   double acc[12];
   for (int k = 0; k < 12; k++) acc[k] = 0.0;
+  const int halves = mirror >= 0 ? 2 : 1;
   for (int e = body_start[bm] + t; e < body_start[bm + 1]; e += 256) {
-    double2 xv = X[(size_t)r * n + e];
-    cd phi{xv.x, xv.y};
-    if (!rad && w.isol == 1) phi = phi + d_vinp(m.cx[e], m.cy[e], m.cz[e], f, w, w.beta[r - nd]);
     const double ds = m.area[e];
-    for (int k = 0; k < 6; k++) {
-      double nk = m.nrm[(size_t)k * n + e];
-      if (rad) {
-        acc[k] += w.rho * phi.r * ds * nk;            // Im(i rho phi DS n_k)
-        acc[6 + k] += w.rho * f.w1 * phi.i * ds * nk;  // -Re(i rho w phi DS n_k)
-      } else {
-        acc[k] += phi.r * ds * nk;
-        acc[6 + k] += phi.i * ds * nk;
+    for (int ip = 0; ip < halves; ip++) {
+      const double r1 = (ip == 1 && mirror == 0) ? -1.0 : 1.0, r2 = (ip == 1 && mirror == 1) ? -1.0 : 1.0;
+      const double sg[6] = {r1, r2, 1.0, r2, r1, r1 * r2};
+      double2 xv = (ip == 0 ? X : Xm)[(size_t)r * n + e];
+      cd phi{xv.x, xv.y};
+      if (!rad && w.isol == 1) phi = phi + d_vinp(r1 * m.cx[e], r2 * m.cy[e], m.cz[e], f, w, w.beta[r - nd]);
+      for (int k = 0; k < 6; k++) {
+        double nk = sg[k] * m.nrm[(size_t)k * n + e];
+        if (rad) {
+          acc[k] += w.rho * phi.r * ds * nk;            // Im(i rho phi DS n_k)
+          acc[6 + k] += w.rho * f.w1 * phi.i * ds * nk;  // -Re(i rho w phi DS n_k)
+        } else {
+          acc[k] += phi.r * ds * nk;
+          acc[6 + k] += phi.i * ds * nk;
+        }
       }
     }
   }
@@ -428,13 +437,17 @@ __global__ __launch_bounds__(256) void k_forces(MeshDev m, FreqDev f, WaveDev w,
 // W(j, 6nb+k) = -DINCP(j, k), X the solved potentials MXPOT -- the sum over
 // JEL of TERM1 - TERM2 (:151-200) is the product M Z.
 // ---------------------------------------------------------------------------
+// With a symmetry plane the mirror half's column set takes the mode signs of
+// the plane (flip: bit m set when mode m + 1 changes sign, the SX / SY factors
+// of :133-185), DINCP at the mirrored sources and the mirror half's MXPOT.
 __global__ void k_field_weights(MeshDev m, int nbody, const double2* __restrict__ dincp,
-                                const double2* __restrict__ X, double2* __restrict__ Z) {
+                                const double2* __restrict__ X, double2* __restrict__ Z, int flip) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y, n = m.n, nd = 6 * nbody;
   if (j >= n) return;
   double2 w;
   if (col < nd) {
-    w = make_double2(m.body[j] == col / 6 ? m.nrm[(size_t)(col % 6) * n + j] : 0.0, 0.0);
+    const double sg = (flip >> (col % 6)) & 1 ? -1.0 : 1.0;
+    w = make_double2(m.body[j] == col / 6 ? sg * m.nrm[(size_t)(col % 6) * n + j] : 0.0, 0.0);
   } else {
     const double2 d = dincp[(size_t)(col - nd) * n + j];
     w = make_double2(-d.x, -d.y);
@@ -664,6 +677,85 @@ __global__ __launch_bounds__(256) void k_motion(int nd, double w1, const double*
   if (t == 0) iters[b] = steps;
 }
 
+// ---------------------------------------------------------------------------
+// Geometric symmetry (NSYS = 2): with T2 / S2 the entries of the mirrored
+// field points (CalGreenFuncMulti.f90:141-177, IS = 2), the two systems of
+// ASSB_LEFT_MULTI (AssbMatxMulti.f90:66-80; RXY = [1 1; 1 -1]):
+// A1 = A + T2, A2 = A - T2 (A holds the 2 pi diagonal), and the single-layer
+// combinations S+ = S + S2, S- = S - S2 that carry the right-hand sides.
+// ---------------------------------------------------------------------------
+__global__ void k_sym_combine(double2* __restrict__ A, double2* __restrict__ T2, double2* __restrict__ S,
+                              double2* __restrict__ S2, size_t count) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = A[k], t = T2[k], s = S[k], u = S2[k];
+    A[k] = make_double2(a.x + t.x, a.y + t.y);
+    T2[k] = make_double2(a.x - t.x, a.y - t.y);
+    S[k] = make_double2(s.x + u.x, s.y + u.y);
+    S2[k] = make_double2(s.x - u.x, s.y - u.y);
+  }
+}
+
+// DINCP at the mirror image of panel j: DINP at the mirrored centroid, the
+// mirrored normal component negated (DBC_MULTI_RIGHT, BodyIntgrMulti.f90:150-170
+// with IS != IP).  mirror 0: x -> -x, 1: y -> -y.
+__global__ void k_dum2_mirror(MeshDev m, FreqDev f, WaveDev w, int mirror, double2* dum2) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (j >= m.n) return;
+  const double sx = mirror == 0 ? -1.0 : 1.0, sy = mirror == 1 ? -1.0 : 1.0;
+  cd d[3];
+  d_dinp(sx * m.cx[j], sy * m.cy[j], m.cz[j], f, w, w.beta[b], d);
+  const cd v = (sx * d[0]) * m.nrm[j] + (sy * d[1]) * m.nrm[m.n + j] + d[2] * m.nrm[2 * m.n + j];
+  dum2[(size_t)b * m.n + j] = make_double2(v.r, v.i);
+}
+
+// The right-hand-side generators of the two systems (ASSB_RBC/DBC_MULTI with
+// the RXY combination, AssbMatxMulti.f90:124-251): radiation mode m of body b
+// is symmetric (system 1) or antisymmetric (system 2) under the mirror --
+// x-mirror: sway, heave, roll symmetric; y-mirror: surge, heave, pitch -- and
+// enters its system with weight 2 n_m on body b's panels; heading k enters
+// system 1 with -(DINCP + DINCP_mirror) and system 2 with -(DINCP - DINCP_mirror).
+__global__ void k_sym_weights(MeshDev m, int nbody, int nbeta, int mirror, const double2* __restrict__ dinc,
+                              const double2* __restrict__ dincm, double2* __restrict__ W1,
+                              double2* __restrict__ W2) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y, n = m.n, nd = 6 * nbody;
+  if (j >= n) return;
+  double2 w1, w2;
+  if (col < nd) {
+    const int md = col % 6;
+    const bool sym = mirror == 0 ? (md == 1 || md == 2 || md == 3) : (md == 0 || md == 2 || md == 4);
+    const double v = m.body[j] == col / 6 ? 2.0 * m.nrm[(size_t)md * n + j] : 0.0;
+    w1 = make_double2(sym ? v : 0.0, 0.0);
+    w2 = make_double2(sym ? 0.0 : v, 0.0);
+  } else {
+    const double2 a = dinc[(size_t)(col - nd) * n + j], b = dincm[(size_t)(col - nd) * n + j];
+    w1 = make_double2(-(a.x + b.x), -(a.y + b.y));
+    w2 = make_double2(-(a.x - b.x), -(a.y - b.y));
+  }
+  W1[(size_t)col * n + j] = w1;
+  W2[(size_t)col * n + j] = w2;
+}
+
+// MXPOT(:, col, IP) = (X1 +- X2) / 2 (RADIATION/DIFFRACTION_SOLVER_MULTI,
+// AssbMatxMulti.f90:306-316, :373-381): X <- the mesh half, X2 <- the mirror.
+__global__ void k_sym_recombine(double2* __restrict__ X, double2* __restrict__ X2, size_t count) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = X[k], b = X2[k];
+    X[k] = make_double2((a.x + b.x) / 2.0, (a.y + b.y) / 2.0);
+    X2[k] = make_double2((a.x - b.x) / 2.0, (a.y - b.y) / 2.0);
+  }
+}
+
+// The mirror images of count points (mirror 0: x -> -x, 1: y -> -y).
+__global__ void k_reflect_points(int count, int mirror, const double* __restrict__ px, const double* __restrict__ py,
+                                 const double* __restrict__ pz, double* __restrict__ qx, double* __restrict__ qy,
+                                 double* __restrict__ qz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  qx[k] = mirror == 0 ? -px[k] : px[k];
+  qy[k] = mirror == 1 ? -py[k] : py[k];
+  qz[k] = pz[k];
+}
+
 __global__ void k_clear_flag(int* flags) { flags[0] = 0; }
 
 namespace launch {
@@ -743,18 +835,29 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
             nchunk, chunk_body_d, partial, X);
 }
 
-void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out) {
-  if (w.nbeta > 0) HX_LAUNCH(k_dum2, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, out);
+void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out, int mirror) {
+  if (w.nbeta == 0) return;
+  if (mirror < 0)
+    HX_LAUNCH(k_dum2, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, out);
+  else
+    HX_LAUNCH(k_dum2_mirror, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, mirror, out);
+}
+
+void reflect_points(hipStream_t s, int count, int mirror, const double* px, const double* py, const double* pz,
+                    double* qx, double* qy, double* qz) {
+  if (count > 0)
+    HX_LAUNCH(k_reflect_points, dim3((count + 255) / 256), dim3(256), 0, s, count, mirror, px, py, pz, qx, qy, qz);
 }
 
 void field_weights(hipStream_t s, const MeshDev& m, int nbody, int nbeta, const double2* dincp, const double2* X,
-                   double2* Z) {
-  HX_LAUNCH(k_field_weights, dim3((m.n + 255) / 256, 6 * nbody + nbeta), dim3(256), 0, s, m, nbody, dincp, X, Z);
+                   double2* Z, int flip) {
+  HX_LAUNCH(k_field_weights, dim3((m.n + 255) / 256, 6 * nbody + nbeta), dim3(256), 0, s, m, nbody, dincp, X, Z,
+            flip);
 }
 
 void on_body(hipStream_t s, const MeshDev& m, const double* vtx, int nfp, const double* px, const double* py,
-             const double* pz, int* onb) {
-  HX_LAUNCH(k_fill_int, dim3((nfp + 255) / 256), dim3(256), 0, s, onb, nfp);
+             const double* pz, int* onb, bool accumulate) {
+  if (!accumulate) HX_LAUNCH(k_fill_int, dim3((nfp + 255) / 256), dim3(256), 0, s, onb, nfp);
   HX_LAUNCH(k_on_body, dim3((m.n + 255) / 256, nfp), dim3(256), 0, s, m, vtx, nfp, px, py, pz, onb);
 }
 
@@ -771,9 +874,28 @@ void motions(hipStream_t s, int nd, int nbeta, double w1, const double* coef, co
 }
 
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
-            const double2* X, double* out) {
-  HX_LAUNCH(k_forces, dim3(nbody, 6 * nbody + w.nbeta), dim3(256), 0, s, m, f, w, nbody, body_start_d, X,
-                     out);
+            const double2* X, double* out, const double2* Xm, int mirror) {
+  HX_LAUNCH(k_forces, dim3(nbody, 6 * nbody + w.nbeta), dim3(256), 0, s, m, f, w, nbody, body_start_d, X, Xm,
+            mirror, out);
+}
+
+void sym_combine(hipStream_t s, double2* A, double2* T2, double2* S, double2* S2, size_t count) {
+  HX_LAUNCH(k_sym_combine, dim3(4096), dim3(256), 0, s, A, T2, S, S2, count);
+}
+
+void sym_weights(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, int mirror,
+                 double2* dinc, double2* W1, double2* W2) {
+  if (w.nbeta > 0) {
+    HX_LAUNCH(k_dum2, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, dinc);
+    HX_LAUNCH(k_dum2_mirror, dim3((m.n + 255) / 256, w.nbeta), dim3(256), 0, s, m, f, w, mirror,
+              dinc + (size_t)m.n * w.nbeta);
+  }
+  HX_LAUNCH(k_sym_weights, dim3((m.n + 255) / 256, 6 * nbody + w.nbeta), dim3(256), 0, s, m, nbody, w.nbeta, mirror,
+            dinc, dinc + (size_t)m.n * w.nbeta, W1, W2);
+}
+
+void sym_recombine(hipStream_t s, double2* X, double2* X2, size_t count) {
+  HX_LAUNCH(k_sym_recombine, dim3(1024), dim3(256), 0, s, X, X2, count);
 }
 
 }  // namespace launch

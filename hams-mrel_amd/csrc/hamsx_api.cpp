@@ -286,7 +286,7 @@ struct hamsx_ctx {
   unsigned long long field_gen = 0;
   DevBuf<double> fpx, fpy, fpz, vtx, ffx, ffy, ffz, fmodetab;
   PairBins fbins;
-  DevBuf<double2> fM, fZ, fP, fOut, fInc, fDincp;
+  DevBuf<double2> fM, fZ, fZ2, fP, fOut, fInc, fDincp;
   DevBuf<int> fOnb;
   // motion equation: the block-diagonal body matrices MATX, BLNR, BQDR, CRS,
   // KSTF (6nb x 6nb each) for bodymat_nbody bodies; motion_quad: max |BQDR| >= 1e-6
@@ -295,6 +295,16 @@ struct hamsx_ctx {
   DevBuf<double2> mwork, mdspl;
   DevBuf<int> miters;
   DevBuf<double> modetab;  // per-frequency finite-depth mode tables
+  // geometric symmetry, one plane (NSYS = 2): sym_mirror 0 = x = 0 (ISX), 1 = y = 0
+  // (ISY), -1 none.  The mirror view's points mview = [fx | fy | fz | gx | gy | gz],
+  // 2n each: rows n .. 2n - 1 are the mirrored hull centroids, FLAG taken at the
+  // unmirrored ones.  T2 / S2: the mirrored entries, then system 2's matrix and S-;
+  // W1 / W2 the two systems' generator columns; X2 the mirror half's potentials.
+  int sym_mirror = -1;
+  DevBuf<double> mview, mmodetab;
+  PairBins mbins;
+  DevBuf<double2> T2, S2, X2, W1, W2, sdinc;
+  DevBuf<int> ipiv2, perm2;
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
   DevBuf<double> gNorm, gScale;
@@ -320,8 +330,23 @@ struct hamsx_ctx {
   int stage_cnt[HAMSX_NSTAGE] = {0};
 
   hamsx::MeshDev mesh() const {
-    return hamsx::MeshDev{n, nf, fx.p, fy.p, fz.p, cx.p, cy.p, cz.p, area.p, pnsz.p, nrm.p, ncn.p, frame.p, body.p};
+    return hamsx::MeshDev{n,      nf,     fx.p,   fy.p, fz.p, cx.p, cy.p, cz.p, area.p, pnsz.p,
+                          nrm.p,  ncn.p,  frame.p, body.p, 0,   fx.p, fy.p, fz.p};
   }
+  hamsx::MeshDev mirror_view() const {
+    hamsx::MeshDev m = mesh();
+    const size_t n2 = 2 * (size_t)n;
+    m.fx = mview.p;
+    m.fy = mview.p + n2;
+    m.fz = mview.p + 2 * n2;
+    m.gx = mview.p + 3 * n2;
+    m.gy = mview.p + 4 * n2;
+    m.gz = mview.p + 5 * n2;
+    m.nf = 2 * n;
+    m.row0 = n;
+    return m;
+  }
+  bool sym() const { return sym_mirror >= 0; }
   hamsx::WaveDev waves() const { return hamsx::WaveDev{nbeta, beta.p, amp, xw0, xw1, rho, isol}; }
   int nrhs() const { return 6 * nbody + nbeta; }
   size_t out_len() const { return 2 * (size_t)(6 * nbody) * (6 * nbody) + 2 * (size_t)(6 * nbody) * nbeta; }
@@ -471,6 +496,9 @@ int ensure_work(hamsx_ctx* c) {
       (rc = c->partial.ensure((size_t)c->nchunk * (6 + c->nbeta) * std::max<size_t>(n, c->nf))) ||
       (rc = c->out.ensure(c->out_len() + 8)))
     return rc;
+  if (c->sym() && ((rc = c->X2.ensure(n * nr)) || (rc = c->W1.ensure(n * nr)) || (rc = c->W2.ensure(n * nr)) ||
+                   (rc = c->sdinc.ensure(2 * n * std::max(1, c->nbeta)))))
+    return rc;
   if (c->prec == HAMSX_PREC_MIXED) {
     if ((rc = c->B.ensure(n * nr)) || (rc = c->R.ensure(n * nr)) || (rc = c->D.ensure(n * nr)) ||
         (rc = c->Af.ensure(n * n)) || (rc = c->norms.ensure(2 * nr)) ||
@@ -496,6 +524,21 @@ int do_rhs(hamsx_ctx* c) {
   int rc = ensure_work(c);
   if (rc) return rc;
   StageTimer tm(c, 1);
+  if (c->sym()) {
+    // the two systems' right-hand sides: X = S+ W1, X2 = S- W2
+    hamsx::launch::sym_weights(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->sym_mirror, c->sdinc.p,
+                               c->W1.p, c->W2.p);
+    if ((rc = check_launch()) || (rc = blas_handle(c))) return rc;
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    for (int ip = 0; ip < 2; ip++)
+      if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->n, c->nrhs(), c->n, &one,
+                        reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->S.p : c->S2.p), c->n,
+                        reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->W1.p : c->W2.p), c->n, &zero,
+                        reinterpret_cast<rocblas_double_complex*>(ip == 0 ? c->X.p : c->X2.p),
+                        c->n) != rocblas_status_success)
+        return fail(HAMSX_EDEVICE, "rocblas_zgemm (symmetric right-hand sides) failed");
+    return 0;
+  }
   const bool lid = c->nlid > 0;
   if (lid && (rc = c->Xe.ensure((size_t)c->nf * c->nrhs()))) return rc;
   hamsx::launch::rhs(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start.data(), c->body_start_d.p,
@@ -555,7 +598,18 @@ int do_factor(hamsx_ctx* c) {
       (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)) ||
       (rc = c->ludiag.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB)))
     return rc;
+  if (c->sym() && ((rc = c->ipiv2.ensure(n)) || (rc = c->perm2.ensure(n)))) return rc;
   StageTimer tm(c, 2);
+  if (c->sym()) {  // the two systems A +- T2 (INFO of system 2 in flags[5])
+    if (c->prec != HAMSX_PREC_FP64) return fail(HAMSX_EARG, "a symmetry plane needs FP64 precision");
+    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
+                             c->ludiag.p, lu_lookahead(c));
+    hamsx::launch::lu_factor(c->stream, c->T2.p, n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p, c->flags.p + 5,
+                             c->ludiag.p, lu_lookahead(c));
+    c->have_A = false;
+    c->have_LU = true;
+    return check_launch();
+  }
   if ((rc = ensure_normal_matrix(c))) return rc;
   double2* M = system_matrix(c);
   if (c->prec == HAMSX_PREC_FP64) {
@@ -988,8 +1042,20 @@ int do_solve_gmres(hamsx_ctx* c) {
 // X <- A^{-1} X for all 6nb + nbeta columns (with refinement in mixed mode)
 int do_solve(hamsx_ctx* c) {
   if (c->have_X) return 0;
+  if (c->sym() && (c->solver != HAMSX_SOLVER_LU || c->prec != HAMSX_PREC_FP64))
+    return fail(HAMSX_EARG, "a symmetry plane needs the FP64 LU solver");
   int rc = do_rhs(c);
   if (rc) return rc;
+  if (c->sym()) {
+    if ((rc = do_factor(c))) return rc;
+    const int n = c->n, nr = c->nrhs();
+    StageTimer tm(c, 3);
+    hamsx::launch::lu_solve(c->stream, c->A.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    hamsx::launch::lu_solve(c->stream, c->T2.p, n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
+    hamsx::launch::sym_recombine(c->stream, c->X.p, c->X2.p, (size_t)n * nr);
+    c->have_X = true;
+    return check_launch();
+  }
   if (c->solver == HAMSX_SOLVER_GMRES) {
     StageTimer tm(c, 3);
     if ((rc = do_solve_gmres(c))) return rc;
@@ -1028,8 +1094,11 @@ int do_solve(hamsx_ctx* c) {
 
 int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
   if (!dst || ncol <= 0) return 0;
-  HX_TRY(hipMemcpyAsync(dst, c->X.p + (size_t)c->n * col0, sizeof(double2) * (size_t)c->n * ncol,
-                        hipMemcpyDeviceToHost, c->stream));
+  const size_t len = (size_t)c->n * ncol;
+  HX_TRY(hipMemcpyAsync(dst, c->X.p + (size_t)c->n * col0, sizeof(double2) * len, hipMemcpyDeviceToHost, c->stream));
+  if (c->sym())  // MXPOT(:, :, 2): the mirror half
+    HX_TRY(hipMemcpyAsync(dst + 2 * len, c->X2.p + (size_t)c->n * col0, sizeof(double2) * len,
+                          hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1038,20 +1107,20 @@ int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
 // refinement norms into the pinned staging area (no host wait).
 int enqueue_forces(hamsx_ctx* c) {
   const size_t len = c->out_len();
-  // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 2 ints][norms: 2 * nrhs]
-  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 2 + 2 * (size_t)c->nrhs());
+  // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 6 ints][norms: 2 * nrhs]
+  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 3 + 2 * (size_t)c->nrhs());
   if (rc) return rc;
   double* hout = c->h_pinned + hamsx::kNK + c->nbeta;
   {
     StageTimer tm(c, 4);
     hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p,
-                          c->out.p);
+                          c->out.p, c->sym() ? c->X2.p : nullptr, c->sym_mirror);
   }
   if ((rc = check_launch())) return rc;
   HX_TRY(hipMemcpyAsync(hout, c->out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
-  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 6, hipMemcpyDeviceToHost, c->stream));
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0)
-    HX_TRY(hipMemcpyAsync(hout + len + 2, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
+    HX_TRY(hipMemcpyAsync(hout + len + 3, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
                           c->stream));
   return 0;
 }
@@ -1066,12 +1135,12 @@ int complete_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int*
   if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
   if (bdmp) std::memcpy(bdmp, hout + (size_t)nd * nd, sizeof(double) * nd * nd);
   if (exfc) std::memcpy(exfc, hout + 2 * (size_t)nd * nd, sizeof(double) * 2 * nd * c->nbeta);
-  int fl[4];  // quadrature overflow, INFO, refinement converged, correction steps
+  int fl[6];  // quadrature overflow, INFO, refinement converged, correction steps, -, INFO of symmetry system 2
   std::memcpy(fl, hout + len, sizeof fl);
-  if (info_out) *info_out = fl[1];
+  if (info_out) *info_out = fl[1] ? fl[1] : (c->sym() ? fl[5] : 0);
   bool noconv = false;
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0 && c->solver == HAMSX_SOLVER_LU) {
-    const double* nm = hout + len + 2;
+    const double* nm = hout + len + 3;
     double worst = 0.0;
     for (int q = 0; q < c->nrhs(); q++) {
       double rn = nm[q], xn = nm[c->nrhs() + q];
@@ -1146,13 +1215,13 @@ int hamsx_create(hamsx_ctx** out, int device) {
     hamsx_destroy(c);
     return fail(HAMSX_EDEVICE, "stream/event creation failed on device %d", device);
   }
-  int rc = c->flags.ensure(5);
+  int rc = c->flags.ensure(6);
   if (rc == 0) rc = c->wvn.ensure(hamsx::kNK);
   if (rc) {
     hamsx_destroy(c);
     return rc;
   }
-  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 5, c->stream) != hipSuccess) {
+  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 6, c->stream) != hipSuccess) {
     hamsx_destroy(c);
     return fail(HAMSX_EDEVICE, "flag initialisation failed");
   }
@@ -1409,6 +1478,8 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   c->nfp = 0;
   c->bodymat_nbody = 0;
   c->field_gen++;
+  c->sym_mirror = -1;
+  c->mbins.n = 0;
   c->nlid = 0;
   c->nf = nelem;
   if ((rc = c->fx.ensure(n)) || (rc = c->fy.ensure(n)) || (rc = c->fz.ensure(n))) return rc;
@@ -1424,6 +1495,7 @@ int hamsx_set_lid(hamsx_ctx* c, int nlid, const double* lid_cen) {
   if (c->pending) return busy(c);
   if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_lid needs hamsx_set_mesh first");
   if (nlid < 0 || (nlid > 0 && !lid_cen)) return fail(HAMSX_EARG, "bad lid arguments");
+  if (nlid > 0 && c->sym()) return fail(HAMSX_EARG, "a lid with a symmetry plane is not supported");
   if (c->n + nlid > 0xFFFF) return fail(HAMSX_EARG, "hull + lid panels exceed 65535");
   int rc = set_device(c);
   if (rc) return rc;
@@ -1451,7 +1523,7 @@ int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, 
   if (h > 0.0 && (!wvn || nk < hamsx::kNK)) return fail(HAMSX_EARG, "finite depth needs %d dispersion roots", hamsx::kNK);
   int rc = set_device(c);
   if (rc) return rc;
-  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 2 + 2 * (size_t)c->nrhs())))
+  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 3 + 2 * (size_t)c->nrhs())))
     return rc;
   // the staging slot is free: every earlier use ended in a stream synchronisation
   HX_TRY(hipStreamSynchronize(c->stream));
@@ -1509,8 +1581,65 @@ int hamsx_assemble(hamsx_ctx* c) {
       hamsx::launch::assemble(c->stream, c->mesh(), c->freq, c->A.p, c->S.p, c->flags.p);
   }
   if ((rc = check_launch())) return rc;
+  if (c->sym()) {
+    // the mirrored field points (IS = 2), then A +- T2 and S +- S2
+    const size_t nn = (size_t)c->n * c->n;
+    const hamsx::MeshDev mv = c->mirror_view();
+    if ((rc = c->T2.ensure(nn)) || (rc = c->S2.ensure(nn)) ||
+        (finite && (rc = c->mmodetab.ensure(hamsx::mode_tab_len(2 * c->n)))))
+      return rc;
+    const bool mbinned = finite && 2 * c->n <= 0xFFFF && !force_grid_assembly();
+    if (mbinned && (rc = ensure_bins(c, c->mbins, mv, 1 + (unsigned long long)c->sym_mirror))) return rc;
+    hamsx::FreqDev fm = c->freq;
+    fm.tab = finite ? c->mmodetab.p : nullptr;
+    StageTimer tm(c, 0);
+    if (finite && !limit) hamsx::launch::mode_tables(c->stream, mv, fm);
+    if (mbinned)
+      assemble_binned(c, c->mbins, mv, fm, c->T2.p, c->S2.p, c->flags.p, false);
+    else
+      hamsx::launch::assemble(c->stream, mv, fm, c->T2.p, c->S2.p, c->flags.p);
+    hamsx::launch::sym_combine(c->stream, c->A.p, c->T2.p, c->S.p, c->S2.p, nn);
+    if ((rc = check_launch())) return rc;
+  }
   c->have_A = c->have_S = true;
   c->have_LU = c->have_X = false;
+  return 0;
+}
+
+int hamsx_set_symmetry(hamsx_ctx* c, int isx, int isy) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_symmetry needs hamsx_set_mesh first");
+  if ((isx != 0 && isx != 1) || (isy != 0 && isy != 1)) return fail(HAMSX_EARG, "ISX and ISY must be 0 or 1");
+  if (isx && isy) return fail(HAMSX_EARG, "ISX and ISY cannot be simultaneously 1");
+  if ((isx || isy) && c->nlid > 0) return fail(HAMSX_EARG, "a symmetry plane with a lid is not supported");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HX_TRY(hipStreamSynchronize(c->stream));
+  const int mirror = isx ? 0 : (isy ? 1 : -1);
+  if (mirror >= 0) {
+    const size_t n = c->n, n2 = 2 * n;
+    std::vector<double> h(6 * n2);
+    for (int d = 0; d < 3; d++) {
+      HX_TRY(hipMemcpy(h.data() + d * n2, (d == 0 ? c->cx.p : d == 1 ? c->cy.p : c->cz.p), sizeof(double) * n,
+                       hipMemcpyDeviceToHost));
+      const double sg = d == mirror ? -1.0 : 1.0;
+      for (size_t i = 0; i < n; i++) {
+        const double v = h[d * n2 + i];
+        h[d * n2 + n + i] = sg * v;           // field point: the mirrored centroid
+        h[(3 + d) * n2 + i] = v;               // FLAG point: the centroid itself
+        h[(3 + d) * n2 + n + i] = v;
+      }
+    }
+    if ((rc = c->mview.ensure(6 * n2))) return rc;
+    HX_TRY(hipMemcpy(c->mview.p, h.data(), sizeof(double) * 6 * n2, hipMemcpyHostToDevice));
+  }
+  if (mirror != c->sym_mirror) {
+    c->sym_mirror = mirror;
+    c->mbins.n = 0;
+    c->field_gen++;  // the field rows change layout
+    c->have_A = c->have_S = c->have_LU = c->have_X = false;
+  }
   return 0;
 }
 
@@ -1630,19 +1759,30 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
   const int n = c->n, nfp = c->nfp, nd = 6 * c->nbody, nr = c->nrhs(), nbeta = c->nbeta;
   const bool finite = c->freq.h >= 0.0;
   const bool binned = finite && n < 0xFFFF && !force_grid_assembly();
-  const int F = field_chunk(c, binned);
+  // with a symmetry plane every point brings its mirror image: images = 2
+  // rows per point, and the mirror rows meet the second column set Z2
+  const int images = c->sym() ? 2 : 1;
+  const int F = std::max(1, field_chunk(c, binned) / images);
   const bool want_inc = incident && nbeta > 0;
-  if ((rc = c->fDincp.ensure((size_t)n * std::max(1, nbeta))) || (rc = c->fZ.ensure((size_t)2 * n * nr)) ||
-      (rc = c->fM.ensure((size_t)2 * n * F)) || (rc = c->fP.ensure((size_t)F * nr)) ||
-      (rc = c->fOut.ensure((size_t)nfp * nr)) || (rc = c->fOnb.ensure(nfp)) || (rc = c->ffx.ensure(n + F)) ||
-      (rc = c->ffy.ensure(n + F)) || (rc = c->ffz.ensure(n + F)) ||
-      (want_inc && (rc = c->fInc.ensure((size_t)nfp * nbeta))) ||
-      (finite && (rc = c->fmodetab.ensure(hamsx::mode_tab_len(n + F)))) || (rc = blas_handle(c)))
+  const size_t dn = (size_t)n * std::max(1, nbeta);
+  if ((rc = c->fDincp.ensure(dn * images)) || (rc = c->fZ.ensure((size_t)2 * n * nr)) ||
+      (c->sym() && (rc = c->fZ2.ensure((size_t)2 * n * nr))) ||
+      (rc = c->fM.ensure((size_t)2 * n * images * F)) || (rc = c->fP.ensure((size_t)F * nr)) ||
+      (rc = c->fOut.ensure((size_t)nfp * nr)) || (rc = c->fOnb.ensure(nfp)) ||
+      (rc = c->ffx.ensure(n + images * F)) || (rc = c->ffy.ensure(n + images * F)) ||
+      (rc = c->ffz.ensure(n + images * F)) || (want_inc && (rc = c->fInc.ensure((size_t)nfp * nbeta))) ||
+      (finite && (rc = c->fmodetab.ensure(hamsx::mode_tab_len(n + images * F)))) || (rc = blas_handle(c)))
     return rc;
   const hamsx::MeshDev m = c->mesh();
   StageTimer tm(c, 9);
   hamsx::launch::dincp(c->stream, m, c->freq, c->waves(), c->fDincp.p);
   hamsx::launch::field_weights(c->stream, m, c->nbody, nbeta, c->fDincp.p, c->X.p, c->fZ.p);
+  if (c->sym()) {
+    // antisymmetric modes: x = 0: surge, pitch, yaw; y = 0: sway, roll, yaw
+    const int flip = c->sym_mirror == 0 ? 0x31 : 0x2A;
+    hamsx::launch::dincp(c->stream, m, c->freq, c->waves(), c->fDincp.p + dn, c->sym_mirror);
+    hamsx::launch::field_weights(c->stream, m, c->nbody, nbeta, c->fDincp.p + dn, c->X2.p, c->fZ2.p, flip);
+  }
   if ((rc = check_launch())) return rc;
   HX_TRY(hipMemcpyAsync(c->ffx.p, c->cx.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   HX_TRY(hipMemcpyAsync(c->ffy.p, c->cy.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1656,15 +1796,20 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
     HX_TRY(hipMemcpyAsync(c->ffx.p + n, c->fpx.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
     HX_TRY(hipMemcpyAsync(c->ffy.p + n, c->fpy.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
     HX_TRY(hipMemcpyAsync(c->ffz.p + n, c->fpz.p + p0, sizeof(double) * nc, hipMemcpyDeviceToDevice, c->stream));
-    // the field view: sources are the hull panels, rows n .. n + nc - 1 the chunk's points
+    if (c->sym())
+      hamsx::launch::reflect_points(c->stream, nc, c->sym_mirror, c->fpx.p + p0, c->fpy.p + p0, c->fpz.p + p0,
+                                    c->ffx.p + n + nc, c->ffy.p + n + nc, c->ffz.p + n + nc);
+    // the field view: sources are the hull panels, rows n .. n + nc - 1 the
+    // chunk's points (then their mirror images); FLAG from the point itself
+    const int rows = images * nc;
     hamsx::MeshDev fm = m;
-    fm.fx = c->ffx.p;
-    fm.fy = c->ffy.p;
-    fm.fz = c->ffz.p;
-    fm.nf = n + nc;
+    fm.fx = fm.gx = c->ffx.p;
+    fm.fy = fm.gy = c->ffy.p;
+    fm.fz = fm.gz = c->ffz.p;
+    fm.nf = n + rows;
     fm.row0 = n;
     double2* S = c->fM.p;
-    double2* T = c->fM.p + (size_t)nc * n;
+    double2* T = c->fM.p + (size_t)rows * n;
     if (finite && c->freq.v != 0.0 && c->freq.v != -1.0) hamsx::launch::mode_tables(c->stream, fm, ff);
     if (binned) {
       if ((rc = ensure_bins(c, c->fbins, fm, (c->field_gen << 20) | (unsigned long long)chunk))) return rc;
@@ -1673,12 +1818,18 @@ int hamsx_field_values(hamsx_ctx* c, int kind, double* values, double* incident,
       hamsx::launch::assemble(c->stream, fm, ff, T, S, ovf);
     }
     hamsx::launch::on_body(c->stream, m, c->vtx.p, nc, c->fpx.p + p0, c->fpy.p + p0, c->fpz.p + p0, c->fOnb.p + p0);
+    if (c->sym())  // the free term counts the mirror image too
+      hamsx::launch::on_body(c->stream, m, c->vtx.p, nc, c->ffx.p + n + nc, c->ffy.p + n + nc, c->ffz.p + n + nc,
+                             c->fOnb.p + p0, true);
     if ((rc = check_launch())) return rc;
-    if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nc, nr, 2 * n, &one,
-                      reinterpret_cast<const rocblas_double_complex*>(c->fM.p), nc,
-                      reinterpret_cast<const rocblas_double_complex*>(c->fZ.p), 2 * n, &zero,
-                      reinterpret_cast<rocblas_double_complex*>(c->fP.p), nc) != rocblas_status_success)
-      return fail(HAMSX_EDEVICE, "rocblas_zgemm (field points) failed");
+    // P = M1 Z1 (+ M2 Z2): M = [S | T] with leading dimension rows, M2 its mirror rows
+    for (int im = 0; im < images; im++)
+      if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, nc, nr, 2 * n, &one,
+                        reinterpret_cast<const rocblas_double_complex*>(c->fM.p + (size_t)im * nc), rows,
+                        reinterpret_cast<const rocblas_double_complex*>(im == 0 ? c->fZ.p : c->fZ2.p), 2 * n,
+                        im == 0 ? &zero : &one, reinterpret_cast<rocblas_double_complex*>(c->fP.p),
+                        nc) != rocblas_status_success)
+        return fail(HAMSX_EDEVICE, "rocblas_zgemm (field points) failed");
     hamsx::launch::field_finish(c->stream, nc, c->fpx.p + p0, c->fpy.p + p0, c->fpz.p + p0, c->fOnb.p + p0, c->freq,
                                 c->waves(), nd, kind, c->fP.p, (size_t)nfp, c->fOut.p + p0,
                                 want_inc ? c->fInc.p + p0 : nullptr);
@@ -1742,7 +1893,8 @@ int hamsx_motions(hamsx_ctx* c, double* dspl, int* iters) {
   if ((rc = c->mwork.ensure((size_t)nd * nd * nbeta)) || (rc = c->mdspl.ensure((size_t)nd * nbeta)) ||
       (rc = c->miters.ensure(nbeta)))
     return rc;
-  hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p, c->out.p);
+  hamsx::launch::forces(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->body_start_d.p, c->X.p, c->out.p,
+                        c->sym() ? c->X2.p : nullptr, c->sym_mirror);
   hamsx::launch::motions(c->stream, nd, nbeta, c->freq.w1, c->out.p, c->bmat.p, c->motion_quad, c->mwork.p,
                          c->mdspl.p, c->miters.p);
   if ((rc = check_launch())) return rc;

@@ -36,6 +36,8 @@ struct MeshDev {
   const int* body;      // body index of each panel
   int row0;             // first field row assembled: rows i >= row0 are stored at i - row0, leading
                         // dimension nf - row0 (0 for the system; n for field-point rows)
+  const double *gx, *gy, *gz;  // the points FLAG is decided from (nf each): the field points, except
+                               // for mirrored rows, which keep the unmirrored centroid
 };
 
 // Regions of the finite-depth Green function by R/h (FinGreen3D.f90:183-191)
@@ -141,17 +143,33 @@ void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work
 // X <- A^{-1} X using the factors, nrhs columns (leading dimension n)
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
 // AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
+// Xm / mirror: the mirror half's potentials and the symmetry plane
+// (0: x = 0, 1: y = 0); mirror = -1 without symmetry.
 void forces(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, const int* body_start_d,
-            const double2* X, double* out);
+            const double2* X, double* out, const double2* Xm = nullptr, int mirror = -1);
+// Geometric symmetry: A, T2 <- A + T2, A - T2; S, S2 <- S + S2, S - S2 (count entries each)
+void sym_combine(hipStream_t s, double2* A, double2* T2, double2* S, double2* S2, size_t count);
+// Right-hand-side generators of the two symmetry systems into W1 / W2 (n x (6 nbody + nbeta));
+// dinc: scratch of 2 n nbeta
+void sym_weights(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, int nbody, int mirror,
+                 double2* dinc, double2* W1, double2* W2);
+// X, X2 <- (X + X2) / 2, (X - X2) / 2
+void sym_recombine(hipStream_t s, double2* X, double2* X2, size_t count);
 // single-precision copy of A for the mixed path, and the helpers of refinement
 // Field points: DINCP(j, k) = grad(phi_I) . n at each hull panel (n x nbeta);
 // Z = [W; -X] (2n x nrhs); the free-term test onb[nfp]; the finished values
 // out[col * ld + p] (and inc[k * ld + p] when inc is not null).
-void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out);
+// mirror >= 0: DINCP at the mirrored sources; flip: the modes whose generator
+// changes sign (bit m for mode m + 1); accumulate: OR into onb instead of
+// clearing it first.
+void dincp(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, double2* out, int mirror = -1);
 void field_weights(hipStream_t s, const MeshDev& m, int nbody, int nbeta, const double2* dincp, const double2* X,
-                   double2* Z);
+                   double2* Z, int flip = 0);
 void on_body(hipStream_t s, const MeshDev& m, const double* vtx, int nfp, const double* px, const double* py,
-             const double* pz, int* onb);
+             const double* pz, int* onb, bool accumulate = false);
+// q = the mirror images of count points (mirror 0: x = 0, 1: y = 0)
+void reflect_points(hipStream_t s, int count, int mirror, const double* px, const double* py, const double* pz,
+                    double* qx, double* qy, double* qz);
 void field_finish(hipStream_t s, int nfp, const double* px, const double* py, const double* pz, const int* onb,
                   const FreqDev& f, const WaveDev& w, int nd, int kind, const double2* P, size_t ld, double2* out,
                   double2* inc);

@@ -38,7 +38,7 @@ module hamsx_mod
             hamsx_set_solver, hamsx_set_lid, hamsx_set_refinement, hamsx_set_hmatrix, &
             hamsx_hmatrix_stats, hamsx_hmatrix_matvec, hamsx_hmatrix_partition, &
             hamsx_stage_times, hamsx_set_field_points, hamsx_field_values, hamsx_set_body_matrices, &
-            hamsx_motions
+            hamsx_motions, hamsx_set_symmetry
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
             PANEL_GEOMETRY_GPU, hamsx_error_message
@@ -173,6 +173,13 @@ module hamsx_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: nlid
       real(c_double), intent(in) :: lid_cen(*)
+      integer(c_int) :: rc
+    end function
+    ! ISX / ISY of ReadControlFile: the mesh is the half on the positive side of the plane
+    function hamsx_set_symmetry(ctx, isx, isy) bind(C, name="hamsx_set_symmetry") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: isx, isy
       integer(c_int) :: rc
     end function
     ! XFP(NFP,3) of ReadControlFile; pass any array with nfp = 0 to remove the points

@@ -42,6 +42,7 @@ SIGNATURES = {
     "hamsx_wave_properties": (C.c_int, [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
     "hamsx_set_mesh": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip]),
     "hamsx_set_lid": (C.c_int, [_vp, C.c_int, _dp]),
+    "hamsx_set_symmetry": (C.c_int, [_vp, C.c_int, C.c_int]),
     "hamsx_set_frequency": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_int]),
     "hamsx_set_waves": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "hamsx_assemble": (C.c_int, [_vp]),

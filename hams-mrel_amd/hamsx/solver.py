@@ -39,6 +39,7 @@ class FrequencySolver:
         self.set_precision(precision)
         self.nbeta = 0
         self.nfp = 0
+        self.nsys = 1
         self.set_mesh(mesh)
 
     def set_mesh(self, mesh: Combined):
@@ -50,6 +51,7 @@ class FrequencySolver:
             _lib.ptr(m.body_start, C.c_int)), "hamsx_set_mesh")
         self.mesh = mesh
         self.nfp = 0
+        self.nsys = 1
 
     # -- lifecycle -------------------------------------------------------------
     def close(self):
@@ -83,6 +85,13 @@ class FrequencySolver:
             return
         lc = np.asfortranarray(lid_cen, dtype=np.float64)
         _lib.check(self._lib.hamsx_set_lid(self._h, lc.shape[0], _lib.ptr(lc)), "hamsx_set_lid")
+
+    def set_symmetry(self, isx: int = 0, isy: int = 0):
+        """Geometric symmetry (ISX / ISY of ControlFile.in): the mesh is the
+        half on the positive side of x = 0 (isx) or y = 0 (isy); solves then
+        cover the whole body.  Both 0 removes the plane."""
+        _lib.check(self._lib.hamsx_set_symmetry(self._h, int(isx), int(isy)), "hamsx_set_symmetry")
+        self.nsys = 2 if (isx or isy) else 1
 
     def set_field_points(self, xfp):
         """Field points (P x 3) for pressure / elevation output (the XFP array
@@ -167,18 +176,22 @@ class FrequencySolver:
         return _lib.check(self._lib.hamsx_factor(self._h), "hamsx_factor", allow_info=True)
 
     def radiation_solver(self) -> np.ndarray:
+        """MXPOT of the radiation modes: n x 6 nbody, or n x 6 nbody x 2 (the
+        mesh half, then the mirror half) with a symmetry plane."""
         n, nd = self.mesh.nelem, 6 * self.mesh.nbody
-        out = np.zeros((n, nd), dtype=np.complex128, order="F")
+        out = np.zeros((n, nd, self.nsys), dtype=np.complex128, order="F")
         _lib.check(self._lib.hamsx_solve_radiation(self._h, _lib.ptr(out)),
                    "hamsx_solve_radiation")
-        return out
+        return out[:, :, 0] if self.nsys == 1 else out
 
     def diffraction_solver(self) -> np.ndarray:
+        """MXPOT of the headings: n x nbeta (x 2 with a symmetry plane)."""
         n = self.mesh.nelem
-        out = np.zeros((n, max(1, self.nbeta)), dtype=np.complex128, order="F")
+        out = np.zeros((n, max(1, self.nbeta), self.nsys), dtype=np.complex128, order="F")
         _lib.check(self._lib.hamsx_solve_diffraction(self._h, _lib.ptr(out)),
                    "hamsx_solve_diffraction")
-        return out[:, :self.nbeta]
+        out = out[:, :self.nbeta]
+        return out[:, :, 0] if self.nsys == 1 else out
 
     def forces(self):
         nd = 6 * self.mesh.nbody

@@ -94,6 +94,20 @@ int hamsx_set_mesh(hamsx_ctx *ctx, int nelem, int nnode, const double *node_xyz,
  * hamsx_set_mesh also removes it.  hamsx_get_matrix(which = 0) then returns C. */
 int hamsx_set_lid(hamsx_ctx *ctx, int nlid, const double *lid_cen);
 
+/* Geometric symmetry (ISX / ISY of the control file, NSYS = 2;
+ * src/HAMS_Prog.f90:495-504, CalGreenFuncMulti.f90:141-177,
+ * AssbMatxMulti.f90:53-383): the mesh set by hamsx_set_mesh is the half on
+ * the positive side of the plane x = 0 (isx = 1) or y = 0 (isy = 1), and
+ * every solve then works on the whole body: the mirrored panels' entries,
+ * the two systems A1 = A + T2, A2 = A - T2 with their right-hand sides, and
+ * forces summed over both halves.  isx = isy = 1 is refused as the reference
+ * refuses it; isx = isy = 0 removes the plane, and so does a new
+ * hamsx_set_mesh.  Needs the FP64 LU solver; refused with a lid.  With a
+ * plane set, the mxpot outputs of hamsx_solve_radiation / _diffraction hold
+ * both halves, MXPOT(:, :, 1) then MXPOT(:, :, 2), and hamsx_field_values
+ * sums over both (PressureElevationMulti.f90:89-190). */
+int hamsx_set_symmetry(hamsx_ctx *ctx, int isx, int isy);
+
 /* ---- per-frequency state -------------------------------------------------
  * The scalars CalWaveProperts leaves in WaveDyn_mod (V, WK, W1, H, WVN). */
 int hamsx_set_frequency(hamsx_ctx *ctx, double v, double wk, double w1, double h,

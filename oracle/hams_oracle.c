@@ -1886,6 +1886,97 @@ void orc_field_values(int nelem, int nnode, const double *xyz, const int *ncn, c
     free(bodyof);
 }
 
+/* Field values with one symmetry plane (NSYS = 2; the IS loops of
+ * CalPotentialMulti, PressureElevationMulti.f90:89-190 and :204-249): the
+ * half mesh's panels seen from the field point and from its mirror image
+ * (FLAG from the mirrored distance), the radiation generators with the
+ * SX / SY signs of the mode, the diffraction generator DINCP at the mirrored
+ * source with the mirrored normal, MXPOT(:, :, IS) of phi2 (nelem x nr x 2,
+ * the symmetric solve's potentials); the free term from both images of the
+ * point and VINP at the point itself.  isx = 1: plane x = 0, else y = 0. */
+void orc_field_values_sym(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
+                          const double *cen, const double *area, const double *pnsz, const double *dxyz,
+                          int nbody, const int *body_start, int isx, double w1, double h, int nbeta,
+                          const double *beta, double amp, double xw0, double xw1, double rho, int isol,
+                          const double *phi2, int nfp, const double *xfp, int kind, double *out, int *onbody,
+                          int nthreads)
+{
+    const int nk = 200;
+    double wvn[200] = {0};
+    double v, wk;
+    wave_setup(w1, h, nk, &v, &wk, wvn);
+    const int nd = 6 * nbody, nr = nd + nbeta;
+    const int limit = fabs(v) < F1EM8 || fabs(v + 1.0) < F1EM8;
+    int *bodyof = (int *)malloc(sizeof(int) * (size_t)nelem);
+    for (int b = 0; b < nbody; b++)
+        for (int j = body_start[b]; j < body_start[b + 1]; j++) bodyof[j] = b;
+    /* SY(1, IS) / SX(1, IS) of the mirrored coordinate: +1 for IS = 1, -1 for IS = 2 */
+    cplx *dincp = (cplx *)malloc(sizeof(cplx) * (size_t)nelem * (nbeta > 0 ? nbeta : 1) * 2);
+    for (int is = 1; is <= 2; is++)
+        for (int b = 0; b < nbeta; b++)
+            for (int j = 0; j < nelem; j++) {
+                const double s = is == 2 ? -1.0 : 1.0, sx = isx ? s : 1.0, sy = isx ? 1.0 : s;
+                double dpo[6];
+                orc_dinp(sx * cen[j], sy * cen[j + nelem], cen[j + 2 * nelem], xw0, xw1, beta[b], v, w1, wk, h, amp,
+                         dpo);
+                cplx px = dpo[0] + I * dpo[1], py = dpo[2] + I * dpo[3], pz = dpo[4] + I * dpo[5];
+                dincp[j + (size_t)nelem * (b + (size_t)nbeta * (is - 1))] =
+                    px * (sx * dxyz[j]) + py * (sy * dxyz[j + nelem]) + pz * dxyz[j + 2 * nelem];
+            }
+    const cplx *ph = (const cplx *)phi2;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int p = 0; p < nfp; p++) {
+        const double xe[3] = {xfp[p], xfp[p + nfp], xfp[p + 2 * nfp]};
+        cplx *pot = (cplx *)calloc((size_t)nr, sizeof(cplx));
+        int ob = 0;
+        for (int is = 1; is <= 2; is++) {
+            const double s = is == 2 ? -1.0 : 1.0;
+            const double xp[3] = {isx ? s * xe[0] : xe[0], isx ? xe[1] : s * xe[1], xe[2]};
+            /* the mode's sign: ISX: surge, pitch, yaw; ISY: sway, roll, yaw */
+            double sg[6] = {1, 1, 1, 1, 1, 1};
+            if (isx) sg[0] = sg[4] = sg[5] = s;
+            else sg[1] = sg[3] = sg[5] = s;
+            for (int j = 0; j < nelem; j++) {
+                cplx dum2, dum1;
+                field_entry(xp[0], xp[1], xp[2], j, nelem, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn,
+                            nk, &dum2, &dum1);
+                const cplx *phs = ph + (size_t)nelem * nr * (is - 1);
+                for (int col = 0; col < nd; col++) {
+                    const int md = col % 6;
+                    const double fn = bodyof[j] == col / 6 ? 1.0 : 0.0;
+                    const cplx term1 = dum1 * sg[md] * dxyz[j + nelem * md] * fn;
+                    const cplx term2 = dum2 * phs[j + (size_t)nelem * col];
+                    pot[col] = pot[col] + (term1 - term2);
+                }
+                for (int b = 0; b < nbeta; b++) {
+                    const cplx term1 = -dum1 * dincp[j + (size_t)nelem * (b + (size_t)nbeta * (is - 1))];
+                    const cplx term2 = dum2 * phs[j + (size_t)nelem * (nd + b)];
+                    pot[nd + b] = pot[nd + b] + (term1 - term2);
+                }
+            }
+            if (orc_on_body(xp, nelem, nnode, xyz, ncn, ncon, pnsz, dxyz)) ob = 1;
+        }
+        if (onbody) onbody[p] = ob;
+        const double sld = ob ? 2.0 * PI_C : 4.0 * PI_C;
+        for (int col = 0; col < nr; col++) {
+            cplx val = pot[col] / sld;
+            if (col >= nd && isol == 1 && !limit) {
+                double f0[2];
+                orc_vinp(xe[0], xe[1], xe[2], xw0, xw1, beta[col - nd], v, w1, wk, h, amp, f0);
+                val = val + (f0[0] + I * f0[1]);
+            }
+            if (kind == 1) val = limit ? rho * val : (I * w1 * rho) * val;
+            else if (kind == 2) val = limit ? val : (I * w1 / G_C) * val;
+            out[2 * ((size_t)p + (size_t)nfp * col)] = creal(val);
+            out[2 * ((size_t)p + (size_t)nfp * col) + 1] = cimag(val);
+        }
+        free(pot);
+    }
+    free(dincp);
+    free(bodyof);
+}
+
 /* ========================================================================
  * Motions: SolveMotionMulti (SolveMotionMulti.f90:46-150) for one heading.
  * nd = 6 nbody; amas, bdmp (nd x nd, column-major), exfc (nd complex) of the

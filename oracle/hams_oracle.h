@@ -133,6 +133,14 @@ void orc_field_values(int nelem, int nnode, const double *node_xyz, const int *n
                       double w1, double h, int nbeta, const double *beta, double amp, double xw0,
                       double xw1, double rho, int isol, const double *phi, int nfp,
                       const double *xfp, int kind, double *out, int *onbody, int nthreads);
+/* the same with one symmetry plane (isx = 1: x = 0, else y = 0) and the
+ * symmetric solve's potentials phi2 (nelem x nr x 2) */
+void orc_field_values_sym(int nelem, int nnode, const double *node_xyz, const int *ncn,
+                          const int *ncon, const double *cen, const double *area,
+                          const double *pnsz, const double *dxyz, int nbody, const int *body_start,
+                          int isx, double w1, double h, int nbeta, const double *beta, double amp,
+                          double xw0, double xw1, double rho, int isol, const double *phi2, int nfp,
+                          const double *xfp, int kind, double *out, int *onbody, int nthreads);
 
 
 /* ---- motions: SolveMotionMulti (SolveMotionMulti.f90:46-150) for one heading;

@@ -90,6 +90,9 @@ def _load(path):
         o.orc_field_values.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d, _d,
                                        C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _ip,
                                        C.c_int]
+        o.orc_field_values_sym.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip,
+                                           C.c_int, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, C.c_int,
+                                           _dp, C.c_int, _dp, _ip, C.c_int]
         o.orc_motion.restype = C.c_int
         o.orc_motion.argtypes = [C.c_int, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         o.orc_solve_frequency_sym.restype = C.c_int
@@ -261,6 +264,24 @@ def field_values(m, phi, xfp, w1: float, h: float, headings_rad, kind=0, amp=1.0
     return out, ob.astype(bool)
 
 
+def field_values_sym(m, isx: int, phi2, xfp, w1: float, h: float, headings_rad, kind=0, amp=1.0, xw=(0.0, 0.0),
+                     rho=1000.0, isol=1, nthreads: int = 8, fma: bool = False):
+    """field_values on the half mesh m with one symmetry plane (isx = 1: x = 0,
+    else y = 0); phi2 (nelem x nr x 2) the symmetric solve's potentials."""
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    xf = np.asfortranarray(xfp, dtype=np.float64).reshape(-1, 3)
+    nfp, nr = xf.shape[0], 6 * m.nbody + len(beta)
+    ph = np.asfortranarray(phi2, dtype=np.complex128)
+    assert ph.shape == (m.nelem, nr, 2)
+    out = np.zeros((nfp, nr), dtype=np.complex128, order="F")
+    ob = np.zeros(nfp, dtype=np.int32)
+    lib(fma).orc_field_values_sym(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int), _p(m.cen),
+                                  _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody, _p(m.body_start, C.c_int), int(isx),
+                                  w1, h, len(beta), _p(beta), amp, xw[0], xw[1], rho, isol, _p(ph), nfp, _p(xf), kind,
+                                  _p(out), _p(ob, C.c_int), nthreads)
+    return out, ob.astype(bool)
+
+
 def _blocks(mats):
     """nbody (6 x 6) matrices -> the nbody x 36 column-major block array."""
     return np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=np.float64).T.ravel() for m in mats]))
@@ -300,7 +321,7 @@ def forces(m, phi, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rh
 
 
 def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0,
-                        isol=1, nthreads: int = 8, potentials: bool = False):
+                        isol=1, nthreads: int = 8, potentials: bool = False, fma: bool = False):
     """NSYS = 2 on the half mesh m (isx = 1: mirrored in x, else in y)."""
     nd = 6 * m.nbody
     beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
@@ -308,7 +329,7 @@ def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0,
     bdmp = np.zeros((nd, nd), order="F")
     exfc = np.zeros((nd, max(1, len(beta))), dtype=np.complex128, order="F")
     phi2 = np.zeros((m.nelem, nd + len(beta), 2), dtype=np.complex128, order="F")
-    info = lib().orc_solve_frequency_sym(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
+    info = lib(fma).orc_solve_frequency_sym(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
                                          _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
                                          _p(m.body_start, C.c_int), int(isx), w1, h, len(beta), _p(beta), amp, xw[0],
                                          xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), _p(phi2), nthreads)

@@ -1,0 +1,162 @@
+"""GPU geometric symmetry (hamsx_set_symmetry, NSYS = 2: the mirrored field
+points of CalGreenFuncMulti.f90:141-177, the two systems and right-hand sides
+of AssbMatxMulti.f90:53-251, the MXPOT recombination :306-316 / :373-381 and
+the force sums over both halves of PotentWavForceMulti.f90) against the
+oracle's symmetric solve on the same half mesh, and against the GPU's own
+solve of the full mesh.
+
+Tolerances as in test_gpu_parity.py: deep water 1e-8 on the coefficients and
+1e-10 on the potentials; finite depth max(1e-8, 4x the FMA / no-FMA distance
+of the oracle on the same case)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from hamsx._lib import HamsxError
+from hamsx.mesh import combine, cylinder
+from hamsx.solver import FrequencySolver
+from test_oracle_symmetry import half
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def two_bodies(axis):
+    """Two cylinders side by side along the other axis, each symmetric about
+    the plane axis = 0."""
+    if axis == 1:
+        bodies = [cylinder(4.0, 8.0, 24, 6, 3, -12.0), cylinder(3.0, 6.0, 20, 5, 3, 12.0)]
+        xr = [np.array([-12.0, 0, 0]), np.array([12.0, 0, 0])]
+    else:
+        bodies = [cylinder(4.0, 8.0, 24, 6, 3), cylinder(3.0, 6.0, 20, 5, 3)]
+        for b, y0 in zip(bodies, (-12.0, 12.0)):
+            b.xyz[:, 1] += y0
+        xr = [np.array([0, -12.0, 0]), np.array([0, 12.0, 0])]
+    return bodies, xr
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("depth,w", [(-1.0, 0.8), (30.0, 1.2)])
+def test_symmetric_solve_matches_oracle(axis, depth, w):
+    bodies, xr = two_bodies(axis)
+    m = combine([half(b, axis) for b in bodies], xr)
+    heads = np.deg2rad([0.0, 45.0, 120.0])
+    isx, isy = (1, 0) if axis == 0 else (0, 1)
+    say(f"[gpu] symmetry isx={isx} isy={isy} n={m.nelem} depth={depth} w={w}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        s.set_symmetry(isx, isy)
+        am, bd, ex, info = s.solve_frequency(w, depth)
+        assert info == 0
+        s.calwaveproperts(w, depth)
+        s.assemble()
+        rad, dif = s.radiation_solver(), s.diffraction_solver()
+    ao, bo, eo, io, phi = oracle.solve_frequency_sym(m, isx, w, depth, heads, potentials=True)
+    assert io == 0
+    errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+    floor = (0.0, 0.0, 0.0)
+    if depth > 0:
+        af, bf, ef, _ = oracle.solve_frequency_sym(m, isx, w, depth, heads, fma=True)
+        floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+    nd = 6 * m.nbody
+    ep = rel(np.concatenate([rad, dif], axis=1), phi)
+    say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e} MXPOT {ep:.3e}  "
+        f"floor {floor[0]:.1e} {floor[1]:.1e} {floor[2]:.1e}")
+    assert rad.shape == (m.nelem, nd, 2) and dif.shape == (m.nelem, len(heads), 2)
+    for e, f in zip(errs, floor):
+        assert e <= max(1e-8, 4 * f)
+    if depth < 0:
+        assert ep <= 1e-10
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_half_mesh_equals_full_mesh_on_gpu(axis):
+    """The half mesh with the plane and the full mesh without it are the same
+    discrete problem: deep water agrees to rounding."""
+    bodies, xr = two_bodies(axis)
+    heads = np.deg2rad([0.0, 60.0])
+    say(f"[gpu] symmetry half vs full axis={axis}")
+    with FrequencySolver(combine(bodies, xr)) as s:
+        s.set_waves(heads)
+        af, bf, ef, _ = s.solve_frequency(1.0, -1.0)
+    with FrequencySolver(combine([half(b, axis) for b in bodies], xr)) as s:
+        s.set_waves(heads)
+        s.set_symmetry(*((1, 0) if axis == 0 else (0, 1)))
+        ah, bh, eh, info = s.solve_frequency(1.0, -1.0)
+    assert info == 0
+    errs = rel(ah, af), rel(bh, bf), rel(eh, ef)
+    say(f"    rel err {errs}")
+    assert max(errs) <= 1e-10
+
+
+def test_symmetry_arguments_and_state():
+    """Both planes are refused as the reference refuses them; the plane
+    needs the FP64 LU solver; a new mesh removes it."""
+    bodies, xr = two_bodies(1)
+    m = combine([half(b, 1) for b in bodies], xr)
+    with FrequencySolver(m) as s:
+        s.set_waves([0.0])
+        with pytest.raises(HamsxError):
+            s.set_symmetry(1, 1)
+        s.set_symmetry(0, 1)
+        s.set_solver("gmres")
+        with pytest.raises(HamsxError):
+            s.solve_frequency(1.0, -1.0)
+        s.set_solver("lu")
+        a1, _, _, _ = s.solve_frequency(1.0, -1.0)
+        s.set_mesh(m)
+        assert s.nsys == 1
+        s.set_waves([0.0])
+        a0, _, _, _ = s.solve_frequency(1.0, -1.0)
+        assert rel(a0, a1) > 1e-3      # the half mesh alone is a different body
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("depth", [-1.0, 30.0])
+@pytest.mark.parametrize("chunk", [None, "2"])
+def test_symmetric_field_values_match_oracle(axis, depth, chunk, monkeypatch):
+    """hamsx_field_values with a symmetry plane (each point and its mirror
+    image against the half mesh, PressureElevationMulti.f90:89-249) against
+    the oracle's restatement fed the GPU's own potentials of both halves.
+    Points in the fluid, on the free surface, on a panel of the mesh half and
+    on a panel of the mirror half (found through the mirrored point);
+    chunk = 2 splits the points into one-point passes."""
+    if chunk:
+        monkeypatch.setenv("HAMSX_FIELD_CHUNK", chunk)
+    bodies, xr = two_bodies(axis)
+    m = combine([half(b, axis) for b in bodies], xr)
+    isx = 1 if axis == 0 else 0
+    heads = np.deg2rad([0.0, 45.0, 120.0])
+    w = 0.9
+    on_half = m.cen[5]
+    on_mirror = on_half.copy()
+    on_mirror[axis] = -on_mirror[axis]
+    xfp = np.array([[30.0, 25.0, -3.0], [-20.0, 5.0, 0.0], [2.0, -33.0, -9.0], on_half, on_mirror])
+    say(f"[gpu] symmetric field values axis={axis} depth={depth} chunk={chunk}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        s.set_symmetry(isx, 1 - isx)
+        s.calwaveproperts(w, depth)
+        s.assemble()
+        phi2 = np.concatenate([s.radiation_solver(), s.diffraction_solver()], axis=1)
+        s.set_field_points(xfp)
+        vals, ob = s.field_values("potential")
+    ref, obr = oracle.field_values_sym(m, isx, phi2, xfp, w, depth, heads, kind=0)
+    e = rel(vals, ref)
+    floor = 0.0
+    if depth > 0:
+        rf, _ = oracle.field_values_sym(m, isx, phi2, xfp, w, depth, heads, kind=0, fma=True)
+        floor = rel(rf, ref)
+    say(f"    onbody {list(ob)}  rel err {e:.3e}  floor {floor:.1e}")
+    assert list(ob) == list(obr) == [False, False, False, True, True]
+    assert e <= max(1e-10, 4 * floor)

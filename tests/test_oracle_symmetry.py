@@ -58,3 +58,28 @@ def test_two_bodies_symmetric_about_y():
     ah, bh, eh, info = oracle.solve_frequency_sym(mh, 0, 1.1, -1.0, heads)
     assert info == 0
     assert rel(ah, af) <= 1e-13 and rel(bh, bf) <= 1e-13 and rel(eh, ef) <= 1e-13
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("depth", [-1.0, 8.0])
+def test_field_values_half_mesh_equal_full_mesh(axis, depth):
+    """Field points with one symmetry plane (the IS loops of
+    CalPotentialMulti): the half mesh's potentials seen from the point and
+    its mirror image give the full mesh's field values -- in the fluid, on
+    the free surface, and at a panel centroid of the mirror half (free term
+    2 pi found through the mirrored point)."""
+    full = cylinder(2.0, 3.0, 16, 5, 3)
+    mf = combined_with_oracle_geometry([full], [np.zeros(3)])
+    mh = combined_with_oracle_geometry([half(full, axis)], [np.zeros(3)])
+    heads = np.deg2rad([0.0, 50.0])
+    w = 1.1
+    mirror_cen = mf.cen[np.argmax(-mf.cen[:, axis])]          # a centroid on the mirror side
+    xfp = np.array([[4.0, 1.5, -1.0], [-3.5, 2.5, 0.0], [1.0, -4.0, -2.5], [0.5, 0.3, -5.0], mirror_cen])
+    phi_full, _ = oracle.solve_potentials(mf, w, depth, heads)
+    vf, of = oracle.field_values(mf, phi_full, xfp, w, depth, heads, kind=0)
+    *_, info, phi2 = oracle.solve_frequency_sym(mh, 1 if axis == 0 else 0, w, depth, heads, potentials=True)
+    assert info == 0
+    vh, oh = oracle.field_values_sym(mh, 1 if axis == 0 else 0, phi2, xfp, w, depth, heads, kind=0)
+    assert list(of) == list(oh) and oh[-1] and not oh[0]
+    tol = 1e-12 if depth < 0 else 1e-6
+    assert rel(vh, vf) <= tol, rel(vh, vf)
