@@ -112,6 +112,29 @@ def roofline(case, mesh, ms, precision):
     return out
 
 
+def solve_roofline(case, mesh, nbeta, ms_per_solve, lu_peak):
+    """Whole-solve roofline: the time one solve would take with every part at
+    its peak -- the assembly's FP64 work (the counter-measured operations per
+    pair of each region, where measured) on the FP64 VALU, the LU's
+    (8/3) N^3 on MFMA, the right-hand sides and the two triangular solves
+    (8 N^2 per column each) on the FP64 VALU -- against the measured time per
+    solve of the run (the throughput interval, solves in flight included)."""
+    if case.depth < 0:
+        return None
+    cnt = REGION_COUNTERS.get(case.name)
+    if not cnt:
+        return None
+    n = mesh.nelem
+    pairs = region_pairs(mesh, case.depth)
+    f_asm = sum(cnt[k]["flops_per_pair"] * pairs[k] for k in REGIONS)
+    nrhs = 6 * mesh.nbody + nbeta
+    f_lu = 8.0 * n ** 3 / 3.0
+    f_vec = 3 * 8.0 * n * n * nrhs
+    t_roof = (f_asm / PEAK_FP64_TFLOPS + f_lu / lu_peak + f_vec / PEAK_FP64_TFLOPS) / 1e9  # ms
+    return {"bound": "fp64 valu (assembly) + mfma (LU)", "flops": {"assembly": f_asm, "lu": f_lu, "rhs_solves": f_vec},
+            "t_roof_ms": t_roof, "ms_per_solve": ms_per_solve, "frac": t_roof / ms_per_solve}
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +489,7 @@ def main():
     lu_tf = 8.0 * n ** 3 / 3.0 / (ms["factor"] * 1e-3) / 1e12
     roof_lu = {"kernel": "LU factorisation", "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
                "frac": lu_tf / lu_peak, "launch_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
+    roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
@@ -478,6 +502,7 @@ def main():
         "stage_ms_per_solve": ms,
         "roofline": roof,
         "roofline_lu": roof_lu,
+        "roofline_solve": roof_solve,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
