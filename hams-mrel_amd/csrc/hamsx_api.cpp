@@ -297,13 +297,14 @@ struct hamsx_ctx {
   DevBuf<double> modetab;  // per-frequency finite-depth mode tables
   // geometric symmetry, one plane (NSYS = 2): sym_mirror 0 = x = 0 (ISX), 1 = y = 0
   // (ISY), -1 none.  The mirror view's points mview = [fx | fy | fz | gx | gy | gz],
-  // 2n each: rows n .. 2n - 1 are the mirrored hull centroids, FLAG taken at the
-  // unmirrored ones.  T2 / S2: the mirrored entries, then system 2's matrix and S-;
+  // n + nf each: rows n .. 2n - 1 are the mirrored hull centroids, FLAG taken at
+  // the unmirrored ones, then the mirrored lid points (FLAG at the mirrored point).
+  // With a lid, Cm2 / Xe2 are system 2's normal matrix and extended right-hand sides.  T2 / S2: the mirrored entries, then system 2's matrix and S-;
   // W1 / W2 the two systems' generator columns; X2 the mirror half's potentials.
   int sym_mirror = -1;
   DevBuf<double> mview, mmodetab;
   PairBins mbins;
-  DevBuf<double2> T2, S2, X2, W1, W2, sdinc;
+  DevBuf<double2> T2, S2, X2, W1, W2, sdinc, Cm2, Xe2;
   DevBuf<int> ipiv2, perm2;
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
@@ -335,14 +336,14 @@ struct hamsx_ctx {
   }
   hamsx::MeshDev mirror_view() const {
     hamsx::MeshDev m = mesh();
-    const size_t n2 = 2 * (size_t)n;
+    const size_t n2 = (size_t)n + nf;
     m.fx = mview.p;
     m.fy = mview.p + n2;
     m.fz = mview.p + 2 * n2;
     m.gx = mview.p + 3 * n2;
     m.gy = mview.p + 4 * n2;
     m.gz = mview.p + 5 * n2;
-    m.nf = 2 * n;
+    m.nf = n + nf;
     m.row0 = n;
     return m;
   }
@@ -529,14 +530,25 @@ int do_rhs(hamsx_ctx* c) {
     hamsx::launch::sym_weights(c->stream, c->mesh(), c->freq, c->waves(), c->nbody, c->sym_mirror, c->sdinc.p,
                                c->W1.p, c->W2.p);
     if ((rc = check_launch()) || (rc = blas_handle(c))) return rc;
+    // with a lid the products run over all nf rows and then D = A_ip^T B_ip
+    const bool lid = c->nlid > 0;
+    if (lid && ((rc = c->Xe.ensure((size_t)c->nf * c->nrhs())) || (rc = c->Xe2.ensure((size_t)c->nf * c->nrhs()))))
+      return rc;
     const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-    for (int ip = 0; ip < 2; ip++)
-      if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->n, c->nrhs(), c->n, &one,
-                        reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->S.p : c->S2.p), c->n,
+    for (int ip = 0; ip < 2; ip++) {
+      double2* dst = lid ? (ip == 0 ? c->Xe.p : c->Xe2.p) : (ip == 0 ? c->X.p : c->X2.p);
+      if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->nf, c->nrhs(), c->n, &one,
+                        reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->S.p : c->S2.p), c->nf,
                         reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->W1.p : c->W2.p), c->n, &zero,
-                        reinterpret_cast<rocblas_double_complex*>(ip == 0 ? c->X.p : c->X2.p),
-                        c->n) != rocblas_status_success)
+                        reinterpret_cast<rocblas_double_complex*>(dst), c->nf) != rocblas_status_success)
         return fail(HAMSX_EDEVICE, "rocblas_zgemm (symmetric right-hand sides) failed");
+      if (lid && rocblas_zgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->n, c->nrhs(), c->nf,
+                               &one, reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->A.p : c->T2.p),
+                               c->nf, reinterpret_cast<const rocblas_double_complex*>(dst), c->nf, &zero,
+                               reinterpret_cast<rocblas_double_complex*>(ip == 0 ? c->X.p : c->X2.p),
+                               c->n) != rocblas_status_success)
+        return fail(HAMSX_EDEVICE, "rocblas_zgemm (A^T B, symmetric) failed");
+    }
     return 0;
   }
   const bool lid = c->nlid > 0;
@@ -565,12 +577,16 @@ int ensure_normal_matrix(hamsx_ctx* c) {
   if (c->nlid == 0 || c->have_C) return 0;
   int rc;
   if ((rc = c->Cm.ensure((size_t)c->n * c->n)) || (rc = blas_handle(c))) return rc;
+  if (c->sym() && (rc = c->Cm2.ensure((size_t)c->n * c->n))) return rc;
   const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-  if (rocblas_zsyrk(c->blas, rocblas_fill_lower, rocblas_operation_transpose, c->n, c->nf, &one,
-                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf, &zero,
-                    reinterpret_cast<rocblas_double_complex*>(c->Cm.p), c->n) != rocblas_status_success)
-    return fail(HAMSX_EDEVICE, "rocblas_zsyrk (A^T A) failed");
-  hamsx::launch::mirror_lower(c->stream, c->Cm.p, c->n);
+  for (int ip = 0; ip < (c->sym() ? 2 : 1); ip++) {  // with a symmetry plane: C2 = A2^T A2 as well
+    double2* C = ip == 0 ? c->Cm.p : c->Cm2.p;
+    if (rocblas_zsyrk(c->blas, rocblas_fill_lower, rocblas_operation_transpose, c->n, c->nf, &one,
+                      reinterpret_cast<const rocblas_double_complex*>(ip == 0 ? c->A.p : c->T2.p), c->nf, &zero,
+                      reinterpret_cast<rocblas_double_complex*>(C), c->n) != rocblas_status_success)
+      return fail(HAMSX_EDEVICE, "rocblas_zsyrk (A^T A) failed");
+    hamsx::launch::mirror_lower(c->stream, C, c->n);
+  }
   if ((rc = check_launch())) return rc;
   c->have_C = true;
   return 0;
@@ -586,6 +602,8 @@ const hamsx::LuAux* lu_lookahead(hamsx_ctx* c) {
 
 // the square matrix the solvers work on: A, or C = A^T A with a lid
 double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
+// symmetry system 2's: A2 (in T2), or C2 = A2^T A2 with a lid
+double2* system_matrix2(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm2.p : c->T2.p; }
 
 int do_factor(hamsx_ctx* c) {
   if (c->have_LU) return 0;
@@ -600,13 +618,14 @@ int do_factor(hamsx_ctx* c) {
     return rc;
   if (c->sym() && ((rc = c->ipiv2.ensure(n)) || (rc = c->perm2.ensure(n)))) return rc;
   StageTimer tm(c, 2);
-  if (c->sym()) {  // the two systems A +- T2 (INFO of system 2 in flags[5])
+  if (c->sym()) {  // the two systems A +- T2, or their normal matrices (INFO of system 2 in flags[5])
     if (c->prec != HAMSX_PREC_FP64) return fail(HAMSX_EARG, "a symmetry plane needs FP64 precision");
-    hamsx::launch::lu_factor(c->stream, c->A.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
-                             c->ludiag.p, lu_lookahead(c));
-    hamsx::launch::lu_factor(c->stream, c->T2.p, n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p, c->flags.p + 5,
-                             c->ludiag.p, lu_lookahead(c));
-    c->have_A = false;
+    if ((rc = ensure_normal_matrix(c))) return rc;
+    hamsx::launch::lu_factor(c->stream, system_matrix(c), n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
+                             c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
+    hamsx::launch::lu_factor(c->stream, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
+                             c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
+    if (c->nlid == 0) c->have_A = false;
     c->have_LU = true;
     return check_launch();
   }
@@ -1050,8 +1069,8 @@ int do_solve(hamsx_ctx* c) {
     if ((rc = do_factor(c))) return rc;
     const int n = c->n, nr = c->nrhs();
     StageTimer tm(c, 3);
-    hamsx::launch::lu_solve(c->stream, c->A.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
-    hamsx::launch::lu_solve(c->stream, c->T2.p, n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
+    hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
     hamsx::launch::sym_recombine(c->stream, c->X.p, c->X2.p, (size_t)n * nr);
     c->have_X = true;
     return check_launch();
@@ -1490,12 +1509,13 @@ int hamsx_set_mesh(hamsx_ctx* c, int nelem, int nnode, const double* xyz, const 
   return 0;
 }
 
+static int build_mirror_view(hamsx_ctx* c, int mirror);
+
 int hamsx_set_lid(hamsx_ctx* c, int nlid, const double* lid_cen) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
   if (c->pending) return busy(c);
   if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_lid needs hamsx_set_mesh first");
   if (nlid < 0 || (nlid > 0 && !lid_cen)) return fail(HAMSX_EARG, "bad lid arguments");
-  if (nlid > 0 && c->sym()) return fail(HAMSX_EARG, "a lid with a symmetry plane is not supported");
   if (c->n + nlid > 0xFFFF) return fail(HAMSX_EARG, "hull + lid panels exceed 65535");
   int rc = set_device(c);
   if (rc) return rc;
@@ -1513,6 +1533,8 @@ int hamsx_set_lid(hamsx_ctx* c, int nlid, const double* lid_cen) {
   HX_TRY(hipMemcpy(c->fz.p, f.data() + 2 * nf, sizeof(double) * nf, hipMemcpyHostToDevice));
   c->nlid = nlid;
   c->nf = (int)nf;
+  c->bins.n = 0;  // new field rows: the pair bins are rebuilt
+  if (c->sym() && (rc = build_mirror_view(c, c->sym_mirror))) return rc;
   c->have_A = c->have_S = c->have_LU = c->have_X = c->have_C = false;
   return 0;
 }
@@ -1583,13 +1605,14 @@ int hamsx_assemble(hamsx_ctx* c) {
   if ((rc = check_launch())) return rc;
   if (c->sym()) {
     // the mirrored field points (IS = 2), then A +- T2 and S +- S2
-    const size_t nn = (size_t)c->n * c->n;
+    const size_t nn = (size_t)c->nf * c->n;
     const hamsx::MeshDev mv = c->mirror_view();
     if ((rc = c->T2.ensure(nn)) || (rc = c->S2.ensure(nn)) ||
-        (finite && (rc = c->mmodetab.ensure(hamsx::mode_tab_len(2 * c->n)))))
+        (finite && (rc = c->mmodetab.ensure(hamsx::mode_tab_len(mv.nf)))))
       return rc;
-    const bool mbinned = finite && 2 * c->n <= 0xFFFF && !force_grid_assembly();
-    if (mbinned && (rc = ensure_bins(c, c->mbins, mv, 1 + (unsigned long long)c->sym_mirror))) return rc;
+    const bool mbinned = finite && mv.nf <= 0xFFFF && !force_grid_assembly();
+    if (mbinned && (rc = ensure_bins(c, c->mbins, mv, 1 + (unsigned long long)c->sym_mirror + 2 * (unsigned long long)c->nlid)))
+      return rc;
     hamsx::FreqDev fm = c->freq;
     fm.tab = finite ? c->mmodetab.p : nullptr;
     StageTimer tm(c, 0);
@@ -1606,39 +1629,49 @@ int hamsx_assemble(hamsx_ctx* c) {
   return 0;
 }
 
+// The mirror view's points for plane `mirror` from the main view's field
+// points (hull centroids, then lid points): rows n .. n + nf - 1 evaluate at
+// the mirrored points; FLAG is taken at the unmirrored hull centroid
+// (CalGreenFuncMulti.f90:134-139) and at the mirrored lid point (the
+// distance CALGREEN_MULTI_IRR gives the Green function, :278-284).
+static int build_mirror_view(hamsx_ctx* c, int mirror) {
+  const size_t n = c->n, nf = c->nf, m2 = n + nf;
+  std::vector<double> f(3 * nf), h(6 * m2);
+  for (int d = 0; d < 3; d++)
+    HX_TRY(hipMemcpy(f.data() + d * nf, (d == 0 ? c->fx.p : d == 1 ? c->fy.p : c->fz.p), sizeof(double) * nf,
+                     hipMemcpyDeviceToHost));
+  for (int d = 0; d < 3; d++) {
+    const double sg = d == mirror ? -1.0 : 1.0;
+    for (size_t i = 0; i < n; i++) h[d * m2 + i] = h[(3 + d) * m2 + i] = f[d * nf + i];
+    for (size_t i = 0; i < nf; i++) {
+      const double v = f[d * nf + i];
+      h[d * m2 + n + i] = sg * v;
+      h[(3 + d) * m2 + n + i] = i < n ? v : sg * v;
+    }
+  }
+  int rc;
+  if ((rc = c->mview.ensure(6 * m2))) return rc;
+  HX_TRY(hipMemcpy(c->mview.p, h.data(), sizeof(double) * 6 * m2, hipMemcpyHostToDevice));
+  c->mbins.n = 0;
+  return 0;
+}
+
 int hamsx_set_symmetry(hamsx_ctx* c, int isx, int isy) {
   if (!c) return fail(HAMSX_EARG, "ctx is NULL");
   if (c->pending) return busy(c);
   if (c->n == 0) return fail(HAMSX_ESTATE, "hamsx_set_symmetry needs hamsx_set_mesh first");
   if ((isx != 0 && isx != 1) || (isy != 0 && isy != 1)) return fail(HAMSX_EARG, "ISX and ISY must be 0 or 1");
   if (isx && isy) return fail(HAMSX_EARG, "ISX and ISY cannot be simultaneously 1");
-  if ((isx || isy) && c->nlid > 0) return fail(HAMSX_EARG, "a symmetry plane with a lid is not supported");
   int rc = set_device(c);
   if (rc) return rc;
   HX_TRY(hipStreamSynchronize(c->stream));
   const int mirror = isx ? 0 : (isy ? 1 : -1);
-  if (mirror >= 0) {
-    const size_t n = c->n, n2 = 2 * n;
-    std::vector<double> h(6 * n2);
-    for (int d = 0; d < 3; d++) {
-      HX_TRY(hipMemcpy(h.data() + d * n2, (d == 0 ? c->cx.p : d == 1 ? c->cy.p : c->cz.p), sizeof(double) * n,
-                       hipMemcpyDeviceToHost));
-      const double sg = d == mirror ? -1.0 : 1.0;
-      for (size_t i = 0; i < n; i++) {
-        const double v = h[d * n2 + i];
-        h[d * n2 + n + i] = sg * v;           // field point: the mirrored centroid
-        h[(3 + d) * n2 + i] = v;               // FLAG point: the centroid itself
-        h[(3 + d) * n2 + n + i] = v;
-      }
-    }
-    if ((rc = c->mview.ensure(6 * n2))) return rc;
-    HX_TRY(hipMemcpy(c->mview.p, h.data(), sizeof(double) * 6 * n2, hipMemcpyHostToDevice));
-  }
+  if (mirror >= 0 && (rc = build_mirror_view(c, mirror))) return rc;
   if (mirror != c->sym_mirror) {
     c->sym_mirror = mirror;
     c->mbins.n = 0;
     c->field_gen++;  // the field rows change layout
-    c->have_A = c->have_S = c->have_LU = c->have_X = false;
+    c->have_A = c->have_S = c->have_LU = c->have_X = c->have_C = false;
   }
   return 0;
 }

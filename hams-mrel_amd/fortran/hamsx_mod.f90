@@ -41,7 +41,7 @@ module hamsx_mod
             hamsx_motions, hamsx_set_symmetry
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
-            PANEL_GEOMETRY_GPU, hamsx_error_message
+            PANEL_GEOMETRY_GPU, SET_SYMMETRY_GPU, hamsx_error_message
 
   ! ---------------------------------------------------------------- part 1
   interface
@@ -390,18 +390,29 @@ contains
     INFO = int(rc)
   end subroutine
 
-  ! MXPOT(TNELEM, 6*NB) receives the radiation potentials.
+  ! MXPOT(TNELEM, 6*NB) receives the radiation potentials (MXPOT(TNELEM, 6*NB, 2)
+  ! with a symmetry plane).
   subroutine RADIATION_SOLVER_GPU(ctx, MXPOT)
     type(c_ptr), intent(in) :: ctx
-    complex(c_double_complex), intent(inout), target, contiguous :: MXPOT(:, :)
-    call check(hamsx_solve_radiation(ctx, c_loc(MXPOT)), "hamsx_solve_radiation")
+    complex(c_double_complex), intent(inout), target :: MXPOT(*)
+    call check(hamsx_solve_radiation(ctx, c_loc(MXPOT(1))), "hamsx_solve_radiation")
   end subroutine
 
-  ! MXPOT(TNELEM, NBETA) receives the diffraction potentials.
+  ! MXPOT(TNELEM, NBETA) receives the diffraction potentials (MXPOT(TNELEM, NBETA, 2)
+  ! with a symmetry plane).
   subroutine DIFFRACTION_SOLVER_GPU(ctx, MXPOT)
     type(c_ptr), intent(in) :: ctx
-    complex(c_double_complex), intent(inout), target, contiguous :: MXPOT(:, :)
-    call check(hamsx_solve_diffraction(ctx, c_loc(MXPOT)), "hamsx_solve_diffraction")
+    complex(c_double_complex), intent(inout), target :: MXPOT(*)
+    call check(hamsx_solve_diffraction(ctx, c_loc(MXPOT(1))), "hamsx_solve_diffraction")
+  end subroutine
+
+  ! ISX / ISY of the .pnl header (NSYS = 2 when one is 1): the mesh is the half
+  ! on the positive side of the plane; the solver MXPOT arguments then hold
+  ! MXPOT(TNELEM, ncol, 2).
+  subroutine SET_SYMMETRY_GPU(ctx, ISX, ISY)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: ISX, ISY
+    call check(hamsx_set_symmetry(ctx, int(ISX, c_int), int(ISY, c_int)), "hamsx_set_symmetry")
   end subroutine
 
   ! AMAS/BDMP(6NB, 6NB) as AMAS_MULTI_COMB(KK,:,:); EXFC(6NB, NBETA).

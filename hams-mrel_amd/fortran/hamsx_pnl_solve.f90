@@ -16,11 +16,11 @@ program hamsx_pnl_solve
   use hamsx_mod
   implicit none
   character(len=1024) :: inp, pnl
-  integer :: u, nelem, nnode, isx, isy, i, k, nfreq, nbeta, prec, info, kf, idum
+  integer :: u, nelem, nnode, isx, isy, i, k, nfreq, nbeta, prec, info, kf, idum, nsys
   real(c_double) :: xr(3), h, v, wk, w1, wvn(HAMSX_NK)
   real(c_double), allocatable :: xyz(:, :), cen(:, :), ds(:), pnsz(:), dxyz(:, :), freqs(:), beta(:)
   real(c_double), allocatable :: amas(:, :), bdmp(:, :)
-  complex(c_double_complex), allocatable :: exfc(:, :), rad(:, :), dif(:, :)
+  complex(c_double_complex), allocatable :: exfc(:, :), rad(:, :, :), dif(:, :, :)
   integer, allocatable :: ncn(:), ncon(:, :)
   type(c_ptr) :: ctx
   real(c_double), parameter :: pi = 3.141592653589793d0
@@ -68,8 +68,14 @@ program hamsx_pnl_solve
 
   call GPU_INIT(ctx, 0, prec)
   call SET_MESH_MULTI_GPU(ctx, nelem, nnode, xyz, ncn, ncon, cen, ds, pnsz, dxyz, 1, [nelem])
+  ! ISX / ISY = 1: the mesh is the half on the positive side of x = 0 / y = 0 (NSYS = 2)
+  nsys = 1
+  if (isx /= 0 .or. isy /= 0) then
+    call SET_SYMMETRY_GPU(ctx, isx, isy)
+    nsys = 2
+  end if
   call SET_WAVES_MULTI_GPU(ctx, nbeta, beta, 1.0d0, [0.0d0, 0.0d0], 1000.0d0, 1)
-  allocate(amas(6, 6), bdmp(6, 6), exfc(6, max(nbeta, 1)), rad(nelem, 6), dif(nelem, max(nbeta, 1)))
+  allocate(amas(6, 6), bdmp(6, 6), exfc(6, max(nbeta, 1)), rad(nelem, 6, nsys), dif(nelem, max(nbeta, 1), nsys))
   do kf = 1, nfreq
     call CALWAVEPROPERTS_GPU(ctx, 3, freqs(kf), h, v, wk, w1, wvn)
     call CALGREEN_ASSB_LEFT_GPU(ctx, info)

@@ -2087,47 +2087,60 @@ static double rxy_tab(int a, int b) { return (a == 2 && b == 2) ? -1.0 : 1.0; }
 
 int orc_solve_frequency_sym(int nelem, int nnode, const double *xyz, const int *ncn, const int *ncon,
                             const double *cen, const double *area, const double *pnsz, const double *dxyz,
-                            int nbody, const int *body_start, int isx, double w1, double h, int nbeta,
-                            const double *beta, double amp, double xw0, double xw1, double rho, int isol,
-                            double *amas, double *bdmp, double *exfc, double *phi2, int nthreads)
+                            int nbody, const int *body_start, int isx, int nlid, const double *lid_cen, double w1,
+                            double h, int nbeta, const double *beta, double amp, double xw0, double xw1, double rho,
+                            int isol, double *amas, double *bdmp, double *exfc, double *phi2, int nthreads)
 {
-    const int nk = 200, n = nelem, nd = 6 * nbody, nr = nd + nbeta;
+    const int nk = 200, n = nelem, nd = 6 * nbody, nr = nd + nbeta, nf = nelem + nlid;
     double wvn[200] = {0};
     double v, wk;
     wave_setup(w1, h, nk, &v, &wk, wvn);
     /* reflection of a point for mirror index m (1: identity, 2: mirrored) */
     #define REFX(m, x) ((isx && (m) == 2) ? -(x) : (x))
     #define REFY(m, y) ((!isx && (m) == 2) ? -(y) : (y))
-    cplx *T = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2), *S = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2);
+    /* rows: the hull centroids, then the lid points (IRSP = 1).  FLAG of a
+     * hull row comes from the unmirrored centroid (CalGreenFuncMulti.f90:134-139);
+     * of a lid row from the point evaluated, mirrored for IS = 2, as
+     * CALGREEN_MULTI_IRR computes it for the Green function (:278-284; its
+     * assembly, AssbMatx_irrMulti.f90:125-131, takes the unmirrored distance,
+     * which differs only for lid points near 50 PNSZ from one image and not
+     * the other) */
+    cplx *T = (cplx *)malloc(sizeof(cplx) * (size_t)nf * n * 2), *S = (cplx *)malloc(sizeof(cplx) * (size_t)nf * n * 2);
     (void)nthreads;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < nf; i++)
         for (int j = 0; j < n; j++)
             for (int is = 1; is <= 2; is++) {
-                const double xi = REFX(is, cen[i]), yi = REFY(is, cen[i + n]), zi = cen[i + 2 * n];
+                const int l = i - n;
+                const double px = i < n ? cen[i] : lid_cen[l], py = i < n ? cen[i + n] : lid_cen[l + nlid];
+                const double pz = i < n ? cen[i + 2 * n] : lid_cen[l + 2 * nlid];
+                const double xi = REFX(is, px), yi = REFY(is, py), zi = pz;
                 cplx t, s;
-                field_entry_flag(xi, yi, zi, cen[i], cen[i + n], cen[i + 2 * n], j, n, nnode, xyz, ncn, ncon, cen, area,
-                                 pnsz, dxyz, v, h, wvn, nk, &t, &s);
-                T[i + (size_t)n * j + (size_t)n * n * (is - 1)] = t;
-                S[i + (size_t)n * j + (size_t)n * n * (is - 1)] = s;
+                if (i < n)
+                    field_entry_flag(xi, yi, zi, px, py, pz, j, n, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h,
+                                     wvn, nk, &t, &s);
+                else
+                    field_entry(xi, yi, zi, j, n, nnode, xyz, ncn, ncon, cen, area, pnsz, dxyz, v, h, wvn, nk, &t, &s);
+                T[i + (size_t)nf * j + (size_t)nf * n * (is - 1)] = t;
+                S[i + (size_t)nf * j + (size_t)nf * n * (is - 1)] = s;
             }
-    /* AMAT(:,:,IP) */
-    cplx *A = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2);
+    /* AMAT(:,:,IP): 2 pi on the hull diagonal, none on lid rows */
+    cplx *A = (cplx *)malloc(sizeof(cplx) * (size_t)nf * n * 2);
     for (int ip = 1; ip <= 2; ip++)
         for (int j = 0; j < n; j++)
-            for (int i = 0; i < n; i++) {
+            for (int i = 0; i < nf; i++) {
                 cplx a = (i == j) ? 2.0 * PI_C : 0.0;
-                for (int is = 1; is <= 2; is++) a = a + rxy_tab(is, ip) * T[i + (size_t)n * j + (size_t)n * n * (is - 1)];
-                A[i + (size_t)n * j + (size_t)n * n * (ip - 1)] = a;
+                for (int is = 1; is <= 2; is++) a = a + rxy_tab(is, ip) * T[i + (size_t)nf * j + (size_t)nf * n * (is - 1)];
+                A[i + (size_t)nf * j + (size_t)nf * n * (ip - 1)] = a;
             }
-    /* right-hand sides, BRMAT / BDMAT (N x nr x 2) */
-    cplx *B = (cplx *)calloc((size_t)n * nr * 2, sizeof(cplx));
-    for (int i = 0; i < n; i++) {
+    /* right-hand sides, BRMAT / BDMAT (nf x nr x 2) */
+    cplx *B = (cplx *)calloc((size_t)nf * nr * 2, sizeof(cplx));
+    for (int i = 0; i < nf; i++) {
         for (int b = 0; b < nbody; b++) {
             cplx bt[6][2] = {{0}};
             for (int j = body_start[b]; j < body_start[b + 1]; j++)
                 for (int is = 1; is <= 2; is++) {
-                    const cplx dum = S[i + (size_t)n * j + (size_t)n * n * (is - 1)];
+                    const cplx dum = S[i + (size_t)nf * j + (size_t)nf * n * (is - 1)];
                     for (int ip = 1; ip <= 2; ip++) {
                         const double sy = sy_tab(is, ip);
                         /* f(IS,IP,MD): ISX: MD 1,5,6 carry SY; ISY: MD 2,4,6 carry SY */
@@ -2143,14 +2156,14 @@ int orc_solve_frequency_sym(int nelem, int nnode, const double *xyz, const int *
                 for (int ip = 1; ip <= 2; ip++) {
                     cplx acc = 0;
                     for (int is = 1; is <= 2; is++) acc = acc + rxy_tab(ip, is) * bt[md][is - 1];
-                    B[i + (size_t)n * (6 * b + md) + (size_t)n * nr * (ip - 1)] = acc;
+                    B[i + (size_t)nf * (6 * b + md) + (size_t)nf * nr * (ip - 1)] = acc;
                 }
         }
         for (int k = 0; k < nbeta; k++) {
             cplx bt[2] = {0, 0};
             for (int j = 0; j < n; j++)
                 for (int is = 1; is <= 2; is++) {
-                    const cplx dum1 = S[i + (size_t)n * j + (size_t)n * n * (is - 1)];
+                    const cplx dum1 = S[i + (size_t)nf * j + (size_t)nf * n * (is - 1)];
                     for (int ip = 1; ip <= 2; ip++) {
                         const double sy = sy_tab(is, ip);
                         const double xq = isx ? sy * cen[j] : cen[j], yq = isx ? cen[j + n] : sy * cen[j + n];
@@ -2165,11 +2178,35 @@ int orc_solve_frequency_sym(int nelem, int nnode, const double *xyz, const int *
             for (int ip = 1; ip <= 2; ip++) {
                 cplx acc = 0;
                 for (int is = 1; is <= 2; is++) acc = acc + rxy_tab(ip, is) * bt[is - 1];
-                B[i + (size_t)n * (nd + k) + (size_t)n * nr * (ip - 1)] = acc;
+                B[i + (size_t)nf * (nd + k) + (size_t)nf * nr * (ip - 1)] = acc;
             }
         }
     }
     free(T); free(S);
+    if (nlid > 0) {
+        /* CMAT(:,:,IP) = A_IP^T A_IP, DRMAT / DDMAT = A_IP^T B_IP (plain transpose,
+         * AssbMatx_irrMulti.f90:154-171, :300-316, :447-461) in place of A, B */
+        cplx *Cn = (cplx *)malloc(sizeof(cplx) * (size_t)n * n * 2), *Dn = (cplx *)malloc(sizeof(cplx) * (size_t)n * nr * 2);
+        for (int ip = 0; ip < 2; ip++) {
+            const cplx *Ap = A + (size_t)nf * n * ip, *Bp = B + (size_t)nf * nr * ip;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    cplx sum = 0;
+                    for (int k = 0; k < nf; k++) sum = sum + Ap[k + (size_t)nf * i] * Ap[k + (size_t)nf * j];
+                    Cn[i + (size_t)n * j + (size_t)n * n * ip] = sum;
+                }
+            for (int q = 0; q < nr; q++)
+                for (int i = 0; i < n; i++) {
+                    cplx sum = 0;
+                    for (int k = 0; k < nf; k++) sum = sum + Bp[k + (size_t)nf * q] * Ap[k + (size_t)nf * i];
+                    Dn[i + (size_t)n * q + (size_t)n * nr * ip] = sum;
+                }
+        }
+        free(A); free(B);
+        A = Cn;
+        B = Dn;
+    }
     int info = 0;
     int *ipiv = (int *)malloc(sizeof(int) * n);
     for (int ip = 0; ip < 2; ip++) {

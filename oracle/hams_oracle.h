@@ -157,10 +157,10 @@ int orc_motion(int nbody, double w1, const double *amas, const double *bdmp, con
 int orc_solve_frequency_sym(int nelem, int nnode, const double *node_xyz, const int *ncn,
                             const int *ncon, const double *cen, const double *area,
                             const double *pnsz, const double *dxyz, int nbody,
-                            const int *body_start, int isx, double w1, double h, int nbeta,
-                            const double *beta, double amp, double xw0, double xw1, double rho,
-                            int isol, double *amas, double *bdmp, double *exfc, double *phi2,
-                            int nthreads);
+                            const int *body_start, int isx, int nlid, const double *lid_cen,
+                            double w1, double h, int nbeta, const double *beta, double amp,
+                            double xw0, double xw1, double rho, int isol, double *amas,
+                            double *bdmp, double *exfc, double *phi2, int nthreads);
 
 #ifdef __cplusplus
 }

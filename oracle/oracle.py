@@ -97,8 +97,8 @@ def _load(path):
         o.orc_motion.argtypes = [C.c_int, _d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         o.orc_solve_frequency_sym.restype = C.c_int
         o.orc_solve_frequency_sym.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip,
-                                              C.c_int, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp,
-                                              _dp, C.c_int]
+                                              C.c_int, C.c_int, _dp, _d, _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int,
+                                              _dp, _dp, _dp, _dp, C.c_int]
         o.orc_solve_frequency.restype = C.c_int
         o.orc_solve_frequency.argtypes = [C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip, _d,
                                           _d, C.c_int, _dp, _d, _d, _d, _d, C.c_int, _dp, _dp, _dp, C.c_int]
@@ -321,8 +321,10 @@ def forces(m, phi, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rh
 
 
 def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0,
-                        isol=1, nthreads: int = 8, potentials: bool = False, fma: bool = False):
-    """NSYS = 2 on the half mesh m (isx = 1: mirrored in x, else in y)."""
+                        isol=1, nthreads: int = 8, potentials: bool = False, fma: bool = False, lid_cen=None):
+    """NSYS = 2 on the half mesh m (isx = 1: mirrored in x, else in y);
+    lid_cen (L x 3): the half lid's points (IRSP = 1), or None."""
+    lc = np.asfortranarray(np.zeros((0, 3)) if lid_cen is None else lid_cen, dtype=np.float64)
     nd = 6 * m.nbody
     beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
     amas = np.zeros((nd, nd), order="F")
@@ -331,7 +333,8 @@ def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0,
     phi2 = np.zeros((m.nelem, nd + len(beta), 2), dtype=np.complex128, order="F")
     info = lib(fma).orc_solve_frequency_sym(m.nelem, m.nnode, _p(m.xyz), _p(m.ncn, C.c_int), _p(m.ncon, C.c_int),
                                          _p(m.cen), _p(m.area), _p(m.pnsz), _p(m.dxyz), m.nbody,
-                                         _p(m.body_start, C.c_int), int(isx), w1, h, len(beta), _p(beta), amp, xw[0],
+                                         _p(m.body_start, C.c_int), int(isx), lc.shape[0],
+                                         _p(lc) if lc.shape[0] else None, w1, h, len(beta), _p(beta), amp, xw[0],
                                          xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), _p(phi2), nthreads)
     out = (amas, bdmp, exfc[:, :len(beta)], info)
     return out + (phi2,) if potentials else out

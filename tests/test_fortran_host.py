@@ -74,3 +74,33 @@ def test_fortran_host_equals_python_host(tmp_path):
                 np.testing.assert_array_equal(am, a2)
                 np.testing.assert_array_equal(bd, b2)
                 np.testing.assert_array_equal(ex, e2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("isx", [1, 0])
+def test_fortran_host_half_mesh_with_symmetry(tmp_path, isx):
+    """A half-mesh .pnl with ISX (or ISY) = 1 in its header: the Fortran host
+    sets the symmetry plane and gives the Python host's numbers bit for bit."""
+    from hamsx.deck import PanelMesh
+    from hamsx.solver import FrequencySolver
+    from test_oracle_symmetry import half
+    freqs, heads = [0.9], [0.0, 30.0]
+    h = half(cylinder(4.0, 6.0, 20, 5, 3), 0 if isx else 1)
+    hm = PanelMesh(h.xyz, h.ncn, h.ncon, isx, 1 - isx)
+    pnl = tmp_path / "half.pnl"
+    write_pnl(str(pnl), hm)
+    inp = tmp_path / "driver.in"
+    inp.write_text(f"'{pnl}'\n0.0 0.0 0.0\n30.0\n1\n0.9\n2\n0.0 30.0\n0\n")
+    print(f"[gpu] fortran host symmetry isx={isx}", flush=True)
+    p = subprocess.run([DRIVER, str(inp)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    (w, am, bd, ex), = parse(p.stdout, len(heads))
+    mesh = combine([read_pnl(str(pnl))], [np.zeros(3)])
+    with FrequencySolver(mesh) as s:
+        s.set_waves(np.deg2rad(heads))
+        s.set_symmetry(isx, 1 - isx)
+        a2, b2, e2, info = s.solve_frequency(freqs[0], 30.0)
+    assert info == 0
+    np.testing.assert_array_equal(am, a2)
+    np.testing.assert_array_equal(bd, b2)
+    np.testing.assert_array_equal(ex, e2)

@@ -160,3 +160,76 @@ def test_symmetric_field_values_match_oracle(axis, depth, chunk, monkeypatch):
     say(f"    onbody {list(ob)}  rel err {e:.3e}  floor {floor:.1e}")
     assert list(ob) == list(obr) == [False, False, False, True, True]
     assert e <= max(1e-10, 4 * floor)
+
+
+def half_lid(axis):
+    """Waterplane lid points of the two cylinders of two_bodies(axis) on the
+    positive side of the plane, and the full set."""
+    from hamsx.mesh import panel_geometry, waterplane_disk
+    pts = []
+    for r, c in ((4.0, -12.0), (3.0, 12.0)):
+        lid, _, _, _ = panel_geometry(waterplane_disk(r, 24 if r == 4.0 else 20, 3), np.zeros(3))
+        lid = np.array(lid)
+        lid[:, 1 - axis] += c
+        pts.append(lid)
+    full = np.vstack(pts)
+    return full[full[:, axis] > 0], full
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("depth,w", [(-1.0, 1.4), (30.0, 1.2)])
+def test_symmetric_solve_with_lid_matches_oracle(axis, depth, w):
+    """IRSP = 1 with a symmetry plane: half hull and half lid, each system
+    through its own normal equations, against the oracle; the order of
+    hamsx_set_lid and hamsx_set_symmetry does not matter (bit for bit)."""
+    bodies, xr = two_bodies(axis)
+    m = combine([half(b, axis) for b in bodies], xr)
+    lh, _ = half_lid(axis)
+    heads = np.deg2rad([0.0, 45.0])
+    isx = 1 if axis == 0 else 0
+    say(f"[gpu] symmetry + lid axis={axis} n={m.nelem} lid={len(lh)} depth={depth} w={w}")
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        s.set_lid(lh)
+        s.set_symmetry(isx, 1 - isx)
+        am, bd, ex, info = s.solve_frequency(w, depth)
+        assert info == 0
+    with FrequencySolver(m) as s:
+        s.set_waves(heads)
+        s.set_symmetry(isx, 1 - isx)
+        s.set_lid(lh)
+        am2, bd2, ex2, _ = s.solve_frequency(w, depth)
+    for a, b in ((am, am2), (bd, bd2), (ex, ex2)):
+        np.testing.assert_array_equal(a, b)
+    ao, bo, eo, io = oracle.solve_frequency_sym(m, isx, w, depth, heads, lid_cen=lh)
+    assert io == 0
+    errs = rel(am, ao), rel(bd, bo), rel(ex, eo)
+    floor = (0.0, 0.0, 0.0)
+    if depth > 0:
+        af, bf, ef, _ = oracle.solve_frequency_sym(m, isx, w, depth, heads, lid_cen=lh, fma=True)
+        floor = rel(af, ao), rel(bf, bo), rel(ef, eo)
+    say(f"    rel err AMAS {errs[0]:.3e} BDMP {errs[1]:.3e} EXFC {errs[2]:.3e}  "
+        f"floor {floor[0]:.1e} {floor[1]:.1e} {floor[2]:.1e}")
+    for e, f in zip(errs, floor):
+        assert e <= max(1e-8, 4 * f)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_half_mesh_with_lid_equals_full_mesh_on_gpu(axis):
+    bodies, xr = two_bodies(axis)
+    lh, lf = half_lid(axis)
+    heads = np.deg2rad([0.0, 60.0])
+    say(f"[gpu] symmetry + lid half vs full axis={axis}")
+    with FrequencySolver(combine(bodies, xr)) as s:
+        s.set_waves(heads)
+        s.set_lid(lf)
+        af, bf, ef, _ = s.solve_frequency(1.4, -1.0)
+    with FrequencySolver(combine([half(b, axis) for b in bodies], xr)) as s:
+        s.set_waves(heads)
+        s.set_lid(lh)
+        s.set_symmetry(*((1, 0) if axis == 0 else (0, 1)))
+        ah, bh, eh, info = s.solve_frequency(1.4, -1.0)
+    assert info == 0
+    errs = rel(ah, af), rel(bh, bf), rel(eh, ef)
+    say(f"    rel err {errs}")
+    assert max(errs) <= 1e-9

@@ -83,3 +83,31 @@ def test_field_values_half_mesh_equal_full_mesh(axis, depth):
     assert list(of) == list(oh) and oh[-1] and not oh[0]
     tol = 1e-12 if depth < 0 else 1e-6
     assert rel(vh, vf) <= tol, rel(vh, vf)
+
+
+def lid_half(lid_cen, axis):
+    return lid_cen[lid_cen[:, axis] > 0]
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("depth,w,tol", [(-1.0, 1.6, 1e-12), (8.0, 1.3, 1e-6)])
+def test_half_mesh_with_lid_and_symmetry_equals_full_mesh(axis, depth, w, tol):
+    """IRSP = 1 with one symmetry plane (CALGREEN_MULTI_IRR's IS loops,
+    ASSB_LEFT/RBC/DBC_IRR_MULTI with NSYS = 2): the half hull and half lid
+    with the plane against the full hull and lid, each through the normal
+    equations C = A^T A of its own system."""
+    from hamsx.mesh import panel_geometry, waterplane_disk
+    full = cylinder(2.0, 3.0, 16, 5, 3)
+    lid, _, _, _ = panel_geometry(waterplane_disk(2.0, 16, 2), np.zeros(3))
+    mf = combined_with_oracle_geometry([full], [np.zeros(3)])
+    mh = combined_with_oracle_geometry([half(full, axis)], [np.zeros(3)])
+    lh = lid_half(lid, axis)
+    assert 2 * len(lh) == len(lid)
+    heads = np.deg2rad([0.0, 30.0, 90.0])
+    af, bf, ef, _ = oracle.solve_frequency_irr(mf, lid, w, depth, heads)
+    ah, bh, eh, info = oracle.solve_frequency_sym(mh, 1 if axis == 0 else 0, w, depth, heads, lid_cen=lh)
+    assert info == 0
+    errs = rel(ah, af), rel(bh, bf), rel(eh, ef)
+    assert max(errs) <= tol, errs
+    a0, _, _, _ = oracle.solve_frequency_sym(mh, 1 if axis == 0 else 0, w, depth, heads)
+    assert rel(a0, af) > 10 * tol          # the lid changes the answer
