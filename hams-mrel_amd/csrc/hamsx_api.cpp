@@ -273,7 +273,7 @@ struct hamsx_ctx {
   bool have_waves = false;
   // matrices and workspace
   DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag, mvw;
-  DevBuf<float2> Af;
+  DevBuf<float2> Af, Af2;  // complex64 factors (Af2: symmetry system 2)
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins of the system's rows
   PairBins bins;
@@ -313,7 +313,7 @@ struct hamsx_ctx {
   DevBuf<size_t> pLuOff, pXoff;  // per body: factor offset, packed right-hand-side offset
   int pMax = 0;
   std::vector<size_t> pOff;
-  DevBuf<double> out, norms;
+  DevBuf<double> out, norms, norms2;  // refinement norms (norms2: symmetry system 2)
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   bool pending = false;  // a submitted frequency solve not yet collected
   bool have_C = false;   // C = A^T A is current (lid only)
@@ -505,6 +505,7 @@ int ensure_work(hamsx_ctx* c) {
         (rc = c->Af.ensure(n * n)) || (rc = c->norms.ensure(2 * nr)) ||
         (rc = c->mvw.ensure(hamsx::launch::matvec_workspace((int)n, (int)nr))))
       return rc;
+    if (c->sym() && ((rc = c->Af2.ensure(n * n)) || (rc = c->norms2.ensure(2 * nr)))) return rc;
   }
   return 0;
 }
@@ -619,13 +620,21 @@ int do_factor(hamsx_ctx* c) {
   if (c->sym() && ((rc = c->ipiv2.ensure(n)) || (rc = c->perm2.ensure(n)))) return rc;
   StageTimer tm(c, 2);
   if (c->sym()) {  // the two systems A +- T2, or their normal matrices (INFO of system 2 in flags[5])
-    if (c->prec != HAMSX_PREC_FP64) return fail(HAMSX_EARG, "a symmetry plane needs FP64 precision");
     if ((rc = ensure_normal_matrix(c))) return rc;
-    hamsx::launch::lu_factor(c->stream, system_matrix(c), n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                             c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
-    hamsx::launch::lu_factor(c->stream, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
-                             c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
-    if (c->nlid == 0) c->have_A = false;
+    if (c->prec == HAMSX_PREC_FP64) {
+      hamsx::launch::lu_factor(c->stream, system_matrix(c), n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
+                               c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
+      hamsx::launch::lu_factor(c->stream, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
+                               c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
+      if (c->nlid == 0) c->have_A = false;
+    } else {  // complex64 factors; the FP64 matrices stay for the residuals
+      hamsx::launch::to_c64(c->stream, system_matrix(c), c->Af.p, (size_t)n * n);
+      hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
+                                   c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
+      hamsx::launch::to_c64(c->stream, system_matrix2(c), c->Af2.p, (size_t)n * n);
+      hamsx::launch::lu_factor_c64(c->stream, c->Af2.p, n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
+                                   c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
+    }
     c->have_LU = true;
     return check_launch();
   }
@@ -658,7 +667,7 @@ int do_factor(hamsx_ctx* c) {
 namespace {
 using cplx = std::complex<double>;
 
-int gmres_precond_setup(hamsx_ctx* c) {
+int gmres_precond_setup(hamsx_ctx* c, const double2* M) {
   const int nb = c->nbody;
   c->pOff.assign(nb + 1, 0);
   size_t maxb = 0;
@@ -676,7 +685,7 @@ int gmres_precond_setup(hamsx_ctx* c) {
   for (int b = 0; b < nb; b++) {
     const int r0 = c->body_start[b], sz = c->body_start[b + 1] - r0;
     double2* blk = c->pLU.p + c->pOff[b];
-    hamsx::launch::block_copy(c->stream, system_matrix(c), c->n, r0, sz, blk);
+    hamsx::launch::block_copy(c->stream, M, c->n, r0, sz, blk);
     hamsx::launch::lu_factor(c->stream, blk, sz, c->pIpiv.p, c->pPerm.p + r0, c->widx.p, c->wcnt.p,
                              c->flags.p + 1, c->ludiag.p);
   }
@@ -909,7 +918,9 @@ void op_matvec(hamsx_ctx* c, const double2* Aop, const double2* X, int nrhs, con
     hamsx::launch::matvec(c->stream, Aop, c->n, X, nrhs, B, alpha, beta, Y, c->mvw.p);
 }
 
-int do_solve_gmres(hamsx_ctx* c) {
+// X <- Aop^{-1} X; the iteration count adds to last_iters, the final relative
+// residual raises last_resid
+int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
   const int n = c->n, nr = c->nrhs(), m = c->gm_restart;
   const size_t blk = (size_t)n * nr;
   int rc;
@@ -919,14 +930,12 @@ int do_solve_gmres(hamsx_ctx* c) {
       (rc = c->mvw.ensure(hamsx::launch::matvec_workspace(n, nr))))
     return rc;
   if ((rc = c->gP.ensure(blk)) || (rc = c->gPt.ensure(blk))) return rc;
-  if ((rc = ensure_normal_matrix(c))) return rc;
-  const double2* Aop = system_matrix(c);
-  if ((rc = gmres_precond_setup(c))) return rc;
+  if ((rc = gmres_precond_setup(c, Aop))) return rc;
   c->hm_built = false;
   if (c->hm_on && (rc = hm_build(c, Aop, nr))) return rc;
   // B = right-hand sides, X = 0
-  HX_TRY(hipMemcpyAsync(c->gB.p, c->X.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
-  HX_TRY(hipMemsetAsync(c->X.p, 0, sizeof(double2) * blk, c->stream));
+  HX_TRY(hipMemcpyAsync(c->gB.p, X, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
+  HX_TRY(hipMemsetAsync(X, 0, sizeof(double2) * blk, c->stream));
   std::vector<double> bnorm(nr), beta(nr), hn(nr), scale(nr);
   std::vector<cplx> H((size_t)(m + 1) * m * nr), g((size_t)(m + 1) * nr), sn((size_t)m * nr);
   std::vector<double> cs((size_t)m * nr);
@@ -943,7 +952,7 @@ int do_solve_gmres(hamsx_ctx* c) {
     if (iter == 0)
       HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
     else
-      op_matvec(c, Aop, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p);
+      op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
     hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
     HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
     HX_TRY(hipStreamSynchronize(c->stream));
@@ -1041,43 +1050,63 @@ int do_solve_gmres(hamsx_ctx* c) {
                           c->stream));
     hamsx::launch::block_combine(c->stream, c->gV.p, j, n, nr, c->gDots.p, c->gU.p);
     gmres_precond_apply(c, c->gU.p, c->gZ.p, nr);
-    hamsx::launch::axpy_update(c->stream, c->X.p, c->gZ.p, blk);
+    hamsx::launch::axpy_update(c->stream, X, c->gZ.p, blk);
     HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above are reused next cycle
   }
   // final residual of the operator GMRES worked with, for the statistics
-  op_matvec(c, Aop, c->X.p, nr, c->gB.p, -1.0, 1.0, c->gW.p);
+  op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
   hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
   HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
   worst = 0.0;
   for (int q = 0; q < nr; q++)
     if (bnorm[q] >= 1.0e-30) worst = std::max(worst, beta[q] / bnorm[q]);
-  c->last_iters = iter;
-  c->last_resid = worst;
+  c->last_iters += iter;
+  c->last_resid = std::max(c->last_resid, worst);
   return check_launch();
 }
 }  // namespace
 
+// Mixed precision: X <- M^{-1} X by the complex64 factors Af, then FP64
+// residual / complex64 correction steps until the residual test passes; the
+// test runs on the device and the steps after convergence are launched as
+// no-ops, so nothing waits on the host.  state: [converged, steps]; norms:
+// the residual and solution norms of the last test.
+int refine_solve(hamsx_ctx* c, const double2* M, const float2* Af, const int* perm, double2* X, int* state,
+                 double* norms) {
+  const int n = c->n, nr = c->nrhs();
+  const size_t cnt = (size_t)n * nr;
+  hamsx::launch::refine_reset(c->stream, state);
+  HX_TRY(hipMemcpyAsync(c->B.p, X, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
+  hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, X, c->Xtmp.p, nr);
+  for (int it = 0; it < c->refine_iters; it++) {
+    hamsx::launch::residual(c->stream, M, n, X, c->B.p, c->D.p, nr, norms, c->mvw.p, state);
+    hamsx::launch::refine_check(c->stream, norms, nr, c->refine_tol, state);
+    hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, c->D.p, c->Xtmp.p, nr, state);
+    hamsx::launch::axpy_update(c->stream, X, c->D.p, cnt, state);
+  }
+  // residual of the final iterate (a no-op when the loop converged: the
+  // norms then hold the residual that passed the test)
+  hamsx::launch::residual(c->stream, M, n, X, c->B.p, c->R.p, nr, norms, c->mvw.p, state);
+  return 0;
+}
+
 // X <- A^{-1} X for all 6nb + nbeta columns (with refinement in mixed mode)
 int do_solve(hamsx_ctx* c) {
   if (c->have_X) return 0;
-  if (c->sym() && (c->solver != HAMSX_SOLVER_LU || c->prec != HAMSX_PREC_FP64))
-    return fail(HAMSX_EARG, "a symmetry plane needs the FP64 LU solver");
   int rc = do_rhs(c);
   if (rc) return rc;
-  if (c->sym()) {
-    if ((rc = do_factor(c))) return rc;
-    const int n = c->n, nr = c->nrhs();
-    StageTimer tm(c, 3);
-    hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
-    hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
-    hamsx::launch::sym_recombine(c->stream, c->X.p, c->X2.p, (size_t)n * nr);
-    c->have_X = true;
-    return check_launch();
-  }
   if (c->solver == HAMSX_SOLVER_GMRES) {
     StageTimer tm(c, 3);
-    if ((rc = do_solve_gmres(c))) return rc;
+    if ((rc = ensure_normal_matrix(c))) return rc;
+    c->last_iters = 0;
+    c->last_resid = 0.0;
+    if ((rc = do_solve_gmres(c, system_matrix(c), c->X.p))) return rc;
+    if (c->sym()) {  // the second system, then MXPOT of both halves
+      if ((rc = do_solve_gmres(c, system_matrix2(c), c->X2.p))) return rc;
+      hamsx::launch::sym_recombine(c->stream, c->X.p, c->X2.p, (size_t)c->n * c->nrhs());
+      if ((rc = check_launch())) return rc;
+    }
     c->have_X = true;
     return 0;
   }
@@ -1086,27 +1115,14 @@ int do_solve(hamsx_ctx* c) {
   StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    if (c->sym()) hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
   } else {
-    // complex64 solve, then FP64 residual / complex64 correction steps until
-    // the residual test passes; the test runs on the device and the steps
-    // after convergence are launched as no-ops, so nothing waits on the host
-    const size_t cnt = (size_t)n * nr;
-    int* state = c->flags.p + 2;
-    hamsx::launch::refine_reset(c->stream, state);
-    HX_TRY(hipMemcpyAsync(c->B.p, c->X.p, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
-    hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->X.p, c->Xtmp.p, nr);
-    for (int it = 0; it < c->refine_iters; it++) {
-      hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->D.p, nr, c->norms.p, c->mvw.p,
-                              state);
-      hamsx::launch::refine_check(c->stream, c->norms.p, nr, c->refine_tol, state);
-      hamsx::launch::lu_solve_c64(c->stream, c->Af.p, n, c->perm.p, c->D.p, c->Xtmp.p, nr, state);
-      hamsx::launch::axpy_update(c->stream, c->X.p, c->D.p, cnt, state);
-    }
-    // residual of the final iterate (a no-op when the loop converged: the
-    // norms then hold the residual that passed the test)
-    hamsx::launch::residual(c->stream, system_matrix(c), n, c->X.p, c->B.p, c->R.p, nr, c->norms.p, c->mvw.p,
-                            state);
+    if ((rc = refine_solve(c, system_matrix(c), c->Af.p, c->perm.p, c->X.p, c->flags.p + 2, c->norms.p))) return rc;
+    if (c->sym() &&
+        (rc = refine_solve(c, system_matrix2(c), c->Af2.p, c->perm2.p, c->X2.p, c->flags.p + 6, c->norms2.p)))
+      return rc;
   }
+  if (c->sym()) hamsx::launch::sym_recombine(c->stream, c->X.p, c->X2.p, (size_t)n * nr);
   c->have_X = true;
   return check_launch();
 }
@@ -1126,8 +1142,8 @@ int fetch_mxpot(hamsx_ctx* c, int col0, int ncol, double* dst) {
 // refinement norms into the pinned staging area (no host wait).
 int enqueue_forces(hamsx_ctx* c) {
   const size_t len = c->out_len();
-  // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 6 ints][norms: 2 * nrhs]
-  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 3 + 2 * (size_t)c->nrhs());
+  // staging layout: [wvn: NK][beta: nbeta][results: len][flags: 8 ints][norms: 2 * nrhs][norms2: 2 * nrhs]
+  int rc = ensure_pinned(c, hamsx::kNK + c->nbeta + len + 4 + 4 * (size_t)c->nrhs());
   if (rc) return rc;
   double* hout = c->h_pinned + hamsx::kNK + c->nbeta;
   {
@@ -1137,9 +1153,12 @@ int enqueue_forces(hamsx_ctx* c) {
   }
   if ((rc = check_launch())) return rc;
   HX_TRY(hipMemcpyAsync(hout, c->out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c->stream));
-  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 6, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(hout + len, c->flags.p, sizeof(int) * 8, hipMemcpyDeviceToHost, c->stream));
+  if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0 && c->sym() && c->solver == HAMSX_SOLVER_LU)
+    HX_TRY(hipMemcpyAsync(hout + len + 4 + 2 * (size_t)c->nrhs(), c->norms2.p, sizeof(double) * 2 * c->nrhs(),
+                          hipMemcpyDeviceToHost, c->stream));
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0)
-    HX_TRY(hipMemcpyAsync(hout + len + 3, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
+    HX_TRY(hipMemcpyAsync(hout + len + 4, c->norms.p, sizeof(double) * 2 * c->nrhs(), hipMemcpyDeviceToHost,
                           c->stream));
   return 0;
 }
@@ -1154,20 +1173,28 @@ int complete_forces(hamsx_ctx* c, double* amas, double* bdmp, double* exfc, int*
   if (amas) std::memcpy(amas, hout, sizeof(double) * nd * nd);
   if (bdmp) std::memcpy(bdmp, hout + (size_t)nd * nd, sizeof(double) * nd * nd);
   if (exfc) std::memcpy(exfc, hout + 2 * (size_t)nd * nd, sizeof(double) * 2 * nd * c->nbeta);
-  int fl[6];  // quadrature overflow, INFO, refinement converged, correction steps, -, INFO of symmetry system 2
+  // quadrature overflow, INFO, refinement converged, correction steps, -, INFO of symmetry
+  // system 2, its refinement converged and steps
+  int fl[8];
   std::memcpy(fl, hout + len, sizeof fl);
   if (info_out) *info_out = fl[1] ? fl[1] : (c->sym() ? fl[5] : 0);
   bool noconv = false;
   if (c->prec == HAMSX_PREC_MIXED && c->nrhs() > 0 && c->solver == HAMSX_SOLVER_LU) {
-    const double* nm = hout + len + 3;
+    const int nsys = c->sym() ? 2 : 1, nr = c->nrhs();
     double worst = 0.0;
-    for (int q = 0; q < c->nrhs(); q++) {
-      double rn = nm[q], xn = nm[c->nrhs() + q];
-      worst = std::max(worst, xn > 0 ? rn / xn : rn);
+    c->last_iters = 0;
+    for (int ip = 0; ip < nsys; ip++) {
+      const double* nm = hout + len + 4 + 2 * (size_t)nr * ip;
+      double w = 0.0;
+      for (int q = 0; q < nr; q++) {
+        double rn = nm[q], xn = nm[nr + q];
+        w = std::max(w, xn > 0 ? rn / xn : rn);
+      }
+      worst = std::max(worst, w);
+      c->last_iters += fl[3 + 4 * ip];
+      noconv = noconv || (!fl[2 + 4 * ip] && c->refine_tol > 0.0 && w > c->refine_tol);
     }
     c->last_resid = worst;
-    c->last_iters = fl[3];
-    noconv = !fl[2] && c->refine_tol > 0.0 && worst > c->refine_tol;
   }
   if (fl[0] & 1) return fail(HAMSX_EDEVICE, "adaptive quadrature stack exhausted in the finite-depth kernel");
   if (noconv)
@@ -1234,13 +1261,13 @@ int hamsx_create(hamsx_ctx** out, int device) {
     hamsx_destroy(c);
     return fail(HAMSX_EDEVICE, "stream/event creation failed on device %d", device);
   }
-  int rc = c->flags.ensure(6);
+  int rc = c->flags.ensure(8);
   if (rc == 0) rc = c->wvn.ensure(hamsx::kNK);
   if (rc) {
     hamsx_destroy(c);
     return rc;
   }
-  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 6, c->stream) != hipSuccess) {
+  if (hipMemsetAsync(c->flags.p, 0, sizeof(int) * 8, c->stream) != hipSuccess) {
     hamsx_destroy(c);
     return fail(HAMSX_EDEVICE, "flag initialisation failed");
   }
@@ -1545,7 +1572,7 @@ int hamsx_set_frequency(hamsx_ctx* c, double v, double wk, double w1, double h, 
   if (h > 0.0 && (!wvn || nk < hamsx::kNK)) return fail(HAMSX_EARG, "finite depth needs %d dispersion roots", hamsx::kNK);
   int rc = set_device(c);
   if (rc) return rc;
-  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 3 + 2 * (size_t)c->nrhs())))
+  if ((rc = ensure_pinned(c, hamsx::kNK + std::max(1, c->nbeta) + c->out_len() + 4 + 4 * (size_t)c->nrhs())))
     return rc;
   // the staging slot is free: every earlier use ended in a stream synchronisation
   HX_TRY(hipStreamSynchronize(c->stream));

@@ -100,8 +100,8 @@ def test_half_mesh_equals_full_mesh_on_gpu(axis):
 
 
 def test_symmetry_arguments_and_state():
-    """Both planes are refused as the reference refuses them; the plane
-    needs the FP64 LU solver; a new mesh removes it."""
+    """Both planes are refused as the reference refuses them; a new mesh
+    removes the plane."""
     bodies, xr = two_bodies(1)
     m = combine([half(b, 1) for b in bodies], xr)
     with FrequencySolver(m) as s:
@@ -109,16 +109,51 @@ def test_symmetry_arguments_and_state():
         with pytest.raises(HamsxError):
             s.set_symmetry(1, 1)
         s.set_symmetry(0, 1)
-        s.set_solver("gmres")
-        with pytest.raises(HamsxError):
-            s.solve_frequency(1.0, -1.0)
-        s.set_solver("lu")
         a1, _, _, _ = s.solve_frequency(1.0, -1.0)
         s.set_mesh(m)
         assert s.nsys == 1
         s.set_waves([0.0])
         a0, _, _, _ = s.solve_frequency(1.0, -1.0)
         assert rel(a0, a1) > 1e-3      # the half mesh alone is a different body
+
+
+@pytest.mark.parametrize("lid", [False, True])
+@pytest.mark.parametrize("depth", [-1.0, 30.0])
+def test_symmetric_gmres_and_mixed_match_fp64_lu(lid, depth):
+    """The other solver paths under a symmetry plane, each running the two
+    systems: GMRES (dense operator, tight tolerance) and the complex64 LU
+    with FP64 refinement against the FP64 LU, with and without a lid."""
+    bodies, xr = two_bodies(0)
+    m = combine([half(b, 0) for b in bodies], xr)
+    lh, _ = half_lid(0)
+    heads = np.deg2rad([0.0, 45.0])
+    w = 1.1
+    say(f"[gpu] symmetry gmres / mixed lid={lid} depth={depth} n={m.nelem}")
+
+    def run(precision="fp64", solver=None):
+        with FrequencySolver(m, precision=precision) as s:
+            s.set_waves(heads)
+            s.set_symmetry(1, 0)
+            if lid:
+                s.set_lid(lh)
+            if solver == "gmres":
+                s.set_hmatrix(False)
+                s.set_solver("gmres", tol=1e-11, restart=25, maxiter=400)
+            out = s.solve_frequency(w, depth)
+            its, res = s.refine_stats()
+        assert out[3] == 0
+        return out[:3], its, res
+
+    ref, _, _ = run()
+    gm, gits, gres = run(solver="gmres")
+    mx, mits, mres = run(precision="mixed")
+    eg = [rel(a, b) for a, b in zip(gm, ref)]
+    em = [rel(a, b) for a, b in zip(mx, ref)]
+    say(f"    gmres {gits} iterations, residual {gres:.1e}, rel err {max(eg):.2e}; "
+        f"mixed {mits} steps, residual {mres:.1e}, rel err {max(em):.2e}")
+    assert max(eg) <= 1e-8
+    assert max(em) <= 1e-6
+
 
 
 @pytest.mark.parametrize("axis", [0, 1])
