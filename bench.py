@@ -144,8 +144,8 @@ def parse():
                     "stream each)")
     ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
     ap.add_argument("--config", default="two-cylinders",
-                    choices=["two-cylinders", "five-cylinders", "one-cylinder", "serving", "large-array",
-                             "wave-field"])
+                    choices=["two-cylinders", "two-cylinders-half", "five-cylinders", "one-cylinder", "serving",
+                             "large-array", "wave-field"])
     ap.add_argument("--requests", type=int, default=40, help="serving: requests in the Poisson stream")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrival rate, requests/s")
     ap.add_argument("--seed", type=int, default=7, help="serving: seed of the request stream")
@@ -181,6 +181,8 @@ def pick_case(name):
         return configs.two_cylinders()
     if name == "five-cylinders":
         return configs.five_cylinders()
+    if name == "two-cylinders-half":
+        return configs.two_cylinders_half()
     return configs.single_cylinder()
 
 
@@ -414,6 +416,20 @@ def wave_field(args):
 
 def main():
     args = parse()
+    # Each context drives two HIP streams (its own and the LU look-ahead's, or
+    # a symmetry plane's second system).  With several contexts in flight the
+    # HIP runtime's default of four hardware queues makes streams share
+    # queues, and a kernel waiting on an event in one stream then stalls the
+    # others: measured 18.9 -> 19.8 solves/s and p99 307 -> 219 ms on the
+    # default workload with eight queues.  Raised (never lowered) before the
+    # runtime starts.
+    want = "8" if args.streams <= 4 else "16"
+    try:
+        have = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        have = 4
+    if have < int(want):
+        os.environ["GPU_MAX_HW_QUEUES"] = want
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -443,6 +459,8 @@ def main():
     for _ in range(nctx):
         sv = FrequencySolver(mesh, precision=args.precision)
         sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        if "symmetry" in case.meta:
+            sv.set_symmetry(*case.meta["symmetry"])
         solvers.append(sv)
     print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(omegas)} frequencies, "
           f"{len(heads)} headings, precision {args.precision}, {nctx} solves in flight", file=sys.stderr, flush=True)
@@ -490,13 +508,33 @@ def main():
     roof_lu = {"kernel": "LU factorisation", "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
                "frac": lu_tf / lu_peak, "launch_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
     roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
+    vs_full = None
+    if "symmetry" in case.meta:
+        # the same frequency on the whole mesh: the plane must not change the answer
+        from hamsx import configs
+        full = configs.two_cylinders(nfreq=len(omegas))
+        with FrequencySolver(combine(full.bodies, full.xr), precision=args.precision) as sv:
+            sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+            t1 = time.perf_counter()
+            fa, fb, fe, _ = sv.solve_frequency(omegas[0], case.depth)
+            t_full = time.perf_counter() - t1
+        with FrequencySolver(mesh, precision=args.precision) as sv:
+            sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+            sv.set_symmetry(*case.meta["symmetry"])
+            ha, hb, he, _ = sv.solve_frequency(omegas[0], case.depth)
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        vs_full = {"w": omegas[0], "amas": rel(ha, fa), "bdmp": rel(hb, fb), "exfc": rel(he, fe),
+                   "full_mesh_solve_ms": t_full * 1e3}
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "c64-lu+f64-refine",
         "data": "synthetic seeded cylinder meshes (no reference files read)",
-        "config": {"workload": case.name, "panels": n, "depth_m": case.depth, "frequencies": len(omegas),
+        "config": {"workload": case.name, "panels": n, "symmetry": case.meta.get("symmetry"),
+                   "depth_m": case.depth, "frequencies": len(omegas),
                    "headings_deg": case.headings_deg, "precision": args.precision, "in_flight": nctx},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
@@ -504,6 +542,8 @@ def main():
         "roofline_lu": roof_lu,
         "roofline_solve": roof_solve,
     }
+    if vs_full is not None:
+        out["vs_full_mesh"] = vs_full
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)

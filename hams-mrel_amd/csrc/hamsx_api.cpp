@@ -279,6 +279,13 @@ struct hamsx_ctx {
   PairBins bins;
   // the LU's look-ahead stream and events (HAMSX_LU_LOOKAHEAD=0 factors on the one stream)
   hamsx::LuAux lu_aux{nullptr, nullptr, nullptr};
+  // symmetry system 2 is factored on its own stream (sym_stream, with its own
+  // scratch) while system 1 is factored on `stream`; created on the first
+  // hamsx_set_symmetry
+  hipStream_t sym_stream = nullptr;
+  hipEvent_t sym_go = nullptr, sym_done = nullptr;
+  DevBuf<int> widx2, wcnt2;
+  DevBuf<double2> ludiag2;
   // field points (CalPotentialMulti): nfp points (SoA); vtx: panel vertices,
   // vtx[(3q + d) n + j]; the field view ffx/ffy/ffz holds the hull centroids
   // and then one chunk of field points; field_gen names the point set
@@ -601,6 +608,7 @@ const hamsx::LuAux* lu_lookahead(hamsx_ctx* c) {
   return c->lu_aux.s2 ? &c->lu_aux : nullptr;
 }
 
+
 // the square matrix the solvers work on: A, or C = A^T A with a lid
 double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
 // symmetry system 2's: A2 (in T2), or C2 = A2^T A2 with a lid
@@ -617,23 +625,48 @@ int do_factor(hamsx_ctx* c) {
       (rc = c->widx.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) || (rc = c->wcnt.ensure((size_t)2 * nsets0)) ||
       (rc = c->ludiag.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB)))
     return rc;
-  if (c->sym() && ((rc = c->ipiv2.ensure(n)) || (rc = c->perm2.ensure(n)))) return rc;
+  if (c->sym() && ((rc = c->ipiv2.ensure(n)) || (rc = c->perm2.ensure(n)) ||
+                   (rc = c->widx2.ensure((size_t)2 * nsets0 * hamsx::kPanelNB)) ||
+                   (rc = c->wcnt2.ensure((size_t)2 * nsets0)) ||
+                   (rc = c->ludiag2.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB))))
+    return rc;
   StageTimer tm(c, 2);
-  if (c->sym()) {  // the two systems A +- T2, or their normal matrices (INFO of system 2 in flags[5])
+  if (c->sym()) {
+    // the two systems A +- T2, or their normal matrices (INFO of system 2 in
+    // flags[5]): system 2 on sym_stream with its own scratch, concurrently
+    // with system 1 on the context stream, which then waits for it.  Both
+    // factorisations are latency-bound chains, so they overlap well; neither
+    // takes the look-ahead stream, which keeps two streams busy per context
+    // (with four contexts in flight, more streams than hardware queues
+    // serialise them: measured slower).
     if ((rc = ensure_normal_matrix(c))) return rc;
+    const bool two = c->sym_stream != nullptr;
+    hipStream_t s2 = two ? c->sym_stream : c->stream;
+    if (two) {
+      HX_TRY(hipEventRecord(c->sym_go, c->stream));
+      HX_TRY(hipStreamWaitEvent(s2, c->sym_go, 0));
+    }
+    int* widx2 = two ? c->widx2.p : c->widx.p;
+    int* wcnt2 = two ? c->wcnt2.p : c->wcnt.p;
+    double2* diag2 = two ? c->ludiag2.p : c->ludiag.p;
+    const hamsx::LuAux* aux1 = two ? nullptr : lu_lookahead(c);
     if (c->prec == HAMSX_PREC_FP64) {
+      hamsx::launch::lu_factor(s2, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, widx2, wcnt2, c->flags.p + 5,
+                               diag2, aux1);
       hamsx::launch::lu_factor(c->stream, system_matrix(c), n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                               c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
-      hamsx::launch::lu_factor(c->stream, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
-                               c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
+                               c->flags.p + 1, c->ludiag.p, aux1);
       if (c->nlid == 0) c->have_A = false;
     } else {  // complex64 factors; the FP64 matrices stay for the residuals
+      hamsx::launch::to_c64(s2, system_matrix2(c), c->Af2.p, (size_t)n * n);
+      hamsx::launch::lu_factor_c64(s2, c->Af2.p, n, c->ipiv2.p, c->perm2.p, widx2, wcnt2, c->flags.p + 5, diag2,
+                                   aux1);
       hamsx::launch::to_c64(c->stream, system_matrix(c), c->Af.p, (size_t)n * n);
       hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                                   c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
-      hamsx::launch::to_c64(c->stream, system_matrix2(c), c->Af2.p, (size_t)n * n);
-      hamsx::launch::lu_factor_c64(c->stream, c->Af2.p, n, c->ipiv2.p, c->perm2.p, c->widx.p, c->wcnt.p,
-                                   c->flags.p + 5, c->ludiag.p, lu_lookahead(c));
+                                   c->flags.p + 1, c->ludiag.p, aux1);
+    }
+    if (two) {
+      HX_TRY(hipEventRecord(c->sym_done, s2));
+      HX_TRY(hipStreamWaitEvent(c->stream, c->sym_done, 0));
     }
     c->have_LU = true;
     return check_launch();
@@ -1301,6 +1334,10 @@ int hamsx_destroy(hamsx_ctx* c) {
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->sym_stream) (void)hipStreamSynchronize(c->sym_stream);
+  for (hipEvent_t ev : {c->sym_go, c->sym_done})
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->sym_stream) (void)hipStreamDestroy(c->sym_stream);
   if (c->lu_aux.s2) (void)hipStreamSynchronize(c->lu_aux.s2);
   if (c->lu_aux.panel) (void)hipEventDestroy(c->lu_aux.panel);
   if (c->lu_aux.update) (void)hipEventDestroy(c->lu_aux.update);
@@ -1694,6 +1731,12 @@ int hamsx_set_symmetry(hamsx_ctx* c, int isx, int isy) {
   HX_TRY(hipStreamSynchronize(c->stream));
   const int mirror = isx ? 0 : (isy ? 1 : -1);
   if (mirror >= 0 && (rc = build_mirror_view(c, mirror))) return rc;
+  if (mirror >= 0 && !c->sym_stream) {  // system 2's stream (default priority)
+    if (hipStreamCreateWithFlags(&c->sym_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->sym_go, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->sym_done, hipEventDisableTiming) != hipSuccess)
+      return fail(HAMSX_EDEVICE, "stream/event creation failed on device %d", c->device);
+  }
   if (mirror != c->sym_mirror) {
     c->sym_mirror = mirror;
     c->mbins.n = 0;

@@ -9,7 +9,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .mesh import box_for_panels, cylinder, cylinder_for_panels, waterplane_disk
+from .mesh import box_for_panels, cylinder, cylinder_for_panels, half_mesh, waterplane_disk
 
 
 @dataclass
@@ -46,6 +46,16 @@ def two_cylinders(panels_each=1984, nfreq=30) -> Case:
         bodies = [cylinder_for_panels(5.0, 10.0, panels_each, x0) for x0 in (-20.0, 20.0)]
     xr = [np.array([x0, 0.0, 0.0]) for x0 in (-20.0, 20.0)]
     return Case("cylinders-2x1984-h50", bodies, xr, 50.0, np.linspace(0.2, 2.5, nfreq), [0.0, 45.0, 90.0])
+
+
+def two_cylinders_half(nfreq=30) -> Case:
+    """two_cylinders() through its symmetry plane y = 0 (ISY = 1): the halves
+    y > 0 of both cylinders (992 panels each), the same depth, frequencies
+    and headings."""
+    full = two_cylinders(nfreq=nfreq)
+    bodies = [half_mesh(b, 1) for b in full.bodies]
+    return Case("cylinders-2x1984-h50-half-isy", bodies, full.xr, full.depth, full.omegas, full.headings_deg,
+                meta={"symmetry": (0, 1)})
 
 
 def five_cylinders(seed=2024, total=10000, nfreq=100) -> Case:

@@ -181,6 +181,21 @@ class Combined:
         return len(self.body_start) - 1
 
 
+def half_mesh(mesh: PanelMesh, axis: int) -> PanelMesh:
+    """The panels of a mesh symmetric about the plane axis = 0 (0: x, 1: y)
+    whose centroids lie on its positive side, nodes renumbered, with ISX /
+    ISY set: the input the reference expects with a symmetry plane."""
+    xyz = np.asarray(mesh.xyz)
+    keep = [e for e in range(len(mesh.ncn)) if xyz[mesh.ncon[e, :mesh.ncn[e]] - 1].mean(axis=0)[axis] > 0]
+    ncon = mesh.ncon[keep]
+    used = np.unique(ncon[ncon > 0])
+    remap = np.zeros(len(xyz) + 1, dtype=np.int32)
+    remap[used] = np.arange(1, len(used) + 1)
+    nc = np.where(ncon > 0, remap[ncon], 0).astype(np.int32)
+    return PanelMesh(np.asfortranarray(xyz[used - 1]), mesh.ncn[keep].copy(), np.asfortranarray(nc),
+                     1 if axis == 0 else 0, 1 if axis == 1 else 0)
+
+
 def panel_geometry(mesh: PanelMesh, xr) -> tuple:
     """hamsx_panel_geometry on one body (NormalProcess.f90 semantics)."""
     lib = _lib.load()
