@@ -85,6 +85,13 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // step's reads.
 // ---------------------------------------------------------------------------
 constexpr int kMerge = kSelRows / kPanelNB;
+constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one far trailing update
+// Outer blocks from this order on.  Measured (FP64, look-ahead): N = 3,968
+// factors in 30.3 ms per panel and 34.0 ms in outer blocks (the far U chain
+// at each block's end sits on the critical path of a selection-bound LU);
+// N = 10,026 in 137 and 133 ms, and four solves in flight then run 4.4 and
+// 5.1 solves/s (the far trailing matrix is read and written once per block).
+constexpr int kOuterMinN = 6144;
 constexpr int kSelWaves = kSelRows / 64;
 static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
 template <typename R, bool FINAL>
@@ -284,21 +291,24 @@ __global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, 
 
 // With the diagonal block factored (diag, from the final tournament level):
 // L21 = A21 U11^-1 (blocks [0, nrb), one row per thread) and U12 = L11^-1 A12
-// (blocks [nrb, ..), one column per thread); block 0 also stores the factored
-// diagonal block into A.  No block reads what another writes.
+// for the columns [uc0, uc1) (blocks [nrb, ..), one column per thread).  The
+// factored diagonal block is read from diag (leading dimension ldd); with
+// store set, block 0 also stores it into A.  No block reads what another
+// writes.
 template <typename R>
 __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
-                                                     int nb, int nrb, const typename CT<R>::c2* __restrict__ diag) {
+                                                     int nb, int nrb, const typename CT<R>::c2* __restrict__ diag,
+                                                     int ldd, int store, int uc0, int uc1) {
   __shared__ R dr[kPanelNB][kPanelNB + 1];
   __shared__ R di[kPanelNB][kPanelNB + 1];
   __shared__ R rr[kPanelNB], ri[kPanelNB];  // reciprocals of the pivots
   const int t = threadIdx.x;
   for (int e = t; e < nb * nb; e += 256) {
     int r = e % nb, c = e / nb;
-    auto v = diag[r + kPanelNB * c];
+    auto v = diag[r + (size_t)ldd * c];
     dr[r][c] = v.x;
     di[r][c] = v.y;
-    if (blockIdx.x == 0) A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = v;
+    if (store && blockIdx.x == 0) A[(size_t)(col0 + r) + (size_t)lda * (col0 + c)] = v;
   }
   __syncthreads();
   if (t < nb) {
@@ -338,8 +348,8 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
     for (int c = 0; c < kPanelNB; c++)
       if (c < nb) A[(size_t)i + (size_t)lda * (col0 + c)] = {xr[c], xi[c]};
   } else {
-    const int j = col0 + nb + (b - nrb) * 256 + t;
-    if (j >= n) return;
+    const int j = uc0 + (b - nrb) * 256 + t;
+    if (j >= uc1) return;
     R yr[kPanelNB], yi[kPanelNB];
 #pragma unroll
     for (int c = 0; c < kPanelNB; c++) {
@@ -365,9 +375,9 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
   }
 }
 
-// Trailing update C -= L U over rows [r0, n) and the column tiles tn0,
-// tn0 + 1, ... (64 columns each, from column r0) of the block starting at
-// (r0, r0), K = kk <= 32; ntiles = tiles_m x the number of column tiles.  MFMA 16x16x4 computes the transposed product
+// Update C -= L U of the rows [ib, ie) and the column tiles tn0, tn0 + 1, ...
+// (64 columns each, from column jb, ending at je), L = A[:, kb .. kb + kk),
+// U = A[kb .. kb + kk, :]; ntiles = tiles_m x the number of column tiles.  MFMA 16x16x4 computes the transposed product
 // (U^T L^T) so that the 16 consecutive lanes of an accumulator row map to
 // 16 consecutive rows of column-major C.  Operand layout (both types): lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16]; the accumulator row held
@@ -376,11 +386,12 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
 // imaginary accumulators).  Block ids are remapped so that each of the eight
 // XCDs works on a contiguous band of tiles (L21 rows stay in its L2).
 template <typename R>
-__global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __restrict__ A, int lda, int n, int r0,
-                                                         int kk, int tiles_m, int ntiles, int tn0) {
+__global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __restrict__ A, int lda, int ib, int ie,
+                                                         int jb, int je, int kb, int kk, int tiles_m, int ntiles,
+                                                         int tn0) {
   using v4 = typename CT<R>::v4;
-  __shared__ R lr_[kPanelNB][64], li_[kPanelNB][64];  // L21 tile: [k][row]
-  __shared__ R ur_[kPanelNB][64], ui_[kPanelNB][64];  // U12 tile: [k][col]
+  __shared__ R lr_[kPanelNB][64], li_[kPanelNB][64];  // L tile: [k][row]
+  __shared__ R ur_[kPanelNB][64], ui_[kPanelNB][64];  // U tile: [k][col]
   int bid = blockIdx.x;
   {
     const int per = (ntiles + 7) / 8, xcd = bid % 8, k = bid / 8;
@@ -389,31 +400,8 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
     bid = rb;
   }
   const int tm = bid % tiles_m, tn = tn0 + bid / tiles_m;
-  const int i0 = r0 + tm * 64, j0 = r0 + tn * 64;
+  const int i0 = ib + tm * 64, j0 = jb + tn * 64;
   const int t = threadIdx.x;
-  {
-    // unconditional loads at clamped, in-bounds addresses, all in flight
-    // together; out-of-range entries are zeroed when staged to LDS
-    constexpr int kPer = 64 * kPanelNB / 256;
-    typename CT<R>::c2 lv[kPer], uv[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-      const int e = t + 256 * u, r = e % 64, k = e / 64;
-      const int kc = r0 - kk + min(k, kk - 1);
-      lv[u] = A[(size_t)min(i0 + r, n - 1) + (size_t)lda * kc];
-      uv[u] = A[(size_t)kc + (size_t)lda * min(j0 + r, n - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-      const int e = t + 256 * u, r = e % 64, k = e / 64;
-      const bool lk = k < kk && i0 + r < n, uk = k < kk && j0 + r < n;
-      lr_[k][r] = lk ? lv[u].x : R(0);
-      li_[k][r] = lk ? lv[u].y : R(0);
-      ur_[k][r] = uk ? uv[u].x : R(0);
-      ui_[k][r] = uk ? uv[u].y : R(0);
-    }
-  }
-  __syncthreads();
   const int w = t >> 6, lane = t & 63;
   const int wi = (w & 1) * 32, wj = (w >> 1) * 32;  // wave quarter within the tile
   const int l16 = lane & 15, kq = lane >> 4;
@@ -423,7 +411,35 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
       accR[a][b] = v4{0, 0, 0, 0};
       accI[a][b] = v4{0, 0, 0, 0};
     }
-  for (int k0 = 0; k0 < kk; k0 += 4) {
+  // K in chunks of kPanelNB through the same LDS tiles; the accumulators
+  // carry the whole sum, so C is read and written once
+  for (int kc0 = 0; kc0 < kk; kc0 += kPanelNB) {
+  const int kn = min(kPanelNB, kk - kc0);
+  if (kc0 > 0) __syncthreads();  // every wave is done with the previous chunk
+  {
+    // unconditional loads at clamped, in-bounds addresses, all in flight
+    // together; out-of-range entries are zeroed when staged to LDS
+    constexpr int kPer = 64 * kPanelNB / 256;
+    typename CT<R>::c2 lv[kPer], uv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = t + 256 * u, r = e % 64, k = e / 64;
+      const int kc = kb + kc0 + min(k, kn - 1);
+      lv[u] = A[(size_t)min(i0 + r, ie - 1) + (size_t)lda * kc];
+      uv[u] = A[(size_t)kc + (size_t)lda * min(j0 + r, je - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = t + 256 * u, r = e % 64, k = e / 64;
+      const bool lk = k < kn && i0 + r < ie, uk = k < kn && j0 + r < je;
+      lr_[k][r] = lk ? lv[u].x : R(0);
+      li_[k][r] = lk ? lv[u].y : R(0);
+      ur_[k][r] = uk ? uv[u].x : R(0);
+      ui_[k][r] = uk ? uv[u].y : R(0);
+    }
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < kn; k0 += 4) {
     const int k = k0 + kq;
     R Ur[2], Ui[2], Lr[2], Li[2];
     for (int b = 0; b < 2; b++) {  // A operand: U^T rows = C columns
@@ -442,6 +458,7 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
         accI[a][b] = CT<R>::mfma(Ur[b], Li[a], accI[a][b]);
       }
   }
+  }
   // read the 16 C entries first (clamped addresses, all loads in flight),
   // then write back the in-range ones
   typename CT<R>::c2 cv[2][2][4];
@@ -451,8 +468,8 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int row = min(i0 + wi + a * 16 + l16, n - 1);
-        const int col = min(j0 + wj + b * 16 + CT<R>::out_row(kq, q), n - 1);
+        const int row = min(i0 + wi + a * 16 + l16, ie - 1);
+        const int col = min(j0 + wj + b * 16 + CT<R>::out_row(kq, q), je - 1);
         cv[a][b][q] = A[(size_t)row + (size_t)lda * col];
       }
 #pragma unroll
@@ -463,7 +480,7 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
       for (int q = 0; q < 4; q++) {
         const int row = i0 + wi + a * 16 + l16;
         const int col = j0 + wj + b * 16 + CT<R>::out_row(kq, q);
-        if (row < n && col < n) {
+        if (row < ie && col < je) {
           auto v = cv[a][b][q];
           v.x -= accR[a][b][q];
           v.y -= accI[a][b][q];
@@ -515,27 +532,79 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
     pending = false;
     HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
+    // Outer blocks of kOuter columns: within the block each panel updates only
+    // the block's columns; at the block's end the columns to its right get
+    // U = L11^-1 A12 for all of the block's rows and one rank-kOuter update,
+    // so the far trailing matrix is read and written once per block instead
+    // of once per panel.
     const int rest = n - col0 - nb;
-    const int nrb = (rest + 255) / 256;
-    HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
-              (const c2*)diag);
-    if (rest > 0) {
-      const int tiles = (rest + 63) / 64;
-      if (!la || tiles == 1) {
-        const int ntiles = tiles * tiles;
-        const int grid = ((ntiles + 7) / 8) * 8;
-        HX_LAUNCH(k_trailing_update<R>, dim3(grid), dim3(256), 0, s, A, n, n, col0 + nb, nb, tiles, ntiles, 0);
+    if (n < kOuterMinN) {  // one panel at a time: U12 and the update over every trailing column
+      const int nrb = (rest + 255) / 256;
+      HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
+                (const c2*)diag, kPanelNB, 1, col0 + nb, n);
+      if (rest > 0) {
+        const int tiles = (rest + 63) / 64;
+        if (!la || tiles == 1) {
+          const int ntiles = tiles * tiles;
+          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n,
+                    col0 + nb, n, col0, nb, tiles, ntiles, 0);
+        } else {
+          // look-ahead: the first 64 trailing columns (the next panel) here, the
+          // rest on the second stream, overlapping the next pivot selection,
+          // which reads only the next panel's columns
+          HX_LAUNCH(k_trailing_update<R>, dim3(((tiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n,
+                    col0 + nb, n, col0, nb, tiles, tiles, 0);
+          (void)hipEventRecord(aux->panel, s);
+          (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
+          const int ntiles = tiles * (tiles - 1);
+          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, col0 + nb, n,
+                    col0 + nb, n, col0, nb, tiles, ntiles, 1);
+          (void)hipEventRecord(aux->update, aux->s2);
+          pending = true;
+        }
+      }
+      continue;
+    }
+    const int blk0 = col0 - col0 % kOuter, blk1 = min(blk0 + kOuter, n);
+    const int near = blk1 - col0 - nb;
+    const int nrb = (rest + 255) / 256, ncb = (near + 255) / 256;
+    HX_LAUNCH(k_panel_solve<R>, dim3(nrb + ncb > 0 ? nrb + ncb : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
+              (const c2*)diag, kPanelNB, 1, col0 + nb, blk1);
+    if (near > 0) {
+      const int tm = (rest + 63) / 64, tn = (near + 63) / 64, ntiles = tm * tn;
+      HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n, col0 + nb,
+                blk1, col0, nb, tm, ntiles, 0);
+    }
+    if (col0 + nb == blk1 && blk1 < n) {
+      const int far = n - blk1, fcb = (far + 255) / 256;
+      // U of the far columns, one panel's rows after another
+      for (int q0 = blk0; q0 < blk1; q0 += kPanelNB) {
+        const int qnb = min(kPanelNB, blk1 - q0);
+        if (q0 > blk0) {
+          const int tn = (far + 63) / 64;
+          HX_LAUNCH(k_trailing_update<R>, dim3(((tn + 7) / 8) * 8), dim3(256), 0, s, A, n, q0, q0 + qnb, blk1, n,
+                    blk0, q0 - blk0, 1, tn, 0);
+        }
+        HX_LAUNCH(k_panel_solve<R>, dim3(fcb), dim3(256), 0, s, A, n, n, q0, qnb, 0,
+                  (const c2*)(A + (size_t)q0 + (size_t)n * q0), n, 0, blk1, n);
+      }
+      // the rank-kOuter update of the far trailing matrix
+      const int tm = (far + 63) / 64, tiles = tm;
+      if (!la || tiles <= 2) {
+        const int ntiles = tm * tiles;
+        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, blk1, n, blk1, n,
+                  blk0, blk1 - blk0, tm, ntiles, 0);
       } else {
-        // look-ahead: the first 64 trailing columns (the next panel) here, the
+        // look-ahead: the next outer block's columns (two tiles) here, the
         // rest on the second stream, overlapping the next pivot selection,
         // which reads only the next panel's columns
-        HX_LAUNCH(k_trailing_update<R>, dim3(((tiles + 7) / 8) * 8), dim3(256), 0, s, A, n, n, col0 + nb, nb,
-                  tiles, tiles, 0);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((2 * tm + 7) / 8) * 8), dim3(256), 0, s, A, n, blk1, n, blk1, n,
+                  blk0, blk1 - blk0, tm, 2 * tm, 0);
         (void)hipEventRecord(aux->panel, s);
         (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
-        const int ntiles = tiles * (tiles - 1);
-        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, n, col0 + nb, nb,
-                  tiles, ntiles, 1);
+        const int ntiles = tm * (tiles - 2);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, blk1, n, blk1,
+                  n, blk0, blk1 - blk0, tm, ntiles, 2);
         (void)hipEventRecord(aux->update, aux->s2);
         pending = true;
       }

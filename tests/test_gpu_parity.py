@@ -131,6 +131,37 @@ def test_lu_backward_error():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lu_outer_blocks_backward_error(precision):
+    """From N = 6,144 on the LU works in outer blocks of four panels (the far
+    columns get U = L11^-1 A12 for the block's 128 rows at its end and one
+    rank-128 update).  N = 6,400 here: ||PA x - L (U x)|| / (||A|| ||x||) for
+    random probes x, and the solves against A."""
+    m = small_pair(na=80, nz=35, nr=5)
+    assert m.nelem >= 6144
+    say(f"[gpu] LU outer blocks n={m.nelem} {precision}")
+    with FrequencySolver(m, precision=precision) as s:
+        s.set_waves([0.0])
+        s.calwaveproperts(1.0, -1.0)
+        s.assemble()
+        A = s.get_matrix("A")
+        info = s.factor()
+        LU, ipiv = s.get_matrix("LU")
+    assert info == 0
+    n = m.nelem
+    perm = np.arange(n)
+    for k in range(n):
+        p = ipiv[k] - 1
+        perm[[k, p]] = perm[[p, k]]
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4))
+    ux = np.triu(LU) @ x
+    lux = np.tril(LU, -1) @ ux + ux
+    err = np.linalg.norm(A[perm] @ x - lux) / (np.linalg.norm(A) * np.linalg.norm(x))
+    say(f"    probe backward error {err:.3e}")
+    assert err < (1e-14 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_lu_lookahead_gives_the_same_factors(precision, monkeypatch):
     """The look-ahead schedule (the next panel's columns first, the rest of
     the trailing update on a second stream overlapping the next pivot
