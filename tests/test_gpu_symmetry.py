@@ -268,3 +268,32 @@ def test_half_mesh_with_lid_equals_full_mesh_on_gpu(axis):
     errs = rel(ah, af), rel(bh, bf), rel(eh, ef)
     say(f"    rel err {errs}")
     assert max(errs) <= 1e-9
+
+
+@pytest.mark.parametrize("quad", [False, True])
+def test_symmetric_motions_equal_full_mesh(quad):
+    """hamsx_motions under a symmetry plane (forces from both halves) against
+    the full mesh's, deep water.  Linear damping: one complex solve, 1e-10.
+    Quadratic damping: the fixed point stops at RERR <= 1e-6, and last-bit
+    differences of the coefficients can move the stop by one step, so the
+    bar is that tolerance."""
+    from test_gpu_motions import buoy_matrices
+    bodies, xr = two_bodies(1)
+    mats = buoy_matrices(2, np.random.default_rng(11), quad)
+    args = [mats[k] for k in ("mass", "blnr", "bqdr", "crs", "kstf")]
+    heads = np.deg2rad([0.0, 60.0])
+    say(f"[gpu] symmetric motions vs full mesh quadratic={quad}")
+    out = []
+    for m, sym in ((combine(bodies, xr), None), (combine([half(b, 1) for b in bodies], xr), (0, 1))):
+        with FrequencySolver(m) as s:
+            s.set_waves(heads)
+            if sym:
+                s.set_symmetry(*sym)
+            s.set_body_matrices(*args)
+            _, _, _, info = s.solve_frequency(0.9, -1.0)
+            assert info == 0
+            out.append(s.motions())
+    (df, itf), (dh, ith) = out
+    e = rel(dh, df)
+    say(f"    rel err {e:.3e}  steps {list(itf)} / {list(ith)}")
+    assert e <= (1e-6 if quad else 1e-10)
