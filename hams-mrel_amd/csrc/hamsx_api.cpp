@@ -1752,10 +1752,10 @@ int hamsx_factor(hamsx_ctx* c) {
   int rc = set_device(c);
   if (rc) return rc;
   if ((rc = do_factor(c))) return rc;
-  int info = 0;
-  HX_TRY(hipMemcpyAsync(&info, c->flags.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int fl[8] = {0};  // INFO in fl[1]; a symmetry plane's system 2 in fl[5]
+  HX_TRY(hipMemcpyAsync(fl, c->flags.p, sizeof fl, hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
-  return info;
+  return fl[1] ? fl[1] : (c->sym() ? fl[5] : 0);
 }
 
 int hamsx_solve_radiation(hamsx_ctx* c, double* mxpot) {
