@@ -102,8 +102,9 @@ int hamsx_set_lid(hamsx_ctx *ctx, int nlid, const double *lid_cen);
  * the two systems A1 = A + T2, A2 = A - T2 with their right-hand sides, and
  * forces summed over both halves.  isx = isy = 1 is refused as the reference
  * refuses it; isx = isy = 0 removes the plane, and so does a new
- * hamsx_set_mesh.  Every solver path (FP64 or mixed LU, GMRES) then runs
- * the two systems one after the other.  With a lid (hamsx_set_lid,
+ * hamsx_set_mesh.  Every solver path runs both systems: the LU paths (FP64
+ * or mixed) factor them at once on two streams, GMRES one after the other.
+ * With a lid (hamsx_set_lid,
  * before or after; the half lid's points) each system is solved through its
  * own normal equations (ASSB_LEFT/RBC/DBC_IRR_MULTI with NSYS = 2).  With a
  * plane set, the mxpot outputs of hamsx_solve_radiation / _diffraction hold
