@@ -311,7 +311,7 @@ struct hamsx_ctx {
   int sym_mirror = -1;
   DevBuf<double> mview, mmodetab;
   PairBins mbins;
-  DevBuf<double2> T2, S2, X2, W1, W2, sdinc, Cm2, Xe2;
+  DevBuf<double2> T2, S2, X2, W1, W2, sdinc, Cm2, Xe2, Xtmp2;
   DevBuf<int> ipiv2, perm2;
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
@@ -1147,8 +1147,16 @@ int do_solve(hamsx_ctx* c) {
   const int n = c->n, nr = c->nrhs();
   StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
+    if (c->sym() && c->sym_stream) {  // system 2's solves on its own stream, at once with system 1's
+      if ((rc = c->Xtmp2.ensure((size_t)n * nr))) return rc;
+      HX_TRY(hipEventRecord(c->sym_go, c->stream));
+      HX_TRY(hipStreamWaitEvent(c->sym_stream, c->sym_go, 0));
+      hamsx::launch::lu_solve(c->sym_stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp2.p, nr);
+      HX_TRY(hipEventRecord(c->sym_done, c->sym_stream));
+    }
     hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
-    if (c->sym()) hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
+    if (c->sym() && c->sym_stream) HX_TRY(hipStreamWaitEvent(c->stream, c->sym_done, 0));
+    else if (c->sym()) hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
   } else {
     if ((rc = refine_solve(c, system_matrix(c), c->Af.p, c->perm.p, c->X.p, c->flags.p + 2, c->norms.p))) return rc;
     if (c->sym() &&
