@@ -8,6 +8,26 @@
 // epsilon-table rows live in caller-owned per-lane arrays, and every input of
 // a discrete decision (FLAG, the R/h region, series lengths) is formed with
 // FMA contraction off so that it takes the same branch as the FP64 reference.
+//
+// Acknowledgements and licences of the methods restated here:
+//  * The special-function routines (error function, exponential integrals
+//    E1/En/Ei, modified Bessel I0/I1/K0/K1 and Bessel J0/J1/Y0/Y1 series of
+//    IK01A and JY01B) follow "Computation of Special Functions" by Shanjie
+//    Zhang and Jianming Jin (Wiley, 1996).  The routines are copyrighted by
+//    Shanjie Zhang and Jianming Jin, who give permission to incorporate them
+//    into a user program provided that the copyright is acknowledged; it is
+//    acknowledged here (as in the reference, src/FinGrnExtSubs.f90:27-31,
+//    :98-102 and the headers of its other routines).
+//  * The finite-depth Green function (the four R/h regions, their series
+//    lengths, Wynn's epsilon extrapolation, the Gauss-Kronrod quadrature and
+//    the Chebyshev image sums) follows FinGreen3D by Yingyi Liu (2013-2016),
+//    and the deep-water Green function follows the global approximation of
+//    Hui Liang (2016-2018).  Both are free software under the GNU Lesser
+//    General Public License, version 3 or later (src/FinGreen3D.f90:19-29,
+//    src/InfGreen_Appr.f90:9-18); this file is a restatement of those
+//    methods and is distributed under the same licence terms.  References:
+//    Y. Liu et al., Energy Conversion and Management (2018); H. Liang, H. Wu
+//    and F. Noblesse, Journal of Fluid Mechanics (2018).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>

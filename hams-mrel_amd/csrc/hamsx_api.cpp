@@ -609,6 +609,15 @@ const hamsx::LuAux* lu_lookahead(hamsx_ctx* c) {
 }
 
 
+// Symmetry system 2's stream, or null when HAMSX_SYM_STREAMS=0 (diagnostic:
+// both systems on the context stream, one after the other; the same results
+// bit for bit).
+hipStream_t sym_second_stream(hamsx_ctx* c) {
+  const char* e = std::getenv("HAMSX_SYM_STREAMS");
+  if (e && *e == '0') return nullptr;
+  return c->sym_stream;
+}
+
 // the square matrix the solvers work on: A, or C = A^T A with a lid
 double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
 // symmetry system 2's: A2 (in T2), or C2 = A2^T A2 with a lid
@@ -640,7 +649,7 @@ int do_factor(hamsx_ctx* c) {
     // (with four contexts in flight, more streams than hardware queues
     // serialise them: measured slower).
     if ((rc = ensure_normal_matrix(c))) return rc;
-    const bool two = c->sym_stream != nullptr;
+    const bool two = sym_second_stream(c) != nullptr;
     hipStream_t s2 = two ? c->sym_stream : c->stream;
     if (two) {
       HX_TRY(hipEventRecord(c->sym_go, c->stream));
@@ -1147,7 +1156,8 @@ int do_solve(hamsx_ctx* c) {
   const int n = c->n, nr = c->nrhs();
   StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
-    if (c->sym() && c->sym_stream) {  // system 2's solves on its own stream, at once with system 1's
+    const bool two = c->sym() && sym_second_stream(c) != nullptr;
+    if (two) {  // system 2's solves on its own stream, at once with system 1's
       if ((rc = c->Xtmp2.ensure((size_t)n * nr))) return rc;
       HX_TRY(hipEventRecord(c->sym_go, c->stream));
       HX_TRY(hipStreamWaitEvent(c->sym_stream, c->sym_go, 0));
@@ -1155,7 +1165,7 @@ int do_solve(hamsx_ctx* c) {
       HX_TRY(hipEventRecord(c->sym_done, c->sym_stream));
     }
     hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
-    if (c->sym() && c->sym_stream) HX_TRY(hipStreamWaitEvent(c->stream, c->sym_done, 0));
+    if (two) HX_TRY(hipStreamWaitEvent(c->stream, c->sym_done, 0));
     else if (c->sym()) hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
   } else {
     if ((rc = refine_solve(c, system_matrix(c), c->Af.p, c->perm.p, c->X.p, c->flags.p + 2, c->norms.p))) return rc;
@@ -1739,11 +1749,21 @@ int hamsx_set_symmetry(hamsx_ctx* c, int isx, int isy) {
   HX_TRY(hipStreamSynchronize(c->stream));
   const int mirror = isx ? 0 : (isy ? 1 : -1);
   if (mirror >= 0 && (rc = build_mirror_view(c, mirror))) return rc;
-  if (mirror >= 0 && !c->sym_stream) {  // system 2's stream (default priority)
-    if (hipStreamCreateWithFlags(&c->sym_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->sym_go, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->sym_done, hipEventDisableTiming) != hipSuccess)
+  if (mirror >= 0 && !(c->sym_stream && c->sym_go && c->sym_done)) {  // system 2's stream (default priority)
+    if (!c->sym_stream && hipStreamCreateWithFlags(&c->sym_stream, hipStreamNonBlocking) != hipSuccess)
+      c->sym_stream = nullptr;
+    if (c->sym_stream && !c->sym_go && hipEventCreateWithFlags(&c->sym_go, hipEventDisableTiming) != hipSuccess)
+      c->sym_go = nullptr;
+    if (c->sym_stream && !c->sym_done && hipEventCreateWithFlags(&c->sym_done, hipEventDisableTiming) != hipSuccess)
+      c->sym_done = nullptr;
+    if (!(c->sym_stream && c->sym_go && c->sym_done)) {  // all three or none
+      if (c->sym_go) (void)hipEventDestroy(c->sym_go);
+      if (c->sym_done) (void)hipEventDestroy(c->sym_done);
+      if (c->sym_stream) (void)hipStreamDestroy(c->sym_stream);
+      c->sym_stream = nullptr;
+      c->sym_go = c->sym_done = nullptr;
       return fail(HAMSX_EDEVICE, "stream/event creation failed on device %d", c->device);
+    }
   }
   if (mirror != c->sym_mirror) {
     c->sym_mirror = mirror;

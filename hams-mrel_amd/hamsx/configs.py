@@ -73,30 +73,33 @@ def five_cylinders(seed=2024, total=10000, nfreq=100) -> Case:
                 meta={"seed": seed})
 
 
-def large_array(panels_each=1875, nfreq=50) -> Case:
+def large_array(panels_each=1875, nfreq=50, grid=4) -> Case:
     """16 cylinders R = 4 m, draft 8 m on a 4 x 4 grid, 30 m spacing
     (about 30,000 hull panels), depth 80 m, 50 frequencies in [0.2, 2.0]
     rad/s, heading 0.  Each cylinder carries a waterplane lid for
     irregular-frequency removal, with panels about twice the hull's size
-    (meta["lids"])."""
+    (meta["lids"]).  grid < 4: the same layout on a grid x grid array."""
     bodies, xr, lids = [], [], []
-    for k in range(16):
-        x0, y0 = 30.0 * (k % 4) - 45.0, 30.0 * (k // 4) - 45.0
+    c0 = 15.0 * (grid - 1)
+    for k in range(grid * grid):
+        x0, y0 = 30.0 * (k % grid) - c0, 30.0 * (k // grid) - c0
         b = cylinder_for_panels(4.0, 8.0, panels_each, x0, y0)
         bodies.append(b)
         xr.append(np.array([x0, y0, 0.0]))
         na = max(8, int(round(2 * math.pi * 4.0 / (2.0 * math.sqrt(4.0 * 8.0 * 2 * math.pi / panels_each)))))
         lids.append(waterplane_disk(4.0, na, max(1, na // 6), x0, y0))
-    return Case(f"array-16x{panels_each}-lid-h80", bodies, xr, 80.0, np.linspace(0.2, 2.0, nfreq), [0.0],
+    return Case(f"array-{grid * grid}x{panels_each}-lid-h80", bodies, xr, 80.0, np.linspace(0.2, 2.0, nfreq), [0.0],
                 meta={"lids": lids})
 
 
-def random_request(rng: np.random.Generator, pmin=500, pmax=12000) -> Case:
+def random_request(rng: np.random.Generator, pmin=500, pmax=12000, total=None) -> Case:
     """One serving request: 1-9 cylinders or boxes (L, B ~ U[5,20] m, draft
     ~ U[4,10] m), 500-12,000 panels, w ~ U[0.1,3.0], heading ~ U[0,180) deg,
-    depth infinite or U[30,200] m."""
+    depth infinite or U[30,200] m.  total: the panel count to aim at instead
+    of a draw from [pmin, pmax] (the other draws are unchanged)."""
     nb = int(rng.integers(1, 10))
-    total = int(rng.integers(pmin, pmax + 1))
+    drawn = int(rng.integers(pmin, pmax + 1))
+    total = drawn if total is None else int(total)
     per = max(60, total // nb)
     bodies, xr = [], []
     for k in range(nb):

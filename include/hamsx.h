@@ -79,7 +79,8 @@ int hamsx_wave_properties(int inft, double value, double h, double *v, double *w
  * NCN/NCON_MULTI_COMB, centroids XYZ_GLOBAL_MULTI_COMB_P(nelem,3),
  * DS/PNSZ_MULTI_COMB, DXYZ_MULTI_COMB(nelem,6).  Bodies occupy consecutive
  * panel ranges: body b owns panels [body_start[b], body_start[b+1]) (0-based,
- * body_start has nbody+1 entries).  Only NSYS = 1 (no geometric symmetry). */
+ * body_start has nbody+1 entries).  The mesh is the whole body (NSYS = 1)
+ * unless hamsx_set_symmetry declares it the half on one side of a plane. */
 int hamsx_set_mesh(hamsx_ctx *ctx, int nelem, int nnode, const double *node_xyz,
                    const int *ncn, const int *ncon, const double *cen, const double *area,
                    const double *pnsz, const double *dxyz, int nbody, const int *body_start);

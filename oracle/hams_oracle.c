@@ -15,6 +15,15 @@
  * Parity of every function here is pinned in tests/test_oracle_golden.py
  * against vectors written by the reference's own Fortran (oracle/_ref), and
  * end to end against the reference's shipped test/data/config1 outputs.
+ *
+ * Acknowledgements: the special-function routines follow Shanjie Zhang and
+ * Jianming Jin, "Computation of Special Functions" (Wiley, 1996); the
+ * routines are copyrighted by Zhang and Jin, who permit their use in a user
+ * program provided that the copyright is acknowledged
+ * (src/FinGrnExtSubs.f90:27-31).  The finite-depth Green function restates
+ * FinGreen3D (Yingyi Liu) and the deep-water one the global approximation of
+ * Hui Liang; both are LGPL-3.0-or-later (src/FinGreen3D.f90:19-29,
+ * src/InfGreen_Appr.f90:9-18), and this restatement keeps those terms.
  */
 #include "hams_oracle.h"
 

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 ORACLE_FMA_SO = os.path.join(HERE, "liboracle_fma.so")
 REF_SO = os.path.join(HERE, "_ref", "libhams_ref.so")
+REF_FMA_SO = os.path.join(HERE, "_ref", "fma", "libhams_ref.so")
 NK = 200
 G = 9.80665
 
@@ -23,6 +24,7 @@ _ip = C.POINTER(C.c_int)
 _d = C.c_double
 _o = None
 _r = None
+_rf = None
 _variant = {}
 
 
@@ -105,15 +107,25 @@ def _load(path):
     return o
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_SO)
+def ref_available(fma: bool = False) -> bool:
+    return os.path.exists(REF_FMA_SO if fma else REF_SO)
 
 
-def ref():
-    """The reference's own Fortran routines (built here from /root/reference)."""
-    global _r
+def ref(fma: bool = False):
+    """The reference's own Fortran routines (built here from /root/reference);
+    fma=True: the same sources built with FMA contraction."""
+    global _r, _rf
+    if fma:
+        if _rf is None:
+            _rf = _bind_ref(C.CDLL(REF_FMA_SO))
+        return _rf
     if _r is None:
-        r = C.CDLL(REF_SO)
+        _r = _bind_ref(C.CDLL(REF_SO))
+    return _r
+
+
+def _bind_ref(r):
+    if True:
         for name in ("ref_error", "ref_e1xa", "ref_eix", "ref_struve_h0", "ref_struve_h1", "ref_bessel_j0",
                      "ref_bessel_j1"):
             getattr(r, name).restype = _d
@@ -134,8 +146,7 @@ def ref():
         r.ref_dinp.argtypes = [_d] * 11 + [_dp]
         r.ref_vinp.argtypes = [_d] * 11 + [_dp]
         r.ref_geometry.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
-        _r = r
-    return _r
+    return r
 
 
 # ---------------------------------------------------------------------------
@@ -338,3 +349,54 @@ def solve_frequency_sym(m, isx: int, w1: float, h: float, headings_rad, amp=1.0,
                                          xw[1], rho, isol, _p(amas), _p(bdmp), _p(exfc), _p(phi2), nthreads)
     out = (amas, bdmp, exfc[:, :len(beta)], info)
     return out + (phi2,) if potentials else out
+
+
+# ---------------------------------------------------------------------------
+# Rounding samples and solves from assembled matrices (parity-gate helpers)
+# ---------------------------------------------------------------------------
+def perturbed(m, kind: int):
+    """A copy of the combined mesh m whose centroids are moved by one unit in
+    the last place: kind 1 moves z towards -inf, kind 2 moves x and y towards
+    +inf, kind 3 moves x and y towards -inf and z towards +inf.  The reference's answer on such a mesh differs from the unperturbed
+    one only by the rounding sensitivity of its algorithm (the geometry moves
+    by ~1e-16 relative), so each kind is one more sample of that spread."""
+    import copy
+    q = copy.copy(m)
+    cen = np.array(m.cen, dtype=np.float64, order="F", copy=True)
+    if kind == 1:
+        cen[:, 2] = np.nextafter(cen[:, 2], -np.inf)
+    elif kind == 2:
+        cen[:, 0] = np.nextafter(cen[:, 0], np.inf)
+        cen[:, 1] = np.nextafter(cen[:, 1], np.inf)
+    elif kind == 3:
+        cen[:, 0] = np.nextafter(cen[:, 0], -np.inf)
+        cen[:, 1] = np.nextafter(cen[:, 1], -np.inf)
+        cen[:, 2] = np.nextafter(cen[:, 2], np.inf)
+    else:
+        raise ValueError("kind must be 1, 2 or 3")
+    q.cen = np.asfortranarray(cen)
+    return q
+
+
+def rhs(m, S, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0)):
+    """ASSB_RBC_MULTI + ASSB_DBC_MULTI right-hand sides (N x (6 nbody + nbeta))
+    from the single-layer matrix S."""
+    v, wk, _ = wave(w1, h)
+    beta = np.ascontiguousarray(headings_rad, dtype=np.float64)
+    Sf = np.asfortranarray(S, dtype=np.complex128)
+    out = np.zeros((m.nelem, 6 * m.nbody + len(beta)), dtype=np.complex128, order="F")
+    lib().orc_rhs(m.nelem, _p(Sf), _p(m.cen), _p(m.dxyz), m.nbody, _p(m.body_start, C.c_int), len(beta), _p(beta),
+                  v, w1, wk, h, amp, xw[0], xw[1], _p(out))
+    return out
+
+
+def solve_assembled(m, A, S, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0, 0.0), rho=1000.0, isol=1):
+    """The frequency solve from oracle-assembled A and S: right-hand sides,
+    LAPACK ZGETRF/ZGETRS (scipy's LAPACK; the reference calls MKL's at
+    AssbMatxMulti.f90:112, :279, :346) and the force sums.  Returns
+    (amas, bdmp, exfc, potentials)."""
+    import scipy.linalg as sl
+    B = rhs(m, S, w1, h, headings_rad, amp, xw)
+    X = sl.lu_solve(sl.lu_factor(np.asfortranarray(A), check_finite=False), B, check_finite=False)
+    am, bd, ex = forces(m, X, w1, h, headings_rad, amp, xw, rho, isol)
+    return am, bd, ex, X
