@@ -35,8 +35,15 @@ void report_launch(const char* name, hipError_t e) {
 }  // namespace hamsx
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+int hamsx::set_error(int code, const char* msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];

@@ -10,6 +10,8 @@ namespace hamsx {
 // followed by hipGetLastError, and a failure is reported with the kernel name.
 bool debug_launches();
 void report_launch(const char* name, hipError_t e);
+// Records msg as the calling thread's hamsx_last_error and returns code.
+int set_error(int code, const char* msg);
 #define HX_LAUNCH(kernel, ...)                                                      \
   do {                                                                              \
     hipLaunchKernelGGL(kernel, __VA_ARGS__);                                        \

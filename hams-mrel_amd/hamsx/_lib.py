@@ -18,6 +18,7 @@ HAMSX_ENOMEM = -3
 HAMSX_ESTATE = -4
 HAMSX_ENOCONV = -5
 HAMSX_NK = 200
+HAMSX_NOALLOC_MAX_N = 32768  # largest order hamsx_factor_noalloc takes
 PREC_FP64 = 0
 PREC_MIXED = 1
 SOLVER_LU = 0
@@ -47,6 +48,7 @@ SIGNATURES = {
     "hamsx_set_waves": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "hamsx_assemble": (C.c_int, [_vp]),
     "hamsx_factor": (C.c_int, [_vp]),
+    "hamsx_factor_noalloc": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
     "hamsx_solve_radiation": (C.c_int, [_vp, _dp]),
     "hamsx_solve_diffraction": (C.c_int, [_vp, _dp]),
     "hamsx_forces": (C.c_int, [_vp, _dp, _dp, _dp]),

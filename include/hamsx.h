@@ -133,6 +133,21 @@ int hamsx_assemble(hamsx_ctx *ctx);
 /* ZGETRF of A (src/AssbMatxMulti.f90:110-114).  Returns INFO. */
 int hamsx_factor(hamsx_ctx *ctx);
 
+/* hamsx_factor on caller-owned device memory, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Factors the
+ * n x n complex128 matrix a (device pointer, column-major, leading dimension
+ * n, interleaved re/im doubles) in place with the kernels of hamsx_factor;
+ * ipiv[n] (device int, 1-based LAPACK pivots) and info[0] (device int,
+ * LAPACK INFO) are written on `stream` (a hipStream_t; NULL: the null
+ * stream).  The call only enqueues work and returns 0, or a HAMSX_E* code
+ * (HAMSX_EARG when n > HAMSX_NOALLOC_MAX_N).  Its workspace (the factored
+ * diagonal block, the permutation and the pivot tournament's lists, about
+ * 180 KB) is one fixed array per GPU, held by the library until the process
+ * ends: calls that overlap on one GPU race on it. */
+#define HAMSX_NOALLOC_MAX_N 32768
+int hamsx_factor_noalloc(int n, double *a, int *ipiv, int *info, void *stream);
+
 /* ASSB_RBC_MULTI + RADIATION_SOLVER_MULTI (src/AssbMatxMulti.f90:124-181,
  * :257-317): mxpot[nelem*6*nbody] (complex) receives the radiation
  * potentials; pass NULL to keep them on the device only. */
