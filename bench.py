@@ -152,6 +152,8 @@ def parse():
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default fp64 (serving: mixed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision and five-cylinder runs that "
+                    "the default FP64 line carries beside its value")
     ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -447,61 +449,16 @@ def main():
         wave_field(args)
         return
     from hamsx.mesh import combine
-    from hamsx.solver import FrequencySolver
 
     case = pick_case(args.config)
     mesh = combine(case.bodies, case.xr)
     n = mesh.nelem
-    heads = np.deg2rad(case.headings_deg)
-    omegas = [float(w) for w in case.omegas]
-    nctx = max(1, args.streams)
-    solvers = []
-    for _ in range(nctx):
-        sv = FrequencySolver(mesh, precision=args.precision)
-        sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
-        if "symmetry" in case.meta:
-            sv.set_symmetry(*case.meta["symmetry"])
-        solvers.append(sv)
-    print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(omegas)} frequencies, "
-          f"{len(heads)} headings, precision {args.precision}, {nctx} solves in flight", file=sys.stderr, flush=True)
-    # per-stage device time of isolated solves (one context, nothing else on the GPU)
-    solvers[0].solve_frequency(omegas[0], case.depth)
-    solvers[0].set_profiling(True)
-    solvers[0].stage_times(reset=True)
-    for k in range(args.stage_steps):
-        solvers[0].solve_frequency(omegas[(k * 7) % len(omegas)], case.depth)
-    stages = solvers[0].stage_times(reset=True)
-    solvers[0].set_profiling(False)
-    for k, sv in enumerate(solvers):
-        for _ in range(args.warmup):
-            sv.solve_frequency(omegas[k % len(omegas)], case.depth)
-    # timed: args.steps frequency solves, nctx in flight (one HIP stream per context)
-    lat = []
-    t_sub = [0.0] * nctx
-    submitted = done = 0
-    t0 = time.perf_counter()
-    while submitted < min(nctx, args.steps):
-        t_sub[submitted] = time.perf_counter()
-        solvers[submitted].submit(omegas[submitted % len(omegas)], case.depth)
-        submitted += 1
-    while done < args.steps:
-        k = done % nctx
-        _, _, _, info = solvers[k].collect()
-        lat.append(time.perf_counter() - t_sub[k])
-        if info != 0:
-            raise SystemExit(f"zero pivot in solve {done}: INFO={info}")
-        done += 1
-        if submitted < args.steps:
-            t_sub[k] = time.perf_counter()
-            solvers[k].submit(omegas[submitted % len(omegas)], case.depth)
-            submitted += 1
-    elapsed = time.perf_counter() - t0
-    for sv in solvers:
-        sv.close()
+    print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(case.omegas)} frequencies, "
+          f"{len(case.headings_deg)} headings, precision {args.precision}, {max(1, args.streams)} solves in flight",
+          file=sys.stderr, flush=True)
+    run = throughput(case, mesh, args.precision, args.streams, args.steps, args.warmup, args.stage_steps)
+    ms, elapsed, lat_ms = run["stages"], run["elapsed"], run["lat_ms"]
     print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
-
-    lat_ms = np.array(lat) * 1e3
-    ms = {k: v[0] / max(1, v[1]) for k, v in stages.items()}
     roof = roofline(case, mesh, ms, args.precision)
     lu_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
     lu_tf = 8.0 * n ** 3 / 3.0 / (ms["factor"] * 1e-3) / 1e12
@@ -510,23 +467,7 @@ def main():
     roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
     vs_full = None
     if "symmetry" in case.meta:
-        # the same frequency on the whole mesh: the plane must not change the answer
-        from hamsx import configs
-        full = configs.two_cylinders(nfreq=len(omegas))
-        with FrequencySolver(combine(full.bodies, full.xr), precision=args.precision) as sv:
-            sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
-            t1 = time.perf_counter()
-            fa, fb, fe, _ = sv.solve_frequency(omegas[0], case.depth)
-            t_full = time.perf_counter() - t1
-        with FrequencySolver(mesh, precision=args.precision) as sv:
-            sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
-            sv.set_symmetry(*case.meta["symmetry"])
-            ha, hb, he, _ = sv.solve_frequency(omegas[0], case.depth)
-
-        def rel(a, b):
-            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-        vs_full = {"w": omegas[0], "amas": rel(ha, fa), "bdmp": rel(hb, fb), "exfc": rel(he, fe),
-                   "full_mesh_solve_ms": t_full * 1e3}
+        vs_full = half_vs_full(case, mesh, args.precision)
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "freq-solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
@@ -534,8 +475,9 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "c64-lu+f64-refine",
         "data": "synthetic seeded cylinder meshes (no reference files read)",
         "config": {"workload": case.name, "panels": n, "symmetry": case.meta.get("symmetry"),
-                   "depth_m": case.depth, "frequencies": len(omegas),
-                   "headings_deg": case.headings_deg, "precision": args.precision, "in_flight": nctx},
+                   "depth_m": case.depth, "frequencies": len(case.omegas),
+                   "headings_deg": case.headings_deg, "precision": args.precision,
+                   "in_flight": max(1, args.streams)},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
         "roofline": roof,
@@ -544,11 +486,136 @@ def main():
     }
     if vs_full is not None:
         out["vs_full_mesh"] = vs_full
+    if args.precision == "fp64" and not args.no_mixed and args.config in ("two-cylinders", "five-cylinders"):
+        # the headline: the mixed-precision path on the same workload, and both
+        # paths on the five-cylinder (10,026-panel) workload
+        out["mixed"] = mixed_vs_fp64(case, mesh, run, args.streams, args.steps, args.warmup)
+        if args.config == "two-cylinders":
+            c5 = pick_case("five-cylinders")
+            m5 = combine(c5.bodies, c5.xr)
+            steps5 = max(4, args.steps // 2)
+            print(f"[bench] {c5.name}: N={m5.nelem}, fp64 and mixed, {steps5} solves each", file=sys.stderr, flush=True)
+            r5 = throughput(c5, m5, "fp64", args.streams, steps5, args.warmup, 1)
+            out["five_cylinders"] = {"workload": c5.name, "panels": m5.nelem, "depth_m": c5.depth,
+                                     "fp64": summary(r5, steps5),
+                                     "mixed": mixed_vs_fp64(c5, m5, r5, args.streams, steps5, args.warmup)}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case, mesh, args.cpu_rows, threads)
     print(json.dumps(out), flush=True)
+
+
+def throughput(case, mesh, precision, streams, steps, warmup, stage_steps):
+    """`steps` frequency solves of case.omegas (in order, cycling) with
+    `streams` solves in flight (one context and HIP stream each), after
+    per-stage timing of `stage_steps` isolated solves and `warmup` untimed
+    solves per context.  Returns the stage times, the wall time, per-solve
+    latencies (submit to collect) and every solve's coefficients in order."""
+    from hamsx.solver import FrequencySolver
+    heads = np.deg2rad(case.headings_deg)
+    omegas = [float(w) for w in case.omegas]
+    nctx = max(1, streams)
+    solvers = []
+    for _ in range(nctx):
+        sv = FrequencySolver(mesh, precision=precision)
+        sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        if "symmetry" in case.meta:
+            sv.set_symmetry(*case.meta["symmetry"])
+        solvers.append(sv)
+    # per-stage device time of isolated solves (one context, nothing else on the GPU)
+    solvers[0].solve_frequency(omegas[0], case.depth)
+    solvers[0].set_profiling(True)
+    solvers[0].stage_times(reset=True)
+    for k in range(stage_steps):
+        solvers[0].solve_frequency(omegas[(k * 7) % len(omegas)], case.depth)
+    stages = solvers[0].stage_times(reset=True)
+    solvers[0].set_profiling(False)
+    for k, sv in enumerate(solvers):
+        for _ in range(warmup):
+            sv.solve_frequency(omegas[k % len(omegas)], case.depth)
+    lat, coef = [], [None] * steps
+    t_sub = [0.0] * nctx
+    owner = [0] * nctx  # solve index each context holds
+    submitted = done = 0
+    t0 = time.perf_counter()
+    while submitted < min(nctx, steps):
+        t_sub[submitted] = time.perf_counter()
+        owner[submitted] = submitted
+        solvers[submitted].submit(omegas[submitted % len(omegas)], case.depth)
+        submitted += 1
+    while done < steps:
+        k = done % nctx
+        am, bd, ex, info = solvers[k].collect()
+        lat.append(time.perf_counter() - t_sub[k])
+        if info != 0:
+            raise SystemExit(f"zero pivot in solve {done}: INFO={info}")
+        coef[owner[k]] = (am, bd, ex)
+        done += 1
+        if submitted < steps:
+            t_sub[k] = time.perf_counter()
+            owner[k] = submitted
+            solvers[k].submit(omegas[submitted % len(omegas)], case.depth)
+            submitted += 1
+    elapsed = time.perf_counter() - t0
+    for sv in solvers:
+        sv.close()
+    return {"stages": {k: v[0] / max(1, v[1]) for k, v in stages.items()}, "elapsed": elapsed,
+            "lat_ms": np.array(lat) * 1e3, "coef": coef}
+
+
+def summary(run, steps):
+    return {"value": steps / run["elapsed"], "unit": "freq-solves/s",
+            "latency_ms": {"p50": float(np.percentile(run["lat_ms"], 50)),
+                           "p99": float(np.percentile(run["lat_ms"], 99))},
+            "stage_ms_per_solve": {k: v for k, v in run["stages"].items() if v > 0}}
+
+
+def mixed_vs_fp64(case, mesh, fp64_run, streams, steps, warmup):
+    """The mixed-precision path (complex64 LU on MFMA + FP64 refinement) on the
+    same workload and solve sequence as fp64_run: its throughput, latency, the
+    ratio to the FP64 path, and the largest difference of every coefficient
+    table from the FP64 path's over the run (relative to that table's largest
+    entry at that frequency, and per coefficient against its own magnitude
+    for those above 1e-3 of the largest)."""
+    print(f"[bench] {case.name}: mixed precision, {steps} solves", file=sys.stderr, flush=True)
+    mx = throughput(case, mesh, "mixed", streams, steps, warmup, 1)
+    out = summary(mx, steps)
+    out["ratio_vs_fp64"] = out["value"] / (steps / fp64_run["elapsed"])
+    err = {"amas": 0.0, "bdmp": 0.0, "exfc": 0.0}
+    own = dict(err)
+    for a, b in zip(mx["coef"], fp64_run["coef"]):
+        for key, x, y in zip(("amas", "bdmp", "exfc"), a, b):
+            m = float(np.max(np.abs(y)))
+            err[key] = max(err[key], float(np.max(np.abs(x - y))) / m)
+            big = np.abs(y) >= 1e-3 * m
+            own[key] = max(own[key], float(np.max(np.abs(x - y)[big] / np.abs(y)[big])))
+    out["max_rel_err_vs_fp64"] = err
+    out["max_rel_err_vs_fp64_own_magnitude"] = own
+    return out
+
+
+def half_vs_full(case, mesh, precision):
+    """The same frequency on the whole mesh: the plane must not change the answer."""
+    from hamsx import configs
+    from hamsx.mesh import combine
+    from hamsx.solver import FrequencySolver
+    heads = np.deg2rad(case.headings_deg)
+    w = float(case.omegas[0])
+    full = configs.two_cylinders(nfreq=len(case.omegas))
+    with FrequencySolver(combine(full.bodies, full.xr), precision=precision) as sv:
+        sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        t1 = time.perf_counter()
+        fa, fb, fe, _ = sv.solve_frequency(w, case.depth)
+        t_full = time.perf_counter() - t1
+    with FrequencySolver(mesh, precision=precision) as sv:
+        sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        sv.set_symmetry(*case.meta["symmetry"])
+        ha, hb, he, _ = sv.solve_frequency(w, case.depth)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return {"w": w, "amas": rel(ha, fa), "bdmp": rel(hb, fb), "exfc": rel(he, fe), "full_mesh_solve_ms": t_full * 1e3}
 
 
 if __name__ == "__main__":

@@ -180,25 +180,49 @@ def test_config4_serving_stream_504_to_12120_panels():
 
 
 def test_config5_lidded_array_gmres_hmatrix():
+    """GMRES (tolerance 1e-9) on the H-matrix of the normal matrix C = A^T A
+    against the dense FP64 LU of C, both on the GPU, at the workload's ACA
+    tolerance 1e-6 (rank <= 50) and at 1e-8 (rank <= 64).  Added mass and
+    excitation are held to 5e-6 (ACA 1e-6; measured up to 1.9e-6) and 1e-6
+    (ACA 1e-8; measured 2e-7) of their tables' largest entries.  Damping is
+    held to the error the added mass implies for it, that bound x
+    max(1, w max|AMAS| / max|BDMP|): at low frequency the damping is a small part of the
+    potentials' effect, so the same potential error is a larger fraction of
+    it -- why the reference's own GMRES test allows 5e-2 on damping against
+    1e-3 on added mass (test/test_integration_gmres.f90:69,97)."""
     case = configs.large_array(600, grid=2)
     m = combine(case.bodies, case.xr)
     lid = np.concatenate([panel_geometry(lm, np.zeros(3))[0] for lm in case.meta["lids"]])
     heads = np.deg2rad(case.headings_deg)
     say(f"[gpu] config 5 (2 x 2): {case.name} N={m.nelem} lid={len(lid)} depth={case.depth}")
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    checks = []
     for w in (0.4, 1.4):
         with FrequencySolver(m) as s:
             s.set_lid(lid)
             s.set_waves(heads)
             lu = s.solve_frequency(w, case.depth)
-            s.set_solver("gmres", tol=1e-8, restart=25, maxiter=400)
-            s.set_hmatrix(True, 2.0, 1e-6, 32, 50)
-            t0 = time.perf_counter()
-            gm = s.solve_frequency(w, case.depth)
-            t = time.perf_counter() - t0
-            it, res = s.refine_stats()
-            nlr, nd, mr, ratio = s.hmatrix_stats()
-        say(f"    w={w}: GMRES {it} iterations, residual {res:.1e}, {t * 1e3:.0f} ms; H-matrix {nlr} low-rank "
-            f"(mean rank {mr:.1f}), {nd} dense, compression {ratio:.1f}")
-        assert lu[3] == 0 and gm[3] == 0 and nlr > 0
-        gates.coef_gates(gm, lu, [], 1e-6, say, "gmres+H vs LU ")
-        report_own("gmres+H vs LU", gm, lu)
+            assert lu[3] == 0
+            for aca, rank in ((1e-6, 50), (1e-8, 64)):
+                s.set_solver("gmres", tol=1e-9, restart=25, maxiter=400)
+                s.set_hmatrix(True, 2.0, aca, 32, rank)
+                t0 = time.perf_counter()
+                gm = s.solve_frequency(w, case.depth)
+                t = time.perf_counter() - t0
+                it, res = s.refine_stats()
+                nlr, nd, mr, ratio = s.hmatrix_stats()
+                errs = [rel(a, b) for a, b in zip(gm[:3], lu[:3])]
+                bound = 5e-6 if aca >= 1e-6 else 1e-6
+                bd_bound = bound * max(1.0, w * np.max(np.abs(lu[0])) / np.max(np.abs(lu[1])))
+                say(f"    w={w} ACA {aca:.0e} rank <= {rank}: GMRES {it} iterations, residual {res:.1e}, "
+                    f"{t * 1e3:.0f} ms; H-matrix {nlr} low-rank (mean rank {mr:.1f}), {nd} dense, compression "
+                    f"{ratio:.1f}; vs LU: added mass {errs[0]:.2e} damping {errs[1]:.2e} (bound {bd_bound:.1e}) "
+                    f"excitation {errs[2]:.2e}")
+                report_own(f"ACA {aca:.0e} gmres+H vs LU", gm, lu)
+                checks.append((gm[3] == 0 and nlr > 0, errs, bound, bd_bound))
+    for ok, errs, bound, bd_bound in checks:
+        assert ok
+        assert errs[0] <= bound and errs[2] <= bound and errs[1] <= bd_bound
