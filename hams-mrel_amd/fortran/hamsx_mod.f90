@@ -15,6 +15,23 @@
 !   FORCES_GPU               RFORCE_MULTI / EFORCE_MULTI sums (PotentWavForceMulti.f90:45-226)
 !   PANEL_GEOMETRY_GPU       NormalProcess for one body (NormalProcess.f90:46-441)
 !
+! Part 3 keeps the reference's own names and argument lists for the calls of
+! the multi-body loop (src/AssbMatxMulti.f90:53-383), acting on the context
+! GPU_INIT opened last, so the loop body of src/HAMS_Prog.f90:629-724 runs
+! unchanged:
+!
+!   CALGREEN_MULTI()                                   assembly on the device
+!   ASSB_LEFT_MULTI(AMAT, IPIV, N, NSYS)               ZGETRF on the device
+!   ASSB_RBC_MULTI(BRMAT, N, NSYS, JSTART, JEND)       (right-hand sides formed by the solver)
+!   RADIATION_SOLVER_MULTI(AMAT, BRMAT, IPIV, MXPOT, N, NSYS)
+!   ASSB_DBC_MULTI(BDMAT, N, NSYS)                     (right-hand side formed by the solver)
+!   DIFFRACTION_SOLVER_MULTI(AMAT, BDMAT, IPIV, MXPOT, N, NSYS)   one heading per call
+!
+! AMAT, IPIV, BRMAT and BDMAT are the caller's storage in the reference; here
+! the matrices, factors and right-hand sides stay in device memory and those
+! arrays are left untouched.  MXPOT(N, 6 NBODY + 1, NSYS) receives the
+! potentials as the reference fills it.
+!
 ! The influence matrices and their factors stay in device memory; only the
 ! potentials (optional) and the force coefficients come back.  Every wrapper
 ! stops the program with the library's message on an error, the way the
@@ -42,6 +59,14 @@ module hamsx_mod
   public :: GPU_INIT, GPU_FINALIZE, SET_MESH_MULTI_GPU, SET_WAVES_MULTI_GPU, CALWAVEPROPERTS_GPU, &
             CALGREEN_ASSB_LEFT_GPU, RADIATION_SOLVER_GPU, DIFFRACTION_SOLVER_GPU, FORCES_GPU, &
             PANEL_GEOMETRY_GPU, SET_SYMMETRY_GPU, hamsx_error_message
+  public :: CALGREEN_MULTI, ASSB_LEFT_MULTI, ASSB_RBC_MULTI, RADIATION_SOLVER_MULTI, ASSB_DBC_MULTI, &
+            DIFFRACTION_SOLVER_MULTI
+
+  ! state of the reference-named calls (part 3): the active context, the mesh's
+  ! body count, the headings and the diffraction potentials of all headings
+  type(c_ptr), save :: g_ctx = c_null_ptr
+  integer, save :: g_nbody = 0, g_nbeta = 0, g_nsys = 1, g_heading = 0
+  complex(c_double_complex), allocatable, target, save :: g_dif(:, :, :)
 
   ! ---------------------------------------------------------------- part 1
   interface
@@ -319,11 +344,13 @@ contains
     integer, intent(in) :: device, precision
     call check(hamsx_create(ctx, int(device, c_int)), "hamsx_create")
     call check(hamsx_set_precision(ctx, int(precision, c_int)), "hamsx_set_precision")
+    g_ctx = ctx
   end subroutine
 
   subroutine GPU_FINALIZE(ctx)
     type(c_ptr), intent(inout) :: ctx
     call check(hamsx_destroy(ctx), "hamsx_destroy")
+    if (c_associated(ctx, g_ctx)) g_ctx = c_null_ptr
     ctx = c_null_ptr
   end subroutine
 
@@ -357,6 +384,8 @@ contains
     ncon_c = int(NCON_COMB, c_int)
     call check(hamsx_set_mesh(ctx, int(TNELEM, c_int), int(TNNODE, c_int), XYZ_COMB, ncn_c, ncon_c, XYZ_P_COMB, &
                               DS_COMB, PNSZ_COMB, DXYZ_COMB, int(NB, c_int), starts), "hamsx_set_mesh")
+    g_nbody = NB
+    g_nsys = 1
   end subroutine
 
   ! Headings (radians), AMP, XW(2), RHO and ISOL as read from ControlFile.in.
@@ -366,6 +395,7 @@ contains
     real(c_double), intent(in) :: WVHD(NBETA), AMP, XW(2), RHO
     call check(hamsx_set_waves(ctx, int(NBETA, c_int), WVHD, AMP, XW(1), XW(2), RHO, int(ISOL, c_int)), &
                "hamsx_set_waves")
+    g_nbeta = NBETA
   end subroutine
 
   ! INFT/OUFT as in ControlFile.in; returns V, WK, W1 and WVN(NK) and makes
@@ -413,6 +443,7 @@ contains
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: ISX, ISY
     call check(hamsx_set_symmetry(ctx, int(ISX, c_int), int(ISY, c_int)), "hamsx_set_symmetry")
+    g_nsys = merge(2, 1, ISX /= 0 .or. ISY /= 0)
   end subroutine
 
   ! AMAS/BDMP(6NB, 6NB) as AMAS_MULTI_COMB(KK,:,:); EXFC(6NB, NBETA).
@@ -421,6 +452,71 @@ contains
     real(c_double), intent(out), target, contiguous :: AMAS(:, :), BDMP(:, :)
     complex(c_double_complex), intent(out), target, contiguous :: EXFC(:, :)
     call check(hamsx_forces(ctx, c_loc(AMAS), c_loc(BDMP), c_loc(EXFC)), "hamsx_forces")
+  end subroutine
+
+  ! ---------------------------------------------------------------- part 3
+  ! CALGREEN_MULTI (CalGreenFuncMulti.f90:112-183): the Green functions and
+  ! panel integrals of every pair, fused on the device with the influence
+  ! matrix of ASSB_LEFT_MULTI (no CGRN / RKBN arrays).
+  subroutine CALGREEN_MULTI()
+    call check(hamsx_assemble(g_ctx), "hamsx_assemble")
+    g_heading = 0
+  end subroutine
+
+  ! ASSB_LEFT_MULTI (AssbMatxMulti.f90:53-114): ZGETRF of the assembled
+  ! matrix, on the device; INFO is ignored as the reference ignores it.
+  subroutine ASSB_LEFT_MULTI(AMAT, IPIV, N, NSYS)
+    integer, intent(in) :: N, NSYS
+    complex(c_double_complex), intent(inout) :: AMAT(N, N, NSYS)
+    integer, intent(inout) :: IPIV(N, NSYS)
+    call check(hamsx_factor(g_ctx), "hamsx_factor")
+  end subroutine
+
+  ! ASSB_RBC_MULTI (AssbMatxMulti.f90:124-181): the radiation right-hand
+  ! sides of panels JSTART..JEND; RADIATION_SOLVER_MULTI forms them on the device.
+  subroutine ASSB_RBC_MULTI(BRMAT, N, NSYS, JSTART, JEND)
+    integer, intent(in) :: N, NSYS, JSTART, JEND
+    complex(c_double_complex), intent(inout) :: BRMAT(N, 6, NSYS)
+  end subroutine
+
+  ! RADIATION_SOLVER_MULTI (AssbMatxMulti.f90:257-317): MXPOT(:, 1:6 NBODY, :).
+  subroutine RADIATION_SOLVER_MULTI(AMAT, BRMAT, IPIV, MXPOT, N, NSYS)
+    integer, intent(in) :: N, NSYS
+    complex(c_double_complex), intent(in) :: AMAT(N, N, NSYS), BRMAT(N, 6 * g_nbody, NSYS)
+    integer, intent(in) :: IPIV(N, NSYS)
+    complex(c_double_complex), intent(inout) :: MXPOT(N, 6 * g_nbody + 1, NSYS)
+    complex(c_double_complex), allocatable, target :: x(:, :, :)
+    if (NSYS /= g_nsys) call check(-1_c_int, "RADIATION_SOLVER_MULTI: NSYS differs from the symmetry set")
+    allocate(x(N, 6 * g_nbody, NSYS))
+    call check(hamsx_solve_radiation(g_ctx, c_loc(x)), "hamsx_solve_radiation")
+    MXPOT(:, 1:6 * g_nbody, :) = x
+  end subroutine
+
+  ! ASSB_DBC_MULTI (AssbMatxMulti.f90:187-251): the diffraction right-hand
+  ! side of the current heading; DIFFRACTION_SOLVER_MULTI forms it on the device.
+  subroutine ASSB_DBC_MULTI(BDMAT, N, NSYS)
+    integer, intent(in) :: N, NSYS
+    complex(c_double_complex), intent(inout) :: BDMAT(N, NSYS)
+  end subroutine
+
+  ! DIFFRACTION_SOLVER_MULTI (AssbMatxMulti.f90:323-383): MXPOT(:, 6 NBODY + 1, :)
+  ! for the next heading of the driver's loop DO II = 1, NBETA
+  ! (HAMS_Prog.f90:700-727).  The first call after CALGREEN_MULTI solves every
+  ! heading at once on the device; the following calls return the next one.
+  subroutine DIFFRACTION_SOLVER_MULTI(AMAT, BDMAT, IPIV, MXPOT, N, NSYS)
+    integer, intent(in) :: N, NSYS
+    complex(c_double_complex), intent(in) :: AMAT(N, N, NSYS), BDMAT(N, NSYS)
+    integer, intent(in) :: IPIV(N, NSYS)
+    complex(c_double_complex), intent(inout) :: MXPOT(N, 6 * g_nbody + 1, NSYS)
+    if (NSYS /= g_nsys) call check(-1_c_int, "DIFFRACTION_SOLVER_MULTI: NSYS differs from the symmetry set")
+    if (g_heading == 0) then
+      if (allocated(g_dif)) deallocate(g_dif)
+      allocate(g_dif(N, max(g_nbeta, 1), NSYS))
+      call check(hamsx_solve_diffraction(g_ctx, c_loc(g_dif)), "hamsx_solve_diffraction")
+    end if
+    if (g_heading >= g_nbeta) call check(-1_c_int, "DIFFRACTION_SOLVER_MULTI: more calls than headings")
+    g_heading = g_heading + 1
+    MXPOT(:, 6 * g_nbody + 1, :) = g_dif(:, g_heading, :)
   end subroutine
 
 end module hamsx_mod

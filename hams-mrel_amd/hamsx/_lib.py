@@ -18,6 +18,7 @@ HAMSX_ENOMEM = -3
 HAMSX_ESTATE = -4
 HAMSX_ENOCONV = -5
 HAMSX_NK = 200
+FREQ_LIMIT = 0  # HAMSX_FREQ_LIMIT: hamsx_wave_properties of the zero / infinite frequency limits
 HAMSX_NOALLOC_MAX_N = 32768  # largest order hamsx_factor_noalloc takes
 PREC_FP64 = 0
 PREC_MIXED = 1

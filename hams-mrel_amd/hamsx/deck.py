@@ -196,7 +196,7 @@ def write_pnl(path: str, mesh: PanelMesh, title: str = "Hull Mesh File") -> None
         f.write(f"{mesh.nelem:12d}{mesh.nnode:12d}{mesh.isx:12d}{mesh.isy:12d}\n \n")
         f.write("    # Start Definition of Node Coordinates     ! node_number   x   y   z\n")
         for i, p in enumerate(mesh.xyz):
-            f.write(f"{i + 1:8d}{p[0]:24.15E}{p[1]:24.15E}{p[2]:24.15E}\n")
+            f.write(f"{i + 1:8d}" + "".join(f"{_num(x):>25s}" for x in p[:3]) + "\n")  # exact round trip
         f.write("   # End Definition of Node Coordinates\n \n")
         f.write("   # Start Definition of Node Relations   ! panel_number  number_of_vertices   Vertex1_ID   Vertex2_ID   Vertex3_ID   (Vertex4_ID)\n")
         for e in range(mesh.nelem):
@@ -204,6 +204,83 @@ def write_pnl(path: str, mesh: PanelMesh, title: str = "Hull Mesh File") -> None
             ids = " ".join(f"{int(v):8d}" for v in mesh.ncon[e, :k])
             f.write(f"{e + 1:8d}{k:6d} {ids}\n")
         f.write("   # End Definition of Node Relations\n")
+
+
+def _num(x: float) -> str:
+    """A real for a list-directed or F-format field: the shortest repr that
+    reads back to the same double, with a decimal point."""
+    t = repr(float(x))
+    return t if ("." in t or "e" in t or "n" in t) else t + ".0"
+
+
+def _f12(x: float) -> str:
+    """A real for a 12-column F12.3 field (LCS, reference body centres,
+    InputFiles.f90:135,164): the exact value when its shortest form fits in
+    11 columns, else the fixed-point form with as many decimals as fit."""
+    t = _num(x)
+    if len(t) <= 11 and "e" not in t:
+        return f"{t:>12s}"
+    for d in range(10, 0, -1):
+        t = f"{float(x):.{d}f}"
+        if len(t) <= 11:
+            return f"{t:>12s}"
+    raise ValueError(f"{x} does not fit an F12.3 field")
+
+
+def write_control(path: str, ctl: ControlDeck) -> None:
+    """ControlFile.in in the layout ReadControlFile reads
+    (``src/InputFiles.f90:46-224``): every fixed-column field at the column
+    its READ format starts, frequencies and headings as list-directed
+    records (positive counts), the optional Solver_type line when ISOLV = 2,
+    and the field points."""
+    L = ["   --------------HAMS Control file---------------", "",
+         f"{'   Waterdepth':<14}{_num(ctl.depth)}", "",
+         "   #Start Definition of Wave Frequencies",
+         f"{'    0_inf_frequency_limits':<27}{ctl.sybo:16d}",
+         f"{'    Input_frequency_type':<25}{ctl.inft:16d}",
+         f"{'    Output_frequency_type':<25}{ctl.ouft:17d}",
+         f"{'    Number_of_frequencies':<26}{len(ctl.wvnb):16d}",
+         "    " + "  ".join(_num(v) for v in ctl.wvnb),
+         "   #End Definition of Wave Frequencies", ""]
+    if ctl.nbody > 1:
+        L += ["   # Start of definition of bodies for multi-body interaction",
+              f"{'    Number_of_bodies':<24}{ctl.nbody:16d}"]
+        for b in range(ctl.nbody):
+            L.append(f"{'    LCS_' + str(b + 1):<26}" + "".join(_f12(x) for x in ctl.lcs[b]))
+        L += ["   # End of definition of bodies for multi-body interaction", ""]
+    L += ["   #Start Definition of Wave Headings",
+          f"{'    Number_of_headings':<23}{len(ctl.headings_deg):16d}",
+          "    " + "  ".join(_num(v) for v in ctl.headings_deg),
+          "   #End Definition of Wave Headings", ""]
+    for b in range(ctl.nbody):
+        name = "    Reference_body_center" + (f"_{b + 1}" if ctl.nbody > 1 else "")
+        L.append(f"{name:<28}" + "".join(_f12(x) for x in ctl.xr[b]))
+    L += [f"{'    Reference_body_length':<26}{_num(ctl.refl)}",
+          f"{'    Wave_diffrac_solution':<26}{ctl.isol:16d}",
+          f"{'    If_remove_irr_freq':<23}{ctl.irsp:16d}",
+          f"{'    Number of threads':<23}{ctl.nthread:16d}"]
+    if ctl.isolv == 2:
+        L.append(f"{'    Solver_type':<26}{2:16d}")
+    L += ["", "   #Start Definition of Pressure and/or Elevation (PE)",
+          f"{'    Number_of_field_points':<27}{len(ctl.xfp):16d}"]
+    L += ["    " + "  ".join(_num(x) for x in p) for p in np.asarray(ctl.xfp).reshape(-1, 3)]
+    L += ["   #End Definition of Pressure and/or Elevation", "",
+          "   ----------End HAMS Control file---------------"]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def write_hydrostatic(path: str, h: "Hydrostatic") -> None:
+    """Hydrostatic*.in in the layout ReadHydroStaticMulti reads
+    (``src/HydroStaticMulti.f90:68-98``)."""
+    L = [" Center of Gravity:", "  " + "  ".join(_num(x) for x in h.xg)]
+    for title, m in (("Body Mass Matrix:", h.mass), ("External Linear Damping Matrix:", h.blnr),
+                     ("External Quadratic Damping Matrix:", h.bqdr), ("Hydrostatic Restoring Matrix:", h.crs),
+                     ("External Restoring Matrix:", h.kstf)):
+        L.append(" " + title)
+        L += ["  " + "   ".join(_num(x) for x in row) for row in np.asarray(m)]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
 
 
 @dataclass

@@ -104,3 +104,34 @@ def test_fortran_host_half_mesh_with_symmetry(tmp_path, isx):
     np.testing.assert_array_equal(am, a2)
     np.testing.assert_array_equal(bd, b2)
     np.testing.assert_array_equal(ex, e2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("half", [False, True])
+def test_reference_named_calls_equal_the_gpu_wrappers(tmp_path, half):
+    """The loop body of HAMS_Prog.f90:629-724 under the reference's own names
+    and argument lists (CALGREEN_MULTI, ASSB_LEFT_MULTI, ASSB_RBC_MULTI,
+    RADIATION_SOLVER_MULTI, ASSB_DBC_MULTI, DIFFRACTION_SOLVER_MULTI; one
+    heading per diffraction call) gives the *_GPU wrappers' potentials and
+    coefficients bit for bit, with and without a symmetry plane."""
+    pnl, inp = write_case(tmp_path, 30.0, [0.7, 1.6], [0.0, 60.0, 120.0])
+    if half:  # the half y > 0 with ISY = 1 in the header: NSYS = 2
+        from hamsx.deck import PanelMesh
+        from test_oracle_symmetry import half as half_of
+        h = half_of(cylinder(4.0, 6.0, 20, 5, 3), 1)
+        write_pnl(str(pnl), PanelMesh(h.xyz, h.ncn, h.ncon, 0, 1))
+    out = {}
+    for style in (0, 1):
+        text = inp.read_text().splitlines()[:8] + [str(style)]
+        inp.write_text("\n".join(text) + "\n")
+        print(f"[gpu] Fortran host style {style} half={half}", flush=True)
+        p = subprocess.run([DRIVER, str(inp)], capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stdout + p.stderr
+        toks = p.stdout.split()
+        diffs = [float(toks[k + 1]) for k in range(len(toks)) if toks[k] == "MXPOT"]
+        keep = [t for k, t in enumerate(toks) if t != "MXPOT" and (k == 0 or toks[k - 1] != "MXPOT")]
+        out[style] = (parse(" ".join(keep), 3), diffs)
+    assert out[1][1] == [0.0, 0.0]
+    for a, b in zip(out[0][0], out[1][0]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
