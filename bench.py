@@ -189,12 +189,19 @@ def pick_case(name):
 
 
 def cpu_baseline(case, mesh, rows, threads):
-    """Seconds per frequency solve on the host: the oracle's assembly on an
-    evenly strided sample of `rows` field panels against all N source panels
-    (scaled by N / rows), plus LAPACK zgetrf + zgetrs (scipy's OpenBLAS) on a
-    full N x N matrix with the same number of right-hand sides.  Right-hand
-    sides and force sums are O(N^2) and O(N) and are left out (they favour
-    the CPU)."""
+    """Seconds per frequency solve on the host.
+
+    With the reference checker built (oracle/_ref/libhams_ref.so, the
+    reference's own Fortran compiled from its sources): the reference's own
+    pair loop CALGREEN_MULTI (CalGreenFuncMulti.f90:112-183, its OpenMP loop,
+    Green values stored in FP32 as the shipped build does) over all N^2
+    pairs on `threads` threads -- kind "reference".  Otherwise the oracle's
+    assembly (a C restatement of the same loop) on an evenly strided sample of
+    `rows` field panels, scaled by N / rows -- kind "port".  Either way LAPACK
+    zgetrf + zgetrs (scipy's OpenBLAS, for the reference's MKL) on a full
+    N x N matrix with the same number of right-hand sides is added.  The
+    right-hand sides and force sums are O(N^2) and O(N) and are left out
+    (they favour the CPU)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import scipy.linalg as sl
@@ -204,7 +211,19 @@ def cpu_baseline(case, mesh, rows, threads):
     t0 = time.perf_counter()
     oracle.assemble_rows(mesh, idx, w, case.depth, nthreads=threads)
     t_sample = time.perf_counter() - t0
-    t_asm = t_sample * n / len(idx)
+    t_port = t_sample * n / len(idx)
+    kind, t_asm = "port", t_port
+    sample = (f"oracle assembly of {len(idx)} of {n} rows (all {n} columns) at w={w:.3f} rad/s, "
+              f"{t_sample:.1f} s on {threads} threads, scaled to N rows = {t_port:.1f} s")
+    if oracle.ref_available():
+        t0 = time.perf_counter()
+        chk = oracle.ref_calgreen(mesh, w, case.depth, threads)
+        t_ref = time.perf_counter() - t0
+        if np.isfinite(chk):
+            kind, t_asm = "reference", t_ref
+            sample = (f"the reference's CALGREEN_MULTI (its OpenMP pair loop, built from its sources) over all "
+                      f"{n}x{n} pairs at w={w:.3f} rad/s: {t_ref:.1f} s on {threads} threads (the C port on a "
+                      f"{len(idx)}-row sample, scaled: {t_port:.1f} s)")
     rng = np.random.default_rng(0)
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 0.01 + 2 * math.pi * np.eye(n)
     A = np.asfortranarray(A)
@@ -215,11 +234,9 @@ def cpu_baseline(case, mesh, rows, threads):
     sl.lu_solve(lu, B, check_finite=False)
     t_lu = time.perf_counter() - t0
     per = t_asm + t_lu
-    return {"value": 1.0 / per, "unit": "freq-solves/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle assembly of {len(idx)} of {n} rows (all {n} columns) at w={w:.3f} rad/s, "
-                       f"{t_sample:.1f} s on {threads} threads, scaled to N rows = {t_asm:.1f} s; "
-                       f"+ scipy/OpenBLAS zgetrf+zgetrs N={n}, {nr} rhs = {t_lu:.2f} s"),
-            "seconds_per_solve": per}
+    return {"value": 1.0 / per, "unit": "freq-solves/s", "cores": threads, "kind": kind,
+            "sample": sample + f"; + scipy/OpenBLAS zgetrf+zgetrs N={n}, {nr} rhs = {t_lu:.2f} s",
+            "seconds_per_solve": per, "port_seconds_per_solve": t_port + t_lu}
 
 
 def serving(args):

@@ -141,6 +141,8 @@ def _bind_ref(r):
         r.ref_infgreen3d.argtypes = [_d] * 7 + [C.c_int, _dp]
         r.ref_fingreen3d.argtypes = [_d] * 7 + [_dp, C.c_int, _d, C.c_int, _dp]
         r.ref_set_mesh.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _dp]
+        r.ref_calgreen.restype = _d
+        r.ref_calgreen.argtypes = [C.c_int, C.c_int, _dp, _d, _dp, C.c_int, _d]
         r.ref_sglint.argtypes = [C.c_int, C.c_int, _dp, _dp]
         r.ref_sglint2.argtypes = [_dp, C.c_int, _dp]
         r.ref_dinp.argtypes = [_d] * 11 + [_dp]
@@ -400,3 +402,15 @@ def solve_assembled(m, A, S, w1: float, h: float, headings_rad, amp=1.0, xw=(0.0
     X = sl.lu_solve(sl.lu_factor(np.asfortranarray(A), check_finite=False), B, check_finite=False)
     am, bd, ex = forces(m, X, w1, h, headings_rad, amp, xw, rho, isol)
     return am, bd, ex, X
+
+
+def ref_calgreen(m, w1: float, h: float, nthreads: int = 8) -> float:
+    """Runs the reference's own pair loop CALGREEN_MULTI (its OpenMP loop over
+    field panels, CalGreenFuncMulti.f90:112-183, built as shipped) over every
+    pair of mesh m at (w1, h) on `nthreads` threads; returns its checksum.
+    For timing the reference on the host (the CPU baseline)."""
+    r = ref()
+    v, wk, wvn = wave(w1, h)
+    r.ref_set_mesh(m.nnode, _p(np.asfortranarray(m.xyz)), m.nelem, _p(np.ascontiguousarray(m.ncn), C.c_int),
+                   _p(np.asfortranarray(m.ncon), C.c_int), _p(np.asfortranarray(m.cen)))
+    return r.ref_calgreen(nthreads, m.nelem, _p(np.ascontiguousarray(m.pnsz)), v, _p(wvn), NK, h)

@@ -2,7 +2,8 @@
 !
 ! TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's source
 ! files where they lie (WavDynMods, FinGrnExtSubs, FinGreen3D, InfGreen_Appr,
-! SingularIntgrMulti, WavDynSubs, PatclVelct, NormalProcess, ImplementSubs)
+! SingularIntgrMulti, WavDynSubs, PatclVelct, NormalProcess, ImplementSubs,
+! CalGreenFuncMulti)
 ! together with this file into oracle/_ref/libhams_ref.so.  It is used only to
 ! write and check golden vectors for the C oracle (tests/golden/make_golden.py,
 ! tests/test_oracle_golden.py).  Nothing in the product links it.
@@ -23,6 +24,9 @@ module ref_glue
    use NormalProcess
    use ImplementSubs
    use HAMS_mod
+   use Inerfs_mod
+   use Potentials_mod
+   use CalGreenFuncMulti
    implicit none
 
 contains
@@ -197,6 +201,35 @@ contains
       ISX = 0
       ISY = 0
    end subroutine
+
+   ! The reference's own pair loop, CALGREEN_MULTI (CalGreenFuncMulti.f90:112-183,
+   ! OpenMP over field panels), on the mesh of ref_set_mesh with panel sizes
+   ! pnsz_: NSYS = 1, nthreads threads, the frequency (vv, wvn_, hh).  For the
+   ! CPU baseline's timing.  Returns the sum of |CGRN(:,:,1,1)| as a checksum.
+   function ref_calgreen(nthreads, nelem, pnsz_, vv, wvn_, nk_, hh) result(chk) bind(C, name='ref_calgreen')
+      integer(c_int), value :: nthreads, nelem, nk_
+      real(c_double), intent(in) :: pnsz_(nelem), wvn_(nk_)
+      real(c_double), value :: vv, hh
+      real(c_double) :: chk
+      TNELEM = nelem
+      NSYS = 1
+      ISX = 0
+      ISY = 0
+      NTHREAD = nthreads
+      V = vv
+      H = hh
+      WVN = 0.d0
+      WVN(1:min(nk_, NK)) = wvn_(1:min(nk_, NK))
+      if (allocated(PNSZ_MULTI_COMB)) deallocate(PNSZ_MULTI_COMB)
+      allocate(PNSZ_MULTI_COMB(nelem))
+      PNSZ_MULTI_COMB = pnsz_
+      if (allocated(RKBN_MULTI_COMB)) deallocate(RKBN_MULTI_COMB)
+      if (allocated(CGRN_MULTI_COMB)) deallocate(CGRN_MULTI_COMB)
+      allocate(RKBN_MULTI_COMB(nelem, nelem, 1, 4), CGRN_MULTI_COMB(nelem, nelem, 1, 4))
+      call CALGREEN_MULTI
+      chk = sum(abs(CGRN_MULTI_COMB(:, :, 1, 1)))
+      deallocate(RKBN_MULTI_COMB, CGRN_MULTI_COMB)
+   end function
 
    ! Newman integral of source panel jel at the centroid of field panel iel (1-based)
    subroutine ref_sglint(iel, jel, sij, dij) bind(C, name='ref_sglint')

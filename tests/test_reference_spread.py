@@ -152,3 +152,28 @@ def test_gates_pass_a_valid_rounding_and_catch_mistakes(h, w):
     if h == 50.0:
         with pytest.raises(AssertionError):
             gates.coef_gate("damping", badc, co[1], [c[1] for c in cd], 1e-8)
+
+
+@needs_ref
+def test_reference_pair_loop_used_for_the_cpu_baseline():
+    """oracle.ref_calgreen runs the reference's own CALGREEN_MULTI (its OpenMP
+    pair loop): its checksum, the sum of |CGRN(:,:,1,1)| stored in FP32 as the
+    shipped build stores it, equals the sum over the same pairs of the
+    reference's FINGREEN3D value with the loop's FLAG rule, to FP32 rounding."""
+    bodies = [cylinder(5.0, 10.0, 12, 3, 2, -20.0), cylinder(5.0, 10.0, 12, 3, 2, 20.0)]
+    m = combine(bodies, [np.array([-20.0, 0, 0]), np.array([20.0, 0, 0])])
+    w, h = 1.2, 50.0
+    chk = oracle.ref_calgreen(m, w, h, nthreads=4)
+    v, wk, wvn = oracle.wave(w, h)
+    r = oracle.ref()
+    c = m.cen
+    g = np.zeros(8)
+    total = 0.0
+    for i in range(m.nelem):
+        for j in range(m.nelem):
+            dist = np.sqrt((c[i, 0] - c[j, 0]) ** 2 + (c[i, 1] - c[j, 1]) ** 2 + (c[i, 2] - c[j, 2]) ** 2)
+            flag = 1 if dist <= 50.0 * m.pnsz[j] else 0
+            r.ref_fingreen3d(c[j, 0], c[i, 0], c[j, 1], c[i, 1], c[j, 2], c[i, 2], v, _p(wvn), 200, h, flag, _p(g))
+            total += abs(np.complex64(complex(g[0], g[1])))
+    print(f"\nchecksum {chk:.9g} vs per-pair {total:.9g}")
+    assert abs(chk - total) <= 1e-5 * total
