@@ -62,6 +62,21 @@ REGION_COUNTERS = {
 REGIONS = ("asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
 
 
+def region_ops(workload):
+    """Algorithmic operations per entry of each finite-depth region, counted
+    once on the CPU restatement of the reference's algorithm (region_ops.json,
+    written by oracle/opcount.py: add/sub/mul/div/sqrt and each libm call = 1
+    operation), or None when the file holds another workload."""
+    try:
+        with open(os.path.join(ROOT, "region_ops.json")) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("workload") != workload:
+        return None
+    return {k: d["regions"][k[4:].upper()]["per_entry"]["ops"] for k in REGIONS if k[4:].upper() in d["regions"]}
+
+
 def region_pairs(mesh, depth):
     """Pairs per finite-depth region (R/h < 5e-4, < 0.05, < 0.5, >= 0.5),
     classified with the kernels' own expression |sqrt(dx^2 + dy^2) / h|."""
@@ -109,6 +124,16 @@ def roofline(case, mesh, ms, precision):
                     "traffic": cnt["traffic_per_pair"] * pairs, "flops_per_pair": cnt["flops_per_pair"]})
     else:
         out.update({"achieved": None, "frac": None, "traffic": None})
+    alg = region_ops(case.name)
+    if alg and dom in alg:
+        # beside the issued rate above: the algorithm's own operation count
+        # (a division or a libm call is one operation here, several issued
+        # instructions on the GPU) at the measured launch time
+        rate = alg[dom] * pairs / (ms[dom] * 1e-3) / 1e12
+        out["algorithmic"] = {"ops_per_entry": alg[dom], "ops": alg[dom] * pairs, "achieved": rate,
+                              "unit": "Tops/s", "frac": rate / PEAK_FP64_TFLOPS,
+                              "issued_per_algorithmic": (cnt["flops_per_pair"] / alg[dom]) if cnt else None,
+                              "source": "region_ops.json (oracle/opcount.py)"}
     return out
 
 
@@ -131,8 +156,14 @@ def solve_roofline(case, mesh, nbeta, ms_per_solve, lu_peak):
     f_lu = 8.0 * n ** 3 / 3.0
     f_vec = 3 * 8.0 * n * n * nrhs
     t_roof = (f_asm / PEAK_FP64_TFLOPS + f_lu / lu_peak + f_vec / PEAK_FP64_TFLOPS) / 1e9  # ms
-    return {"bound": "fp64 valu (assembly) + mfma (LU)", "flops": {"assembly": f_asm, "lu": f_lu, "rhs_solves": f_vec},
-            "t_roof_ms": t_roof, "ms_per_solve": ms_per_solve, "frac": t_roof / ms_per_solve}
+    out = {"bound": "fp64 valu (assembly) + mfma (LU)", "flops": {"assembly": f_asm, "lu": f_lu, "rhs_solves": f_vec},
+           "t_roof_ms": t_roof, "ms_per_solve": ms_per_solve, "frac": t_roof / ms_per_solve}
+    alg = region_ops(case.name)
+    if alg:
+        a_asm = sum(alg[k] * pairs[k] for k in REGIONS if k in alg)
+        t_alg = (a_asm / PEAK_FP64_TFLOPS + f_lu / lu_peak + f_vec / PEAK_FP64_TFLOPS) / 1e9
+        out["algorithmic"] = {"assembly_ops": a_asm, "t_roof_ms": t_alg, "frac": t_alg / ms_per_solve}
+    return out
 
 
 def parse():
