@@ -284,8 +284,10 @@ struct hamsx_ctx {
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins of the system's rows
   PairBins bins;
-  // the LU's look-ahead stream and events (HAMSX_LU_LOOKAHEAD=0 factors on the one stream)
-  hamsx::LuAux lu_aux{nullptr, nullptr, nullptr};
+  // the LU's look-ahead stream and events, and the tuning options
+  // (hamsx_set_option); lookahead -1: on unless HAMSX_LU_LOOKAHEAD=0
+  hamsx::LuAux lu_aux{};
+  int opt_lookahead = -1;
   // symmetry system 2 is factored on its own stream (sym_stream, with its own
   // scratch) while system 1 is factored on `stream`; created on the first
   // hamsx_set_symmetry
@@ -607,12 +609,19 @@ int ensure_normal_matrix(hamsx_ctx* c) {
   return 0;
 }
 
-// The LU's look-ahead resources, or null when HAMSX_LU_LOOKAHEAD=0
-// (diagnostic: one stream, the same factors bit for bit).
-const hamsx::LuAux* lu_lookahead(hamsx_ctx* c) {
-  const char* e = std::getenv("HAMSX_LU_LOOKAHEAD");
-  if (e && *e == '0') return nullptr;
-  return c->lu_aux.s2 ? &c->lu_aux : nullptr;
+// The LU's schedule and shape for this context: the look-ahead stream unless
+// switched off (HAMSX_OPT_LU_LOOKAHEAD, or HAMSX_LU_LOOKAHEAD=0 by default:
+// one stream, the same factors bit for bit) or `allow` is false, and the
+// selection-block and outer-block options.
+hamsx::LuAux lu_options(hamsx_ctx* c, bool allow = true) {
+  hamsx::LuAux a = c->lu_aux;
+  bool on = c->opt_lookahead != 0;
+  if (c->opt_lookahead < 0) {
+    const char* e = std::getenv("HAMSX_LU_LOOKAHEAD");
+    on = !(e && *e == '0');
+  }
+  if (!on || !allow) a.s2 = nullptr;
+  return a;
 }
 
 
@@ -665,7 +674,8 @@ int do_factor(hamsx_ctx* c) {
     int* widx2 = two ? c->widx2.p : c->widx.p;
     int* wcnt2 = two ? c->wcnt2.p : c->wcnt.p;
     double2* diag2 = two ? c->ludiag2.p : c->ludiag.p;
-    const hamsx::LuAux* aux1 = two ? nullptr : lu_lookahead(c);
+    const hamsx::LuAux o1 = lu_options(c, !two);
+    const hamsx::LuAux* aux1 = &o1;
     if (c->prec == HAMSX_PREC_FP64) {
       hamsx::launch::lu_factor(s2, system_matrix2(c), n, c->ipiv2.p, c->perm2.p, widx2, wcnt2, c->flags.p + 5,
                                diag2, aux1);
@@ -689,14 +699,15 @@ int do_factor(hamsx_ctx* c) {
   }
   if ((rc = ensure_normal_matrix(c))) return rc;
   double2* M = system_matrix(c);
+  const hamsx::LuAux o = lu_options(c);
   if (c->prec == HAMSX_PREC_FP64) {
     hamsx::launch::lu_factor(c->stream, M, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->flags.p + 1,
-                             c->ludiag.p, lu_lookahead(c));
+                             c->ludiag.p, &o);
     if (c->nlid == 0) c->have_A = false;  // factored in place (with a lid, C is factored and A kept)
   } else {
     hamsx::launch::to_c64(c->stream, M, c->Af.p, (size_t)n * n);
     hamsx::launch::lu_factor_c64(c->stream, c->Af.p, n, c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p,
-                                 c->flags.p + 1, c->ludiag.p, lu_lookahead(c));
+                                 c->flags.p + 1, c->ludiag.p, &o);
   }
   c->have_LU = true;
   return check_launch();
@@ -1370,6 +1381,37 @@ int hamsx_destroy(hamsx_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
+}
+
+int hamsx_set_option(hamsx_ctx* c, int option, int value) {
+  if (!c) return fail(HAMSX_EARG, "ctx is NULL");
+  if (c->pending) return busy(c);
+  switch (option) {
+    case HAMSX_OPT_LU_LOOKAHEAD:
+      if (value < -1 || value > 1) return fail(HAMSX_EARG, "HAMSX_OPT_LU_LOOKAHEAD is -1, 0 or 1");
+      c->opt_lookahead = value;
+      return 0;
+    case HAMSX_OPT_LU_SELECT_ROWS:
+      if (value != 256 && value != 512 && value != 1024)
+        return fail(HAMSX_EARG, "HAMSX_OPT_LU_SELECT_ROWS is 256, 512 or 1024");
+      c->lu_aux.sel_rows = value;
+      return 0;
+    case HAMSX_OPT_LU_OUTER_MIN_N:
+      c->lu_aux.outer_min_n = value;
+      return 0;
+    default:
+      return fail(HAMSX_EARG, "unknown option %d", option);
+  }
+}
+
+int hamsx_get_option(hamsx_ctx* c, int option, int* value) {
+  if (!c || !value) return fail(HAMSX_EARG, "ctx or value is NULL");
+  switch (option) {
+    case HAMSX_OPT_LU_LOOKAHEAD: *value = c->opt_lookahead; return 0;
+    case HAMSX_OPT_LU_SELECT_ROWS: *value = c->lu_aux.sel_rows; return 0;
+    case HAMSX_OPT_LU_OUTER_MIN_N: *value = c->lu_aux.outer_min_n; return 0;
+    default: return fail(HAMSX_EARG, "unknown option %d", option);
+  }
 }
 
 int hamsx_set_precision(hamsx_ctx* c, int mode) {

@@ -111,9 +111,15 @@ struct HmDev {
 
 // Look-ahead resources of the LU: a second stream (default priority) and two
 // events; null or s2 == null factors on the one stream.
+// LU schedule and shape options.  s2 (with its two events) set: look-ahead
+// on a second stream.  sel_rows: rows per pivot-selection block (256, 512 or
+// 1024).  outer_min_n: outer blocks of four panels from this order on (<= 0:
+// never).
 struct LuAux {
-  hipStream_t s2;
-  hipEvent_t panel, update;
+  hipStream_t s2 = nullptr;
+  hipEvent_t panel = nullptr, update = nullptr;
+  int sel_rows = kSelRows;
+  int outer_min_n = 6144;
 };
 
 namespace launch {

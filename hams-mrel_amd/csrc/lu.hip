@@ -63,10 +63,13 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 }
 
 // ---------------------------------------------------------------------------
-// Tournament pivot selection.  One thread per candidate row, kSelRows per
-// block.  Level 0 (idx_in == nullptr): block b takes rows
-// [row0 + b*kSelRows, ...).  Level > 0: block b merges candidate sets
-// [b*kMerge, b*kMerge + kMerge) of the previous level.  Each block runs
+// Tournament pivot selection.  One thread per candidate row, ROWS (256, 512
+// or 1024) per block.  Level 0 (idx_in == nullptr): block b takes rows
+// [row0 + b*ROWS, ...).  Level > 0: block b merges candidate sets
+// [b*merge, b*merge + merge) of the previous level, merge = ROWS / NB.
+// Larger blocks need fewer levels (at N = 3,968: three levels of 256-row
+// blocks, two of 512) at a higher cost per step (more waves meet at each
+// barrier); the choice is a tuning option (LuAux::sel_rows).  Each block runs
 // partial pivoting on its candidates' panel entries (pivot = largest
 // |re| + |im| as IZAMAX, ties to the first candidate).  Outputs the chosen
 // rows in pivot order; the final level (FINAL, diag set) also writes the
@@ -84,7 +87,6 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // alternate between steps, so a step's writes never meet the previous
 // step's reads.
 // ---------------------------------------------------------------------------
-constexpr int kMerge = kSelRows / kPanelNB;
 constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one far trailing update
 // Outer blocks from this order on.  Measured (FP64, look-ahead): N = 3,968
 // factors in 30.3 ms per panel and 34.0 ms in outer blocks (the far U chain
@@ -92,15 +94,15 @@ constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one f
 // N = 10,026 in 137 and 133 ms, and four solves in flight then run 4.4 and
 // 5.1 solves/s (the far trailing matrix is read and written once per block).
 constexpr int kOuterMinN = 6144;
-constexpr int kSelWaves = kSelRows / 64;
 static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
-template <typename R, bool FINAL>
-__global__ __launch_bounds__(kSelRows) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
+template <typename R, bool FINAL, int ROWS>
+__global__ __launch_bounds__(ROWS) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
                                                 int ncols, int row0, const int* __restrict__ idx_in,
                                                 const int* __restrict__ cnt_in, int nsets_in, int* __restrict__ idx_out,
                                                 int* __restrict__ cnt_out, typename CT<R>::c2* __restrict__ diag,
                                                 int* __restrict__ info) {
-  __shared__ int sidx[kSelRows];
+  constexpr int kMerge = ROWS / kPanelNB, kSelWaves = ROWS / 64;
+  __shared__ int sidx[ROWS];
   __shared__ int off[kMerge + 1];
   __shared__ double wmax[2][kSelWaves];
   __shared__ int wrow[2][kSelWaves];
@@ -109,8 +111,8 @@ __global__ __launch_bounds__(kSelRows) void k_select(const typename CT<R>::c2* _
   const int t = threadIdx.x, b = blockIdx.x;
   int mc;
   if (idx_in == nullptr) {
-    const int r0 = row0 + b * kSelRows;
-    mc = min(kSelRows, n - r0);
+    const int r0 = row0 + b * ROWS;
+    mc = min(ROWS, n - r0);
     if (t < mc) sidx[t] = r0 + t;
   } else {
     if (t == 0) {
@@ -495,35 +497,54 @@ __global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info)
   if (i == 0) info[0] = 0;
 }
 
+// One tournament level: `blocks` selection blocks of `rows` threads.
+template <typename R>
+static void select_level(hipStream_t s, int rows, bool final, int blocks, const typename CT<R>::c2* A, int n,
+                         int col0, int nb, const int* idx_in, const int* cnt_in, int nsets_in, int* idx_out,
+                         int* cnt_out, typename CT<R>::c2* diag, int* info) {
+  using c2 = typename CT<R>::c2;
+  c2* d = final ? diag : nullptr;
+#define HX_SEL(ROWS)                                                                                              \
+  if (final)                                                                                                     \
+    HX_LAUNCH((k_select<R, true, ROWS>), dim3(blocks), dim3(ROWS), 0, s, A, n, n, col0, nb, col0, idx_in, cnt_in, \
+              nsets_in, idx_out, cnt_out, d, info);                                                              \
+  else                                                                                                           \
+    HX_LAUNCH((k_select<R, false, ROWS>), dim3(blocks), dim3(ROWS), 0, s, A, n, n, col0, nb, col0, idx_in,        \
+              cnt_in, nsets_in, idx_out, cnt_out, d, info);
+  if (rows == 1024) {
+    HX_SEL(1024)
+  } else if (rows == 512) {
+    HX_SEL(512)
+  } else {
+    HX_SEL(kSelRows)
+  }
+#undef HX_SEL
+}
+
 template <typename R>
 static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
                         int* info, typename CT<R>::c2* diag, const LuAux* aux) {
   HX_LAUNCH(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
   const bool la = aux && aux->s2;
+  const int rows = aux ? aux->sel_rows : kSelRows;
+  const int outer_min = aux ? aux->outer_min_n : kOuterMinN;
   bool pending = false;  // a deferred trailing update is running on aux->s2
-  // workspace: two ping-pong candidate buffers of nsets0 sets
-  const int nsets0 = (n + kSelRows - 1) / kSelRows;
+  // workspace: two ping-pong candidate buffers of nsets0 sets (sized for
+  // kSelRows-row blocks, the smallest)
+  const int nsets0 = (n + rows - 1) / rows, merge = rows / kPanelNB;
   int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
   int* cnt[2] = {wcnt, wcnt + nsets0};
   using c2 = typename CT<R>::c2;
   for (int col0 = 0; col0 < n; col0 += kPanelNB) {
     const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
     const int m = n - col0;
-    int nsets = (m + kSelRows - 1) / kSelRows, cur = 0;
-    if (nsets == 1)
-      HX_LAUNCH((k_select<R, true>), dim3(nsets), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0,
-                (const int*)nullptr, (const int*)nullptr, 0, idx[cur], cnt[cur], diag, info);
-    else
-      HX_LAUNCH((k_select<R, false>), dim3(nsets), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0,
-                (const int*)nullptr, (const int*)nullptr, 0, idx[cur], cnt[cur], (c2*)nullptr, info);
+    int nsets = (m + rows - 1) / rows, cur = 0;
+    select_level<R>(s, rows, nsets == 1, nsets, A, n, col0, nb, (const int*)nullptr, (const int*)nullptr, 0,
+                    idx[cur], cnt[cur], diag, info);
     while (nsets > 1) {
-      const int nout = (nsets + kMerge - 1) / kMerge;
-      if (nout == 1)
-        HX_LAUNCH((k_select<R, true>), dim3(nout), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0, idx[cur],
-                  cnt[cur], nsets, idx[1 - cur], cnt[1 - cur], diag, info);
-      else
-        HX_LAUNCH((k_select<R, false>), dim3(nout), dim3(kSelRows), 0, s, A, n, n, col0, nb, col0, idx[cur],
-                  cnt[cur], nsets, idx[1 - cur], cnt[1 - cur], (c2*)nullptr, info);
+      const int nout = (nsets + merge - 1) / merge;
+      select_level<R>(s, rows, nout == 1, nout, A, n, col0, nb, idx[cur], cnt[cur], nsets, idx[1 - cur],
+                      cnt[1 - cur], diag, info);
       cur = 1 - cur;
       nsets = nout;
     }
@@ -538,7 +559,7 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     // so the far trailing matrix is read and written once per block instead
     // of once per panel.
     const int rest = n - col0 - nb;
-    if (n < kOuterMinN) {  // one panel at a time: U12 and the update over every trailing column
+    if (outer_min <= 0 || n < outer_min) {  // one panel at a time: U12 and the update over every trailing column
       const int nrb = (rest + 255) / 256;
       HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
                 (const c2*)diag, kPanelNB, 1, col0 + nb, n);

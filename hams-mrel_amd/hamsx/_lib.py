@@ -28,6 +28,11 @@ FIELD_POTENTIAL = 0
 FIELD_PRESSURE = 1
 FIELD_ELEVATION = 2
 FIELD_AUTO = 3
+OPT_LU_LOOKAHEAD = 1
+OPT_LU_SELECT_ROWS = 2
+OPT_LU_OUTER_MIN_N = 3
+OPTIONS = {"lu_lookahead": OPT_LU_LOOKAHEAD, "lu_select_rows": OPT_LU_SELECT_ROWS,
+           "lu_outer_min_n": OPT_LU_OUTER_MIN_N}
 
 #: every symbol include/hamsx.h declares, with (restype, argtypes)
 _dp = C.POINTER(C.c_double)
@@ -40,6 +45,8 @@ SIGNATURES = {
     "hamsx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
     "hamsx_destroy": (C.c_int, [_vp]),
     "hamsx_set_precision": (C.c_int, [_vp, C.c_int]),
+    "hamsx_set_option": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "hamsx_get_option": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "hamsx_panel_geometry": (C.c_int, [C.c_int, _dp, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp]),
     "hamsx_wave_properties": (C.c_int, [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
     "hamsx_set_mesh": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int, _ip]),

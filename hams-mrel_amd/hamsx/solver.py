@@ -77,6 +77,20 @@ class FrequencySolver:
         _lib.check(self._lib.hamsx_set_precision(self._h, mode), "hamsx_set_precision")
         self.precision = precision
 
+    def set_options(self, **opts):
+        """LU tuning options by name (lu_lookahead, lu_select_rows,
+        lu_outer_min_n), taking effect at the next factorisation."""
+        for name, value in opts.items():
+            _lib.check(self._lib.hamsx_set_option(self._h, _lib.OPTIONS[name], int(value)), "hamsx_set_option")
+
+    def options(self) -> dict:
+        out = {}
+        for name, code in _lib.OPTIONS.items():
+            v = C.c_int()
+            _lib.check(self._lib.hamsx_get_option(self._h, code, C.byref(v)), "hamsx_get_option")
+            out[name] = v.value
+        return out
+
     def set_lid(self, lid_cen):
         """Irregular-frequency removal: lid_cen (L x 3) are the lid (waterplane)
         panel centroids; None or an empty array removes the lid."""

@@ -48,6 +48,25 @@ int hamsx_create(hamsx_ctx **ctx, int device);
 int hamsx_destroy(hamsx_ctx *ctx);
 int hamsx_set_precision(hamsx_ctx *ctx, int mode);
 
+/* Tuning options of a context, taking effect at its next factorisation (no
+ * counterpart in the reference, whose ZGETRF has no such choices; picked per
+ * problem size by hamsx/autotune.py):
+ *   HAMSX_OPT_LU_LOOKAHEAD   -1 (default: on unless the environment sets
+ *                            HAMSX_LU_LOOKAHEAD=0), 0 or 1 -- the trailing
+ *                            update beyond the next panel on a second stream;
+ *                            the factors are the same bit for bit either way;
+ *   HAMSX_OPT_LU_SELECT_ROWS 256 (default), 512 or 1024 -- rows per
+ *                            pivot-selection block (fewer tournament levels
+ *                            with larger blocks; the pivots may differ);
+ *   HAMSX_OPT_LU_OUTER_MIN_N 6144 (default) -- outer blocks of four panels
+ *                            from this order on, <= 0 never.
+ * HAMSX_EARG for an unknown option or value. */
+#define HAMSX_OPT_LU_LOOKAHEAD 1
+#define HAMSX_OPT_LU_SELECT_ROWS 2
+#define HAMSX_OPT_LU_OUTER_MIN_N 3
+int hamsx_set_option(hamsx_ctx *ctx, int option, int value);
+int hamsx_get_option(hamsx_ctx *ctx, int option, int *value);
+
 /* ---- host-side set-up routines (no device work) -------------------------
  * Panel centroid, area (two Heron triangles per quad), maximum edge length
  * and the six generalized normals about reference point xr[3].

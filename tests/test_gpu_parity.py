@@ -164,6 +164,44 @@ def test_lu_outer_blocks_backward_error(precision):
     assert err < (1e-14 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("rows,outer_min", [(512, 6144), (1024, 6144), (256, 1024), (512, 1024)])
+def test_lu_tuning_options(rows, outer_min):
+    """The tuning options (selection blocks of 256/512/1024 rows, outer blocks
+    from a given order on) give valid factorisations -- backward error on
+    random probes as in test_lu_outer_blocks_backward_error -- and, for each
+    option set, the look-ahead and one-stream schedules agree bit for bit.
+    N = 2,240: blocks per tournament level 9, 2, 1 with 256-row blocks; 5, 1
+    with 512; 3, 1 with 1,024."""
+    m = small_pair(na=40, nz=24, nr=4)
+    say(f"[gpu] LU options n={m.nelem} rows={rows} outer_min={outer_min}")
+    out = []
+    for la in (0, 1):
+        with FrequencySolver(m) as s:
+            s.set_options(lu_select_rows=rows, lu_outer_min_n=outer_min, lu_lookahead=la)
+            assert s.options() == {"lu_lookahead": la, "lu_select_rows": rows, "lu_outer_min_n": outer_min}
+            s.set_waves([0.0])
+            s.calwaveproperts(1.0, 35.0)
+            s.assemble()
+            A = s.get_matrix("A")
+            assert s.factor() == 0
+            out.append(s.get_matrix("LU"))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    LU, ipiv = out[1]
+    n = m.nelem
+    perm = np.arange(n)
+    for k in range(n):
+        p = ipiv[k] - 1
+        perm[[k, p]] = perm[[p, k]]
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4))
+    ux = np.triu(LU) @ x
+    lux = np.tril(LU, -1) @ ux + ux
+    err = np.linalg.norm(A[perm] @ x - lux) / (np.linalg.norm(A) * np.linalg.norm(x))
+    say(f"    probe backward error {err:.3e}")
+    assert err < 1e-14
+
+
 @pytest.mark.parametrize("precision,outer", [("fp64", False), ("mixed", False), ("fp64", True), ("mixed", True)])
 def test_lu_lookahead_gives_the_same_factors(precision, outer, monkeypatch):
     """The look-ahead schedule (the next panel's columns first, the rest of
