@@ -183,6 +183,8 @@ def parse():
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default fp64 (serving: mixed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", action="store_true", help="replay each context's recorded HIP graph of the solve "
+                    "instead of launching it eagerly")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision and five-cylinder runs that "
                     "the default FP64 line carries beside its value")
     ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
@@ -290,17 +292,30 @@ def serving(args):
     precision = args.precision if args.precision_set else "mixed"
     print(f"[bench] serving {args.requests} requests, {min(sizes)}-{max(sizes)} panels, rate {args.rate}/s, "
           f"{args.streams} slots, {precision}", file=sys.stderr, flush=True)
-    with Server(slots=args.streams, precision=precision) as srv:
-        warm = Request(-1, reqs[0].mesh, 1.0, reqs[0].depth, reqs[0].headings_rad)
-        srv.run([warm])
-        t0 = time.perf_counter()
-        res = srv.run(reqs)
-        wall = time.perf_counter() - t0
-    bad = [r.rid for r in res if r.info != 0]
-    if bad:
-        raise SystemExit(f"zero pivots in requests {bad}")
-    lat = np.array([r.latency for r in res]) * 1e3
-    span = max(r.done for r in res) - min(r.arrival for r in res)
+    def run(stream, tuning=None):
+        with Server(slots=args.streams, precision=precision, tuning=tuning) as srv:
+            warm = Request(-1, reqs[0].mesh, 1.0, reqs[0].depth, reqs[0].headings_rad)
+            srv.run([warm])
+            t0 = time.perf_counter()
+            res = srv.run(stream)
+            wall = time.perf_counter() - t0
+        bad = [r.rid for r in res if r.info != 0]
+        if bad:
+            raise SystemExit(f"zero pivots in requests {bad}")
+        lat = np.array([r.latency for r in res]) * 1e3
+        span = max(r.done for r in res) - min(r.arrival for r in res)
+        return res, lat, span, wall
+
+    res, lat, span, wall = run(reqs)
+    # capacity: the same requests all queued at once (the server never idles),
+    # with the autotuner's per-size LU options and with the library defaults
+    cap = {}
+    for tag, tuning in (("tuned", None), ("defaults", {})):
+        burst = [Request(r.rid, r.mesh, r.w1, r.depth, r.headings_rad, arrival=0.0) for r in reqs]
+        cres, clat, cspan, _ = run(burst, tuning)
+        cap[tag] = {"value": len(cres) / cspan, "unit": "freq-solves/s", "p50_ms": float(np.percentile(clat, 50)),
+                    "p99_ms": float(np.percentile(clat, 99))}
+        print(f"[bench] saturated, {tag} options: {cap[tag]['value']:.2f} solves/s", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": len(res) / span, "unit": "freq-solves/s", "n_gpus": 1, "steps": len(res),
         "warmup": 1, "ms_per_step": span / len(res) * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -310,6 +325,7 @@ def serving(args):
                    "panels_min": min(sizes), "panels_max": max(sizes), "panels_mean": float(np.mean(sizes)),
                    "slots": args.streams, "precision": precision, "wall_s": wall},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "capacity": cap,
     }
     print(json.dumps(out), flush=True)
 
@@ -504,7 +520,8 @@ def main():
     print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(case.omegas)} frequencies, "
           f"{len(case.headings_deg)} headings, precision {args.precision}, {max(1, args.streams)} solves in flight",
           file=sys.stderr, flush=True)
-    run = throughput(case, mesh, args.precision, args.streams, args.steps, args.warmup, args.stage_steps)
+    run = throughput(case, mesh, args.precision, args.streams, args.steps, args.warmup, args.stage_steps,
+                     graphs=args.graphs)
     ms, elapsed, lat_ms = run["stages"], run["elapsed"], run["lat_ms"]
     print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
     roof = roofline(case, mesh, ms, args.precision)
@@ -525,7 +542,7 @@ def main():
         "config": {"workload": case.name, "panels": n, "symmetry": case.meta.get("symmetry"),
                    "depth_m": case.depth, "frequencies": len(case.omegas),
                    "headings_deg": case.headings_deg, "precision": args.precision,
-                   "in_flight": max(1, args.streams)},
+                   "in_flight": max(1, args.streams), "graphs": args.graphs},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
         "roofline": roof,
@@ -543,7 +560,7 @@ def main():
             m5 = combine(c5.bodies, c5.xr)
             steps5 = max(4, args.steps // 2)
             print(f"[bench] {c5.name}: N={m5.nelem}, fp64 and mixed, {steps5} solves each", file=sys.stderr, flush=True)
-            r5 = throughput(c5, m5, "fp64", args.streams, steps5, args.warmup, 1)
+            r5 = throughput(c5, m5, "fp64", args.streams, steps5, args.warmup, 1, graphs=args.graphs)
             out["five_cylinders"] = {"workload": c5.name, "panels": m5.nelem, "depth_m": c5.depth,
                                      "fp64": summary(r5, steps5),
                                      "mixed": mixed_vs_fp64(c5, m5, r5, args.streams, steps5, args.warmup)}
@@ -554,7 +571,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def throughput(case, mesh, precision, streams, steps, warmup, stage_steps):
+def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graphs=False):
     """`steps` frequency solves of case.omegas (in order, cycling) with
     `streams` solves in flight (one context and HIP stream each), after
     per-stage timing of `stage_steps` isolated solves and `warmup` untimed
@@ -567,6 +584,7 @@ def throughput(case, mesh, precision, streams, steps, warmup, stage_steps):
     solvers = []
     for _ in range(nctx):
         sv = FrequencySolver(mesh, precision=precision)
+        sv.set_options(graphs=1 if graphs else 0)
         sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
         if "symmetry" in case.meta:
             sv.set_symmetry(*case.meta["symmetry"])
@@ -580,8 +598,9 @@ def throughput(case, mesh, precision, streams, steps, warmup, stage_steps):
     stages = solvers[0].stage_times(reset=True)
     solvers[0].set_profiling(False)
     for k, sv in enumerate(solvers):
-        for _ in range(warmup):
-            sv.solve_frequency(omegas[k % len(omegas)], case.depth)
+        for _ in range(warmup):  # through the timed path (the first submit at a depth runs eagerly)
+            sv.submit(omegas[k % len(omegas)], case.depth)
+            sv.collect()
     lat, coef = [], [None] * steps
     t_sub = [0.0] * nctx
     owner = [0] * nctx  # solve index each context holds

@@ -35,6 +35,7 @@ __device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int j
 // the per-mode scalars, and thread (0, 0) N_0.
 __global__ __launch_bounds__(256) void k_mode_tables(MeshDev m, FreqDev f) {
 #pragma clang fp contract(off)
+  f = freq_now(f);
   const int panel = blockIdx.x * 256 + threadIdx.x, mode = blockIdx.y, n = m.nf;
   if (panel >= n) return;
   double* t = f.tab;
@@ -139,6 +140,7 @@ __device__ __forceinline__ void pair_entry2(const MeshDev& m, const FreqDev& f, 
 template <bool DEEP>
 __global__ __launch_bounds__(256) void k_assemble(MeshDev m, FreqDev f, double2* __restrict__ A,
                                                   double2* __restrict__ S, int* flags) {
+  f = freq_now(f);
   const int lane = threadIdx.x & 63;
   const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
   const int i = m.row0 + blockIdx.x * 64 + lane;
@@ -212,6 +214,7 @@ template <int REG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REG == kRegEigene || REG == kRegPidcock ? 2 : 1))) void k_assemble_list(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
                                                        int count, double2* __restrict__ A, double2* __restrict__ S,
                                                        int* flags) {
+  f = freq_now(f);
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= count) return;
   const unsigned e = list[t];
@@ -224,6 +227,7 @@ template <int REG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(REG == kRegEigene || REG == kRegPidcock ? 2 : 1))) void k_assemble_pairs(MeshDev m, FreqDev f, const unsigned* __restrict__ list,
                                                         int count, double2* __restrict__ A,
                                                         double2* __restrict__ S) {
+  f = freq_now(f);
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= count) return;
   const unsigned e = list[t];
@@ -293,6 +297,7 @@ __device__ inline cd d_vinp(double x, double y, double z, const FreqDev& f, cons
 
 // dum2(j, beta) = grad(phi_I) . n at panel j (DBC_MULTI_RIGHT, BodyIntgrMulti.f90:150-163)
 __global__ void k_dum2(MeshDev m, FreqDev f, WaveDev w, double2* dum2) {
+  f = freq_now(f);
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int b = blockIdx.y;
   if (j >= m.n) return;
@@ -378,6 +383,7 @@ __global__ void k_rhs_reduce(int n, int nbody, int nbeta, int nchunk, const int*
 __global__ __launch_bounds__(256) void k_forces(MeshDev m, FreqDev f, WaveDev w, int nbody, const int* body_start,
                                                 const double2* __restrict__ X, const double2* __restrict__ Xm,
                                                 int mirror, double* __restrict__ out) {
+  f = freq_now(f);
   __shared__ double red[12][256];
   const int bm = blockIdx.x, r = blockIdx.y, t = threadIdx.x;
   const int n = m.n, nd = 6 * nbody;
@@ -516,6 +522,7 @@ __global__ void k_field_finish(int nfp, const double* __restrict__ px, const dou
                                const double* __restrict__ pz, const int* __restrict__ onb, FreqDev f, WaveDev w,
                                int nd, int kind, const double2* __restrict__ P, size_t ld, double2* __restrict__ out,
                                double2* __restrict__ inc) {
+  f = freq_now(f);
   const int p = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
   if (p >= nfp) return;
   const bool limit = fabs(f.v) < HX_F(1.0e-8) || fabs(f.v + 1.0) < HX_F(1.0e-8);
@@ -699,6 +706,7 @@ __global__ void k_sym_combine(double2* __restrict__ A, double2* __restrict__ T2,
 // mirrored normal component negated (DBC_MULTI_RIGHT, BodyIntgrMulti.f90:150-170
 // with IS != IP).  mirror 0: x -> -x, 1: y -> -y.
 __global__ void k_dum2_mirror(MeshDev m, FreqDev f, WaveDev w, int mirror, double2* dum2) {
+  f = freq_now(f);
   const int j = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
   if (j >= m.n) return;
   const double sx = mirror == 0 ? -1.0 : 1.0, sy = mirror == 1 ? -1.0 : 1.0;

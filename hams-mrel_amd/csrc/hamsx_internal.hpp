@@ -69,7 +69,20 @@ struct FreqDev {
   double v, wk, w1, h;
   const double* wvn;  // kNK roots on the device
   double* tab;        // finite depth: per-frequency mode tables (layout below)
+  // device copy of (v, wk, w1, h) written with the roots: every kernel takes
+  // the scalars from here (freq_now), so a launch sequence recorded once
+  // serves any frequency of the same depth
+  const double* sc = nullptr;
 };
+__host__ __device__ inline FreqDev freq_now(FreqDev f) {
+  if (f.sc) {
+    f.v = f.sc[0];
+    f.wk = f.sc[1];
+    f.w1 = f.sc[2];
+    f.h = f.sc[3];
+  }
+  return f;
+}
 
 // Per-frequency mode tables of the finite-depth series, kModes modes:
 // [pm | ac | ut] (kModes each), nm0 (1), padding to kTabHead, then

@@ -31,8 +31,9 @@ FIELD_AUTO = 3
 OPT_LU_LOOKAHEAD = 1
 OPT_LU_SELECT_ROWS = 2
 OPT_LU_OUTER_MIN_N = 3
+OPT_GRAPHS = 4
 OPTIONS = {"lu_lookahead": OPT_LU_LOOKAHEAD, "lu_select_rows": OPT_LU_SELECT_ROWS,
-           "lu_outer_min_n": OPT_LU_OUTER_MIN_N}
+           "lu_outer_min_n": OPT_LU_OUTER_MIN_N, "graphs": OPT_GRAPHS}
 
 #: every symbol include/hamsx.h declares, with (restype, argtypes)
 _dp = C.POINTER(C.c_double)
@@ -64,6 +65,11 @@ SIGNATURES = {
     "hamsx_submit_frequency": (C.c_int, [_vp, C.c_double, C.c_double]),
     "hamsx_query": (C.c_int, [_vp]),
     "hamsx_collect": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "hamsx_queue_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "hamsx_queue_destroy": (C.c_int, [_vp]),
+    "hamsx_set_queue": (C.c_int, [_vp, _vp, C.c_int]),
+    "hamsx_queue_wait": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int)]),
+    "hamsx_graph_stats": (C.c_int, [_vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "hamsx_get_matrix": (C.c_int, [_vp, C.c_int, _dp, _ip]),
     "hamsx_set_solver": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
     "hamsx_set_refinement": (C.c_int, [_vp, C.c_int, C.c_double]),

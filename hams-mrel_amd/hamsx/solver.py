@@ -232,6 +232,19 @@ class FrequencySolver:
         """Enqueue one whole frequency solve on this context's stream."""
         _lib.check(self._lib.hamsx_submit_frequency(self._h, w1, h), "hamsx_submit_frequency")
 
+    def set_queue(self, queue, tag: int):
+        """Post `tag` to `queue` (a CompletionQueue, or None to detach) when
+        each submitted solve finishes."""
+        _lib.check(self._lib.hamsx_set_queue(self._h, queue._q if queue is not None else None, int(tag)),
+                   "hamsx_set_queue")
+
+    def graph_stats(self):
+        """(recorded, launched): HIP graphs recorded by this context and solves
+        launched from one."""
+        r, n = C.c_ulonglong(), C.c_ulonglong()
+        _lib.check(self._lib.hamsx_graph_stats(self._h, C.byref(r), C.byref(n)), "hamsx_graph_stats")
+        return r.value, n.value
+
     def query(self) -> bool:
         """True when the submitted solve has finished (or none is pending)."""
         return _lib.check(self._lib.hamsx_query(self._h), "hamsx_query", allow_info=True) == 1
@@ -300,6 +313,36 @@ class FrequencySolver:
         _lib.check(self._lib.hamsx_stage_times(self._h, 1 if reset else 0, _lib.ptr(ms), _lib.ptr(cnt, C.c_int)),
                    "hamsx_stage_times")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(_lib.STAGES)}
+
+
+class CompletionQueue:
+    """Completion notices of submitted solves (hamsx_queue_*): contexts
+    attached with FrequencySolver.set_queue post their tag when a solve has
+    finished; wait() blocks for the next tag without polling."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        q = C.c_void_p()
+        _lib.check(self._lib.hamsx_queue_create(C.byref(q)), "hamsx_queue_create")
+        self._q = q
+
+    def wait(self, timeout_s: float | None = None):
+        """The next posted tag, or None when timeout_s passes first."""
+        tag = C.c_int()
+        got = _lib.check(self._lib.hamsx_queue_wait(self._q, -1.0 if timeout_s is None else max(0.0, timeout_s),
+                                                    C.byref(tag)), "hamsx_queue_wait", allow_info=True)
+        return tag.value if got == 1 else None
+
+    def close(self):
+        if getattr(self, "_q", None):
+            self._lib.hamsx_queue_destroy(self._q)
+            self._q = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def hmatrix_partition(cen, leaf: int = 32, eta: float = 2.0):

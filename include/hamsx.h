@@ -64,6 +64,15 @@ int hamsx_set_precision(hamsx_ctx *ctx, int mode);
 #define HAMSX_OPT_LU_LOOKAHEAD 1
 #define HAMSX_OPT_LU_SELECT_ROWS 2
 #define HAMSX_OPT_LU_OUTER_MIN_N 3
+/*   HAMSX_OPT_GRAPHS         0 (default) or 1 -- hamsx_submit_frequency
+ *                            records its launch sequence as a HIP graph on
+ *                            the second solve at a depth and replays it for
+ *                            every later frequency at that depth (dense LU
+ *                            path, no lid, no symmetry plane, no stage
+ *                            timing; otherwise, and with 0, it launches
+ *                            eagerly).  Recordings use the one-stream LU
+ *                            schedule; results are the same bit for bit. */
+#define HAMSX_OPT_GRAPHS 4
 int hamsx_set_option(hamsx_ctx *ctx, int option, int value);
 int hamsx_get_option(hamsx_ctx *ctx, int option, int *value);
 
@@ -205,6 +214,23 @@ int hamsx_solve_frequency(hamsx_ctx *ctx, double w1, double h, double *amas, dou
 int hamsx_submit_frequency(hamsx_ctx *ctx, double w1, double h);
 int hamsx_query(hamsx_ctx *ctx);
 int hamsx_collect(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
+
+/* Completion notices for hosts that keep several contexts in flight (no
+ * counterpart in the reference): a context attached to a queue with a tag
+ * (hamsx_set_queue; q NULL detaches) has a host function appended to each
+ * submitted solve, which posts the tag when the solve's work has finished.
+ * hamsx_queue_wait blocks until a tag is posted (timeout_s < 0: no limit)
+ * and returns 1 with the tag, or 0 when the time runs out; the solve is then
+ * collected with hamsx_collect as usual.  Destroy a queue only after every
+ * context attached to it has been collected or destroyed. */
+typedef struct hamsx_queue hamsx_queue;
+int hamsx_queue_create(hamsx_queue **q);
+int hamsx_queue_destroy(hamsx_queue *q);
+int hamsx_set_queue(hamsx_ctx *ctx, hamsx_queue *q, int tag);
+int hamsx_queue_wait(hamsx_queue *q, double timeout_s, int *tag);
+/* Recorded launch sequences of this context (HAMSX_OPT_GRAPHS): how many
+ * were recorded and how many solves were launched from one. */
+int hamsx_graph_stats(hamsx_ctx *ctx, unsigned long long *records, unsigned long long *launches);
 
 /* Parity/debug: copy A (which = 0), S (1) or the LU factors (2) to host
  * out[nelem*nelem] (complex); ipiv[nelem] (1-based) for which = 2, else NULL. */
