@@ -298,6 +298,7 @@ struct hamsx_ctx {
   bool have_waves = false;
   // matrices and workspace
   DevBuf<double2> A, S, X, Xtmp, B, R, D, dum2, partial, ludiag, mvw;
+  DevBuf<double2> trinv, trinv2;  // inverted diagonal blocks of the triangular solves (system 2: trinv2)
   DevBuf<float2> Af, Af2;  // complex64 factors (Af2: symmetry system 2)
   DevBuf<int> ipiv, perm, widx, wcnt, flags;
   // finite-depth pair bins of the system's rows
@@ -571,6 +572,7 @@ int ensure_work(hamsx_ctx* c) {
   const size_t n = c->n, nr = std::max(1, c->nrhs());
   int rc;
   if ((rc = c->X.ensure(n * nr)) || (rc = c->Xtmp.ensure(n * nr)) ||
+      (rc = c->trinv.ensure(hamsx::launch::tri_inverse_len((int)n))) ||
       (rc = c->dum2.ensure(n * std::max(1, c->nbeta))) ||
       (rc = c->partial.ensure((size_t)c->nchunk * (6 + c->nbeta) * std::max<size_t>(n, c->nf))) ||
       (rc = c->out.ensure(c->out_len() + 8)))
@@ -1200,11 +1202,11 @@ int refine_solve(hamsx_ctx* c, const double2* M, const float2* Af, const int* pe
   const size_t cnt = (size_t)n * nr;
   hamsx::launch::refine_reset(c->stream, state);
   HX_TRY(hipMemcpyAsync(c->B.p, X, sizeof(double2) * cnt, hipMemcpyDeviceToDevice, c->stream));
-  hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, X, c->Xtmp.p, nr);
+  hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, X, c->Xtmp.p, c->trinv.p, nr);
   for (int it = 0; it < c->refine_iters; it++) {
     hamsx::launch::residual(c->stream, M, n, X, c->B.p, c->D.p, nr, norms, c->mvw.p, state);
     hamsx::launch::refine_check(c->stream, norms, nr, c->refine_tol, state);
-    hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, c->D.p, c->Xtmp.p, nr, state);
+    hamsx::launch::lu_solve_c64(c->stream, Af, n, perm, c->D.p, c->Xtmp.p, c->trinv.p, nr, state);
     hamsx::launch::axpy_update(c->stream, X, c->D.p, cnt, state);
   }
   // residual of the final iterate (a no-op when the loop converged: the
@@ -1241,12 +1243,14 @@ int do_solve(hamsx_ctx* c) {
       if ((rc = c->Xtmp2.ensure((size_t)n * nr))) return rc;
       HX_TRY(hipEventRecord(c->sym_go, c->stream));
       HX_TRY(hipStreamWaitEvent(c->sym_stream, c->sym_go, 0));
-      hamsx::launch::lu_solve(c->sym_stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp2.p, nr);
+      if ((rc = c->trinv2.ensure(hamsx::launch::tri_inverse_len(n)))) return rc;
+      hamsx::launch::lu_solve(c->sym_stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp2.p, c->trinv2.p, nr);
       HX_TRY(hipEventRecord(c->sym_done, c->sym_stream));
     }
-    hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, nr);
+    hamsx::launch::lu_solve(c->stream, system_matrix(c), n, c->perm.p, c->X.p, c->Xtmp.p, c->trinv.p, nr);
     if (two) HX_TRY(hipStreamWaitEvent(c->stream, c->sym_done, 0));
-    else if (c->sym()) hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, nr);
+    else if (c->sym())
+      hamsx::launch::lu_solve(c->stream, system_matrix2(c), n, c->perm2.p, c->X2.p, c->Xtmp.p, c->trinv.p, nr);
   } else {
     if ((rc = refine_solve(c, system_matrix(c), c->Af.p, c->perm.p, c->X.p, c->flags.p + 2, c->norms.p))) return rc;
     if (c->sym() &&

@@ -161,8 +161,11 @@ void rhs(hipStream_t s, const MeshDev& m, const FreqDev& f, const WaveDev& w, in
 // overlapping the next pivot selection; the factors are the same bit for bit.
 void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
                double2* diag, const LuAux* aux = nullptr);
-// X <- A^{-1} X using the factors, nrhs columns (leading dimension n)
-void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs);
+// X <- A^{-1} X using the factors, nrhs columns (leading dimension n);
+// inv: tri_inverse_len(n) complex128 of workspace (the inverted diagonal blocks)
+size_t tri_inverse_len(int n);
+void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
+              int nrhs);
 // AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
 // Xm / mirror: the mirror half's potentials and the symmetry plane
 // (0: x = 0, 1: y = 0); mirror = -1 without symmetry.
@@ -206,8 +209,8 @@ void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* w
                    double2* diag, const LuAux* aux = nullptr);
 // skip (may be null, here and below): a device flag that turns every
 // launch into a no-op when set (refinement after convergence)
-void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
-                  const int* skip = nullptr);
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
+                  int nrhs, const int* skip = nullptr);
 // Y = beta B + alpha A X (B may be null), A read once; P: matvec_workspace(n, nrhs) complex128
 size_t matvec_workspace(int n, int nrhs);
 void matvec(hipStream_t s, const double2* A, int n, const double2* X, int nrhs, const double2* B, double alpha,

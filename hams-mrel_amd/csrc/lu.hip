@@ -750,12 +750,193 @@ __device__ __forceinline__ void trsm_block_body(const typename CT<R>::c2* __rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Triangular solves of one factorisation, 256 rows per launch.  The 32 x 32
+// diagonal blocks of L (unit) and U are inverted once per solve
+// (k_tri_inverse, by the substitution above applied to the identity); a
+// launch then takes a kSuper-row block of rows: its right-hand sides sit in
+// LDS, each 32-row diagonal block is solved as a small product with its
+// inverse and eliminated from the block's other rows, and every workgroup
+// finally updates its 256 rows outside the block with the block's solution.
+// Each workgroup solves the block for itself (nothing but the launch order
+// synchronises workgroups), so the chain is n / 256 launches per triangle
+// instead of n / 32.
+// ---------------------------------------------------------------------------
+constexpr int kSuper = 256;
+static_assert(kSuper / kPanelNB * kPanelNB == kSuper && 256 / kPanelNB == kTrsmRhs,
+              "thread (r, q) of a 256-thread block covers one row and one right-hand side of a diagonal block");
+
+// inv[(2 b + u) * 1024 + c * 32 + r] = (T_bb^-1)(r, c): u = 0 the unit lower
+// factor, u = 1 the upper factor; rows and columns past n are zero.
+template <typename R>
+__global__ __launch_bounds__(256) void k_tri_inverse(const typename CT<R>::c2* __restrict__ LU, int n,
+                                                     double2* __restrict__ inv, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
+  __shared__ double dr[kPanelNB], di[kPanelNB];
+  const int b = blockIdx.x, upper = blockIdx.y, t = threadIdx.x;
+  const int k0 = b * kPanelNB, kb = min(kPanelNB, n - k0);
+  for (int e = t; e < kPanelNB * kPanelNB; e += 256) {
+    const int r = e % kPanelNB, c = e / kPanelNB;
+    double vr = 0.0, vi = 0.0;
+    if (r < kb && c < kb) {
+      auto v = LU[(size_t)(k0 + r) + (size_t)n * (k0 + c)];
+      vr = (double)v.x;
+      vi = (double)v.y;
+    }
+    tr[c][r] = vr;
+    ti[c][r] = vi;
+  }
+  __syncthreads();
+  if (upper && t < kb) {  // reciprocal of the diagonal (Smith)
+    double a = tr[t][t], bb = ti[t][t];
+    if (fabs(a) >= fabs(bb)) {
+      double qq = bb / a, dd = a + bb * qq;
+      dr[t] = 1.0 / dd;
+      di[t] = -qq / dd;
+    } else {
+      double qq = a / bb, dd = a * qq + bb;
+      dr[t] = qq / dd;
+      di[t] = -1.0 / dd;
+    }
+  }
+  __syncthreads();
+  const int lane = t & 63, half = lane >> 5, r = lane & 31;
+  double2* out = inv + (size_t)(2 * b + upper) * kPanelNB * kPanelNB;
+  for (int pass = 0; pass < kPanelNB / 8; pass++) {
+    const int col = pass * 8 + (t >> 6) * 2 + half;  // the identity column this half-wave solves for
+    double vr = (r == col && r < kb) ? 1.0 : 0.0, vi = 0.0;
+    if (!upper) {
+      for (int c = 0; c < kb; c++) {
+        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
+        if (r > c && r < kb) {
+          const double a = tr[c][r], bb = ti[c][r];
+          vr -= a * cr - bb * ci;
+          vi -= a * ci + bb * cr;
+        }
+      }
+    } else {
+      for (int c = kb - 1; c >= 0; c--) {
+        if (r == c) {
+          const double a = vr, bb = vi;
+          vr = a * dr[c] - bb * di[c];
+          vi = a * di[c] + bb * dr[c];
+        }
+        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
+        if (r < c) {
+          const double a = tr[c][r], bb = ti[c][r];
+          vr -= a * cr - bb * ci;
+          vi -= a * ci + bb * cr;
+        }
+      }
+    }
+    out[(size_t)col * kPanelNB + r] = (r < kb && col < kb) ? make_double2(vr, vi) : make_double2(0.0, 0.0);
+  }
+}
+
+// One kSuper-row block [k0, k0 + kb) for the right-hand sides of group
+// blockIdx.y: solves it (LOWER: forward, unit L; else backward, U) and
+// updates this workgroup's 256 rows outside it (below it, or above it).
+// The block's solution goes to Y (workgroup 0 of the group); the updated
+// rows stay in X, which no workgroup reads inside the block.
 template <typename R, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsm_block(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
+__global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __restrict__ LU,
+                                                    const double2* __restrict__ inv, int n, int k0, int kb,
                                                     double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
                                                     const int* __restrict__ skip) {
   if (skip && *skip) return;
-  trsm_block_body<R, LOWER>(LU, n, k0, kb, X, Y, nrhs, blockIdx.x);
+  // [q][row]: a thread's own row is contiguous across lanes, a shared row is
+  // one broadcast address
+  __shared__ double2 xs[kTrsmRhs][kSuper];        // the block's right-hand sides, then its solution
+  __shared__ double2 iv[kPanelNB][kPanelNB + 1];  // inverse of the current diagonal block, [c][r]
+  const int t = threadIdx.x, q0 = blockIdx.y * kTrsmRhs;
+  const int nq = min(kTrsmRhs, nrhs - q0);
+  for (int e = t; e < kSuper * kTrsmRhs; e += 256) {
+    const int r = e % kSuper, q = e / kSuper;
+    xs[q][r] = (r < kb && q < nq) ? X[(size_t)(k0 + r) + (size_t)n * (q0 + q)] : make_double2(0.0, 0.0);
+  }
+  const int nsub = (kb + kPanelNB - 1) / kPanelNB;
+  const int rr = t % kPanelNB, qq = t / kPanelNB;  // (row, right-hand side) of a diagonal block
+  for (int ss = 0; ss < nsub; ss++) {
+    const int sblk = LOWER ? ss : nsub - 1 - ss;
+    const int r0 = sblk * kPanelNB, sb = min(kPanelNB, kb - r0);
+    const double2* src = inv + (size_t)(2 * ((k0 + r0) / kPanelNB) + (LOWER ? 0 : 1)) * kPanelNB * kPanelNB;
+    for (int e = t; e < kPanelNB * kPanelNB; e += 256) iv[e / kPanelNB][e % kPanelNB] = src[e];
+    // this thread's factor row against the diagonal block (rows still to
+    // come), all loads issued before the arithmetic
+    const bool upd = LOWER ? (t >= r0 + sb && t < kb) : (t < r0);
+    typename CT<R>::c2 l[kPanelNB];
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++)
+      l[c] = LU[(size_t)(k0 + (upd ? t : r0)) + (size_t)n * (k0 + r0 + min(c, sb - 1))];
+    __syncthreads();
+    double yr = 0.0, yi = 0.0;
+    for (int c = 0; c < sb; c++) {
+      const double2 a = iv[c][rr], x = xs[qq][r0 + c];
+      yr += a.x * x.x - a.y * x.y;
+      yi += a.x * x.y + a.y * x.x;
+    }
+    __syncthreads();  // every read of the block's old values is done
+    if (rr < sb) {
+      xs[qq][r0 + rr] = make_double2(yr, yi);
+      if (blockIdx.x == 0 && qq < nq) Y[(size_t)(k0 + r0 + rr) + (size_t)n * (q0 + qq)] = make_double2(yr, yi);
+    }
+    __syncthreads();
+    // eliminate the solved rows from the block's rows still to come
+    if (upd) {
+      double ar[kTrsmRhs], ai[kTrsmRhs];
+#pragma unroll
+      for (int q = 0; q < kTrsmRhs; q++) ar[q] = ai[q] = 0.0;
+#pragma unroll
+      for (int c = 0; c < kPanelNB; c++) {
+        if (c < sb) {
+          const double lr = (double)l[c].x, li = (double)l[c].y;
+#pragma unroll
+          for (int q = 0; q < kTrsmRhs; q++) {
+            const double2 y = xs[q][r0 + c];
+            ar[q] += lr * y.x - li * y.y;
+            ai[q] += lr * y.y + li * y.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kTrsmRhs; q++) xs[q][t] = make_double2(xs[q][t].x - ar[q], xs[q][t].y - ai[q]);
+    }
+    __syncthreads();
+  }
+  // this workgroup's rows outside the block, the factor row in chunks of 32
+  // columns with each chunk's loads issued before its arithmetic
+  const int i = LOWER ? k0 + kb + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
+  if (LOWER ? i >= n : i >= k0) return;
+  double ar[kTrsmRhs], ai[kTrsmRhs];
+#pragma unroll
+  for (int q = 0; q < kTrsmRhs; q++) ar[q] = ai[q] = 0.0;
+  for (int c0 = 0; c0 < kb; c0 += kPanelNB) {
+    const int cb = min(kPanelNB, kb - c0);
+    typename CT<R>::c2 l[kPanelNB];
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) l[c] = LU[(size_t)i + (size_t)n * (k0 + c0 + min(c, cb - 1))];
+#pragma unroll
+    for (int c = 0; c < kPanelNB; c++) {
+      if (c < cb) {
+        const double lr = (double)l[c].x, li = (double)l[c].y;
+#pragma unroll
+        for (int q = 0; q < kTrsmRhs; q++) {
+          const double2 y = xs[q][c0 + c];
+          ar[q] += lr * y.x - li * y.y;
+          ai[q] += lr * y.y + li * y.x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kTrsmRhs; q++)
+    if (q < nq) {
+      double2 v = X[(size_t)i + (size_t)n * (q0 + q)];
+      v.x -= ar[q];
+      v.y -= ai[q];
+      X[(size_t)i + (size_t)n * (q0 + q)] = v;
+    }
 }
 
 // The same step for a batch of independent systems (blockIdx.z = system z
@@ -795,22 +976,25 @@ __global__ void k_copy_unless(const double2* __restrict__ src, double2* __restri
 // (the refinement loop of the mixed path after convergence).
 template <typename R>
 static void lu_solve_t(hipStream_t s, const typename CT<R>::c2* LU, int n, const int* perm, double2* X,
-                       double2* Xtmp, int nrhs, const int* skip = nullptr) {
+                       double2* Xtmp, double2* inv, int nrhs, const int* skip = nullptr) {
   if (nrhs <= 0) return;
   const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
   // T = P X; forward pass: off-diagonal rows updated in T, solved rows into X;
   // backward pass: off-diagonal rows updated in X, solved rows into T.
   HX_LAUNCH(k_gather_rows, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, perm, X, Xtmp, n, nrhs, skip);
-  for (int k0 = 0; k0 < n; k0 += kPanelNB) {
-    const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+  HX_LAUNCH(k_tri_inverse<R>, dim3((n + kPanelNB - 1) / kPanelNB, 2), dim3(256), 0, s, LU, n, inv, skip);
+  for (int k0 = 0; k0 < n; k0 += kSuper) {
+    const int kb = n - k0 < kSuper ? n - k0 : kSuper;
     const int rest = n - k0 - kb;
     const int g = rest > 0 ? (rest + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block<R, true>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, Xtmp, X, nrhs, skip);
+    HX_LAUNCH((k_trsm_super<R, true>), dim3(g, groups), dim3(256), 0, s, LU, (const double2*)inv, n, k0, kb, Xtmp,
+              X, nrhs, skip);
   }
-  for (int k0 = ((n - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
-    const int kb = n - k0 < kPanelNB ? n - k0 : kPanelNB;
+  for (int k0 = ((n - 1) / kSuper) * kSuper; k0 >= 0; k0 -= kSuper) {
+    const int kb = n - k0 < kSuper ? n - k0 : kSuper;
     const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block<R, false>), dim3(g, groups), dim3(256), 0, s, LU, n, k0, kb, X, Xtmp, nrhs, skip);
+    HX_LAUNCH((k_trsm_super<R, false>), dim3(g, groups), dim3(256), 0, s, LU, (const double2*)inv, n, k0, kb, X,
+              Xtmp, nrhs, skip);
   }
   if (skip)
     HX_LAUNCH(k_copy_unless, dim3(1024), dim3(256), 0, s, Xtmp, X, (size_t)n * nrhs, skip);
@@ -997,8 +1181,10 @@ void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx
                double2* diag, const LuAux* aux) {
   lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info, diag, aux);
 }
-void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs) {
-  lu_solve_t<double>(s, LU, n, perm, X, Xtmp, nrhs);
+size_t tri_inverse_len(int n) { return (size_t)2 * ((n + kPanelNB - 1) / kPanelNB) * kPanelNB * kPanelNB; }
+void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
+              int nrhs) {
+  lu_solve_t<double>(s, LU, n, perm, X, Xtmp, inv, nrhs);
 }
 void mirror_lower(hipStream_t s, double2* C, int n) {
   if (n <= 1) return;
@@ -1012,9 +1198,9 @@ void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* w
                    double2* diag, const LuAux* aux) {
   lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag), aux);
 }
-void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, int nrhs,
-                  const int* skip) {
-  lu_solve_t<float>(s, LU, n, perm, X, Xtmp, nrhs, skip);
+void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
+                  int nrhs, const int* skip) {
+  lu_solve_t<float>(s, LU, n, perm, X, Xtmp, inv, nrhs, skip);
 }
 void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
                       const size_t* x_off, const int* r0, const int* body, const int* perm, int n, const double2* V,

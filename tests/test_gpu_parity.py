@@ -230,6 +230,31 @@ def test_lu_lookahead_gives_the_same_factors(precision, outer, monkeypatch):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape,precision", [((10, 3, 2), "fp64"), ((40, 8, 4), "fp64"), ((40, 8, 4), "mixed"),
+                                              ((40, 24, 4), "fp64"), ((40, 24, 4), "mixed")])
+def test_triangular_solves_match_lapack(shape, precision):
+    """The solves (k_tri_inverse + k_trsm_super: 256-row blocks, 32-row
+    diagonal blocks by their inverses) against LAPACK (scipy lu_factor /
+    lu_solve) on the GPU's own A and right-hand sides from the GPU's S: N =
+    100 (one partial block), 960 (three full blocks and a partial one) and
+    2,240 (eight full, one partial), all 15 right-hand-side columns (two
+    groups, one partial)."""
+    m = small_pair(*shape)
+    heads = np.deg2rad([0.0, 30.0, 90.0])
+    say(f"[gpu] triangular solves n={m.nelem} {precision}")
+    with FrequencySolver(m, precision=precision) as s:
+        s.set_waves(heads)
+        s.calwaveproperts(0.9, 35.0)
+        s.assemble()
+        A, S = s.get_matrix("A"), s.get_matrix("S")
+        assert s.factor() == 0
+        X = np.concatenate([s.radiation_solver(), s.diffraction_solver()], axis=1)
+    Xo = oracle.solve_assembled(m, A, S, 0.9, 35.0, heads)[3]
+    err = rel(X, Xo)
+    say(f"    max |X - X_lapack| / max |X_lapack| = {err:.2e}")
+    assert err < (1e-12 if precision == "fp64" else 1e-9)
+
+
 @pytest.mark.parametrize("depth", [-1.0, 30.0])
 def test_frequency_solve_matches_oracle(mesh_small, depth):
     m = mesh_small
