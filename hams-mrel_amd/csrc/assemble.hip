@@ -28,7 +28,8 @@ __device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int j
   const size_t mn = (size_t)kModes * n;
   const double* p = t + kTabHead;
   return FreqParams{f.v,     f.wk,       f.w1,           f.h,    f.wvn,          t,  t + kModes, t + 2 * kModes,
-                    t[3 * kModes], p, p + mn, p + 2 * mn, p + 3 * mn, p + 4 * mn, p + 4 * mn + n, n, js, jf};
+                    t[3 * kModes], p, p + mn, p + 2 * mn, p + 3 * mn, p + 4 * mn, p + 4 * mn + n, n, js, jf,
+                    f.far32};
 }
 
 // Mode tables: one thread per (panel, mode); panel 0's threads also write

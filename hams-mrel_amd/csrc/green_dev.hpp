@@ -493,6 +493,7 @@ struct FreqParams {
   const double *chz, *shz;     // [n]: cosh, sinh of k (z + h)
   int n;                       // table row stride (panels)
   int js, jf;                  // table columns of zf (source panel) and zp (field panel)
+  int far32 = 0;               // EIGEN region: evanescent series in FP32 (mixed precision)
 };
 
 // The table entries, each formed by the expression the reference evaluates
@@ -766,6 +767,76 @@ __device__ inline void d_eigen2(double r, double zs, double zd, const FreqParams
   }
   d_remove_rankine(r, zs, zd, fgA);
   d_remove_rankine(r, zd, zs, fgB);
+}
+
+// ---- EIGEN region in FP32 (mixed precision; R/h >= 0.5) ----
+// K0 and K1 of x > 0 in single precision by the polynomial approximations of
+// Abramowitz & Stegun 9.8.1, 9.8.3, 9.8.5-9.8.8 (absolute or relative error
+// below 2.2e-7 on their ranges) -- well conditioned in FP32, unlike the
+// power series IK01A sums up to x = 9, whose terms cancel to 1e-8 there.
+__device__ __forceinline__ void f_k01(float x, float& k0, float& k1) {
+  if (x <= 2.0f) {
+    const float t = x * (1.0f / 3.75f), t2 = t * t;
+    const float i0 = 1.0f + t2 * (3.5156229f + t2 * (3.0899424f + t2 * (1.2067492f + t2 * (0.2659732f + t2 * (0.0360768f + t2 * 0.0045813f)))));
+    const float i1 = x * (0.5f + t2 * (0.87890594f + t2 * (0.51498869f + t2 * (0.15084934f + t2 * (0.02658733f + t2 * (0.00301532f + t2 * 0.00032411f))))));
+    const float y = 0.25f * x * x, lg = logf(0.5f * x);
+    k0 = -lg * i0 + (-0.57721566f + y * (0.42278420f + y * (0.23069756f + y * (0.03488590f + y * (0.00262698f + y * (0.00010750f + y * 0.0000074f))))));
+    k1 = lg * i1 + (1.0f + y * (0.15443144f + y * (-0.67278579f + y * (-0.18156897f + y * (-0.01919402f + y * (-0.00110404f + y * -0.00004686f)))))) / x;
+  } else {
+    const float y = 2.0f / x, e = expf(-x) * rsqrtf(x);
+    k0 = e * (1.25331414f + y * (-0.07832358f + y * (0.02189568f + y * (-0.01062446f + y * (0.00587872f + y * (-0.00251540f + y * 0.00053208f))))));
+    k1 = e * (1.25331414f + y * (0.23498619f + y * (-0.03655620f + y * (0.01504268f + y * (-0.00780353f + y * (0.00325614f + y * -0.00068245f))))));
+  }
+}
+
+// The evanescent modes of EIGEN (:321-337) for both orientations in FP32,
+// each with the reference's break test (:330); the propagating mode (FP64)
+// is in fgA / fgB already and the modes are added to it.  The Rankine terms
+// are not removed here: the caller adds the remainder that the removal and
+// FINGREEN3D's re-addition leave (green_rankine_far).
+__device__ inline void f_eigen_modes2(double r, const FreqParams& fpA, const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  bool goA = true, goB = true;
+  const float rf = (float)r;
+  for (int i = 2; i <= 10; i++) {
+    const int mode = i - 1;
+    const float km = (float)fpA.wvn[mode], pm = (float)fpA.pm[mode];
+    float k0, k1;
+    f_k01(km * rf, k0, k1);
+    const size_t row = (size_t)mode * fpA.n;
+    const float cA = (float)fpA.cz[row + fpA.js], cB = (float)fpA.cz[row + fpA.jf];
+    const float cc = cA * cB, pk0 = pm * k0, pk1 = pm * km * k1;
+    if (goA) {
+      const float r0 = pk0 * cc, r1 = -(pk1 * cc), r2 = -(pm * km * k0 * (float)fpA.sz[row + fpA.js] * cB);
+      a0 += r0;
+      a1 += r1;
+      a2 += r2;
+      if (fabsf(r0) < 1.0e-6f && fabsf(r1) < 1.0e-6f && fabsf(r2) < 1.0e-6f) goA = false;
+    }
+    if (goB) {
+      const float r0 = pk0 * cc, r1 = -(pk1 * cc), r2 = -(pm * km * k0 * (float)fpB.sz[row + fpB.js] * cA);
+      b0 += r0;
+      b1 += r1;
+      b2 += r2;
+      if (fabsf(r0) < 1.0e-6f && fabsf(r1) < 1.0e-6f && fabsf(r2) < 1.0e-6f) goB = false;
+    }
+    if (!goA && !goB) break;
+  }
+  fgA[0].r += a0;
+  fgA[1].r += a1;
+  fgA[2].r += a2;
+  fgB[0].r += b0;
+  fgB[1].r += b1;
+  fgB[2].r += b2;
+}
+
+// EIGEN with FP32 evanescent modes, both orientations (no Rankine removal).
+__device__ inline void d_eigen2_far32(double r, const FreqParams& fpA, const FreqParams& fpB, cd fgA[3], cd fgB[3]) {
+  double j0, j1, y0, y1;
+  d_jy01b(fpA.wk * r, j0, j1, y0, y1);
+  wave_mode_from(j0, j1, y0, y1, fpA, false, fgA);
+  wave_mode_from(j0, j1, y0, y1, fpB, false, fgB);
+  f_eigen_modes2(r, fpA, fpB, fgA, fgB);
 }
 
 // EIGENE, both orientations: the value and d/dR series are shared (see
@@ -1170,6 +1241,20 @@ __device__ __forceinline__ void green_rankine(const PairGeom& p, double sgn, int
   }
 }
 
+// What the EIGEN routine's Rankine removal (:344-357) and green_rankine
+// leave together, exactly: the removed 1/r and 1/r1 terms equal the added
+// ones (sr = r, sr1 = r1), so nothing for FLAG = 0 and the source term's
+// negative for FLAG = 1 (the Newman integrals carry 1/r there).  The FP32
+// path adds this instead of subtracting and re-adding the terms.
+__device__ __forceinline__ void green_rankine_far(const PairGeom& p, int flag, cd g[4]) {
+  if (flag == 0) return;
+  const double r3 = p.r * p.r * p.r;
+  g[0].r -= 1.0 / p.r;
+  g[1].r += p.xpq / r3;
+  g[2].r += p.ypq / r3;
+  g[3].r += (p.zsrc - p.zfld) / r3;
+}
+
 // FINGREEN3D with FINGREEN3D_R's region split (:183-191).  REG fixes the
 // region at compile time for pairs binned by fin_region; kRegAny decides per
 // pair.
@@ -1183,6 +1268,13 @@ __device__ inline void d_fingreen_r(const PairGeom& p, const FreqParams& fp, int
   } else {
     cd fg[3];
     const int reg = REG == kRegAny ? fin_region(p.rr, fp.h) : REG;
+    if (reg == kRegEigen && fp.far32) {
+      cd fgB[3];
+      d_eigen2_far32(p.rr, fp, fp, fg, fgB);
+      green_from_fg(p, fg, g);
+      green_rankine_far(p, flag, g);
+      return;
+    }
     if (reg == kRegLinton)
       d_linton(p.rr, p.zsrc, p.zfld, fp, fg, ovf);
     else if (reg == kRegPidcock)
@@ -1210,6 +1302,14 @@ __device__ inline void d_fingreen2(const PairGeom& pA, const PairGeom& pB, const
     sgn = -1.0;
   } else {
     cd fgA[3], fgB[3];
+    if (REG == kRegEigen && fpA.far32) {
+      d_eigen2_far32(pA.rr, fpA, fpB, fgA, fgB);
+      green_from_fg(pA, fgA, gA);
+      green_from_fg(pB, fgB, gB);
+      green_rankine_far(pA, flagA, gA);
+      green_rankine_far(pB, flagB, gB);
+      return;
+    }
     if (REG == kRegPidcock)
       d_pidcock2(pA.rr, pA.zsrc, pA.zfld, fpA, fpB, fgA, fgB);
     else if (REG == kRegEigene)

@@ -375,6 +375,7 @@ struct hamsx_ctx {
     return k;
   }
   bool opt_graphs = false;
+  bool opt_far32 = true;  // mixed precision: the EIGEN region's evanescent series in FP32
   bool capturing = false;  // recording: the LU runs its one-stream schedule
   unsigned long long alloc_epoch = 0;
   hipGraph_t graph = nullptr;
@@ -1474,6 +1475,11 @@ int hamsx_set_option(hamsx_ctx* c, int option, int value) {
       if (value != 0 && value != 1) return fail(HAMSX_EARG, "HAMSX_OPT_GRAPHS is 0 or 1");
       c->opt_graphs = value != 0;
       return 0;
+    case HAMSX_OPT_FP32_FARFIELD:
+      if (value != 0 && value != 1) return fail(HAMSX_EARG, "HAMSX_OPT_FP32_FARFIELD is 0 or 1");
+      c->opt_far32 = value != 0;
+      c->have_A = c->have_S = c->have_LU = c->have_X = false;
+      return 0;
     default:
       return fail(HAMSX_EARG, "unknown option %d", option);
   }
@@ -1486,6 +1492,7 @@ int hamsx_get_option(hamsx_ctx* c, int option, int* value) {
     case HAMSX_OPT_LU_SELECT_ROWS: *value = c->lu_aux.sel_rows; return 0;
     case HAMSX_OPT_LU_OUTER_MIN_N: *value = c->lu_aux.outer_min_n; return 0;
     case HAMSX_OPT_GRAPHS: *value = c->opt_graphs ? 1 : 0; return 0;
+    case HAMSX_OPT_FP32_FARFIELD: *value = c->opt_far32 ? 1 : 0; return 0;
     default: return fail(HAMSX_EARG, "unknown option %d", option);
   }
 }
@@ -1809,6 +1816,7 @@ int hamsx_assemble(hamsx_ctx* c) {
   if (finite && (rc = c->modetab.ensure(hamsx::mode_tab_len(c->nf)))) return rc;
   c->have_C = false;
   c->freq.tab = finite ? c->modetab.p : nullptr;
+  c->freq.far32 = c->prec == HAMSX_PREC_MIXED && c->opt_far32 ? 1 : 0;
   const bool limit = c->freq.v == 0.0 || c->freq.v == -1.0;
   {
     StageTimer tm(c, 0);

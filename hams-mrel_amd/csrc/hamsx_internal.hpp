@@ -73,6 +73,7 @@ struct FreqDev {
   // the scalars from here (freq_now), so a launch sequence recorded once
   // serves any frequency of the same depth
   const double* sc = nullptr;
+  int far32 = 0;  // EIGEN region's evanescent series in FP32 (mixed precision)
 };
 __host__ __device__ inline FreqDev freq_now(FreqDev f) {
   if (f.sc) {

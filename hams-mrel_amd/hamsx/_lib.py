@@ -32,8 +32,9 @@ OPT_LU_LOOKAHEAD = 1
 OPT_LU_SELECT_ROWS = 2
 OPT_LU_OUTER_MIN_N = 3
 OPT_GRAPHS = 4
+OPT_FP32_FARFIELD = 5
 OPTIONS = {"lu_lookahead": OPT_LU_LOOKAHEAD, "lu_select_rows": OPT_LU_SELECT_ROWS,
-           "lu_outer_min_n": OPT_LU_OUTER_MIN_N, "graphs": OPT_GRAPHS}
+           "lu_outer_min_n": OPT_LU_OUTER_MIN_N, "graphs": OPT_GRAPHS, "fp32_farfield": OPT_FP32_FARFIELD}
 
 #: every symbol include/hamsx.h declares, with (restype, argtypes)
 _dp = C.POINTER(C.c_double)

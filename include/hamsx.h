@@ -73,6 +73,13 @@ int hamsx_set_precision(hamsx_ctx *ctx, int mode);
  *                            eagerly).  Recordings use the one-stream LU
  *                            schedule; results are the same bit for bit. */
 #define HAMSX_OPT_GRAPHS 4
+/*   HAMSX_OPT_FP32_FARFIELD  1 (default) or 0 -- with HAMSX_PREC_MIXED, the
+ *                            finite-depth far field (EIGEN region, R/h >= 0.5,
+ *                            FinGreen3D.f90:281-362) sums its evanescent modes
+ *                            in FP32 (K0/K1 by the Abramowitz & Stegun
+ *                            polynomials); the propagating mode stays FP64.
+ *                            No effect in FP64 mode. */
+#define HAMSX_OPT_FP32_FARFIELD 5
 int hamsx_set_option(hamsx_ctx *ctx, int option, int value);
 int hamsx_get_option(hamsx_ctx *ctx, int option, int *value);
 

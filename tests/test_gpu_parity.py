@@ -178,7 +178,9 @@ def test_lu_tuning_options(rows, outer_min):
     for la in (0, 1):
         with FrequencySolver(m) as s:
             s.set_options(lu_select_rows=rows, lu_outer_min_n=outer_min, lu_lookahead=la)
-            assert s.options() == {"lu_lookahead": la, "lu_select_rows": rows, "lu_outer_min_n": outer_min}
+            got = s.options()
+            assert {k: got[k] for k in ("lu_lookahead", "lu_select_rows", "lu_outer_min_n")} == \
+                {"lu_lookahead": la, "lu_select_rows": rows, "lu_outer_min_n": outer_min}
             s.set_waves([0.0])
             s.calwaveproperts(1.0, 35.0)
             s.assemble()
@@ -253,6 +255,38 @@ def test_triangular_solves_match_lapack(shape, precision):
     err = rel(X, Xo)
     say(f"    max |X - X_lapack| / max |X_lapack| = {err:.2e}")
     assert err < (1e-12 if precision == "fp64" else 1e-9)
+
+
+def test_fp32_far_field_changes_only_the_eigen_region():
+    """Mixed precision sums the EIGEN region's evanescent modes (R/h >= 0.5)
+    in FP32: every other entry of A and S equals the FP64 assembly bit for
+    bit, the EIGEN entries stay within 1e-6 of the largest EIGEN entry, and
+    the coefficients within 1e-6 of the FP64 solve's (the mixed path's bar).
+    Two cylinders 60 m apart at depth 40 m: most pairs are EIGEN pairs."""
+    m = small_pair(na=24, nz=6, nr=3, gap=60.0)
+    h, w = 40.0, 1.1
+    heads = np.deg2rad([0.0, 45.0])
+    rr = np.hypot(m.cen[:, None, 0] - m.cen[None, :, 0], m.cen[:, None, 1] - m.cen[None, :, 1])
+    eig = np.abs(rr / h) >= 0.5
+    say(f"[gpu] FP32 far field n={m.nelem}: {eig.mean():.2f} of the pairs in EIGEN")
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with FrequencySolver(m, precision=prec) as s:
+            assert s.options()["fp32_farfield"] == 1
+            s.set_waves(heads)
+            s.calwaveproperts(w, h)
+            s.assemble()
+            out[prec] = (s.get_matrix("A"), s.get_matrix("S"), s.solve_frequency(w, h)[:3])
+    for k in (0, 1):
+        a, b = out["mixed"][k], out["fp64"][k]
+        np.testing.assert_array_equal(a[~eig], b[~eig])
+        d = np.max(np.abs(a[eig] - b[eig])) / np.max(np.abs(b[eig]))
+        say(f"    {'AS'[k]}: EIGEN entries within {d:.1e} of the FP64 assembly's largest")
+        assert 0 < d < 1e-6
+    for name, a, b in zip(("amas", "bdmp", "exfc"), out["mixed"][2], out["fp64"][2]):
+        d = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        say(f"    {name}: {d:.1e}")
+        assert d < 1e-6
 
 
 @pytest.mark.parametrize("depth", [-1.0, 30.0])
