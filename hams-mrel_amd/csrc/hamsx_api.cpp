@@ -275,6 +275,8 @@ struct hamsx_ctx {
   DevBuf<size_t> hmUoff, hmVoff, hmRoff, hmToff, hmDoff;
   DevBuf<unsigned char> hmUsed;
   DevBuf<double2> hmU, hmVH, hmD, hmT, hmXp, hmYp;
+  DevBuf<float2> hmU32, hmVH32, hmD32;  // mixed precision: the blocks in complex64 for the Arnoldi products
+  bool hm_c64 = false;
   int hm_nlr = 0, hm_ndense = 0, hm_nleaf = 0, hm_nrhs = 0;
   double hm_mean_rank = 0.0, hm_ratio = 0.0;
   // mesh; with a lid (irregular-frequency removal) the nf = n + nlid field
@@ -999,6 +1001,16 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
   if (!litem.empty())
     HX_TRY(hipMemcpyAsync(c->hmLitem.p, litem.data(), sizeof(int) * litem.size(), hipMemcpyHostToDevice,
                           c->stream));
+  // mixed precision: complex64 copies of the blocks for the Arnoldi products
+  c->hm_c64 = false;
+  if (c->prec == HAMSX_PREC_MIXED) {
+    if ((rc = c->hmU32.ensure(c->hmU.n)) || (rc = c->hmVH32.ensure(c->hmVH.n)) || (rc = c->hmD32.ensure(c->hmD.n)))
+      return rc;
+    hamsx::launch::to_c64(c->stream, c->hmU.p, c->hmU32.p, c->hmU.n);
+    hamsx::launch::to_c64(c->stream, c->hmVH.p, c->hmVH32.p, c->hmVH.n);
+    hamsx::launch::to_c64(c->stream, c->hmD.p, c->hmD32.p, c->hmD.n);
+    c->hm_c64 = true;
+  }
   HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above go out of scope
   c->hm_nlr = nlr;
   c->hm_ndense = nd;
@@ -1030,15 +1042,23 @@ hamsx::HmDev hm_dev(hamsx_ctx* c) {
   h.leaves = c->hmLeaves.p;
   h.lptr = c->hmLptr.p;
   h.litem = c->hmLitem.p;
+  if (c->hm_c64) {
+    h.U32 = c->hmU32.p;
+    h.VH32 = c->hmVH32.p;
+    h.D32 = c->hmD32.p;
+  }
   return h;
 }
 
 // the GMRES operator: Y = beta B + alpha Op X, Op the H-matrix when built,
-// else the dense matrix
+// else the dense matrix.  inner: a product inside an Arnoldi cycle, which in
+// mixed precision reads the complex64 blocks; the residuals at each restart
+// and the final one use the FP64 blocks.
 void op_matvec(hamsx_ctx* c, const double2* Aop, const double2* X, int nrhs, const double2* B, double alpha,
-               double beta, double2* Y) {
+               double beta, double2* Y, bool inner = false) {
   if (c->hm_built)
-    hamsx::launch::hm_matvec(c->stream, hm_dev(c), c->n, X, nrhs, B, alpha, beta, Y, c->hmXp.p, c->hmYp.p);
+    hamsx::launch::hm_matvec(c->stream, hm_dev(c), c->n, X, nrhs, B, alpha, beta, Y, c->hmXp.p, c->hmYp.p,
+                             inner && c->hm_c64);
   else
     hamsx::launch::matvec(c->stream, Aop, c->n, X, nrhs, B, alpha, beta, Y, c->mvw.p);
 }
@@ -1102,7 +1122,7 @@ int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
       iter++;
       double2* Vj = c->gV.p + blk * j;
       gmres_precond_apply(c, Vj, c->gZ.p, nr);
-      op_matvec(c, Aop, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p);
+      op_matvec(c, Aop, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, true);
       double2* d1 = c->gDots.p;
       double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
       hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1);
@@ -2394,7 +2414,6 @@ int hamsx_set_solver(hamsx_ctx* c, int solver, double tol, int restart, int maxi
   if (solver == HAMSX_SOLVER_GMRES) {
     if (!(tol > 0.0) || restart < 1 || restart > 200 || maxiter < 1)
       return fail(HAMSX_EARG, "GMRES needs tol > 0, 1 <= restart <= 200, maxiter >= 1");
-    if (c->prec != HAMSX_PREC_FP64) return fail(HAMSX_EARG, "GMRES runs in FP64 precision mode");
     c->gm_tol = tol;
     c->gm_restart = restart;
     c->gm_maxiter = maxiter;

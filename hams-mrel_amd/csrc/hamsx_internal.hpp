@@ -121,6 +121,8 @@ struct HmDev {
   const double2* D;
   const int2* leaves;
   const int *lptr, *litem;
+  // complex64 copies of U, VH and D (mixed precision), or null
+  const float2 *U32 = nullptr, *VH32 = nullptr, *D32 = nullptr;
 };
 
 // Look-ahead resources of the LU: a second stream (default priority) and two
@@ -249,8 +251,9 @@ void hm_aca(hipStream_t s, const double2* A, int lda, const int* perm, int nblk,
             int rank_max, double tol);
 void hm_dense_gather(hipStream_t s, const double2* A, int lda, const int* perm, int nblk, const int4* desc,
                      const size_t* doff, double2* D);
+// c64: read the complex64 copies of the blocks (when built)
 void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
-               double beta, double2* Y, double2* Xp, double2* Yp);
+               double beta, double2* Y, double2* Xp, double2* Yp, bool c64 = false);
 }  // namespace launch
 
 }  // namespace hamsx

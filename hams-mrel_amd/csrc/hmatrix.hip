@@ -245,10 +245,15 @@ __device__ __forceinline__ void stage_rows(double2 (&xs)[kHmJ][kHmQ], const doub
 // Workgroup (block, column group of kHmQ): thread (column q = t % 128,
 // parity p = t / 128) holds rows l = p + 2k, k < KPER; VH and the
 // right-hand-side rows go through LDS in chunks of kHmJ columns.
-template <int KPER>
+__device__ __forceinline__ double2 as_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 as_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+
+// S: the storage of the blocks' factors and dense entries -- double2, or
+// float2 (complex64 blocks of the mixed path); arithmetic is FP64 either way.
+template <int KPER, typename S>
 __global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, const int4* __restrict__ desc,
                                                const int* __restrict__ rank, const size_t* __restrict__ voff,
-                                               const size_t* __restrict__ toff, const double2* __restrict__ VH,
+                                               const size_t* __restrict__ toff, const S* __restrict__ VH,
                                                const double2* __restrict__ Xp, int n, int nrhs,
                                                double2* __restrict__ T) {
   __shared__ double2 xs[kHmJ][kHmQ];
@@ -261,14 +266,14 @@ __global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, con
   double ar[KPER], ai[KPER];
 #pragma unroll
   for (int k = 0; k < KPER; k++) ar[k] = ai[k] = 0.0;
-  const double2* v = VH + voff[b];
+  const S* v = VH + voff[b];
   for (int j0 = 0; j0 < d.w; j0 += kHmJ) {
     const int jn = min(kHmJ, d.w - j0);
     __syncthreads();
     stage_rows(xs, Xp + (size_t)d.z + j0 + (size_t)n * q0, (size_t)n, jn, nq);
     for (int e = threadIdx.x; e < 2 * KPER * kHmJ; e += 256) {
       const int l = e / kHmJ, jj = e % kHmJ;
-      vs[l][jj] = (l < rk && jj < jn) ? v[(size_t)l * d.w + j0 + jj] : make_double2(0.0, 0.0);
+      vs[l][jj] = (l < rk && jj < jn) ? as_d2(v[(size_t)l * d.w + j0 + jj]) : make_double2(0.0, 0.0);
     }
     __syncthreads();
     for (int jj = 0; jj < jn; jj++) {
@@ -294,12 +299,13 @@ __global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, con
 // order; item >= 0: dense block, item < 0: low-rank block ~item.
 // Workgroup (leaf, column group of kHmQ); leaves of more than 32 rows are
 // done 32 rows at a time.
+template <typename S>
 __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves, const int* __restrict__ lptr,
                                                  const int* __restrict__ litem, const int4* __restrict__ ddesc,
-                                                 const size_t* __restrict__ doff, const double2* __restrict__ D,
+                                                 const size_t* __restrict__ doff, const S* __restrict__ D,
                                                  const int4* __restrict__ ldesc, const int* __restrict__ rank,
                                                  const size_t* __restrict__ uoff, const size_t* __restrict__ toff,
-                                                 const double2* __restrict__ U, const double2* __restrict__ T,
+                                                 const S* __restrict__ U, const double2* __restrict__ T,
                                                  const double2* __restrict__ Xp, int n, int nrhs,
                                                  double2* __restrict__ Yp) {
   __shared__ double2 xs[kHmJ][kHmQ];
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves
     for (int k = 0; k < kPer; k++) ar[k] = ai[k] = 0.0;
     for (int it = lptr[blockIdx.x]; it < lptr[blockIdx.x + 1]; it++) {
       const int item = litem[it];
-      const double2* a;     // this row of the block's left factor, column stride nr
+      const S* a;           // this row of the block's left factor, column stride nr
       const double2* src;   // right factor rows, leading dimension ld
       size_t ld;
       int nr, kk;
@@ -342,7 +348,7 @@ __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves
         __syncthreads();
         if (live)
           for (int jj = 0; jj < jn; jj++) {
-            const double2 av = a[(size_t)nr * (j0 + jj)];
+            const double2 av = as_d2(a[(size_t)nr * (j0 + jj)]);
 #pragma unroll
             for (int k = 0; k < kPer; k++) {
               const double2 c = xs[jj][ql + 8 * k];
@@ -389,8 +395,10 @@ void hm_dense_gather(hipStream_t s, const double2* A, int lda, const int* perm, 
   if (nblk <= 0) return;
   HX_LAUNCH(k_dense_gather, dim3(nblk), dim3(256), 0, s, A, lda, perm, desc, doff, D);
 }
-void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
-               double beta, double2* Y, double2* Xp, double2* Yp) {
+template <typename S>
+static void hm_matvec_t(hipStream_t s, const HmDev& h, const S* U, const S* VH, const S* D, int n,
+                        const double2* X, int nrhs, const double2* B, double alpha, double beta, double2* Y,
+                        double2* Xp, double2* Yp) {
   const size_t tot = (size_t)n * nrhs;
   const dim3 g1((unsigned)((tot + 255) / 256));
   HX_LAUNCH(k_perm_gather, g1, dim3(256), 0, s, h.perm, X, n, nrhs, Xp);
@@ -402,19 +410,28 @@ void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs,
     if (cnt <= 0) continue;
     const dim3 grid(cnt, qg);
     if (g == 0)
-      HX_LAUNCH(k_hm_vx<4>, grid, dim3(256), 0, s, lists[0], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+      HX_LAUNCH((k_hm_vx<4, S>), grid, dim3(256), 0, s, lists[0], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n, nrhs,
+                h.T);
     else if (g == 1)
-      HX_LAUNCH(k_hm_vx<8>, grid, dim3(256), 0, s, lists[1], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs, h.T);
+      HX_LAUNCH((k_hm_vx<8, S>), grid, dim3(256), 0, s, lists[1], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n, nrhs,
+                h.T);
     else if (g == 2)
-      HX_LAUNCH(k_hm_vx<16>, grid, dim3(256), 0, s, lists[2], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs,
-                h.T);
+      HX_LAUNCH((k_hm_vx<16, S>), grid, dim3(256), 0, s, lists[2], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n,
+                nrhs, h.T);
     else
-      HX_LAUNCH(k_hm_vx<32>, grid, dim3(256), 0, s, lists[3], h.ldesc, h.rank, h.voff, h.toff, h.VH, Xp, n, nrhs,
-                h.T);
+      HX_LAUNCH((k_hm_vx<32, S>), grid, dim3(256), 0, s, lists[3], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n,
+                nrhs, h.T);
   }
-  HX_LAUNCH(k_hm_leaf, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, h.D, h.ldesc,
-            h.rank, h.uoff, h.toff, h.U, h.T, Xp, n, nrhs, Yp);
+  HX_LAUNCH(k_hm_leaf<S>, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, D,
+            h.ldesc, h.rank, h.uoff, h.toff, U, h.T, Xp, n, nrhs, Yp);
   HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);
+}
+void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
+               double beta, double2* Y, double2* Xp, double2* Yp, bool c64) {
+  if (c64 && h.U32)
+    hm_matvec_t<float2>(s, h, h.U32, h.VH32, h.D32, n, X, nrhs, B, alpha, beta, Y, Xp, Yp);
+  else
+    hm_matvec_t<double2>(s, h, h.U, h.VH, h.D, n, X, nrhs, B, alpha, beta, Y, Xp, Yp);
 }
 }  // namespace launch
 }  // namespace hamsx

@@ -250,7 +250,11 @@ int hamsx_get_matrix(hamsx_ctx *ctx, int which, double *out, int *ipiv);
  * ZITER_SOLVE_MULTI with SETUP_BLOCK_PRECONDITIONER,
  * src/IterativeSolvers.f90:56-58 (restart 25, tol 1e-4, maxiter 100),
  * :190-587, called from src/AssbMatxMulti.f90:283-289, :349-357); the
- * operator is the assembled matrix itself.  FP64 precision mode only. */
+ * operator is the H-matrix below, or the assembled matrix itself.  In
+ * HAMSX_PREC_MIXED the products inside each Arnoldi cycle read complex64
+ * copies of the H-matrix blocks, while the Krylov vectors, the
+ * orthogonalisation and every restart residual (and so the stopping test)
+ * stay FP64 on the FP64 blocks. */
 #define HAMSX_SOLVER_LU 0
 #define HAMSX_SOLVER_GMRES 1
 int hamsx_set_solver(hamsx_ctx *ctx, int solver, double tol, int restart, int maxiter);

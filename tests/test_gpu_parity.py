@@ -501,6 +501,33 @@ def test_gmres_hmatrix_matches_lu(depth):
     assert er[0] < 1e-3 and er[1] < 5e-2 and er[2] < 1e-3
 
 
+@pytest.mark.parametrize("depth", [-1.0, 40.0])
+def test_gmres_complex64_hmatrix_blocks(depth):
+    """Mixed precision GMRES: the Arnoldi products read complex64 copies of
+    the H-matrix blocks, while the Krylov vectors and every restart residual
+    stay FP64 on the FP64 blocks.  It must reach the FP64 GMRES result on the
+    same H-matrix (ACA 1e-9, GMRES 1e-10) to 1e-6 of each table's largest
+    entry, and the dense FP64 LU to 1e-6."""
+    m = small_pair(na=28, nz=7, nr=3)
+    heads = np.deg2rad([0.0, 45.0])
+    say(f"[gpu] gmres with complex64 H-matrix blocks n={m.nelem} depth={depth}")
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with FrequencySolver(m, precision=prec) as s:
+            s.set_waves(heads)
+            if prec == "fp64":
+                out["lu"] = s.solve_frequency(1.1, depth)
+            s.set_solver("gmres", tol=1e-10, restart=25, maxiter=400)
+            s.set_hmatrix(True, 2.0, 1e-9, 32, 50)
+            out[prec] = s.solve_frequency(1.1, depth)
+            out[prec + "_it"] = s.refine_stats()
+    e64 = [rel(a, b) for a, b in zip(out["mixed"][:3], out["fp64"][:3])]
+    elu = [rel(a, b) for a, b in zip(out["mixed"][:3], out["lu"][:3])]
+    say(f"    iterations fp64 {out['fp64_it']}, mixed {out['mixed_it']}; mixed vs FP64 GMRES {e64}, vs LU {elu}")
+    assert max(e64) < 1e-6 and max(elu) < 1e-6
+    assert out["mixed_it"][1] < 1e-9  # the final residual is the FP64 one
+
+
 def lid_centroids(lid):
     from hamsx.mesh import panel_geometry
     cen, _, _, _ = panel_geometry(lid, np.zeros(3))
