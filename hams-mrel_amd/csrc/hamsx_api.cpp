@@ -347,7 +347,8 @@ struct hamsx_ctx {
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
   DevBuf<double> gNorm, gScale;
   DevBuf<int> pPerm, pIpiv, pSz;
-  DevBuf<size_t> pLuOff, pXoff;  // per body: factor offset, packed right-hand-side offset
+  DevBuf<size_t> pLuOff, pXoff, pInvOff;  // per body: factor, packed right-hand-side, inverse-block offsets
+  DevBuf<double2> pInv;                   // inverted 32 x 32 diagonal blocks of the body factors
   int pMax = 0;
   std::vector<size_t> pOff;
   DevBuf<double> out, norms, norms2;  // refinement norms (norms2: symmetry system 2)
@@ -825,18 +826,26 @@ int gmres_precond_setup(hamsx_ctx* c, const double2* M) {
     xo[b] = (size_t)nr * c->body_start[b];
   }
   c->pMax = (int)maxb;
-  if ((rc = c->pSz.ensure(nb)) || (rc = c->pLuOff.ensure(nb)) || (rc = c->pXoff.ensure(nb))) return rc;
+  // the inverted 32 x 32 diagonal blocks of every body's factors
+  std::vector<size_t> io(nb + 1, 0);
+  for (int b = 0; b < nb; b++) io[b + 1] = io[b] + hamsx::launch::tri_inverse_len(sz[b]);
+  if ((rc = c->pSz.ensure(nb)) || (rc = c->pLuOff.ensure(nb)) || (rc = c->pXoff.ensure(nb)) ||
+      (rc = c->pInvOff.ensure(nb)) || (rc = c->pInv.ensure(std::max<size_t>(io[nb], 1))))
+    return rc;
   HX_TRY(hipMemcpyAsync(c->pSz.p, sz.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
   HX_TRY(hipMemcpyAsync(c->pLuOff.p, c->pOff.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, c->stream));
   HX_TRY(hipMemcpyAsync(c->pXoff.p, xo.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, c->stream));
+  HX_TRY(hipMemcpyAsync(c->pInvOff.p, io.data(), sizeof(size_t) * nb, hipMemcpyHostToDevice, c->stream));
+  hamsx::launch::tri_inverse_systems(c->stream, nb, c->pMax, c->pLU.p, c->pLuOff.p, c->pSz.p, c->pInvOff.p, c->pInv.p);
   HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
   return check_launch();
 }
 
 // Z = M^-1 V (n x nrhs): all bodies' block solves in one batched pass
 void gmres_precond_apply(hamsx_ctx* c, const double2* V, double2* Z, int nrhs) {
-  hamsx::launch::lu_solve_systems(c->stream, c->nbody, c->pMax, c->pLU.p, c->pLuOff.p, c->pSz.p, c->pXoff.p,
-                                  c->body_start_d.p, c->body.p, c->pPerm.p, c->n, V, Z, c->gP.p, c->gPt.p, nrhs);
+  hamsx::launch::lu_solve_systems(c->stream, c->nbody, c->pMax, c->pLU.p, c->pLuOff.p, c->pSz.p, c->pInv.p,
+                                  c->pInvOff.p, c->pXoff.p, c->body_start_d.p, c->body.p, c->pPerm.p, c->n, V, Z,
+                                  c->gP.p, c->gPt.p, nrhs);
 }
 
 void givens(cplx a, cplx b, double& cs, cplx& sn) {

@@ -227,9 +227,13 @@ void axpy_update(hipStream_t s, double2* X, const double2* D, size_t count, cons
 // perm[r0[z] .. r0[z+1]) (local indices) and owns rows [r0[z], r0[z+1]) of
 // the n x nrhs block V (body[i] = system of row i); Z = the solutions in the
 // same rows.  P, Pt: n x nrhs scratch; x_off[z] = nrhs * r0[z].
+// inv / inv_off: the systems' inverted diagonal blocks, tri_inverse_len(nz[z])
+// each, written by tri_inverse_systems once per factorisation.
+void tri_inverse_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
+                         const size_t* inv_off, double2* inv);
 void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
-                      const size_t* x_off, const int* r0, const int* body, const int* perm, int n, const double2* V,
-                      double2* Z, double2* P, double2* Pt, int nrhs);
+                      const double2* inv, const size_t* inv_off, const size_t* x_off, const int* r0, const int* body,
+                      const int* perm, int n, const double2* V, double2* Z, double2* P, double2* Pt, int nrhs);
 // refinement state[0] = converged, state[1] = correction steps: reset, then
 // after each residual test max_q |R_q|_max / |X_q|_max <= tol
 void refine_reset(hipStream_t s, int* state);

@@ -635,125 +635,15 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves, 32-row diagonal blocks.  Each launch solves one diagonal
-// block for a group of kTrsmRhs right-hand sides and updates the remaining
-// rows of that group:
-//   * diagonal solve, redundantly in every block of the group: a half-wave
-//     holds one right-hand side, lane r one row; at step c the row-c value is
-//     broadcast by a shuffle and rows below (lower, unit diagonal) or above
-//     (upper, divided by the diagonal first) subtract T(r, c) times it;
-//   * update: one thread per remaining row accumulates T(i, 0:kb) y(0:kb, q)
-//     for the group's right-hand sides.
-// X holds the right-hand sides (complex128); the factors are complex128 or
-// complex64 (mixed path: FP32 factors, FP64 right-hand sides).  Solved
-// diagonal rows go to Y, so no block reads a row another block writes.
+// Triangular solves.  X holds the right-hand sides (complex128); the factors
+// are complex128 or complex64 (mixed path: FP32 factors, FP64 right-hand
+// sides); arithmetic is FP64.  kTrsmRhs right-hand sides per workgroup.
 // ---------------------------------------------------------------------------
 constexpr int kTrsmRhs = 8;
-template <typename R, bool LOWER>
-__device__ __forceinline__ void trsm_block_body(const typename CT<R>::c2* __restrict__ LU, int n, int k0, int kb,
-                                                double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
-                                                int bx) {
-  __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
-  __shared__ double dr[kPanelNB], di[kPanelNB];                      // 1 / T(c, c) (upper)
-  __shared__ double yr[kPanelNB][kTrsmRhs], yi[kPanelNB][kTrsmRhs];  // solved block rows
-  const int t = threadIdx.x, q0 = blockIdx.y * kTrsmRhs;
-  const int nq = min(kTrsmRhs, nrhs - q0);
-  for (int e = t; e < kb * kb; e += 256) {
-    const int r = e % kb, c = e / kb;
-    auto v = LU[(size_t)(k0 + r) + (size_t)n * (k0 + c)];
-    tr[c][r] = (double)v.x;
-    ti[c][r] = (double)v.y;
-  }
-  __syncthreads();
-  if (!LOWER && t < kb) {  // reciprocal of the diagonal (Smith)
-    double a = tr[t][t], b = ti[t][t];
-    if (fabs(a) >= fabs(b)) {
-      double qq = b / a, dd = a + b * qq;
-      dr[t] = 1.0 / dd;
-      di[t] = -qq / dd;
-    } else {
-      double qq = a / b, dd = a * qq + b;
-      dr[t] = qq / dd;
-      di[t] = -1.0 / dd;
-    }
-  }
-  __syncthreads();
-  {
-    const int lane = t & 63, half = lane >> 5, r = lane & 31;
-    const int ql = (t >> 6) * 2 + half;  // right-hand side of this half-wave within the group
-    const bool live = ql < nq && r < kb;
-    double vr = 0.0, vi = 0.0;
-    if (live) {
-      double2 v = X[(size_t)(k0 + r) + (size_t)n * (q0 + ql)];
-      vr = v.x;
-      vi = v.y;
-    }
-    if (LOWER) {
-      for (int c = 0; c < kb; c++) {
-        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
-        if (r > c) {
-          const double a = tr[c][r], b = ti[c][r];
-          vr -= a * cr - b * ci;
-          vi -= a * ci + b * cr;
-        }
-      }
-    } else {
-      for (int c = kb - 1; c >= 0; c--) {
-        if (r == c) {
-          const double a = vr, b = vi;
-          vr = a * dr[c] - b * di[c];
-          vi = a * di[c] + b * dr[c];
-        }
-        const double cr = __shfl(vr, (half << 5) + c), ci = __shfl(vi, (half << 5) + c);
-        if (r < c) {
-          const double a = tr[c][r], b = ti[c][r];
-          vr -= a * cr - b * ci;
-          vi -= a * ci + b * cr;
-        }
-      }
-    }
-    if (ql < kTrsmRhs && r < kPanelNB) {
-      yr[r][ql] = live ? vr : 0.0;
-      yi[r][ql] = live ? vi : 0.0;
-    }
-    if (live && bx == 0) Y[(size_t)(k0 + r) + (size_t)n * (q0 + ql)] = make_double2(vr, vi);
-  }
-  __syncthreads();
-  // update the other rows: below the block (LOWER) or above it (upper)
-  const int i = LOWER ? k0 + kb + bx * 256 + t : bx * 256 + t;
-  if (LOWER ? i >= n : i >= k0) return;
-  double ar[kTrsmRhs], ai[kTrsmRhs];
-#pragma unroll
-  for (int q = 0; q < kTrsmRhs; q++) ar[q] = ai[q] = 0.0;
-  // issue all the row's factor loads before the arithmetic
-  typename CT<R>::c2 l[kPanelNB];
-#pragma unroll
-  for (int c = 0; c < kPanelNB; c++) l[c] = LU[(size_t)i + (size_t)n * (k0 + min(c, kb - 1))];
-#pragma unroll
-  for (int c = 0; c < kPanelNB; c++) {
-    if (c < kb) {
-      const double lr = (double)l[c].x, li = (double)l[c].y;
-#pragma unroll
-      for (int q = 0; q < kTrsmRhs; q++) {
-        ar[q] += lr * yr[c][q] - li * yi[c][q];
-        ai[q] += lr * yi[c][q] + li * yr[c][q];
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < kTrsmRhs; q++)
-    if (q < nq) {
-      double2 v = X[(size_t)i + (size_t)n * (q0 + q)];
-      v.x -= ar[q];
-      v.y -= ai[q];
-      X[(size_t)i + (size_t)n * (q0 + q)] = v;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Triangular solves of one factorisation, 256 rows per launch.  The 32 x 32
-// diagonal blocks of L (unit) and U are inverted once per solve
-// (k_tri_inverse, by the substitution above applied to the identity); a
+// One factorisation, 256 rows per launch.  The 32 x 32 diagonal blocks of L
+// (unit) and U are inverted once per solve (k_tri_inverse: substitution on
+// the identity, a half-wave per column, lane r holding row r); a
 // launch then takes a kSuper-row block of rows: its right-hand sides sit in
 // LDS, each 32-row diagonal block is solved as a small product with its
 // inverse and eliminated from the block's other rows, and every workgroup
@@ -769,12 +659,11 @@ static_assert(kSuper / kPanelNB * kPanelNB == kSuper && 256 / kPanelNB == kTrsmR
 // inv[(2 b + u) * 1024 + c * 32 + r] = (T_bb^-1)(r, c): u = 0 the unit lower
 // factor, u = 1 the upper factor; rows and columns past n are zero.
 template <typename R>
-__global__ __launch_bounds__(256) void k_tri_inverse(const typename CT<R>::c2* __restrict__ LU, int n,
-                                                     double2* __restrict__ inv, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+__device__ __forceinline__ void tri_inverse_body(const typename CT<R>::c2* __restrict__ LU, int n,
+                                                 double2* __restrict__ inv, int b, int upper) {
   __shared__ double tr[kPanelNB][kPanelNB], ti[kPanelNB][kPanelNB];  // [c][r]
   __shared__ double dr[kPanelNB], di[kPanelNB];
-  const int b = blockIdx.x, upper = blockIdx.y, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int k0 = b * kPanelNB, kb = min(kPanelNB, n - k0);
   for (int e = t; e < kPanelNB * kPanelNB; e += 256) {
     const int r = e % kPanelNB, c = e / kPanelNB;
@@ -834,17 +723,35 @@ __global__ __launch_bounds__(256) void k_tri_inverse(const typename CT<R>::c2* _
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_tri_inverse(const typename CT<R>::c2* __restrict__ LU, int n,
+                                                     double2* __restrict__ inv, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  tri_inverse_body<R>(LU, n, inv, blockIdx.x, blockIdx.y);
+}
+
+// The same for a batch of systems (blockIdx.z = system z of order nz[z],
+// factors at LU + lu_off[z], inverses at inv + inv_off[z]).
+template <typename R>
+__global__ __launch_bounds__(256) void k_tri_inverse_batched(const typename CT<R>::c2* __restrict__ LU,
+                                                             const size_t* __restrict__ lu_off,
+                                                             const int* __restrict__ nz,
+                                                             const size_t* __restrict__ inv_off,
+                                                             double2* __restrict__ inv) {
+  const int z = blockIdx.z, n = nz[z];
+  if ((int)blockIdx.x * kPanelNB >= n) return;
+  tri_inverse_body<R>(LU + lu_off[z], n, inv + inv_off[z], blockIdx.x, blockIdx.y);
+}
+
 // One kSuper-row block [k0, k0 + kb) for the right-hand sides of group
 // blockIdx.y: solves it (LOWER: forward, unit L; else backward, U) and
 // updates this workgroup's 256 rows outside it (below it, or above it).
 // The block's solution goes to Y (workgroup 0 of the group); the updated
 // rows stay in X, which no workgroup reads inside the block.
 template <typename R, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __restrict__ LU,
-                                                    const double2* __restrict__ inv, int n, int k0, int kb,
-                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
-                                                    const int* __restrict__ skip) {
-  if (skip && *skip) return;
+__device__ __forceinline__ void trsm_super_body(const typename CT<R>::c2* __restrict__ LU,
+                                                const double2* __restrict__ inv, int n, int k0, int kb,
+                                                double2* __restrict__ X, double2* __restrict__ Y, int nrhs, int bx) {
   // [q][row]: a thread's own row is contiguous across lanes, a shared row is
   // one broadcast address
   __shared__ double2 xs[kTrsmRhs][kSuper];        // the block's right-hand sides, then its solution
@@ -879,7 +786,7 @@ __global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __
     __syncthreads();  // every read of the block's old values is done
     if (rr < sb) {
       xs[qq][r0 + rr] = make_double2(yr, yi);
-      if (blockIdx.x == 0 && qq < nq) Y[(size_t)(k0 + r0 + rr) + (size_t)n * (q0 + qq)] = make_double2(yr, yi);
+      if (bx == 0 && qq < nq) Y[(size_t)(k0 + r0 + rr) + (size_t)n * (q0 + qq)] = make_double2(yr, yi);
     }
     __syncthreads();
     // eliminate the solved rows from the block's rows still to come
@@ -906,7 +813,7 @@ __global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __
   }
   // this workgroup's rows outside the block, the factor row in chunks of 32
   // columns with each chunk's loads issued before its arithmetic
-  const int i = LOWER ? k0 + kb + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
+  const int i = LOWER ? k0 + kb + bx * 256 + t : bx * 256 + t;
   if (LOWER ? i >= n : i >= k0) return;
   double ar[kTrsmRhs], ai[kTrsmRhs];
 #pragma unroll
@@ -939,22 +846,31 @@ __global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __
     }
 }
 
-// The same step for a batch of independent systems (blockIdx.z = system z
-// of order nz[z], factors at LU + lu_off[z], right-hand sides at
-// X + x_off[z] with leading dimension nz[z]).  The grid is sized for the
-// largest system; systems that have no block at k0 return, and blocks past
-// a system's rows return inside the step.
 template <typename R, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsm_block_batched(const typename CT<R>::c2* __restrict__ LU,
+__global__ __launch_bounds__(256) void k_trsm_super(const typename CT<R>::c2* __restrict__ LU,
+                                                    const double2* __restrict__ inv, int n, int k0, int kb,
+                                                    double2* __restrict__ X, double2* __restrict__ Y, int nrhs,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  trsm_super_body<R, LOWER>(LU, inv, n, k0, kb, X, Y, nrhs, blockIdx.x);
+}
+
+// The same for a batch of systems (as k_tri_inverse_batched, right-hand sides
+// at X + x_off[z] with leading dimension nz[z]); systems with no block at k0
+// return.
+template <typename R, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_super_batched(const typename CT<R>::c2* __restrict__ LU,
                                                             const size_t* __restrict__ lu_off,
                                                             const int* __restrict__ nz,
+                                                            const size_t* __restrict__ inv_off,
+                                                            const double2* __restrict__ inv,
                                                             const size_t* __restrict__ x_off, int k0,
                                                             double2* __restrict__ X, double2* __restrict__ Y,
                                                             int nrhs) {
   const int z = blockIdx.z, n = nz[z];
   if (k0 >= n) return;
-  trsm_block_body<R, LOWER>(LU + lu_off[z], n, k0, min(kPanelNB, n - k0), X + x_off[z], Y + x_off[z], nrhs,
-                            blockIdx.x);
+  trsm_super_body<R, LOWER>(LU + lu_off[z], inv + inv_off[z], n, k0, min(kSuper, n - k0), X + x_off[z],
+                            Y + x_off[z], nrhs, blockIdx.x);
 }
 
 __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
@@ -1202,25 +1118,30 @@ void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, doubl
                   int nrhs, const int* skip) {
   lu_solve_t<float>(s, LU, n, perm, X, Xtmp, inv, nrhs, skip);
 }
+void tri_inverse_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
+                         const size_t* inv_off, double2* inv) {
+  HX_LAUNCH(k_tri_inverse_batched<double>, dim3((nmax + kPanelNB - 1) / kPanelNB, 2, nsys), dim3(256), 0, s, LU,
+            lu_off, nz, inv_off, inv);
+}
 void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, const size_t* lu_off, const int* nz,
-                      const size_t* x_off, const int* r0, const int* body, const int* perm, int n, const double2* V,
-                      double2* Z, double2* P, double2* Pt, int nrhs) {
+                      const double2* inv, const size_t* inv_off, const size_t* x_off, const int* r0, const int* body,
+                      const int* perm, int n, const double2* V, double2* Z, double2* P, double2* Pt, int nrhs) {
   const size_t tot = (size_t)n * nrhs;
   const dim3 g1((unsigned)((tot + 255) / 256));
   const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
   HX_LAUNCH(k_pack_systems, g1, dim3(256), 0, s, V, n, nrhs, body, r0, P);
   HX_LAUNCH(k_gather_systems, g1, dim3(256), 0, s, P, n, nrhs, body, r0, perm, Pt);
   // forward pass: rows updated in Pt, solved rows into P; backward: updated in P, solved into Pt
-  for (int k0 = 0; k0 < nmax; k0 += kPanelNB) {
-    const int rest = nmax - k0 - min(kPanelNB, nmax - k0);
+  for (int k0 = 0; k0 < nmax; k0 += kSuper) {
+    const int rest = nmax - k0 - min(kSuper, nmax - k0);
     const int g = rest > 0 ? (rest + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block_batched<double, true>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz, x_off,
-              k0, Pt, P, nrhs);
+    HX_LAUNCH((k_trsm_super_batched<double, true>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz,
+              inv_off, inv, x_off, k0, Pt, P, nrhs);
   }
-  for (int k0 = ((nmax - 1) / kPanelNB) * kPanelNB; k0 >= 0; k0 -= kPanelNB) {
+  for (int k0 = ((nmax - 1) / kSuper) * kSuper; k0 >= 0; k0 -= kSuper) {
     const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
-    HX_LAUNCH((k_trsm_block_batched<double, false>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz, x_off,
-              k0, P, Pt, nrhs);
+    HX_LAUNCH((k_trsm_super_batched<double, false>), dim3(g, groups, nsys), dim3(256), 0, s, LU, lu_off, nz,
+              inv_off, inv, x_off, k0, P, Pt, nrhs);
   }
   HX_LAUNCH(k_unpack_systems, g1, dim3(256), 0, s, Pt, n, nrhs, body, r0, Z);
 }
