@@ -21,6 +21,8 @@ the LU's rounding: look-ahead is bit-identical, and selection-block size and
 outer blocks give equally valid pivoted factorisations.
 
 Run on the GPU:  python -m hamsx.autotune [--out PATH] [--quick]
+(with eight hardware queues, GPU_MAX_HW_QUEUES, as bench.py runs: four
+contexts of two streams each)
 
 This has no counterpart in the reference, whose ZGETRF (MKL) has no such
 choices (src/AssbMatxMulti.f90:110-114).
@@ -147,6 +149,11 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE)
     ap.add_argument("--quick", action="store_true", help="two buckets only (a smoke run of the tuner)")
     args = ap.parse_args(argv)
+    # two HIP streams per context (its own and the LU look-ahead's): as many
+    # hardware queues, raised before the runtime starts, as bench.py and the
+    # serving bench run with
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     buckets = (1000, 4000) if args.quick else BUCKETS
     t0 = time.perf_counter()
     table = tune(buckets=buckets, log=lambda s: print(s, flush=True))
