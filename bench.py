@@ -171,8 +171,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves per context before timing")
-    ap.add_argument("--streams", type=int, default=4, help="frequency solves kept in flight (one context and HIP "
-                    "stream each)")
+    ap.add_argument("--streams", type=int, default=None, help="frequency solves kept in flight (one context and HIP "
+                    "stream each); default: the group size, or 4 slots for serving")
+    ap.add_argument("--group", type=int, default=16, help="contexts per frequency group (one batched factorisation "
+                    "for the group's matrices); 1: independent submissions")
     ap.add_argument("--stage-steps", type=int, default=3, help="isolated solves timed per stage for the roofline")
     ap.add_argument("--config", default="two-cylinders",
                     choices=["two-cylinders", "two-cylinders-half", "five-cylinders", "one-cylinder", "serving",
@@ -193,6 +195,8 @@ def parse():
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.streams is None:
+        args.streams = 4 if args.config == "serving" or args.group <= 1 else args.group
     args.precision_set = args.precision is not None
     if args.precision is None:
         args.precision = "fp64"
@@ -533,15 +537,19 @@ def main():
     print(f"[bench] {case.name}: N={n} panels, depth {case.depth}, {len(case.omegas)} frequencies, "
           f"{len(case.headings_deg)} headings, precision {args.precision}, {max(1, args.streams)} solves in flight",
           file=sys.stderr, flush=True)
+    group = args.group if args.group > 1 and "symmetry" not in case.meta else 1
     run = throughput(case, mesh, args.precision, args.streams, args.steps, args.warmup, args.stage_steps,
-                     graphs=args.graphs)
+                     graphs=args.graphs, group=group)
     ms, elapsed, lat_ms = run["stages"], run["elapsed"], run["lat_ms"]
     print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
     roof = roofline(case, mesh, ms, args.precision)
     lu_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
-    lu_tf = 8.0 * n ** 3 / 3.0 / (ms["factor"] * 1e-3) / 1e12
-    roof_lu = {"kernel": "LU factorisation", "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
-               "frac": lu_tf / lu_peak, "launch_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
+    lu_ms = ms.get("factor_group", ms["factor"])
+    lu_tf = 8.0 * n ** 3 / 3.0 / (lu_ms * 1e-3) / 1e12
+    roof_lu = {"kernel": "LU factorisation" + (f" (one batched launch sequence for a group of {group} matrices; "
+                                               "time per matrix)" if group > 1 else ""),
+               "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
+               "frac": lu_tf / lu_peak, "launch_ms": lu_ms, "single_matrix_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
     roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
     vs_full = None
     if "symmetry" in case.meta:
@@ -555,7 +563,7 @@ def main():
         "config": {"workload": case.name, "panels": n, "symmetry": case.meta.get("symmetry"),
                    "depth_m": case.depth, "frequencies": len(case.omegas),
                    "headings_deg": case.headings_deg, "precision": args.precision,
-                   "in_flight": max(1, args.streams), "graphs": args.graphs},
+                   "in_flight": max(1, args.streams), "group": group, "graphs": args.graphs},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
         "roofline": roof,
@@ -567,16 +575,16 @@ def main():
     if args.precision == "fp64" and not args.no_mixed and args.config in ("two-cylinders", "five-cylinders"):
         # the headline: the mixed-precision path on the same workload, and both
         # paths on the five-cylinder (10,026-panel) workload
-        out["mixed"] = mixed_vs_fp64(case, mesh, run, args.streams, args.steps, args.warmup)
+        out["mixed"] = mixed_vs_fp64(case, mesh, run, args.streams, args.steps, args.warmup, group)
         if args.config == "two-cylinders":
             c5 = pick_case("five-cylinders")
             m5 = combine(c5.bodies, c5.xr)
             steps5 = max(4, args.steps // 2)
             print(f"[bench] {c5.name}: N={m5.nelem}, fp64 and mixed, {steps5} solves each", file=sys.stderr, flush=True)
-            r5 = throughput(c5, m5, "fp64", args.streams, steps5, args.warmup, 1, graphs=args.graphs)
+            r5 = throughput(c5, m5, "fp64", args.streams, steps5, args.warmup, 1, graphs=args.graphs, group=group)
             out["five_cylinders"] = {"workload": c5.name, "panels": m5.nelem, "depth_m": c5.depth,
                                      "fp64": summary(r5, steps5),
-                                     "mixed": mixed_vs_fp64(c5, m5, r5, args.streams, steps5, args.warmup)}
+                                     "mixed": mixed_vs_fp64(c5, m5, r5, args.streams, steps5, args.warmup, group)}
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] CPU baseline on {threads} threads", file=sys.stderr, flush=True)
@@ -584,11 +592,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graphs=False):
+def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graphs=False, group=1):
     """`steps` frequency solves of case.omegas (in order, cycling) with
     `streams` solves in flight (one context and HIP stream each), after
     per-stage timing of `stage_steps` isolated solves and `warmup` untimed
-    solves per context.  Returns the stage times, the wall time, per-solve
+    solves per context.  With group > 1 the contexts are submitted in
+    frequency groups of `group` (hamsx_submit_frequency_group: one batched
+    factorisation per group), and the factorisation's time per solve in a
+    group is timed too.  Returns the stage times, the wall time, per-solve
     latencies (submit to collect) and every solve's coefficients in order."""
     from hamsx.solver import FrequencySolver
     heads = np.deg2rad(case.headings_deg)
@@ -610,6 +621,8 @@ def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graph
         solvers[0].solve_frequency(omegas[(k * 7) % len(omegas)], case.depth)
     stages = solvers[0].stage_times(reset=True)
     solvers[0].set_profiling(False)
+    if group > 1:
+        return _throughput_groups(case, solvers, omegas, steps, warmup, stage_steps, group, stages)
     for k, sv in enumerate(solvers):
         for _ in range(warmup):  # through the timed path (the first submit at a depth runs eagerly)
             sv.submit(omegas[k % len(omegas)], case.depth)
@@ -644,6 +657,69 @@ def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graph
             "lat_ms": np.array(lat) * 1e3, "coef": coef}
 
 
+def _throughput_groups(case, solvers, omegas, steps, warmup, stage_steps, group, stages):
+    """The timed loop of throughput() for frequency groups: each group of
+    contexts is submitted with one call and collected context by context;
+    a finished group takes the next frequencies (the last one fewer if fewer
+    remain)."""
+    from hamsx.solver import FrequencySolver
+    groups = [solvers[i:i + group] for i in range(0, len(solvers), group)]
+    for _ in range(warmup):
+        for g in groups:
+            FrequencySolver.submit_group(g, [omegas[k % len(omegas)] for k in range(len(g))], case.depth)
+            for sv in g:
+                sv.collect()
+    # the batched factorisation's device time, per solve of a full group
+    g0 = groups[0]
+    g0[0].set_profiling(True)
+    g0[0].stage_times(reset=True)
+    for k in range(stage_steps):
+        FrequencySolver.submit_group(g0, [omegas[(k * 7 + i) % len(omegas)] for i in range(len(g0))], case.depth)
+        for sv in g0:
+            sv.collect()
+    st = g0[0].stage_times(reset=True)
+    g0[0].set_profiling(False)
+    f_ms, f_cnt = st.get("factor", (0.0, 0))
+    stages["factor_group"] = (f_ms / max(1, f_cnt) / len(g0), 1)
+    lat, coef = [], [None] * steps
+    t_sub = {}
+    owner = {}
+    submitted = done = 0
+    queue = []
+    t0 = time.perf_counter()
+
+    def submit(gi):
+        nonlocal submitted
+        g = groups[gi][:min(len(groups[gi]), steps - submitted)]
+        now = time.perf_counter()
+        for i, sv in enumerate(g):
+            t_sub[id(sv)] = now
+            owner[id(sv)] = submitted + i
+        FrequencySolver.submit_group(g, [omegas[(submitted + i) % len(omegas)] for i in range(len(g))], case.depth)
+        submitted += len(g)
+        queue.append(g)
+
+    for gi in range(len(groups)):
+        if submitted < steps:
+            submit(gi)
+    while queue:
+        g = queue.pop(0)
+        for sv in g:
+            am, bd, ex, info = sv.collect()
+            lat.append(time.perf_counter() - t_sub[id(sv)])
+            if info != 0:
+                raise SystemExit(f"zero pivot in solve {owner[id(sv)]}: INFO={info}")
+            coef[owner[id(sv)]] = (am, bd, ex)
+            done += 1
+        if submitted < steps:
+            submit(next(gi for gi, gg in enumerate(groups) if gg[0] is g[0]))
+    elapsed = time.perf_counter() - t0
+    for sv in solvers:
+        sv.close()
+    return {"stages": {k: v[0] / max(1, v[1]) for k, v in stages.items()}, "elapsed": elapsed,
+            "lat_ms": np.array(lat) * 1e3, "coef": coef}
+
+
 def summary(run, steps):
     return {"value": steps / run["elapsed"], "unit": "freq-solves/s",
             "latency_ms": {"p50": float(np.percentile(run["lat_ms"], 50)),
@@ -651,7 +727,7 @@ def summary(run, steps):
             "stage_ms_per_solve": {k: v for k, v in run["stages"].items() if v > 0}}
 
 
-def mixed_vs_fp64(case, mesh, fp64_run, streams, steps, warmup):
+def mixed_vs_fp64(case, mesh, fp64_run, streams, steps, warmup, group=1):
     """The mixed-precision path (complex64 LU on MFMA + FP64 refinement) on the
     same workload and solve sequence as fp64_run: its throughput, latency, the
     ratio to the FP64 path, and the largest difference of every coefficient
@@ -659,7 +735,7 @@ def mixed_vs_fp64(case, mesh, fp64_run, streams, steps, warmup):
     entry at that frequency, and per coefficient against its own magnitude
     for those above 1e-3 of the largest)."""
     print(f"[bench] {case.name}: mixed precision, {steps} solves", file=sys.stderr, flush=True)
-    mx = throughput(case, mesh, "mixed", streams, steps, warmup, 1)
+    mx = throughput(case, mesh, "mixed", streams, steps, warmup, 1, group=group)
     out = summary(mx, steps)
     out["ratio_vs_fp64"] = out["value"] / (steps / fp64_run["elapsed"])
     err = {"amas": 0.0, "bdmp": 0.0, "exfc": 0.0}

@@ -314,6 +314,9 @@ struct hamsx_ctx {
   // hamsx_set_symmetry
   hipStream_t sym_stream = nullptr;
   hipEvent_t sym_go = nullptr, sym_done = nullptr;
+  // a frequency group (hamsx_submit_frequency_group): this context's
+  // matrix is ready / the group's factorisation is done
+  hipEvent_t grp_ev = nullptr;
   DevBuf<int> widx2, wcnt2;
   DevBuf<double2> ludiag2;
   // field points (CalPotentialMulti): nfp points (SoA); vtx: panel vertices,
@@ -707,9 +710,9 @@ double2* system_matrix(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm.p : c->A.p; }
 // symmetry system 2's: A2 (in T2), or C2 = A2^T A2 with a lid
 double2* system_matrix2(hamsx_ctx* c) { return c->nlid > 0 ? c->Cm2.p : c->T2.p; }
 
-int do_factor(hamsx_ctx* c) {
-  if (c->have_LU) return 0;
-  if (!c->have_A) return fail(HAMSX_ESTATE, "hamsx_factor needs hamsx_assemble first");
+// The factorisation's buffers: pivots, permutation, tournament workspace,
+// diagonal-block scratch (and the solves' workspace).
+int factor_buffers(hamsx_ctx* c) {
   int rc = ensure_work(c);
   if (rc) return rc;
   const int n = c->n;
@@ -723,6 +726,15 @@ int do_factor(hamsx_ctx* c) {
                    (rc = c->wcnt2.ensure((size_t)2 * nsets0)) ||
                    (rc = c->ludiag2.ensure((size_t)hamsx::kPanelNB * hamsx::kPanelNB))))
     return rc;
+  return 0;
+}
+
+int do_factor(hamsx_ctx* c) {
+  if (c->have_LU) return 0;
+  if (!c->have_A) return fail(HAMSX_ESTATE, "hamsx_factor needs hamsx_assemble first");
+  int rc = factor_buffers(c);
+  if (rc) return rc;
+  const int n = c->n;
   StageTimer tm(c, 2);
   if (c->sym()) {
     // the two systems A +- T2, or their normal matrices (INFO of system 2 in
@@ -1246,6 +1258,8 @@ int refine_solve(hamsx_ctx* c, const double2* M, const float2* Af, const int* pe
 }
 
 // X <- A^{-1} X for all 6nb + nbeta columns (with refinement in mixed mode)
+int solve_factored(hamsx_ctx* c);
+
 int do_solve(hamsx_ctx* c) {
   if (c->have_X) return 0;
   int rc = do_rhs(c);
@@ -1265,6 +1279,13 @@ int do_solve(hamsx_ctx* c) {
     return 0;
   }
   if ((rc = do_factor(c))) return rc;
+  return solve_factored(c);
+}
+
+// The triangular solves (FP64) or the refined complex64 solves (mixed) of
+// the factored system, into X.
+int solve_factored(hamsx_ctx* c) {
+  int rc = 0;
   const int n = c->n, nr = c->nrhs();
   StageTimer tm(c, 3);
   if (c->prec == HAMSX_PREC_FP64) {
@@ -1471,7 +1492,7 @@ int hamsx_destroy(hamsx_ctx* c) {
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   if (c->sym_stream) (void)hipStreamSynchronize(c->sym_stream);
-  for (hipEvent_t ev : {c->sym_go, c->sym_done})
+  for (hipEvent_t ev : {c->sym_go, c->sym_done, c->grp_ev})
     if (ev) (void)hipEventDestroy(ev);
   if (c->sym_stream) (void)hipStreamDestroy(c->sym_stream);
   if (c->lu_aux.s2) (void)hipStreamSynchronize(c->lu_aux.s2);
@@ -2132,6 +2153,71 @@ int hamsx_submit_frequency(hamsx_ctx* c, double w1, double h) {
   if ((rc = enqueue_solve(c))) return rc;
   if (c->queue) HX_TRY(hipLaunchHostFunc(c->stream, notify_done, c));
   c->pending = true;
+  return 0;
+}
+
+static_assert(HAMSX_GROUP_MAX == hamsx::kLuBatchMax, "a frequency group is one batched factorisation");
+
+int hamsx_submit_frequency_group(hamsx_ctx** ctx, int count, const double* w1, double h) {
+  if (!ctx || !w1) return fail(HAMSX_EARG, "ctx or w1 is NULL");
+  if (count < 1 || count > hamsx::kLuBatchMax)
+    return fail(HAMSX_EARG, "a frequency group holds 1 to %d contexts", hamsx::kLuBatchMax);
+  hamsx_ctx* c0 = ctx[0];
+  for (int i = 0; i < count; i++) {
+    hamsx_ctx* c = ctx[i];
+    if (!c) return fail(HAMSX_EARG, "context %d of the group is NULL", i);
+    for (int j = 0; j < i; j++)
+      if (ctx[j] == c) return fail(HAMSX_EARG, "context %d appears twice in the group", i);
+    if (c->pending) return busy(c);
+    if (!c->have_waves) return fail(HAMSX_ESTATE, "hamsx_submit_frequency_group needs hamsx_set_waves first");
+    if (c->device != c0->device || c->n != c0->n || c->prec != c0->prec)
+      return fail(HAMSX_EARG, "the contexts of a group share one device, one mesh size and one precision");
+    if (c->solver != HAMSX_SOLVER_LU || c->sym() || c->nlid > 0)
+      return fail(HAMSX_EARG, "a frequency group takes the dense LU path (no GMRES, lid or symmetry plane)");
+  }
+  const bool c64 = c0->prec == HAMSX_PREC_MIXED;
+  const int n = c0->n;
+  // each context: its frequency, assembly and right-hand sides on its own stream
+  for (int i = 0; i < count; i++) {
+    hamsx_ctx* c = ctx[i];
+    double v, wk, w, wvn[HAMSX_NK];
+    int rc = hamsx_wave_properties(w1[i] == 0.0 || w1[i] == -1.0 ? HAMSX_FREQ_LIMIT : 3, w1[i], h, &v, &wk, &w, wvn);
+    if (rc || (rc = hamsx_set_frequency(c, v, wk, w, h, wvn, HAMSX_NK)) || (rc = hamsx_assemble(c)) ||
+        (rc = do_rhs(c)) || (rc = factor_buffers(c)))
+      return rc;
+    if (!c->grp_ev) HX_TRY(hipEventCreateWithFlags(&c->grp_ev, hipEventDisableTiming));
+    if (c64) hamsx::launch::to_c64(c->stream, system_matrix(c), c->Af.p, (size_t)n * n);
+    if ((rc = check_launch())) return rc;
+    HX_TRY(hipEventRecord(c->grp_ev, c->stream));
+  }
+  // one factorisation launch sequence for the whole group, on the first context's stream
+  int rc = set_device(c0);
+  if (rc) return rc;
+  for (int i = 1; i < count; i++) HX_TRY(hipStreamWaitEvent(c0->stream, ctx[i]->grp_ev, 0));
+  hamsx::LuBatch b{};
+  for (int i = 0; i < count; i++) {
+    hamsx_ctx* c = ctx[i];
+    b.m[i] = hamsx::LuMat{c64 ? static_cast<void*>(c->Af.p) : static_cast<void*>(system_matrix(c)),
+                          c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->ludiag.p, c->flags.p + 1};
+  }
+  const hamsx::LuAux o = lu_options(c0);
+  {
+    StageTimer tm(c0, 2);
+    hamsx::launch::lu_factor_batch(c0->stream, b, count, n, c64, &o);
+  }
+  if ((rc = check_launch())) return rc;
+  HX_TRY(hipEventRecord(c0->grp_ev, c0->stream));
+  // each context: its solves and force reductions on its own stream
+  for (int i = 0; i < count; i++) {
+    hamsx_ctx* c = ctx[i];
+    if ((rc = set_device(c))) return rc;
+    if (i > 0) HX_TRY(hipStreamWaitEvent(c->stream, c0->grp_ev, 0));
+    c->have_LU = true;
+    if (!c64) c->have_A = false;  // factored in place
+    if ((rc = solve_factored(c)) || (rc = enqueue_forces(c))) return rc;
+    if (c->queue) HX_TRY(hipLaunchHostFunc(c->stream, notify_done, c));
+    c->pending = true;
+  }
   return 0;
 }
 

@@ -138,6 +138,26 @@ struct LuAux {
   int outer_min_n = 6144;
 };
 
+// One matrix of a batched factorisation: the n x n matrix (complex128 or
+// complex64), its pivots, permutation, tournament workspace (2 nsets0 kPanelNB
+// + 2 nsets0 ints, nsets0 = ceil(n / kSelRows)), diagonal-block scratch and
+// INFO.  Every matrix of a batch has the same order, so one launch sequence
+// factors them all (grid.y = matrix); the descriptors travel in the kernel
+// arguments, so a batch needs no device memory of its own.
+constexpr int kLuBatchMax = 16;
+struct LuMat {
+  void* A;
+  int* ipiv;
+  int* perm;
+  int* widx;
+  int* wcnt;
+  void* diag;
+  int* info;
+};
+struct LuBatch {
+  LuMat m[kLuBatchMax];
+};
+
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
@@ -210,6 +230,10 @@ void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
 void mirror_lower(hipStream_t s, double2* C, int n);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,
                    double2* diag, const LuAux* aux = nullptr);
+// count <= kLuBatchMax matrices of order n factored by one launch sequence
+// (complex128, or complex64 with c64); each matrix's factors, pivots and INFO
+// are those lu_factor / lu_factor_c64 give it alone, bit for bit.
+void lu_factor_batch(hipStream_t s, const LuBatch& b, int count, int n, bool c64, const LuAux* aux = nullptr);
 // skip (may be null, here and below): a device flag that turns every
 // launch into a no-op when set (refinement after convergence)
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,

@@ -95,13 +95,22 @@ constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one f
 // 5.1 solves/s (the far trailing matrix is read and written once per block).
 constexpr int kOuterMinN = 6144;
 static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
+// Batched: matrix blockIdx.y of bat; the tournament's candidate lists
+// alternate between two buffers of wsets sets in the matrix's workspace
+// (in_buf < 0: level 0, rows taken from the matrix itself).
 template <typename R, bool FINAL, int ROWS>
-__global__ __launch_bounds__(ROWS) void k_select(const typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
-                                                int ncols, int row0, const int* __restrict__ idx_in,
-                                                const int* __restrict__ cnt_in, int nsets_in, int* __restrict__ idx_out,
-                                                int* __restrict__ cnt_out, typename CT<R>::c2* __restrict__ diag,
-                                                int* __restrict__ info) {
+__global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int col0, int ncols, int row0, int in_buf,
+                                                int nsets_in, int out_buf, int wsets) {
   constexpr int kMerge = ROWS / kPanelNB, kSelWaves = ROWS / 64;
+  const LuMat& mat = bat.m[blockIdx.y];
+  const typename CT<R>::c2* __restrict__ A = static_cast<const typename CT<R>::c2*>(mat.A);
+  const int lda = n;
+  const int* __restrict__ idx_in = in_buf < 0 ? nullptr : mat.widx + (size_t)in_buf * wsets * kPanelNB;
+  const int* __restrict__ cnt_in = in_buf < 0 ? nullptr : mat.wcnt + (size_t)in_buf * wsets;
+  int* __restrict__ idx_out = mat.widx + (size_t)out_buf * wsets * kPanelNB;
+  int* __restrict__ cnt_out = mat.wcnt + (size_t)out_buf * wsets;
+  typename CT<R>::c2* __restrict__ diag = static_cast<typename CT<R>::c2*>(mat.diag);
+  int* __restrict__ info = mat.info;
   __shared__ int sidx[ROWS];
   __shared__ int off[kMerge + 1];
   __shared__ double wmax[2][kSelWaves];
@@ -228,8 +237,11 @@ __global__ __launch_bounds__(ROWS) void k_select(const typename CT<R>::c2* __res
 
 // Turn the final pivot-row list into LAPACK-style sequential swaps
 // ipiv[col0 + c] (1-based) and record the first zero pivot in info.
-__global__ void k_pivots_to_swaps(const int* __restrict__ winners, int nsel, int col0, int* __restrict__ ipiv) {
+__global__ void k_pivots_to_swaps(const LuBatch bat, int buf, int wsets, int nsel, int col0) {
   if (threadIdx.x != 0) return;
+  const LuMat& mat = bat.m[blockIdx.y];
+  const int* __restrict__ winners = mat.widx + (size_t)buf * wsets * kPanelNB;
+  int* __restrict__ ipiv = mat.ipiv;
   int who[2 * kPanelNB], pos[2 * kPanelNB], nt = 0;  // rows whose position changed
   for (int c = 0; c < nsel; c++) {
     int w = winners[c], p = w;
@@ -272,8 +284,12 @@ __global__ void k_pivots_to_swaps(const int* __restrict__ winners, int nsel, int
 
 // Apply the panel's swaps to every column (and to the permutation vector).
 template <typename R>
-__global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0, int nsel,
-                            const int* __restrict__ ipiv, int* __restrict__ perm) {
+__global__ void k_swap_rows(const LuBatch bat, int n, int col0, int nsel) {
+  const LuMat& mat = bat.m[blockIdx.y];
+  typename CT<R>::c2* __restrict__ A = static_cast<typename CT<R>::c2*>(mat.A);
+  const int lda = n;
+  const int* __restrict__ ipiv = mat.ipiv;
+  int* __restrict__ perm = mat.perm;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j > n) return;
   for (int c = 0; c < nsel; c++) {
@@ -291,16 +307,21 @@ __global__ void k_swap_rows(typename CT<R>::c2* __restrict__ A, int lda, int n, 
   }
 }
 
-// With the diagonal block factored (diag, from the final tournament level):
-// L21 = A21 U11^-1 (blocks [0, nrb), one row per thread) and U12 = L11^-1 A12
-// for the columns [uc0, uc1) (blocks [nrb, ..), one column per thread).  The
-// factored diagonal block is read from diag (leading dimension ldd); with
-// store set, block 0 also stores it into A.  No block reads what another
-// writes.
+// With the diagonal block factored: L21 = A21 U11^-1 (blocks [0, nrb), one
+// row per thread) and U12 = L11^-1 A12 for the columns [uc0, uc1) (blocks
+// [nrb, ..), one column per thread).  The factored diagonal block is read
+// from the matrix's diagonal scratch (from the final tournament level), or
+// with diag_in_A from A itself at (col0, col0); with store set, block 0 also
+// stores it into A.  No block reads what another writes.
 template <typename R>
-__global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restrict__ A, int lda, int n, int col0,
-                                                     int nb, int nrb, const typename CT<R>::c2* __restrict__ diag,
-                                                     int ldd, int store, int uc0, int uc1) {
+__global__ __launch_bounds__(256) void k_panel_solve(const LuBatch bat, int n, int col0, int nb, int nrb,
+                                                     int diag_in_A, int store, int uc0, int uc1) {
+  const LuMat& mat = bat.m[blockIdx.y];
+  typename CT<R>::c2* __restrict__ A = static_cast<typename CT<R>::c2*>(mat.A);
+  const int lda = n;
+  const typename CT<R>::c2* __restrict__ diag =
+      diag_in_A ? A + (size_t)col0 + (size_t)n * col0 : static_cast<const typename CT<R>::c2*>(mat.diag);
+  const int ldd = diag_in_A ? n : kPanelNB;
   __shared__ R dr[kPanelNB][kPanelNB + 1];
   __shared__ R di[kPanelNB][kPanelNB + 1];
   __shared__ R rr[kPanelNB], ri[kPanelNB];  // reciprocals of the pivots
@@ -388,10 +409,10 @@ __global__ __launch_bounds__(256) void k_panel_solve(typename CT<R>::c2* __restr
 // imaginary accumulators).  Block ids are remapped so that each of the eight
 // XCDs works on a contiguous band of tiles (L21 rows stay in its L2).
 template <typename R>
-__global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __restrict__ A, int lda, int ib, int ie,
-                                                         int jb, int je, int kb, int kk, int tiles_m, int ntiles,
-                                                         int tn0) {
+__global__ __launch_bounds__(256) void k_trailing_update(const LuBatch bat, int lda, int ib, int ie, int jb, int je,
+                                                         int kb, int kk, int tiles_m, int ntiles, int tn0) {
   using v4 = typename CT<R>::v4;
+  typename CT<R>::c2* __restrict__ A = static_cast<typename CT<R>::c2*>(bat.m[blockIdx.y].A);
   __shared__ R lr_[kPanelNB][64], li_[kPanelNB][64];  // L tile: [k][row]
   __shared__ R ur_[kPanelNB][64], ui_[kPanelNB][64];  // U tile: [k][col]
   int bid = blockIdx.x;
@@ -491,26 +512,25 @@ __global__ __launch_bounds__(256) void k_trailing_update(typename CT<R>::c2* __r
       }
 }
 
-__global__ void k_lu_init(int* __restrict__ perm, int n, int* __restrict__ info) {
+__global__ void k_lu_init(const LuBatch bat, int n) {
+  int* __restrict__ perm = bat.m[blockIdx.y].perm;
+  int* __restrict__ info = bat.m[blockIdx.y].info;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) perm[i] = i;
   if (i == 0) info[0] = 0;
 }
 
-// One tournament level: `blocks` selection blocks of `rows` threads.
+// One tournament level: `blocks` selection blocks of `rows` threads per matrix.
 template <typename R>
-static void select_level(hipStream_t s, int rows, bool final, int blocks, const typename CT<R>::c2* A, int n,
-                         int col0, int nb, const int* idx_in, const int* cnt_in, int nsets_in, int* idx_out,
-                         int* cnt_out, typename CT<R>::c2* diag, int* info) {
-  using c2 = typename CT<R>::c2;
-  c2* d = final ? diag : nullptr;
+static void select_level(hipStream_t s, int rows, bool final, int blocks, const LuBatch& bat, int nbat, int n,
+                         int col0, int nb, int in_buf, int nsets_in, int out_buf, int wsets) {
 #define HX_SEL(ROWS)                                                                                              \
   if (final)                                                                                                     \
-    HX_LAUNCH((k_select<R, true, ROWS>), dim3(blocks), dim3(ROWS), 0, s, A, n, n, col0, nb, col0, idx_in, cnt_in, \
-              nsets_in, idx_out, cnt_out, d, info);                                                              \
+    HX_LAUNCH((k_select<R, true, ROWS>), dim3(blocks, nbat), dim3(ROWS), 0, s, bat, n, col0, nb, col0, in_buf,    \
+              nsets_in, out_buf, wsets);                                                                         \
   else                                                                                                           \
-    HX_LAUNCH((k_select<R, false, ROWS>), dim3(blocks), dim3(ROWS), 0, s, A, n, n, col0, nb, col0, idx_in,        \
-              cnt_in, nsets_in, idx_out, cnt_out, d, info);
+    HX_LAUNCH((k_select<R, false, ROWS>), dim3(blocks, nbat), dim3(ROWS), 0, s, bat, n, col0, nb, col0, in_buf,   \
+              nsets_in, out_buf, wsets);
   if (rows == 1024) {
     HX_SEL(1024)
   } else if (rows == 512) {
@@ -521,10 +541,13 @@ static void select_level(hipStream_t s, int rows, bool final, int blocks, const 
 #undef HX_SEL
 }
 
+// Factors the nbat matrices of bat (all of order n) with one launch sequence:
+// every launch covers all of them (grid.y = matrix), so each step's latency
+// chain serves the whole batch.  Each matrix gets exactly the operations of a
+// factorisation on its own.
 template <typename R>
-static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt,
-                        int* info, typename CT<R>::c2* diag, const LuAux* aux) {
-  HX_LAUNCH(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, s, perm, n, info);
+static void lu_factor_t(hipStream_t s, const LuBatch& bat, int nbat, int n, const LuAux* aux) {
+  HX_LAUNCH(k_lu_init, dim3((n + 255) / 256, nbat), dim3(256), 0, s, bat, n);
   const bool la = aux && aux->s2;
   const int rows = aux ? aux->sel_rows : kSelRows;
   const int outer_min = aux ? aux->outer_min_n : kOuterMinN;
@@ -532,27 +555,22 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
   // workspace: two ping-pong candidate buffers of nsets0 sets (sized for
   // kSelRows-row blocks, the smallest)
   const int nsets0 = (n + rows - 1) / rows, merge = rows / kPanelNB;
-  int* idx[2] = {widx, widx + (size_t)nsets0 * kPanelNB};
-  int* cnt[2] = {wcnt, wcnt + nsets0};
-  using c2 = typename CT<R>::c2;
   for (int col0 = 0; col0 < n; col0 += kPanelNB) {
     const int nb = n - col0 < kPanelNB ? n - col0 : kPanelNB;
     const int m = n - col0;
     int nsets = (m + rows - 1) / rows, cur = 0;
-    select_level<R>(s, rows, nsets == 1, nsets, A, n, col0, nb, (const int*)nullptr, (const int*)nullptr, 0,
-                    idx[cur], cnt[cur], diag, info);
+    select_level<R>(s, rows, nsets == 1, nsets, bat, nbat, n, col0, nb, -1, 0, cur, nsets0);
     while (nsets > 1) {
       const int nout = (nsets + merge - 1) / merge;
-      select_level<R>(s, rows, nout == 1, nout, A, n, col0, nb, idx[cur], cnt[cur], nsets, idx[1 - cur],
-                      cnt[1 - cur], diag, info);
+      select_level<R>(s, rows, nout == 1, nout, bat, nbat, n, col0, nb, cur, nsets, 1 - cur, nsets0);
       cur = 1 - cur;
       nsets = nout;
     }
-    HX_LAUNCH(k_pivots_to_swaps, dim3(1), dim3(64), 0, s, idx[cur], nb, col0, ipiv);
+    HX_LAUNCH(k_pivots_to_swaps, dim3(1, nbat), dim3(64), 0, s, bat, cur, nsets0, nb, col0);
     // the row swaps touch every column: the previous step's deferred update must be done
     if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
     pending = false;
-    HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256), dim3(256), 0, s, A, n, n, col0, nb, ipiv, perm);
+    HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256, nbat), dim3(256), 0, s, bat, n, col0, nb);
     // Outer blocks of kOuter columns: within the block each panel updates only
     // the block's columns; at the block's end the columns to its right get
     // U = L11^-1 A12 for all of the block's rows and one rank-kOuter update,
@@ -561,25 +579,25 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     const int rest = n - col0 - nb;
     if (outer_min <= 0 || n < outer_min) {  // one panel at a time: U12 and the update over every trailing column
       const int nrb = (rest + 255) / 256;
-      HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
-                (const c2*)diag, kPanelNB, 1, col0 + nb, n);
+      HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1, nbat), dim3(256), 0, s, bat, n, col0, nb, nrb, 0, 1,
+                col0 + nb, n);
       if (rest > 0) {
         const int tiles = (rest + 63) / 64;
         if (!la || tiles == 1) {
           const int ntiles = tiles * tiles;
-          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n,
-                    col0 + nb, n, col0, nb, tiles, ntiles, 0);
+          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, col0 + nb,
+                    n, col0 + nb, n, col0, nb, tiles, ntiles, 0);
         } else {
           // look-ahead: the first 64 trailing columns (the next panel) here, the
           // rest on the second stream, overlapping the next pivot selection,
           // which reads only the next panel's columns
-          HX_LAUNCH(k_trailing_update<R>, dim3(((tiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n,
-                    col0 + nb, n, col0, nb, tiles, tiles, 0);
+          HX_LAUNCH(k_trailing_update<R>, dim3(((tiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, col0 + nb,
+                    n, col0 + nb, n, col0, nb, tiles, tiles, 0);
           (void)hipEventRecord(aux->panel, s);
           (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
           const int ntiles = tiles * (tiles - 1);
-          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, col0 + nb, n,
-                    col0 + nb, n, col0, nb, tiles, ntiles, 1);
+          HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, aux->s2, bat, n,
+                    col0 + nb, n, col0 + nb, n, col0, nb, tiles, ntiles, 1);
           (void)hipEventRecord(aux->update, aux->s2);
           pending = true;
         }
@@ -589,12 +607,12 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
     const int blk0 = col0 - col0 % kOuter, blk1 = min(blk0 + kOuter, n);
     const int near = blk1 - col0 - nb;
     const int nrb = (rest + 255) / 256, ncb = (near + 255) / 256;
-    HX_LAUNCH(k_panel_solve<R>, dim3(nrb + ncb > 0 ? nrb + ncb : 1), dim3(256), 0, s, A, n, n, col0, nb, nrb,
-              (const c2*)diag, kPanelNB, 1, col0 + nb, blk1);
+    HX_LAUNCH(k_panel_solve<R>, dim3(nrb + ncb > 0 ? nrb + ncb : 1, nbat), dim3(256), 0, s, bat, n, col0, nb, nrb, 0,
+              1, col0 + nb, blk1);
     if (near > 0) {
       const int tm = (rest + 63) / 64, tn = (near + 63) / 64, ntiles = tm * tn;
-      HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, col0 + nb, n, col0 + nb,
-                blk1, col0, nb, tm, ntiles, 0);
+      HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, col0 + nb, n,
+                col0 + nb, blk1, col0, nb, tm, ntiles, 0);
     }
     if (col0 + nb == blk1 && blk1 < n) {
       const int far = n - blk1, fcb = (far + 255) / 256;
@@ -603,29 +621,28 @@ static void lu_factor_t(hipStream_t s, typename CT<R>::c2* A, int n, int* ipiv, 
         const int qnb = min(kPanelNB, blk1 - q0);
         if (q0 > blk0) {
           const int tn = (far + 63) / 64;
-          HX_LAUNCH(k_trailing_update<R>, dim3(((tn + 7) / 8) * 8), dim3(256), 0, s, A, n, q0, q0 + qnb, blk1, n,
-                    blk0, q0 - blk0, 1, tn, 0);
+          HX_LAUNCH(k_trailing_update<R>, dim3(((tn + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, q0, q0 + qnb,
+                    blk1, n, blk0, q0 - blk0, 1, tn, 0);
         }
-        HX_LAUNCH(k_panel_solve<R>, dim3(fcb), dim3(256), 0, s, A, n, n, q0, qnb, 0,
-                  (const c2*)(A + (size_t)q0 + (size_t)n * q0), n, 0, blk1, n);
+        HX_LAUNCH(k_panel_solve<R>, dim3(fcb, nbat), dim3(256), 0, s, bat, n, q0, qnb, 0, 1, 0, blk1, n);
       }
       // the rank-kOuter update of the far trailing matrix
       const int tm = (far + 63) / 64, tiles = tm;
       if (!la || tiles <= 2) {
         const int ntiles = tm * tiles;
-        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, s, A, n, blk1, n, blk1, n,
-                  blk0, blk1 - blk0, tm, ntiles, 0);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, blk1, n, blk1,
+                  n, blk0, blk1 - blk0, tm, ntiles, 0);
       } else {
         // look-ahead: the next outer block's columns (two tiles) here, the
         // rest on the second stream, overlapping the next pivot selection,
         // which reads only the next panel's columns
-        HX_LAUNCH(k_trailing_update<R>, dim3(((2 * tm + 7) / 8) * 8), dim3(256), 0, s, A, n, blk1, n, blk1, n,
-                  blk0, blk1 - blk0, tm, 2 * tm, 0);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((2 * tm + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, blk1, n, blk1,
+                  n, blk0, blk1 - blk0, tm, 2 * tm, 0);
         (void)hipEventRecord(aux->panel, s);
         (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
         const int ntiles = tm * (tiles - 2);
-        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8), dim3(256), 0, aux->s2, A, n, blk1, n, blk1,
-                  n, blk0, blk1 - blk0, tm, ntiles, 2);
+        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, aux->s2, bat, n, blk1, n,
+                  blk1, n, blk0, blk1 - blk0, tm, ntiles, 2);
         (void)hipEventRecord(aux->update, aux->s2);
         pending = true;
       }
@@ -1095,7 +1112,16 @@ __global__ void k_refine_reset(int* __restrict__ state) {
 namespace launch {
 void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
                double2* diag, const LuAux* aux) {
-  lu_factor_t<double>(s, A, n, ipiv, perm, widx, wcnt, info, diag, aux);
+  LuBatch b{};
+  b.m[0] = LuMat{A, ipiv, perm, widx, wcnt, diag, info};
+  lu_factor_t<double>(s, b, 1, n, aux);
+}
+void lu_factor_batch(hipStream_t s, const LuBatch& b, int count, int n, bool c64, const LuAux* aux) {
+  if (count < 1 || count > kLuBatchMax) return;
+  if (c64)
+    lu_factor_t<float>(s, b, count, n, aux);
+  else
+    lu_factor_t<double>(s, b, count, n, aux);
 }
 size_t tri_inverse_len(int n) { return (size_t)2 * ((n + kPanelNB - 1) / kPanelNB) * kPanelNB * kPanelNB; }
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
@@ -1112,7 +1138,9 @@ void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
 }
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* widx, int* wcnt, int* info,
                    double2* diag, const LuAux* aux) {
-  lu_factor_t<float>(s, A, n, ipiv, perm, widx, wcnt, info, reinterpret_cast<float2*>(diag), aux);
+  LuBatch b{};
+  b.m[0] = LuMat{A, ipiv, perm, widx, wcnt, diag, info};
+  lu_factor_t<float>(s, b, 1, n, aux);
 }
 void lu_solve_c64(hipStream_t s, const float2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
                   int nrhs, const int* skip) {

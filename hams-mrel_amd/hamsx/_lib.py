@@ -64,6 +64,7 @@ SIGNATURES = {
     "hamsx_forces": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_solve_frequency": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, _dp]),
     "hamsx_submit_frequency": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "hamsx_submit_frequency_group": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(C.c_double), C.c_double]),
     "hamsx_query": (C.c_int, [_vp]),
     "hamsx_collect": (C.c_int, [_vp, _dp, _dp, _dp]),
     "hamsx_queue_create": (C.c_int, [C.POINTER(C.c_void_p)]),

@@ -232,6 +232,23 @@ class FrequencySolver:
         """Enqueue one whole frequency solve on this context's stream."""
         _lib.check(self._lib.hamsx_submit_frequency(self._h, w1, h), "hamsx_submit_frequency")
 
+    @staticmethod
+    def submit_group(solvers, w1s, h: float):
+        """Enqueue one frequency solve on each of `solvers` (at most sixteen
+        contexts of one mesh size and precision, dense LU path), w1s[i] on
+        solvers[i]: each assembles on its own stream, the group's matrices are
+        factored by one batched launch sequence, and each solves on its own
+        stream.  Collect each solver as after submit."""
+        if not solvers:
+            return
+        lib = solvers[0]._lib
+        arr = (C.c_void_p * len(solvers))(*[s._h for s in solvers])
+        ws = np.ascontiguousarray(w1s, dtype=np.float64)
+        if ws.shape != (len(solvers),):
+            raise ValueError("one frequency per solver")
+        _lib.check(lib.hamsx_submit_frequency_group(arr, len(solvers), _lib.ptr(ws), float(h)),
+                   "hamsx_submit_frequency_group")
+
     def set_queue(self, queue, tag: int):
         """Post `tag` to `queue` (a CompletionQueue, or None to detach) when
         each submitted solve finishes."""

@@ -222,6 +222,20 @@ int hamsx_submit_frequency(hamsx_ctx *ctx, double w1, double h);
 int hamsx_query(hamsx_ctx *ctx);
 int hamsx_collect(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
 
+/* A group of 1..HAMSX_GROUP_MAX contexts of one device, one mesh size and
+ * one precision (the dense LU path: no GMRES, lid or symmetry plane), each
+ * solving its own frequency w1[i] at depth h: each context assembles on its
+ * own stream, the group's matrices are factored by one launch sequence on
+ * the first context's stream (every launch covers all of them, so the pivot
+ * selection's dependent steps serve the whole group), and each context then
+ * solves and reduces on its own stream.  Every context ends up pending, as
+ * after hamsx_submit_frequency, and is collected the same way; each result
+ * is the one its context gives alone, bit for bit.  No counterpart in the
+ * reference, which factors each frequency's matrix by its own ZGETRF
+ * (src/AssbMatxMulti.f90:110-114). */
+#define HAMSX_GROUP_MAX 16
+int hamsx_submit_frequency_group(hamsx_ctx **ctx, int count, const double *w1, double h);
+
 /* Completion notices for hosts that keep several contexts in flight (no
  * counterpart in the reference): a context attached to a queue with a tag
  * (hamsx_set_queue; q NULL detaches) has a host function appended to each

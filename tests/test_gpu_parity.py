@@ -417,6 +417,51 @@ def test_submitted_solves_in_flight_equal_blocking_solves(mesh_small):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision,outer,depth", [("fp64", 6144, 40.0), ("mixed", 6144, 40.0),
+                                                   ("fp64", 256, -1.0), ("mixed", 256, 40.0)])
+def test_frequency_group_equals_single_solves(mesh_small, precision, outer, depth):
+    """A frequency group (one batched factorisation for several contexts'
+    matrices) gives each context the coefficients and INFO of its own solve,
+    bit for bit, in the per-panel and the outer-block LU (outer blocks from
+    order 256 on); groups of one and of three, twice on the same contexts in
+    another order.  Contexts of another precision are refused."""
+    from hamsx._lib import HamsxError
+    m = mesh_small
+    heads = np.deg2rad([0.0, 45.0])
+    ws = [0.5, 1.1, 1.7]
+    say(f"[gpu] frequency group n={m.nelem} {precision} outer>={outer} depth={depth}")
+    with FrequencySolver(m, precision=precision) as s:
+        s.set_options(lu_outer_min_n=outer)
+        s.set_waves(heads)
+        ref = {w: s.solve_frequency(w, depth) for w in ws}
+    ctxs = [FrequencySolver(m, precision=precision) for _ in ws]
+    other = FrequencySolver(m, precision="fp64" if precision == "mixed" else "mixed")
+    try:
+        for c in ctxs + [other]:
+            c.set_options(lu_outer_min_n=outer)
+            c.set_waves(heads)
+        FrequencySolver.submit_group(ctxs, ws, depth)
+        got = {w: c.collect() for w, c in zip(ws, ctxs)}
+        FrequencySolver.submit_group(ctxs[::-1], ws, depth)
+        got2 = {w: c.collect() for w, c in zip(ws, ctxs[::-1])}
+        FrequencySolver.submit_group(ctxs[1:2], ws[:1], depth)
+        got1 = ctxs[1].collect()
+        with pytest.raises(HamsxError):
+            FrequencySolver.submit_group([ctxs[0], other], ws[:2], depth)
+        with pytest.raises(HamsxError):
+            FrequencySolver.submit_group([ctxs[0], ctxs[0]], ws[:2], depth)
+    finally:
+        for c in ctxs + [other]:
+            c.close()
+    for w in ws:
+        assert ref[w][3] == 0
+        for a, b, c in zip(ref[w], got[w], got2[w]):
+            np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(a, c)
+    for a, b in zip(ref[ws[0]], got1):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("depth", [-1.0, 35.0])
 def test_gmres_matches_lu(depth):
     """Restarted GMRES with the block-diagonal preconditioner converges to the
