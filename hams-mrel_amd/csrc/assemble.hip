@@ -29,7 +29,7 @@ __device__ __forceinline__ FreqParams freq_params(const FreqDev& f, int n, int j
   const double* p = t + kTabHead;
   return FreqParams{f.v,     f.wk,       f.w1,           f.h,    f.wvn,          t,  t + kModes, t + 2 * kModes,
                     t[3 * kModes], p, p + mn, p + 2 * mn, p + 3 * mn, p + 4 * mn, p + 4 * mn + n, n, js, jf,
-                    f.far32};
+                    f.far32,   f.limes_ieee};
 }
 
 // Mode tables: one thread per (panel, mode); panel 0's threads also write

@@ -322,6 +322,27 @@ __device__ inline double d_limes(int n, double* s, double* a) {
   return rl;
 }
 
+// 1/d as the FP64 division sequence (v_div_scale, v_rcp, two Newton steps,
+// one correction, v_div_fmas, v_div_fixup) computes it when neither operand
+// needs scaling: the same reciprocal, Newton steps and correction without
+// the scale and fix-up steps, so the result is 1.0 / d bit for bit whenever
+// 2^-990 <= |1/d| <= 2^750.  lo / hi record the smallest and largest |1/d|
+// (fmin / fmax ignore the NaN that d = 0 gives in a lane whose quotient is
+// not used), so the caller can tell afterwards that every quotient it used
+// was in that range.
+__device__ __forceinline__ double rcp_fast(double d, double& lo, double& hi) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(e, r, r);
+  lo = __builtin_fmin(lo, fabs(r));
+  hi = __builtin_fmax(hi, fabs(r));
+  return r;
+}
+
 // The same Limes with s[] and a[] held in registers: the two sweeps of each
 // column pair are fused into one pass (a_new[i+1] is formed just before
 // s_new[i] needs it, which reads only entries the unfused order has not yet
@@ -335,10 +356,14 @@ __device__ inline double d_limes(int n, double* s, double* a) {
 // d_limes, and the scan visits them in the same order: the result is
 // identical.  Needs n <= NMAX - 1.
 constexpr int kLimesChunk = 8;
-template <int NMAX>
-__device__ __forceinline__ double d_limes_reg(int n, double (&s)[NMAX], double (&a)[NMAX]) {
-  double rl = 0.0, amax = 0.0;
+// FAST = false: every quotient by IEEE division.  FAST = true: by rcp_fast;
+// ok reports whether every quotient stayed in rcp_fast's exact range (the
+// result is then d_limes's bit for bit; otherwise the caller recomputes).
+template <int NMAX, bool FAST>
+__device__ __forceinline__ double limes_reg_t(int n, double (&s)[NMAX], double (&a)[NMAX], bool& ok) {
+  double rl = 0.0, amax = 0.0, lo = 1.0, hi = 1.0;
   bool done = false;
+  auto recip = [&](double d) { return FAST ? rcp_fast(d, lo, hi) : 1.0 / d; };
 #pragma unroll
   for (int i = 0; i < NMAX; i++) {
     if (i == n) rl = s[i];
@@ -348,8 +373,8 @@ __device__ __forceinline__ double d_limes_reg(int n, double (&s)[NMAX], double (
     const int ls = n - 2 * k;  // last s index of this column pair; the a sweep ends at ls + 1
     if (ls + 1 < 0) break;
     {
-      double d = s[1] - s[0];
-      a[0] = (s[1] != s[0]) ? a[1] + 1.0 / d : a[1];
+      const double r = recip(s[1] - s[0]);
+      a[0] = (s[1] != s[0]) ? a[1] + r : a[1];
     }
     if (ls < 0) break;
 #pragma unroll
@@ -357,20 +382,49 @@ __device__ __forceinline__ double d_limes_reg(int n, double (&s)[NMAX], double (
       if (c <= ls) {
 #pragma unroll
         for (int i = c; i < c + kLimesChunk && i < NMAX - 2; i++) {
-          double d = s[i + 2] - s[i + 1];
-          a[i + 1] = (s[i + 2] != s[i + 1]) ? a[i + 2] + 1.0 / d : a[i + 2];
+          const double r = recip(s[i + 2] - s[i + 1]);
+          a[i + 1] = (s[i + 2] != s[i + 1]) ? a[i + 2] + r : a[i + 2];
           const double si1 = s[i + 1];
-          s[i] = (a[i + 1] != a[i]) ? si1 + 1.0 / (a[i + 1] - a[i]) : si1;
+          const double r2 = recip(a[i + 1] - a[i]);
+          s[i] = (a[i + 1] != a[i]) ? si1 + r2 : si1;
           const bool ev = !done && i <= ls && fabs(a[i]) > amax;
           amax = ev ? fabs(a[i]) : amax;
           rl = ev ? s[i] : rl;
           done = done || (ev && si1 == s[i]);
         }
-        if (done) return rl;
+        if (done) {
+          ok = !FAST || (lo >= 0x1p-990 && hi <= 0x1p750);
+          return rl;
+        }
       }
     }
   }
+  ok = !FAST || (lo >= 0x1p-990 && hi <= 0x1p750);
   return rl;
+}
+template <int NMAX>
+__device__ __forceinline__ double d_limes_reg(int n, double (&s)[NMAX], double (&a)[NMAX]) {
+  bool ok;
+  return limes_reg_t<NMAX, false>(n, s, a, ok);
+}
+
+// The limit of the partial sums ser[0..nt-1] (d_limes of them): the fast
+// pass, and the exact pass over the same partial sums when a quotient of the
+// fast pass left rcp_fast's exact range (in practice never), or the exact
+// pass alone when ieee is set.
+template <int NMAX>
+__device__ __forceinline__ double limes_series(int nt, const double (&ser)[NMAX], int ieee) {
+  double sv[NMAX], av[NMAX];
+  if (!ieee) {
+#pragma unroll
+    for (int i = 0; i < NMAX; i++) sv[i] = i < nt ? ser[i] : 0.0;
+    bool ok;
+    const double r = limes_reg_t<NMAX, true>(nt - 1, sv, av, ok);
+    if (__builtin_expect(ok, 1)) return r;
+  }
+#pragma unroll
+  for (int i = 0; i < NMAX; i++) sv[i] = i < nt ? ser[i] : 0.0;
+  return d_limes_reg<NMAX>(nt - 1, sv, av);
 }
 
 // ---------------------------------------------------------------------------
@@ -528,6 +582,7 @@ struct FreqParams {
   int n;                       // table row stride (panels)
   int js, jf;                  // table columns of zf (source panel) and zp (field panel)
   int far32 = 0;               // EIGEN region: evanescent series in FP32 (mixed precision)
+  int limes_ieee = 0;          // epsilon tables by IEEE division only (diagnostic)
 };
 
 // The table entries, each formed by the expression the reference evaluates
@@ -647,10 +702,7 @@ __device__ __forceinline__ void d_eigene(double r, double zf, double zp, const F
   double res[3];
 #pragma unroll 1
   for (int q = 0; q < 3; q++) {
-    double sv[kEigeneMax], av[kEigeneMax];
-#pragma unroll
-    for (int i = 0; i < kEigeneMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
-    res[q] = d_limes_reg<kEigeneMax>(nt - 1, sv, av);
+    res[q] = limes_series<kEigeneMax>(nt, ser[q], fp.limes_ieee);
   }
   fg[0].r += res[0];
   fg[1].r += res[1];
@@ -748,10 +800,7 @@ __device__ __forceinline__ void d_pidcock(double r, double zf, double zp, const 
   double res[3];
 #pragma unroll 1
   for (int q = 0; q < 3; q++) {
-    double sv[kPidcockMax], av[kPidcockMax];
-#pragma unroll
-    for (int i = 0; i < kPidcockMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
-    res[q] = 4.0 * d_limes_reg<kPidcockMax>(nt - 1, sv, av);
+    res[q] = 4.0 * limes_series<kPidcockMax>(nt, ser[q], fp.limes_ieee);
   }
   double im[3];
   d_image_sums(r, zf, zp, h, im);
@@ -906,10 +955,7 @@ __device__ __forceinline__ void d_eigene2(double r, double zs, double zd, const 
   double res[4];
 #pragma unroll 1
   for (int q = 0; q < 4; q++) {
-    double sv[kEigeneMax], av[kEigeneMax];
-#pragma unroll
-    for (int i = 0; i < kEigeneMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
-    res[q] = d_limes_reg<kEigeneMax>(nt - 1, sv, av);
+    res[q] = limes_series<kEigeneMax>(nt, ser[q], fpA.limes_ieee);
   }
   fgA[0].r += res[0];
   fgA[1].r += res[1];
@@ -960,10 +1006,7 @@ __device__ __forceinline__ void d_pidcock2(double r, double zs, double zd, const
   double res[4];
 #pragma unroll 1
   for (int q = 0; q < 4; q++) {
-    double sv[kPidcockMax], av[kPidcockMax];
-#pragma unroll
-    for (int i = 0; i < kPidcockMax; i++) sv[i] = i < nt ? ser[q][i] : 0.0;
-    res[q] = 4.0 * d_limes_reg<kPidcockMax>(nt - 1, sv, av);
+    res[q] = 4.0 * limes_series<kPidcockMax>(nt, ser[q], fpA.limes_ieee);
   }
   double im[3];
   d_image_sums(r, zs, zd, h, im);

@@ -513,6 +513,14 @@ bool force_grid_assembly() {
   return e && *e && *e != '0';
 }
 
+// Diagnostic switch: HAMSX_LIMES_IEEE=1 computes every quotient of the
+// epsilon tables (PIDCOCK, EIGENE) by IEEE division instead of the fast
+// reciprocal pass (same numbers; used to check that they are).
+bool force_limes_ieee() {
+  const char* e = std::getenv("HAMSX_LIMES_IEEE");
+  return e && *e && *e != '0';
+}
+
 // Counting sort of the pairs of mesh view m by (region, NT, FLAG) for the
 // current depth; done once per (mesh, rows, h) and reused by every frequency
 // at that depth.
@@ -1867,6 +1875,7 @@ int hamsx_assemble(hamsx_ctx* c) {
   c->have_C = false;
   c->freq.tab = finite ? c->modetab.p : nullptr;
   c->freq.far32 = c->prec == HAMSX_PREC_MIXED && c->opt_far32 ? 1 : 0;
+  c->freq.limes_ieee = force_limes_ieee() ? 1 : 0;
   const bool limit = c->freq.v == 0.0 || c->freq.v == -1.0;
   {
     StageTimer tm(c, 0);

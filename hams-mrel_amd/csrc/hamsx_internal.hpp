@@ -74,6 +74,7 @@ struct FreqDev {
   // serves any frequency of the same depth
   const double* sc = nullptr;
   int far32 = 0;  // EIGEN region's evanescent series in FP32 (mixed precision)
+  int limes_ieee = 0;  // diagnostic: every epsilon-table quotient by IEEE division (no fast pass)
 };
 __host__ __device__ inline FreqDev freq_now(FreqDev f) {
   if (f.sc) {

@@ -87,6 +87,27 @@ def test_binned_assembly_equals_grid(mesh_small, depth, w, monkeypatch):
     np.testing.assert_array_equal(out["0"][1], out["1"][1])
 
 
+@pytest.mark.parametrize("w", [0.2, 1.39, 2.5])
+def test_fast_limes_equals_ieee_division(w, monkeypatch):
+    """The epsilon tables' fast reciprocal pass (the division sequence
+    without its scale and fix-up steps, range-checked) gives the tables of
+    IEEE division bit for bit: A and S of the two-cylinder workload (3,968
+    panels at h = 50 m: every PIDCOCK and EIGENE series length, 15.7M
+    entries) with and without HAMSX_LIMES_IEEE."""
+    case = configs.two_cylinders()
+    m = combine(case.bodies, case.xr)
+    say(f"[gpu] fast Limes vs IEEE division n={m.nelem} depth={case.depth} w={w}")
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("HAMSX_LIMES_IEEE", mode)
+        with FrequencySolver(m) as s:
+            s.calwaveproperts(w, case.depth)
+            s.assemble()
+            out[mode] = (s.get_matrix("A"), s.get_matrix("S"))
+    np.testing.assert_array_equal(out["0"][0], out["1"][0])
+    np.testing.assert_array_equal(out["0"][1], out["1"][1])
+
+
 @pytest.mark.parametrize("depth,w", [(30.0, 1.2)])
 def test_binned_assembly_equals_grid_with_lid(depth, w, monkeypatch):
     """With lid field points the binned assembly mixes single entries (lid
