@@ -39,6 +39,7 @@ METRIC = "freq-solves/sec @4k-12k panels, mixed vs FP64 at 1e-6 rel; p50/p99 lat
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector and FP64 matrix (dense)
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second: 256 CUs x 4 SIMDs, 4 cycles each
 
 # FP64 work and HBM traffic per panel pair (one A entry and one S entry) of
 # each finite-depth region, summed over the region's kernels (hull-pair
@@ -50,13 +51,18 @@ PEAK_HBM_GBS = 8000.0
 # SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU).  Traffic = FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors that make a 256 MiB device copy read 256 MiB
 # and write 256 MiB.  The algorithmic bytes of every region are 32 per pair
-# (A and S entries written once).
+# (A and S entries written once).  valu = wave-level VALU instructions
+# (SQ_INSTS_VALU) per pair, for the fraction of the VALU issue slots busy.
 REGION_COUNTERS = {
     "cylinders-2x1984-h50": {
-        "asm_linton": {"flops_per_pair": 571390.4, "traffic_per_pair": (38278104 + 54102251) / 57728},
-        "asm_pidcock": {"flops_per_pair": 187615.2, "traffic_per_pair": (7758628724 + 10256639040) / 1079424},
-        "asm_eigene": {"flops_per_pair": 68951.6, "traffic_per_pair": (6289391727 + 5676591616) / 6735360},
-        "asm_eigen": {"flops_per_pair": 4247.2, "traffic_per_pair": (30479918 + 253196160) / 7872512},
+        "asm_linton": {"flops_per_pair": 571398.4, "traffic_per_pair": (38857279 + 54625269) / 57728,
+                       "valu_per_pair": 8.311e8 / 57728},
+        "asm_pidcock": {"flops_per_pair": 113403.8, "traffic_per_pair": (7475511549 + 9363481643) / 1079424,
+                        "valu_per_pair": 2.601e9 / 1079424},
+        "asm_eigene": {"flops_per_pair": 43750.8, "traffic_per_pair": (6935846112 + 5435031093) / 6735360,
+                       "valu_per_pair": 5.839e9 / 6735360},
+        "asm_eigen": {"flops_per_pair": 1701.5, "traffic_per_pair": (30696267 + 252948651) / 7872512,
+                      "valu_per_pair": 2.688e8 / 7872512},
     },
 }
 REGIONS = ("asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
@@ -121,7 +127,8 @@ def roofline(case, mesh, ms, precision):
         flops = cnt["flops_per_pair"] * pairs
         out.update({"achieved": flops / (ms[dom] * 1e-3) / 1e12,
                     "frac": flops / (ms[dom] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                    "traffic": cnt["traffic_per_pair"] * pairs, "flops_per_pair": cnt["flops_per_pair"]})
+                    "traffic": cnt["traffic_per_pair"] * pairs, "flops_per_pair": cnt["flops_per_pair"],
+                    "valu_issue_busy": cnt["valu_per_pair"] * pairs / (ms[dom] * 1e-3) / VALU_ISSUE_PER_S})
     else:
         out.update({"achieved": None, "frac": None, "traffic": None})
     alg = region_ops(case.name)
@@ -505,8 +512,12 @@ def main():
     # queues, and a kernel waiting on an event in one stream then stalls the
     # others: measured 18.9 -> 19.8 solves/s and p99 307 -> 219 ms on the
     # default workload with eight queues.  Raised (never lowered) before the
-    # runtime starts.
-    want = "8" if args.streams <= 4 else "16"
+    # runtime starts, to eight and no further: every queue that runs the
+    # PIDCOCK kernel keeps 2.37 GB of scratch (4.5 kB per lane for every wave
+    # slot of the GPU), and the GPU's scratch limit shared by all queues is
+    # 32 GiB, which sixteen queues overrun (the dispatch then fails with
+    # HSA_STATUS_ERROR_OUT_OF_RESOURCES).
+    want = "8"
     try:
         have = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
     except ValueError:
