@@ -246,6 +246,9 @@ struct PairBins {
 // pinned staging and the device root buffer start with the kNK dispersion
 // roots and the four frequency scalars (v, wk, w1, h)
 constexpr size_t kStageHead = hamsx::kNK + 4;
+// Arnoldi steps the GMRES host loop runs ahead of the device's convergence
+// counters (it waits for step j - kGmLag before enqueuing step j + 1)
+constexpr int kGmLag = 2;
 
 struct hamsx_ctx {
   int device = 0;
@@ -349,6 +352,15 @@ struct hamsx_ctx {
   // GMRES: Arnoldi blocks, work blocks, per-body preconditioner factors
   DevBuf<double2> gV, gW, gZ, gU, gB, gP, gPt, gDots, pLU;
   DevBuf<double> gNorm, gScale;
+  // GMRES Hessenberg / Givens state on the device, its pinned counters
+  // (unconverged columns per restart and active columns per Arnoldi step,
+  // iterations, worst residual) and the events the host reads them behind
+  DevBuf<double2> gH, gG, gSn;
+  DevBuf<double> gCs, gBn, gWorst;
+  DevBuf<int> gAct, gDone, gMact, gIters, gCnt;
+  int* gm_pin = nullptr;
+  size_t gm_pin_len = 0;
+  hipEvent_t gm_ev[kGmLag + 1] = {};
   DevBuf<int> pPerm, pIpiv, pSz;
   DevBuf<size_t> pLuOff, pXoff, pInvOff;  // per body: factor, packed right-hand-side, inverse-block offsets
   DevBuf<double2> pInv;                   // inverted 32 x 32 diagonal blocks of the body factors
@@ -805,16 +817,15 @@ int do_factor(hamsx_ctx* c) {
 // Restarted GMRES (ISOLV = 2 path of the reference: ZBLOCK_GMRES_SOLVE /
 // ZGMRES_SOLVE, src/IterativeSolvers.f90:324-562, block-diagonal LU
 // preconditioner SETUP_BLOCK_PRECONDITIONER :190-307), right-preconditioned,
-// x0 = 0, all right-hand sides advancing together with one device
-// synchronisation per Arnoldi step.  The operator is the assembled dense
+// x0 = 0, all right-hand sides advancing together; the Hessenberg / Givens
+// work runs on the device and the host waits once per restart cycle (and
+// reads the active-column count kGmLag steps behind).  The operator is the assembled dense
 // matrix (the reference's H-matrix samples that same matrix,
 // src/HMatrix.f90:633-665).  Rotations are the standard complex Givens
 // rotations with real cosine, which annihilate the subdiagonal for complex
 // Hessenberg entries.
 // ---------------------------------------------------------------------------
 namespace {
-using cplx = std::complex<double>;
-
 int gmres_precond_setup(hamsx_ctx* c, const double2* M) {
   const int nb = c->nbody;
   c->pOff.assign(nb + 1, 0);
@@ -866,21 +877,6 @@ void gmres_precond_apply(hamsx_ctx* c, const double2* V, double2* Z, int nrhs) {
   hamsx::launch::lu_solve_systems(c->stream, c->nbody, c->pMax, c->pLU.p, c->pLuOff.p, c->pSz.p, c->pInv.p,
                                   c->pInvOff.p, c->pXoff.p, c->body_start_d.p, c->body.p, c->pPerm.p, c->n, V, Z,
                                   c->gP.p, c->gPt.p, nrhs);
-}
-
-void givens(cplx a, cplx b, double& cs, cplx& sn) {
-  const double ab = std::abs(a), bb = std::abs(b);
-  if (bb == 0.0) {
-    cs = 1.0;
-    sn = 0.0;
-  } else if (ab == 0.0) {
-    cs = 0.0;
-    sn = 1.0;
-  } else {
-    const double d = std::sqrt(ab * ab + bb * bb);
-    cs = ab / d;
-    sn = (a / ab) * std::conj(b) / d;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1104,138 +1100,98 @@ int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
       (rc = c->mvw.ensure(hamsx::launch::matvec_workspace(n, nr))))
     return rc;
   if ((rc = c->gP.ensure(blk)) || (rc = c->gPt.ensure(blk))) return rc;
+  if ((rc = c->gH.ensure((size_t)(m + 1) * m * nr)) || (rc = c->gG.ensure((size_t)(m + 1) * nr)) ||
+      (rc = c->gSn.ensure((size_t)m * nr)) || (rc = c->gCs.ensure((size_t)m * nr)) || (rc = c->gBn.ensure(nr)) ||
+      (rc = c->gWorst.ensure(1)) || (rc = c->gAct.ensure(nr)) || (rc = c->gDone.ensure(nr)) ||
+      (rc = c->gMact.ensure(nr)) || (rc = c->gIters.ensure(1)) || (rc = c->gCnt.ensure((size_t)m + 1)))
+    return rc;
+  // pinned copies of the device counters: [0] unconverged columns at the
+  // last restart test, [1 + j] active columns after Arnoldi step j
+  const size_t pin_len = (size_t)m + 1;
+  if (c->gm_pin_len < pin_len) {
+    HX_TRY(hipStreamSynchronize(c->stream));
+    if (c->gm_pin) (void)hipHostFree(c->gm_pin);
+    c->gm_pin = nullptr;
+    c->gm_pin_len = 0;
+    HX_TRY(hipHostMalloc((void**)&c->gm_pin, sizeof(int) * pin_len, hipHostMallocDefault));
+    c->gm_pin_len = pin_len;
+  }
+  for (hipEvent_t& e : c->gm_ev)
+    if (!e) HX_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if ((rc = gmres_precond_setup(c, Aop))) return rc;
   c->hm_built = false;
   if (c->hm_on && (rc = hm_build(c, Aop, nr))) return rc;
+  const hamsx::GmDev gd{c->gH.p, c->gG.p, c->gSn.p, c->gCs.p, c->gBn.p, c->gAct.p, c->gDone.p, c->gMact.p,
+                        c->gIters.p};
+  volatile int* pin = c->gm_pin;
   // B = right-hand sides, X = 0
   HX_TRY(hipMemcpyAsync(c->gB.p, X, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
   HX_TRY(hipMemsetAsync(X, 0, sizeof(double2) * blk, c->stream));
-  std::vector<double> bnorm(nr), beta(nr), hn(nr), scale(nr);
-  std::vector<cplx> H((size_t)(m + 1) * m * nr), g((size_t)(m + 1) * nr), sn((size_t)m * nr);
-  std::vector<double> cs((size_t)m * nr);
-  std::vector<double2> d12((size_t)2 * (m + 1) * nr), ybuf((size_t)m * nr);
-  std::vector<int> mact(nr), done(nr, 0);
   hamsx::launch::col_norm2(c->stream, c->gB.p, n, nr, c->gNorm.p);
-  HX_TRY(hipMemcpyAsync(bnorm.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
-  HX_TRY(hipStreamSynchronize(c->stream));
-  for (int q = 0; q < nr; q++) done[q] = bnorm[q] < 1.0e-30;
+  hamsx::launch::gm_start(c->stream, gd, nr, c->gNorm.p);
+  double2* d1 = c->gDots.p;
+  double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
   int iter = 0;
-  double worst = 0.0;
   while (iter < c->gm_maxiter) {
-    // R = B - A X (into gW), beta = ||R||
+    // R = B - A X (into gW) and beta = ||R||; the restart test runs on the
+    // device and the host waits once per cycle for the count of columns left
     if (iter == 0)
       HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
     else
       op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
     hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
-    HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+    hamsx::launch::gm_restart(c->stream, gd, nr, m, c->gNorm.p, c->gm_tol, c->gScale.p, c->gCnt.p, c->gWorst.p);
+    if ((rc = check_launch())) return rc;
+    HX_TRY(hipMemcpyAsync((void*)pin, c->gCnt.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HX_TRY(hipStreamSynchronize(c->stream));
-    bool all = true;
-    worst = 0.0;
-    for (int q = 0; q < nr; q++) {
-      if (!done[q] && beta[q] / bnorm[q] < c->gm_tol) done[q] = 1;
-      if (!done[q]) all = false;
-      if (bnorm[q] >= 1.0e-30) worst = std::max(worst, beta[q] / bnorm[q]);
-      scale[q] = done[q] ? 0.0 : 1.0 / beta[q];
-      mact[q] = 0;
-      for (int j = 0; j <= m; j++) g[(size_t)j * nr + q] = 0.0;
-      g[q] = done[q] ? 0.0 : beta[q];
-    }
-    if (all) break;
-    HX_TRY(hipMemcpyAsync(c->gScale.p, scale.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+    if (pin[0] == 0) break;
     hamsx::launch::col_scale(c->stream, c->gW.p, n, nr, c->gScale.p, c->gV.p);
-    std::vector<int> active(nr);
-    for (int q = 0; q < nr; q++) active[q] = !done[q];
+    // Arnoldi steps, enqueued kGmLag steps ahead of the host's look at the
+    // device's count of active columns; a step after every column has
+    // converged changes nothing (its scale is 0 and k_gm_step skips the
+    // columns), so the cycle ends at most kGmLag steps late.
     int j = 0;
-    for (; j < m && iter < c->gm_maxiter; j++) {
+    bool over = false;
+    for (; j < m && iter < c->gm_maxiter && !over; j++) {
       iter++;
       double2* Vj = c->gV.p + blk * j;
       gmres_precond_apply(c, Vj, c->gZ.p, nr);
       op_matvec(c, Aop, c->gZ.p, nr, nullptr, 1.0, 0.0, c->gW.p, true);
-      double2* d1 = c->gDots.p;
-      double2* d2 = c->gDots.p + (size_t)(m + 1) * nr;
       hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1);
       hamsx::launch::block_update(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d1, nullptr);
       hamsx::launch::block_dots(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d2);
       hamsx::launch::block_update(c->stream, c->gV.p, j + 1, c->gW.p, n, nr, d2, nullptr);
       hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
-      HX_TRY(hipMemcpyAsync(d12.data(), c->gDots.p, sizeof(double2) * 2 * (m + 1) * nr, hipMemcpyDeviceToHost,
-                            c->stream));
-      HX_TRY(hipMemcpyAsync(hn.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
-      HX_TRY(hipStreamSynchronize(c->stream));
-      bool any = false;
-      for (int q = 0; q < nr; q++) {
-        scale[q] = 0.0;
-        if (!active[q]) continue;
-        cplx* h = &H[((size_t)q * m + j) * (m + 1)];  // column j of q's Hessenberg matrix
-        for (int i = 0; i <= j; i++) {
-          const double2 a = d12[(size_t)i * nr + q], b = d12[(size_t)(m + 1) * nr + (size_t)i * nr + q];
-          h[i] = cplx(a.x + b.x, a.y + b.y);
-        }
-        h[j + 1] = hn[q];
-        for (int i = 0; i < j; i++) {
-          const double ci = cs[(size_t)i * nr + q];
-          const cplx si = sn[(size_t)i * nr + q];
-          const cplx t = ci * h[i] + si * h[i + 1];
-          h[i + 1] = -std::conj(si) * h[i] + ci * h[i + 1];
-          h[i] = t;
-        }
-        double cj;
-        cplx sj;
-        givens(h[j], h[j + 1], cj, sj);
-        cs[(size_t)j * nr + q] = cj;
-        sn[(size_t)j * nr + q] = sj;
-        h[j] = cj * h[j] + sj * h[j + 1];
-        h[j + 1] = 0.0;
-        cplx& gj = g[(size_t)j * nr + q];
-        cplx& gj1 = g[(size_t)(j + 1) * nr + q];
-        const cplx t = cj * gj + sj * gj1;
-        gj1 = -std::conj(sj) * gj + cj * gj1;
-        gj = t;
-        mact[q] = j + 1;
-        if (std::abs(gj1) / bnorm[q] < c->gm_tol) {
-          active[q] = 0;  // converged in this cycle: its later columns are zero
-        } else {
-          any = true;
-          scale[q] = hn[q] > 1.0e-30 ? 1.0 / hn[q] : 0.0;
-        }
-      }
-      if (!any) {
-        j++;
-        break;
-      }
-      HX_TRY(hipMemcpyAsync(c->gScale.p, scale.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+      hamsx::launch::gm_step(c->stream, gd, nr, m, j, d1, d2, c->gNorm.p, c->gm_tol, c->gScale.p, c->gCnt.p);
       hamsx::launch::col_scale(c->stream, c->gW.p, n, nr, c->gScale.p, c->gV.p + blk * (j + 1));
-    }
-    // y = H^-1 g per right-hand side; X += M^-1 (V y)
-    for (size_t k = 0; k < ybuf.size(); k++) ybuf[k] = make_double2(0.0, 0.0);
-    for (int q = 0; q < nr; q++) {
-      const int mq = mact[q];
-      std::vector<cplx> y(mq);
-      for (int i = 0; i < mq; i++) y[i] = g[(size_t)i * nr + q];
-      for (int i = mq - 1; i >= 0; i--) {
-        const cplx hii = H[((size_t)q * m + i) * (m + 1) + i];
-        y[i] = std::abs(hii) < 1.0e-30 ? cplx(0.0) : y[i] / hii;
-        for (int k = 0; k < i; k++) y[k] -= H[((size_t)q * m + i) * (m + 1) + k] * y[i];
+      if ((rc = check_launch())) return rc;
+      HX_TRY(hipMemcpyAsync((void*)(pin + 1 + j), c->gCnt.p + 1 + j, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HX_TRY(hipEventRecord(c->gm_ev[j % (kGmLag + 1)], c->stream));
+      if (j >= kGmLag) {
+        HX_TRY(hipEventSynchronize(c->gm_ev[(j - kGmLag) % (kGmLag + 1)]));
+        over = pin[1 + j - kGmLag] == 0;
+        if (over) iter -= kGmLag;  // the steps after convergence do not count
       }
-      for (int i = 0; i < mq; i++) ybuf[(size_t)i * nr + q] = make_double2(y[i].real(), y[i].imag());
     }
-    HX_TRY(hipMemcpyAsync(c->gDots.p, ybuf.data(), sizeof(double2) * (size_t)m * nr, hipMemcpyHostToDevice,
-                          c->stream));
+    // y = H^-1 g per right-hand side (device); X += M^-1 (V y)
+    hamsx::launch::gm_solve(c->stream, gd, nr, m, c->gDots.p);
     hamsx::launch::block_combine(c->stream, c->gV.p, j, n, nr, c->gDots.p, c->gU.p);
     gmres_precond_apply(c, c->gU.p, c->gZ.p, nr);
     hamsx::launch::axpy_update(c->stream, X, c->gZ.p, blk);
-    HX_TRY(hipStreamSynchronize(c->stream));  // the host vectors above are reused next cycle
   }
   // final residual of the operator GMRES worked with, for the statistics
   op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
   hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
+  std::vector<double> beta(nr), bnorm(nr);
+  int iters = 0;
   HX_TRY(hipMemcpyAsync(beta.data(), c->gNorm.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(bnorm.data(), c->gBn.p, sizeof(double) * nr, hipMemcpyDeviceToHost, c->stream));
+  HX_TRY(hipMemcpyAsync(&iters, c->gIters.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HX_TRY(hipStreamSynchronize(c->stream));
-  worst = 0.0;
+  double worst = 0.0;
   for (int q = 0; q < nr; q++)
     if (bnorm[q] >= 1.0e-30) worst = std::max(worst, beta[q] / bnorm[q]);
-  c->last_iters += iter;
+  c->last_iters += iters;
   c->last_resid = std::max(c->last_resid, worst);
   return check_launch();
 }
@@ -1499,6 +1455,9 @@ int hamsx_destroy(hamsx_ctx* c) {
   for (auto& m : c->marks) c->pool.insert(c->pool.end(), {m.a, m.b});
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->gm_pin) (void)hipHostFree(c->gm_pin);
+  for (hipEvent_t e : c->gm_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->sym_stream) (void)hipStreamSynchronize(c->sym_stream);
   for (hipEvent_t ev : {c->sym_go, c->sym_done, c->grp_ev})
     if (ev) (void)hipEventDestroy(ev);

@@ -159,6 +159,15 @@ struct LuBatch {
   LuMat m[kLuBatchMax];
 };
 
+// GMRES(m) Hessenberg / Givens state on the device (krylov.hip)
+struct GmDev {
+  double2 *H, *g, *sn;  // (m + 1) m, (m + 1), m per right-hand side
+  double* cs;           // m per right-hand side
+  double* bnorm;
+  int *act, *done, *mact;
+  int* iters;           // [0]: iterations over the restarts (longest cycle each)
+};
+
 namespace launch {
 // influence matrices A and S (complex128, column-major, leading dimension n)
 void assemble(hipStream_t s, const MeshDev& m, const FreqDev& f, double2* A, double2* S, int* flags);
@@ -263,6 +272,12 @@ void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, cons
 // after each residual test max_q |R_q|_max / |X_q|_max <= tol
 void refine_reset(hipStream_t s, int* state);
 void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state);
+void gm_start(hipStream_t s, const GmDev& g, int nrhs, const double* norms);
+void gm_restart(hipStream_t s, const GmDev& g, int nrhs, int m, const double* beta, double tol, double* scale,
+                int* out, double* worst);
+void gm_step(hipStream_t s, const GmDev& g, int nrhs, int m, int j, const double2* d1, const double2* d2,
+             const double* hn, double tol, double* scale, int* out);
+void gm_solve(hipStream_t s, const GmDev& g, int nrhs, int m, double2* y);
 // restarted GMRES helpers (krylov.hip); blocks are n x nrhs, V holds nv of them
 void block_dots(hipStream_t s, const double2* V, int nv, const double2* W, int n, int nrhs, double2* dots);
 void block_update(hipStream_t s, const double2* V, int nv, double2* W, int n, int nrhs, const double2* h,
