@@ -311,6 +311,7 @@ struct hamsx_ctx {
   // the LU's look-ahead stream and events, and the tuning options
   // (hamsx_set_option); lookahead -1: on unless HAMSX_LU_LOOKAHEAD=0
   hamsx::LuAux lu_aux{};
+  bool outer_default = true;  // HAMSX_OPT_LU_OUTER_MIN_N never set: kOuterMinNGroup for groups
   int opt_lookahead = -1;
   // symmetry system 2 is factored on its own stream (sym_stream, with its own
   // scratch) while system 1 is factored on `stream`; created on the first
@@ -703,9 +704,12 @@ int ensure_normal_matrix(hamsx_ctx* c) {
 // The LU's schedule and shape for this context: the look-ahead stream unless
 // switched off (HAMSX_OPT_LU_LOOKAHEAD, or HAMSX_LU_LOOKAHEAD=0 by default:
 // one stream, the same factors bit for bit) or `allow` is false, and the
-// selection-block and outer-block options.
-hamsx::LuAux lu_options(hamsx_ctx* c, bool allow = true) {
+// selection-block and outer-block options (for a batch of two or more
+// matrices, group, outer blocks from kOuterMinNGroup on unless the option
+// was set).
+hamsx::LuAux lu_options(hamsx_ctx* c, bool allow = true, bool group = false) {
   hamsx::LuAux a = c->lu_aux;
+  if (group && c->outer_default) a.outer_min_n = hamsx::kOuterMinNGroup;
   bool on = c->opt_lookahead != 0;
   if (c->opt_lookahead < 0) {
     const char* e = std::getenv("HAMSX_LU_LOOKAHEAD");
@@ -1487,6 +1491,7 @@ int hamsx_set_option(hamsx_ctx* c, int option, int value) {
       return 0;
     case HAMSX_OPT_LU_OUTER_MIN_N:
       c->lu_aux.outer_min_n = value;
+      c->outer_default = false;
       return 0;
     case HAMSX_OPT_GRAPHS:
       if (value != 0 && value != 1) return fail(HAMSX_EARG, "HAMSX_OPT_GRAPHS is 0 or 1");
@@ -2168,7 +2173,7 @@ int hamsx_submit_frequency_group(hamsx_ctx** ctx, int count, const double* w1, d
     b.m[i] = hamsx::LuMat{c64 ? static_cast<void*>(c->Af.p) : static_cast<void*>(system_matrix(c)),
                           c->ipiv.p, c->perm.p, c->widx.p, c->wcnt.p, c->ludiag.p, c->flags.p + 1};
   }
-  const hamsx::LuAux o = lu_options(c0);
+  const hamsx::LuAux o = lu_options(c0, true, count > 1);
   {
     StageTimer tm(c0, 2);
     hamsx::launch::lu_factor_batch(c0->stream, b, count, n, c64, &o);

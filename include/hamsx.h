@@ -59,7 +59,10 @@ int hamsx_set_precision(hamsx_ctx *ctx, int mode);
  *                            pivot-selection block (fewer tournament levels
  *                            with larger blocks; the pivots may differ);
  *   HAMSX_OPT_LU_OUTER_MIN_N 6144 (default) -- outer blocks of four panels
- *                            from this order on, <= 0 never.
+ *                            from this order on, <= 0 never; unless set, a
+ *                            frequency group of two or more matrices uses
+ *                            them from 1024 on (its selection steps serve
+ *                            the whole group).
  * HAMSX_EARG for an unknown option or value. */
 #define HAMSX_OPT_LU_LOOKAHEAD 1
 #define HAMSX_OPT_LU_SELECT_ROWS 2
@@ -230,7 +233,10 @@ int hamsx_collect(hamsx_ctx *ctx, double *amas, double *bdmp, double *exfc);
  * selection's dependent steps serve the whole group), and each context then
  * solves and reduces on its own stream.  Every context ends up pending, as
  * after hamsx_submit_frequency, and is collected the same way; each result
- * is the one its context gives alone, bit for bit.  No counterpart in the
+ * is the one its context gives alone under the same LU options, bit for bit
+ * (a group of two or more takes outer blocks from order 1024 on unless
+ * HAMSX_OPT_LU_OUTER_MIN_N was set, so its default factors differ from a
+ * single solve's by rounding only).  No counterpart in the
  * reference, which factors each frequency's matrix by its own ZGETRF
  * (src/AssbMatxMulti.f90:110-114). */
 #define HAMSX_GROUP_MAX 16

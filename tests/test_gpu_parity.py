@@ -483,6 +483,36 @@ def test_frequency_group_equals_single_solves(mesh_small, precision, outer, dept
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_frequency_group_default_outer_blocks(precision):
+    """With the LU options left alone a group of two or more matrices factors
+    in outer blocks from order 1024 on (single solves from 6144 on): at
+    N = 1,728 the group's coefficients equal the single solves' to the LU's
+    rounding (1e-10 of each table's largest entry)."""
+    m = small_pair(na=48, nz=12, nr=6)
+    assert m.nelem >= 1024
+    heads = np.deg2rad([0.0, 90.0])
+    ws = [0.6, 1.3]
+    say(f"[gpu] default group vs single n={m.nelem} {precision}")
+    with FrequencySolver(m, precision=precision) as s:
+        s.set_waves(heads)
+        ref = [s.solve_frequency(w, 40.0) for w in ws]
+    ctxs = [FrequencySolver(m, precision=precision) for _ in ws]
+    try:
+        for c in ctxs:
+            c.set_waves(heads)
+        FrequencySolver.submit_group(ctxs, ws, 40.0)
+        got = [c.collect() for c in ctxs]
+    finally:
+        for c in ctxs:
+            c.close()
+    for r, g in zip(ref, got):
+        assert r[3] == 0 and g[3] == 0
+        errs = [rel(a, b) for a, b in zip(g[:3], r[:3])]
+        say(f"    group vs single: {errs}")
+        assert max(errs) < 1e-10
+
+
 @pytest.mark.parametrize("depth", [-1.0, 35.0])
 def test_gmres_matches_lu(depth):
     """Restarted GMRES with the block-diagonal preconditioner converges to the
