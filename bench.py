@@ -194,6 +194,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay each context's recorded HIP graph of the solve "
                     "instead of launching it eagerly")
+    ap.add_argument("--solver", default="lu", choices=["lu", "gmres"],
+                    help="gmres: the reference's ISOLV = 2 path (GMRES(25), tolerance 1e-8, block-LU "
+                         "preconditioner) on the H-matrix (eta 2, ACA 1e-6) instead of the dense LU; "
+                         "independent submissions (no frequency groups)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision and five-cylinder runs that "
                     "the default FP64 line carries beside its value")
     ap.add_argument("--cpu-rows", type=int, default=256, help="field panels in the CPU baseline's assembly sample")
@@ -202,6 +206,8 @@ def parse():
                     help="only run a plain device-to-device copy of MIB MiB three times (the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.solver == "gmres":
+        args.group = 1
     if args.streams is None:
         args.streams = 4 if args.config == "serving" or args.group <= 1 else args.group
     args.precision_set = args.precision is not None
@@ -550,18 +556,21 @@ def main():
           file=sys.stderr, flush=True)
     group = args.group if args.group > 1 and "symmetry" not in case.meta else 1
     run = throughput(case, mesh, args.precision, args.streams, args.steps, args.warmup, args.stage_steps,
-                     graphs=args.graphs, group=group)
+                     graphs=args.graphs, group=group, solver=args.solver)
     ms, elapsed, lat_ms = run["stages"], run["elapsed"], run["lat_ms"]
     print(f"[bench] timed {args.steps} solves in {elapsed:.2f} s", file=sys.stderr, flush=True)
     roof = roofline(case, mesh, ms, args.precision)
     lu_peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_MFMA_TFLOPS
     lu_ms = ms.get("factor_group", ms["factor"])
-    lu_tf = 8.0 * n ** 3 / 3.0 / (lu_ms * 1e-3) / 1e12
-    roof_lu = {"kernel": "LU factorisation" + (f" (one batched launch sequence for a group of {group} matrices; "
-                                               "time per matrix)" if group > 1 else ""),
-               "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
-               "frac": lu_tf / lu_peak, "launch_ms": lu_ms, "single_matrix_ms": ms["factor"], "flops": 8.0 * n ** 3 / 3.0}
-    roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
+    roof_lu = roof_solve = None
+    if lu_ms > 0.0:  # the dense LU path (GMRES factors no matrix)
+        lu_tf = 8.0 * n ** 3 / 3.0 / (lu_ms * 1e-3) / 1e12
+        roof_lu = {"kernel": "LU factorisation" + (f" (one batched launch sequence for a group of {group} matrices; "
+                                                   "time per matrix)" if group > 1 else ""),
+                   "bound": "mfma", "unit": "TFLOP/s", "achieved": lu_tf, "peak": lu_peak,
+                   "frac": lu_tf / lu_peak, "launch_ms": lu_ms, "single_matrix_ms": ms["factor"],
+                   "flops": 8.0 * n ** 3 / 3.0}
+        roof_solve = solve_roofline(case, mesh, len(case.headings_deg), elapsed / args.steps * 1e3, lu_peak)
     vs_full = None
     if "symmetry" in case.meta:
         vs_full = half_vs_full(case, mesh, args.precision)
@@ -574,7 +583,8 @@ def main():
         "config": {"workload": case.name, "panels": n, "symmetry": case.meta.get("symmetry"),
                    "depth_m": case.depth, "frequencies": len(case.omegas),
                    "headings_deg": case.headings_deg, "precision": args.precision,
-                   "in_flight": max(1, args.streams), "group": group, "graphs": args.graphs},
+                   "in_flight": max(1, args.streams), "group": group, "graphs": args.graphs,
+                   "solver": args.solver},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "stage_ms_per_solve": ms,
         "roofline": roof,
@@ -583,7 +593,8 @@ def main():
     }
     if vs_full is not None:
         out["vs_full_mesh"] = vs_full
-    if args.precision == "fp64" and not args.no_mixed and args.config in ("two-cylinders", "five-cylinders"):
+    if (args.precision == "fp64" and not args.no_mixed and args.solver == "lu"
+            and args.config in ("two-cylinders", "five-cylinders")):
         # the headline: the mixed-precision path on the same workload, and both
         # paths on the five-cylinder (10,026-panel) workload
         out["mixed"] = mixed_vs_fp64(case, mesh, run, args.streams, args.steps, args.warmup, group)
@@ -603,7 +614,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graphs=False, group=1):
+def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graphs=False, group=1, solver="lu"):
     """`steps` frequency solves of case.omegas (in order, cycling) with
     `streams` solves in flight (one context and HIP stream each), after
     per-stage timing of `stage_steps` isolated solves and `warmup` untimed
@@ -621,6 +632,9 @@ def throughput(case, mesh, precision, streams, steps, warmup, stage_steps, graph
         sv = FrequencySolver(mesh, precision=precision)
         sv.set_options(graphs=1 if graphs else 0)
         sv.set_waves(heads, amp=case.amp, xw=case.xw, rho=case.rho)
+        if solver == "gmres":
+            sv.set_solver("gmres", tol=1e-8, restart=25, maxiter=400)
+            sv.set_hmatrix(True, 2.0, 1e-6, 32, 50)
         if "symmetry" in case.meta:
             sv.set_symmetry(*case.meta["symmetry"])
         solvers.append(sv)
