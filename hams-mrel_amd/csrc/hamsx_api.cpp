@@ -271,7 +271,10 @@ struct hamsx_ctx {
   std::vector<hamsx::HNode> hm_tree;
   std::vector<hamsx::HBlk> hm_blocks;
   std::vector<int> hm_aca_idx;  // partition index of each ACA block
-  DevBuf<int> hmPerm, hmRank, hmLptr, hmLitem, hmVxList;
+  DevBuf<int> hmPerm, hmRank, hmLptr, hmLitem;
+  DevBuf<int2> hmVxItem, hmVxRed;
+  DevBuf<size_t> hmVxOut, hmVxRbase;
+  int hm_vx_nred = 0;
   int hm_vx_start[5] = {0, 0, 0, 0, 0};
   DevBuf<int4> hmAdesc, hmDdesc;
   DevBuf<int2> hmLeaves;
@@ -371,6 +374,12 @@ struct hamsx_ctx {
   bool have_A = false, have_S = false, have_LU = false, have_X = false;
   bool pending = false;  // a submitted frequency solve not yet collected
   bool have_C = false;   // C = A^T A is current (lid only)
+  // C formed in complex64 (mixed precision, GMRES, lid, no symmetry plane):
+  // its FP64 copy carries complex64 accuracy, so GMRES takes its residuals
+  // from A itself; A32 / C32 the complex64 copy of A and the product
+  bool c_fp32 = false;
+  DevBuf<float2> A32, C32;
+  DevBuf<double2> nY;  // A X, (N + L) x nrhs, for those residuals
   int last_iters = 0;
   double last_resid = 0.0;
   // stage timing (measurement hook)
@@ -682,10 +691,45 @@ int do_rhs(hamsx_ctx* c) {
 // the matrix the LU, the refinement and GMRES then work on.  The plain
 // transpose makes C complex symmetric, so only its lower triangle is
 // computed (zsyrk, half the flops of the full product) and then mirrored.
+// Diagnostic switch: HAMSX_NORMAL_FP64=1 forms C = A^T A in FP64 on the
+// mixed-precision GMRES path too (for comparisons).
+bool force_normal_fp64() {
+  const char* e = std::getenv("HAMSX_NORMAL_FP64");
+  return e && *e && *e != '0';
+}
+
+// C = A^T A (plain transpose, AssbMatx_irrMulti.f90:154-171) with a lid.  In
+// mixed precision on the GMRES path (no symmetry plane) the product runs in
+// complex64 on FP32 MFMA (csyrk of a complex64 copy of A) and is widened to
+// FP64 for the H-matrix compression and the preconditioner: the Arnoldi
+// products read complex64 blocks anyway, and every residual GMRES tests is
+// taken from A itself in FP64 (normal_residual), so GMRES still converges
+// on the FP64 normal equations.
 int ensure_normal_matrix(hamsx_ctx* c) {
-  if (c->nlid == 0 || c->have_C) return 0;
+  if (c->nlid == 0) return 0;
+  const bool want32 = c->prec == HAMSX_PREC_MIXED && c->solver == HAMSX_SOLVER_GMRES && !c->sym() &&
+                      !force_normal_fp64();
+  if (c->have_C && c->c_fp32 == want32) return 0;
+  c->have_C = false;
   int rc;
   if ((rc = c->Cm.ensure((size_t)c->n * c->n)) || (rc = blas_handle(c))) return rc;
+  if (want32) {
+    const size_t na = (size_t)c->nf * c->n, nc = (size_t)c->n * c->n;
+    if ((rc = c->A32.ensure(na)) || (rc = c->C32.ensure(nc))) return rc;
+    hamsx::launch::to_c64(c->stream, c->A.p, c->A32.p, na);
+    const rocblas_float_complex onef(1.0f, 0.0f), zerof(0.0f, 0.0f);
+    if (rocblas_csyrk(c->blas, rocblas_fill_lower, rocblas_operation_transpose, c->n, c->nf, &onef,
+                      reinterpret_cast<const rocblas_float_complex*>(c->A32.p), c->nf, &zerof,
+                      reinterpret_cast<rocblas_float_complex*>(c->C32.p), c->n) != rocblas_status_success)
+      return fail(HAMSX_EDEVICE, "rocblas_csyrk (A^T A in complex64) failed");
+    hamsx::launch::to_c128(c->stream, c->C32.p, c->Cm.p, nc);
+    hamsx::launch::mirror_lower(c->stream, c->Cm.p, c->n);
+    if ((rc = check_launch())) return rc;
+    c->c_fp32 = true;
+    c->have_C = true;
+    return 0;
+  }
+  c->c_fp32 = false;
   if (c->sym() && (rc = c->Cm2.ensure((size_t)c->n * c->n))) return rc;
   const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
   for (int ip = 0; ip < (c->sym() ? 2 : 1); ip++) {  // with a symmetry plane: C2 = A2^T A2 as well
@@ -983,21 +1027,51 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
     doff.push_back(ds);
     ds += (size_t)b.nr * b.nc;
   }
-  // low-rank blocks bucketed by rank for the T = VH x pass
-  std::vector<int> vx;
+  // work items of the T = VH x pass: the low-rank blocks bucketed by rank,
+  // each in chunks of kVxCols columns (one workgroup per chunk); a block of
+  // several chunks gets partials after the blocks' T (from ts on) and a
+  // fixed-order reduction into its T
+  std::vector<int2> vx, red;
+  std::vector<size_t> vout, rbase;
+  size_t ps = ts;
   for (int g = 0; g < 4; g++) {
     c->hm_vx_start[g] = (int)vx.size();
     const int lo = g == 0 ? 1 : (4 << g) + 1, hi = 8 << g;
-    for (int a = 0; a < na; a++)
-      if (rank[a] >= lo && rank[a] <= hi) vx.push_back(a);
+    for (int a = 0; a < na; a++) {
+      if (rank[a] < lo || rank[a] > hi) continue;
+      const int nch = (adesc[a].w + hamsx::kVxCols - 1) / hamsx::kVxCols;
+      const size_t len = (size_t)rank[a] * nrhs;
+      if (nch > 1) {
+        red.push_back(make_int2(a, nch));
+        rbase.push_back(ps);
+      }
+      for (int ch = 0; ch < nch; ch++) {
+        vx.push_back(make_int2(a, ch));
+        vout.push_back(nch > 1 ? ps + (size_t)ch * len : toff[a]);
+      }
+      if (nch > 1) ps += (size_t)nch * len;
+    }
   }
   c->hm_vx_start[4] = (int)vx.size();
-  if ((rc = c->hmVxList.ensure(std::max<size_t>(vx.size(), 1)))) return rc;
-  if (!vx.empty())
-    HX_TRY(hipMemcpyAsync(c->hmVxList.p, vx.data(), sizeof(int) * vx.size(), hipMemcpyHostToDevice, c->stream));
+  c->hm_vx_nred = (int)red.size();
+  if ((rc = c->hmVxItem.ensure(std::max<size_t>(vx.size(), 1))) ||
+      (rc = c->hmVxOut.ensure(std::max<size_t>(vx.size(), 1))) ||
+      (rc = c->hmVxRed.ensure(std::max<size_t>(red.size(), 1))) ||
+      (rc = c->hmVxRbase.ensure(std::max<size_t>(red.size(), 1))))
+    return rc;
+  if (!vx.empty()) {
+    HX_TRY(hipMemcpyAsync(c->hmVxItem.p, vx.data(), sizeof(int2) * vx.size(), hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmVxOut.p, vout.data(), sizeof(size_t) * vout.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  if (!red.empty()) {
+    HX_TRY(hipMemcpyAsync(c->hmVxRed.p, red.data(), sizeof(int2) * red.size(), hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmVxRbase.p, rbase.data(), sizeof(size_t) * rbase.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  }
   const int nd = (int)ddesc.size();
   if ((rc = c->hmDdesc.ensure(std::max(nd, 1))) || (rc = c->hmDoff.ensure(std::max(nd, 1))) ||
-      (rc = c->hmD.ensure(std::max<size_t>(ds, 1))) || (rc = c->hmT.ensure(std::max<size_t>(ts, 1))) ||
+      (rc = c->hmD.ensure(std::max<size_t>(ds, 1))) || (rc = c->hmT.ensure(std::max<size_t>(ps, 1))) ||
       (rc = c->hmXp.ensure((size_t)n * nrhs)) || (rc = c->hmYp.ensure((size_t)n * nrhs)))
     return rc;
   if (na > 0) {
@@ -1055,8 +1129,12 @@ hamsx::HmDev hm_dev(hamsx_ctx* c) {
   h.perm = c->hmPerm.p;
   h.nlr = (int)c->hm_aca_idx.size();
   h.nleaf = c->hm_nleaf;
-  h.vx_list = c->hmVxList.p;
+  h.vx_item = c->hmVxItem.p;
+  h.vx_out = c->hmVxOut.p;
   for (int g = 0; g < 5; g++) h.vx_start[g] = c->hm_vx_start[g];
+  h.vx_red = c->hmVxRed.p;
+  h.vx_rbase = c->hmVxRbase.p;
+  h.vx_nred = c->hm_vx_nred;
   h.ldesc = c->hmAdesc.p;
   h.rank = c->hmRank.p;
   h.uoff = c->hmUoff.p;
@@ -1079,10 +1157,29 @@ hamsx::HmDev hm_dev(hamsx_ctx* c) {
   return h;
 }
 
+// W = B - A^T (A X) in FP64 from the assembled A (N + L rows): the residual
+// of the normal equations when C itself was formed in complex64.
+int normal_residual(hamsx_ctx* c, const double2* X, int nrhs, const double2* B, double2* W) {
+  int rc;
+  if ((rc = c->nY.ensure((size_t)c->nf * nrhs)) || (rc = blas_handle(c))) return rc;
+  const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
+  HX_TRY(hipMemcpyAsync(W, B, sizeof(double2) * (size_t)c->n * nrhs, hipMemcpyDeviceToDevice, c->stream));
+  if (rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->nf, nrhs, c->n, &one,
+                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf,
+                    reinterpret_cast<const rocblas_double_complex*>(X), c->n, &zero,
+                    reinterpret_cast<rocblas_double_complex*>(c->nY.p), c->nf) != rocblas_status_success ||
+      rocblas_zgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->n, nrhs, c->nf, &mone,
+                    reinterpret_cast<const rocblas_double_complex*>(c->A.p), c->nf,
+                    reinterpret_cast<const rocblas_double_complex*>(c->nY.p), c->nf, &one,
+                    reinterpret_cast<rocblas_double_complex*>(W), c->n) != rocblas_status_success)
+    return fail(HAMSX_EDEVICE, "rocblas_zgemm (B - A^T A X) failed");
+  return 0;
+}
+
 // the GMRES operator: Y = beta B + alpha Op X, Op the H-matrix when built,
 // else the dense matrix.  inner: a product inside an Arnoldi cycle, which in
 // mixed precision reads the complex64 blocks; the residuals at each restart
-// and the final one use the FP64 blocks.
+// and the final one use the FP64 blocks (or A itself, normal_residual).
 void op_matvec(hamsx_ctx* c, const double2* Aop, const double2* X, int nrhs, const double2* B, double alpha,
                double beta, double2* Y, bool inner = false) {
   if (c->hm_built)
@@ -1127,6 +1224,8 @@ int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
   if (c->hm_on && (rc = hm_build(c, Aop, nr))) return rc;
   const hamsx::GmDev gd{c->gH.p, c->gG.p, c->gSn.p, c->gCs.p, c->gBn.p, c->gAct.p, c->gDone.p, c->gMact.p,
                         c->gIters.p};
+  // C formed in complex64: the residuals come from A itself (system 1 only)
+  const bool exact_normal = c->c_fp32 && Aop == c->Cm.p;
   volatile int* pin = c->gm_pin;
   // B = right-hand sides, X = 0
   HX_TRY(hipMemcpyAsync(c->gB.p, X, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
@@ -1141,7 +1240,9 @@ int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
     // device and the host waits once per cycle for the count of columns left
     if (iter == 0)
       HX_TRY(hipMemcpyAsync(c->gW.p, c->gB.p, sizeof(double2) * blk, hipMemcpyDeviceToDevice, c->stream));
-    else
+    else if (exact_normal) {
+      if ((rc = normal_residual(c, X, nr, c->gB.p, c->gW.p))) return rc;
+    } else
       op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
     hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
     hamsx::launch::gm_restart(c->stream, gd, nr, m, c->gNorm.p, c->gm_tol, c->gScale.p, c->gCnt.p, c->gWorst.p);
@@ -1184,7 +1285,10 @@ int do_solve_gmres(hamsx_ctx* c, const double2* Aop, double2* X) {
     hamsx::launch::axpy_update(c->stream, X, c->gZ.p, blk);
   }
   // final residual of the operator GMRES worked with, for the statistics
-  op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
+  if (exact_normal) {
+    if ((rc = normal_residual(c, X, nr, c->gB.p, c->gW.p))) return rc;
+  } else
+    op_matvec(c, Aop, X, nr, c->gB.p, -1.0, 1.0, c->gW.p);
   hamsx::launch::col_norm2(c->stream, c->gW.p, n, nr, c->gNorm.p);
   std::vector<double> beta(nr), bnorm(nr);
   int iters = 0;

@@ -105,13 +105,24 @@ struct WaveDev {
 // blocks (ddesc/doff/D), and per leaf cluster of rows [leaves[k].x,
 // leaves[k].y) the blocks covering it, litem[lptr[k] .. lptr[k+1]) (item >= 0
 // dense block, ~item low-rank block).  perm: cluster position -> panel.
+// columns of a low-rank block per work item of the T = VH x pass
+constexpr int kVxCols = 256;
 struct HmDev {
   const int* perm;
   int nlr, nleaf;
-  // low-rank blocks with rank > 0 grouped by rank (<= 8, 16, 32, 64):
-  // group g is vx_list[vx_start[g] .. vx_start[g+1])
-  const int* vx_list;
+  // T = VH x work items of the low-rank blocks with rank > 0, grouped by
+  // rank (<= 8, 16, 32, 64): group g is items vx_start[g] .. vx_start[g+1];
+  // item k covers columns [chunk kVxCols, (chunk + 1) kVxCols) of block
+  // vx_item[k] = (block, chunk) and writes its rank x nrhs product at
+  // T + vx_out[k]: the block's T itself when it has one chunk, else a partial
+  // that k_hm_vx_reduce adds up, chunk by chunk, for the vx_nred blocks
+  // vx_red[r] = (block, chunks) with partials from T + vx_rbase[r]
+  const int2* vx_item;
+  const size_t* vx_out;
   int vx_start[5];
+  const int2* vx_red;
+  const size_t* vx_rbase;
+  int vx_nred;
   const int4* ldesc;
   const int* rank;
   const size_t *uoff, *voff, *toff;
@@ -242,6 +253,7 @@ void field_finish(hipStream_t s, int nfp, const double* px, const double* py, co
 void motions(hipStream_t s, int nd, int nbeta, double w1, const double* coef, const double* bm, int quad,
              double2* work, double2* dspl, int* iters);
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count);
+void to_c128(hipStream_t s, const float2* Af, double2* A, size_t count);
 // C(r, c) = C(c, r) above the diagonal (n x n): completes a lower-triangle syrk
 void mirror_lower(hipStream_t s, double2* C, int n);
 void lu_factor_c64(hipStream_t s, float2* A, int n, int* ipiv, int* perm, int* work_idx, int* work_cnt, int* info,

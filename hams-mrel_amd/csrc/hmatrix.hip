@@ -241,34 +241,39 @@ __device__ __forceinline__ void stage_rows(double2 (&xs)[kHmJ][kHmQ], const doub
 }
 
 // T_b(l, q) = sum_j VH_b(l, j) Xp(cs + j, q), rank_b x nrhs (column-major),
-// for the low-rank blocks list[0 .. count) whose ranks are at most 2 KPER.
-// Workgroup (block, column group of kHmQ): thread (column q = t % 128,
-// parity p = t / 128) holds rows l = p + 2k, k < KPER; VH and the
-// right-hand-side rows go through LDS in chunks of kHmJ columns.
+// for low-rank blocks whose ranks are at most 2 KPER, over the columns of one
+// work item: item[k] = (block, chunk) sums the chunk's kVxCols columns, so
+// a wide block spreads over several workgroups instead of one walking all of
+// its columns, and writes the sum at T + out[k] (k_hm_vx_reduce adds a split
+// block's chunks).  Workgroup (item, column group of kHmQ): thread (column
+// q = t % 128, parity p = t / 128) holds rows l = p + 2k, k < KPER; VH and
+// the right-hand-side rows go through LDS in chunks of kHmJ columns.
 __device__ __forceinline__ double2 as_d2(double2 v) { return v; }
 __device__ __forceinline__ double2 as_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 
 // S: the storage of the blocks' factors and dense entries -- double2, or
 // float2 (complex64 blocks of the mixed path); arithmetic is FP64 either way.
 template <int KPER, typename S>
-__global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, const int4* __restrict__ desc,
-                                               const int* __restrict__ rank, const size_t* __restrict__ voff,
-                                               const size_t* __restrict__ toff, const S* __restrict__ VH,
+__global__ __launch_bounds__(256) void k_hm_vx(const int2* __restrict__ item, const size_t* __restrict__ out,
+                                               const int4* __restrict__ desc, const int* __restrict__ rank,
+                                               const size_t* __restrict__ voff, const S* __restrict__ VH,
                                                const double2* __restrict__ Xp, int n, int nrhs,
                                                double2* __restrict__ T) {
   __shared__ double2 xs[kHmJ][kHmQ];
   __shared__ double2 vs[2 * KPER][kHmJ];
-  const int b = list[blockIdx.x];
+  const int2 it = item[blockIdx.x];
+  const int b = it.x;
   const int4 d = desc[b];
   const int rk = rank[b];
+  const int jb = it.y * kVxCols, je = min(d.w, jb + kVxCols);  // this item's columns of the block
   const int q0 = blockIdx.y * kHmQ, nq = min(kHmQ, nrhs - q0);
   const int q = threadIdx.x & (kHmQ - 1), p = threadIdx.x >> 7;
   double ar[KPER], ai[KPER];
 #pragma unroll
   for (int k = 0; k < KPER; k++) ar[k] = ai[k] = 0.0;
   const S* v = VH + voff[b];
-  for (int j0 = 0; j0 < d.w; j0 += kHmJ) {
-    const int jn = min(kHmJ, d.w - j0);
+  for (int j0 = jb; j0 < je; j0 += kHmJ) {
+    const int jn = min(kHmJ, je - j0);
     __syncthreads();
     stage_rows(xs, Xp + (size_t)d.z + j0 + (size_t)n * q0, (size_t)n, jn, nq);
     for (int e = threadIdx.x; e < 2 * KPER * kHmJ; e += 256) {
@@ -290,8 +295,29 @@ __global__ __launch_bounds__(256) void k_hm_vx(const int* __restrict__ list, con
 #pragma unroll
     for (int k = 0; k < KPER; k++) {
       const int l = p + 2 * k;
-      if (l < rk) T[toff[b] + (size_t)l + (size_t)rk * (q0 + q)] = make_double2(ar[k], ai[k]);
+      if (l < rk) T[out[blockIdx.x] + (size_t)l + (size_t)rk * (q0 + q)] = make_double2(ar[k], ai[k]);
     }
+}
+
+// T of the blocks split into several column chunks: the chunks' partial
+// products added in chunk order (a fixed order, so the result does not
+// depend on scheduling).  One workgroup per block.
+__global__ __launch_bounds__(256) void k_hm_vx_reduce(const int2* __restrict__ red, const size_t* __restrict__ rbase,
+                                                      const int* __restrict__ rank, const size_t* __restrict__ toff,
+                                                      int nrhs, double2* __restrict__ T) {
+  const int2 r = red[blockIdx.x];
+  const size_t len = (size_t)rank[r.x] * nrhs;
+  const double2* P = T + rbase[blockIdx.x];
+  double2* dst = T + toff[r.x];
+  for (size_t e = threadIdx.x; e < len; e += 256) {
+    double2 acc = P[e];
+    for (int c = 1; c < r.y; c++) {
+      const double2 v = P[(size_t)c * len + e];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    dst[e] = acc;
+  }
 }
 
 // Per leaf cluster of rows: thread (row lane t / 8, column lane t % 8) sums,
@@ -403,25 +429,25 @@ static void hm_matvec_t(hipStream_t s, const HmDev& h, const S* U, const S* VH, 
   const dim3 g1((unsigned)((tot + 255) / 256));
   HX_LAUNCH(k_perm_gather, g1, dim3(256), 0, s, h.perm, X, n, nrhs, Xp);
   const unsigned qg = (unsigned)((nrhs + kHmQ - 1) / kHmQ);
-  // low-rank blocks bucketed by rank: <= 8, <= 16, <= 32, <= 64
-  const int* lists[4] = {h.vx_list, h.vx_list + h.vx_start[1], h.vx_list + h.vx_start[2], h.vx_list + h.vx_start[3]};
+  // low-rank blocks bucketed by rank (<= 8, <= 16, <= 32, <= 64), in work
+  // items of kVxCols columns each; then the split blocks' partials summed
   for (int g = 0; g < 4; g++) {
     const int cnt = h.vx_start[g + 1] - h.vx_start[g];
     if (cnt <= 0) continue;
     const dim3 grid(cnt, qg);
+    const int2* it = h.vx_item + h.vx_start[g];
+    const size_t* out = h.vx_out + h.vx_start[g];
     if (g == 0)
-      HX_LAUNCH((k_hm_vx<4, S>), grid, dim3(256), 0, s, lists[0], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n, nrhs,
-                h.T);
+      HX_LAUNCH((k_hm_vx<4, S>), grid, dim3(256), 0, s, it, out, h.ldesc, h.rank, h.voff, VH, Xp, n, nrhs, h.T);
     else if (g == 1)
-      HX_LAUNCH((k_hm_vx<8, S>), grid, dim3(256), 0, s, lists[1], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n, nrhs,
-                h.T);
+      HX_LAUNCH((k_hm_vx<8, S>), grid, dim3(256), 0, s, it, out, h.ldesc, h.rank, h.voff, VH, Xp, n, nrhs, h.T);
     else if (g == 2)
-      HX_LAUNCH((k_hm_vx<16, S>), grid, dim3(256), 0, s, lists[2], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n,
-                nrhs, h.T);
+      HX_LAUNCH((k_hm_vx<16, S>), grid, dim3(256), 0, s, it, out, h.ldesc, h.rank, h.voff, VH, Xp, n, nrhs, h.T);
     else
-      HX_LAUNCH((k_hm_vx<32, S>), grid, dim3(256), 0, s, lists[3], h.ldesc, h.rank, h.voff, h.toff, VH, Xp, n,
-                nrhs, h.T);
+      HX_LAUNCH((k_hm_vx<32, S>), grid, dim3(256), 0, s, it, out, h.ldesc, h.rank, h.voff, VH, Xp, n, nrhs, h.T);
   }
+  if (h.vx_nred > 0)
+    HX_LAUNCH(k_hm_vx_reduce, dim3(h.vx_nred), dim3(256), 0, s, h.vx_red, h.vx_rbase, h.rank, h.toff, nrhs, h.T);
   HX_LAUNCH(k_hm_leaf<S>, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, D,
             h.ldesc, h.rank, h.uoff, h.toff, U, h.T, Xp, n, nrhs, Yp);
   HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);

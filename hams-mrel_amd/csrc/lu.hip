@@ -964,6 +964,10 @@ __global__ void k_to_c64(const double2* __restrict__ A, float2* __restrict__ Af,
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < count; i += (size_t)gridDim.x * blockDim.x) Af[i] = make_float2((float)A[i].x, (float)A[i].y);
 }
+__global__ void k_to_c128(const float2* __restrict__ Af, double2* __restrict__ A, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < count; i += (size_t)gridDim.x * blockDim.x) A[i] = make_double2((double)Af[i].x, (double)Af[i].y);
+}
 
 // Y = beta B + alpha A X for up to kMvRhs right-hand sides per pass, reading
 // every entry of A once (the residual of the mixed path; HBM-bound).  Block
@@ -1132,6 +1136,9 @@ void mirror_lower(hipStream_t s, double2* C, int n) {
   if (n <= 1) return;
   const unsigned t = (unsigned)((n + 31) / 32);
   HX_LAUNCH(k_mirror_lower, dim3(t, t), dim3(256), 0, s, C, n);
+}
+void to_c128(hipStream_t s, const float2* Af, double2* A, size_t count) {
+  HX_LAUNCH(k_to_c128, dim3(4096), dim3(256), 0, s, Af, A, count);
 }
 void to_c64(hipStream_t s, const double2* A, float2* Af, size_t count) {
   HX_LAUNCH(k_to_c64, dim3(4096), dim3(256), 0, s, A, Af, count);

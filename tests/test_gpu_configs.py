@@ -226,3 +226,50 @@ def test_config5_lidded_array_gmres_hmatrix():
     for ok, errs, bound, bd_bound in checks:
         assert ok
         assert errs[0] <= bound and errs[2] <= bound and errs[1] <= bd_bound
+
+
+def test_config5_mixed_normal_matrix_in_complex64(monkeypatch):
+    """Mixed precision on the lidded array: C = A^T A formed in complex64
+    (csyrk on FP32 MFMA), its H-matrix blocks complex64 in the Arnoldi
+    products, and every residual GMRES tests taken from A itself in FP64.
+    Against the dense FP64 LU of the FP64 C, at the workload's ACA tolerance
+    1e-6, with test_config5's bounds (added mass and excitation 5e-6 of
+    their tables' largest entries, damping the bound the added mass implies
+    for it); and against the same path with C formed in FP64
+    (HAMSX_NORMAL_FP64=1)."""
+    case = configs.large_array(600, grid=2)
+    m = combine(case.bodies, case.xr)
+    lid = np.concatenate([panel_geometry(lm, np.zeros(3))[0] for lm in case.meta["lids"]])
+    heads = np.deg2rad(case.headings_deg)
+    say(f"[gpu] config 5 mixed, C in complex64: {case.name} N={m.nelem} lid={len(lid)} depth={case.depth}")
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    for w in (0.4, 1.4):
+        with FrequencySolver(m) as s:
+            s.set_lid(lid)
+            s.set_waves(heads)
+            lu = s.solve_frequency(w, case.depth)
+        assert lu[3] == 0
+        got = {}
+        for mode in ("0", "1"):
+            monkeypatch.setenv("HAMSX_NORMAL_FP64", mode)
+            with FrequencySolver(m, precision="mixed") as s:
+                s.set_lid(lid)
+                s.set_waves(heads)
+                s.set_solver("gmres", tol=1e-9, restart=25, maxiter=400)
+                s.set_hmatrix(True, 2.0, 1e-6, 32, 50)
+                t0 = time.perf_counter()
+                got[mode] = s.solve_frequency(w, case.depth)
+                t = time.perf_counter() - t0
+                it, res = s.refine_stats()
+            errs = [rel(a, b) for a, b in zip(got[mode][:3], lu[:3])]
+            say(f"    w={w} C in {'FP64' if mode == '1' else 'complex64'}: GMRES {it} iterations, residual {res:.1e}, "
+                f"{t * 1e3:.0f} ms; vs LU: added mass {errs[0]:.2e} damping {errs[1]:.2e} excitation {errs[2]:.2e}")
+            report_own(f"C {'FP64' if mode == '1' else 'complex64'} vs LU", got[mode], lu)
+            assert got[mode][3] == 0 and res < 1e-9
+            bd_bound = 5e-6 * max(1.0, w * np.max(np.abs(lu[0])) / np.max(np.abs(lu[1])))
+            assert errs[0] <= 5e-6 and errs[2] <= 5e-6 and errs[1] <= bd_bound
+        d = [rel(a, b) for a, b in zip(got["0"][:3], got["1"][:3])]
+        say(f"    complex64 C vs FP64 C: {d}")
