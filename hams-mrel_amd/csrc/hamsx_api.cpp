@@ -277,7 +277,12 @@ struct hamsx_ctx {
   int hm_vx_nred = 0;
   int hm_vx_start[5] = {0, 0, 0, 0, 0};
   DevBuf<int4> hmAdesc, hmDdesc;
-  DevBuf<int2> hmLeaves;
+  DevBuf<int2> hmLeaves, hmLeafRed;
+  DevBuf<int4> hmSeg;
+  DevBuf<size_t> hmSegOut, hmLeafRbase;
+  DevBuf<double2> hmYpart;
+  std::vector<int2> hm_leaves_h;  // the leaves' row ranges, cluster order
+  int hm_nseg = 0, hm_leaf_nred = 0;
   DevBuf<size_t> hmUoff, hmVoff, hmRoff, hmToff, hmDoff;
   DevBuf<unsigned char> hmUsed;
   DevBuf<double2> hmU, hmVH, hmD, hmT, hmXp, hmYp;
@@ -959,6 +964,7 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
       if (nd.left < 0) leaves.push_back(make_int2(nd.s, nd.e + 1));
     std::sort(leaves.begin(), leaves.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
     c->hm_nleaf = (int)leaves.size();
+    c->hm_leaves_h = leaves;
     if ((rc = c->hmLeaves.ensure(leaves.size()))) return rc;
     HX_TRY(hipMemcpyAsync(c->hmLeaves.p, leaves.data(), sizeof(int2) * leaves.size(), hipMemcpyHostToDevice,
                           c->stream));
@@ -1099,6 +1105,46 @@ int hm_build(hamsx_ctx* c, const double2* Aop, int nrhs) {
     litem.insert(litem.end(), lists[q].begin(), lists[q].end());
     lptr[q + 1] = (int)litem.size();
   }
+  // segments of at most kLeafSeg blocks per leaf (one workgroup each); a
+  // leaf of several segments gets partial rows and a fixed-order reduction
+  std::vector<int4> segs;
+  std::vector<size_t> sout, lrbase;
+  std::vector<int2> lred;
+  size_t yps = 0;
+  for (int q = 0; q < c->hm_nleaf; q++) {
+    const int ib = lptr[q], ie = lptr[q + 1];
+    const int nsg = std::max(1, (ie - ib + hamsx::kLeafSeg - 1) / hamsx::kLeafSeg);
+    const size_t len = (size_t)(c->hm_leaves_h[q].y - c->hm_leaves_h[q].x) * nrhs;
+    if (nsg > 1) {
+      lred.push_back(make_int2(q, nsg));
+      lrbase.push_back(yps);
+    }
+    for (int sgi = 0; sgi < nsg; sgi++) {
+      segs.push_back(make_int4(q, ib + sgi * hamsx::kLeafSeg, std::min(ie, ib + (sgi + 1) * hamsx::kLeafSeg),
+                               nsg > 1 ? 1 : 0));
+      sout.push_back(nsg > 1 ? yps + (size_t)sgi * len : 0);
+    }
+    if (nsg > 1) yps += (size_t)nsg * len;
+  }
+  c->hm_nseg = (int)segs.size();
+  c->hm_leaf_nred = (int)lred.size();
+  if ((rc = c->hmSeg.ensure(std::max<size_t>(segs.size(), 1))) ||
+      (rc = c->hmSegOut.ensure(std::max<size_t>(sout.size(), 1))) ||
+      (rc = c->hmLeafRed.ensure(std::max<size_t>(lred.size(), 1))) ||
+      (rc = c->hmLeafRbase.ensure(std::max<size_t>(lrbase.size(), 1))) ||
+      (rc = c->hmYpart.ensure(std::max<size_t>(yps, 1))))
+    return rc;
+  if (!segs.empty()) {
+    HX_TRY(hipMemcpyAsync(c->hmSeg.p, segs.data(), sizeof(int4) * segs.size(), hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmSegOut.p, sout.data(), sizeof(size_t) * sout.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  if (!lred.empty()) {
+    HX_TRY(hipMemcpyAsync(c->hmLeafRed.p, lred.data(), sizeof(int2) * lred.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HX_TRY(hipMemcpyAsync(c->hmLeafRbase.p, lrbase.data(), sizeof(size_t) * lrbase.size(), hipMemcpyHostToDevice,
+                          c->stream));
+  }
   if ((rc = c->hmLptr.ensure(lptr.size())) || (rc = c->hmLitem.ensure(std::max<size_t>(litem.size(), 1)))) return rc;
   HX_TRY(hipMemcpyAsync(c->hmLptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, c->stream));
   if (!litem.empty())
@@ -1149,6 +1195,13 @@ hamsx::HmDev hm_dev(hamsx_ctx* c) {
   h.leaves = c->hmLeaves.p;
   h.lptr = c->hmLptr.p;
   h.litem = c->hmLitem.p;
+  h.seg = c->hmSeg.p;
+  h.seg_out = c->hmSegOut.p;
+  h.nseg = c->hm_nseg;
+  h.Ypart = c->hmYpart.p;
+  h.leaf_red = c->hmLeafRed.p;
+  h.leaf_rbase = c->hmLeafRbase.p;
+  h.leaf_nred = c->hm_leaf_nred;
   if (c->hm_c64) {
     h.U32 = c->hmU32.p;
     h.VH32 = c->hmVH32.p;

@@ -107,6 +107,8 @@ struct WaveDev {
 // dense block, ~item low-rank block).  perm: cluster position -> panel.
 // columns of a low-rank block per work item of the T = VH x pass
 constexpr int kVxCols = 256;
+// blocks of a leaf's list per work item of the per-leaf pass
+constexpr int kLeafSeg = 8;
 struct HmDev {
   const int* perm;
   int nlr, nleaf;
@@ -133,6 +135,18 @@ struct HmDev {
   const double2* D;
   const int2* leaves;
   const int *lptr, *litem;
+  // the per-leaf pass in segments of at most kLeafSeg blocks of a leaf's
+  // list: seg[k] = (leaf, first item, end item, partial?) -- a leaf of one
+  // segment writes Yp itself, the others write rows x nrhs partials at
+  // Ypart + seg_out[k], which k_hm_leaf_reduce adds up for the leaf_nred
+  // leaves leaf_red[r] = (leaf, segments) from Ypart + leaf_rbase[r]
+  const int4* seg;
+  const size_t* seg_out;
+  int nseg;
+  double2* Ypart;
+  const int2* leaf_red;
+  const size_t* leaf_rbase;
+  int leaf_nred;
   // complex64 copies of U, VH and D (mixed precision), or null
   const float2 *U32 = nullptr, *VH32 = nullptr, *D32 = nullptr;
 };

@@ -320,23 +320,32 @@ __global__ __launch_bounds__(256) void k_hm_vx_reduce(const int2* __restrict__ r
   }
 }
 
-// Per leaf cluster of rows: thread (row lane t / 8, column lane t % 8) sums,
-// for its row and columns q0 + lane + 8k, the blocks listed for the leaf in
-// order; item >= 0: dense block, item < 0: low-rank block ~item.
-// Workgroup (leaf, column group of kHmQ); leaves of more than 32 rows are
-// done 32 rows at a time.
+// Per segment of a leaf cluster's block list: thread (row lane t / 8, column
+// lane t % 8) sums, for its row and columns q0 + lane + 8k, the segment's
+// blocks in list order; item >= 0: dense block, item < 0: low-rank block
+// ~item.  Workgroup (segment, column group of kHmQ); leaves of more than 32
+// rows are done 32 rows at a time.  A leaf of one segment writes its rows of
+// Yp; a segment of a longer list writes its partial rows (leading dimension
+// the leaf's row count) at Ypart + seg_out, summed by k_hm_leaf_reduce.
 template <typename S>
-__global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves, const int* __restrict__ lptr,
-                                                 const int* __restrict__ litem, const int4* __restrict__ ddesc,
+__global__ __launch_bounds__(256) void k_hm_leaf(const int4* __restrict__ seg, const size_t* __restrict__ seg_out,
+                                                 const int2* __restrict__ leaves, const int* __restrict__ litem,
+                                                 const int4* __restrict__ ddesc,
                                                  const size_t* __restrict__ doff, const S* __restrict__ D,
                                                  const int4* __restrict__ ldesc, const int* __restrict__ rank,
                                                  const size_t* __restrict__ uoff, const size_t* __restrict__ toff,
                                                  const S* __restrict__ U, const double2* __restrict__ T,
                                                  const double2* __restrict__ Xp, int n, int nrhs,
-                                                 double2* __restrict__ Yp) {
+                                                 double2* __restrict__ Yp, double2* __restrict__ Ypart) {
   __shared__ double2 xs[kHmJ][kHmQ];
-  const int2 lf = leaves[blockIdx.x];
+  const int4 sg = seg[blockIdx.x];
+  const int2 lf = leaves[sg.x];
   const int q0 = blockIdx.y * kHmQ, nq = min(kHmQ, nrhs - q0);
+  const int lrows = lf.y - lf.x;
+  // row r of column q at dst[obase + (r - rbase) + ld q]
+  double2* dst = sg.w ? Ypart : Yp;
+  const size_t obase = sg.w ? seg_out[blockIdx.x] : 0, ld = sg.w ? (size_t)lrows : (size_t)n;
+  const int rbase = sg.w ? lf.x : 0;
   const int rl = threadIdx.x >> 3, ql = threadIdx.x & 7;
   constexpr int kPer = kHmQ / 8;
   for (int r0 = lf.x; r0 < lf.y; r0 += 32) {
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves
     double ar[kPer], ai[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) ar[k] = ai[k] = 0.0;
-    for (int it = lptr[blockIdx.x]; it < lptr[blockIdx.x + 1]; it++) {
+    for (int it = sg.y; it < sg.z; it++) {
       const int item = litem[it];
       const S* a;           // this row of the block's left factor, column stride nr
       const double2* src;   // right factor rows, leading dimension ld
@@ -388,8 +397,31 @@ __global__ __launch_bounds__(256) void k_hm_leaf(const int2* __restrict__ leaves
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
         const int q = ql + 8 * k;
-        if (q < nq) Yp[(size_t)row + (size_t)n * (q0 + q)] = make_double2(ar[k], ai[k]);
+        if (q < nq) dst[obase + (size_t)(row - rbase) + ld * (q0 + q)] = make_double2(ar[k], ai[k]);
       }
+  }
+}
+
+// Rows of the leaves whose block lists were split into segments: the
+// segments' partial rows added in segment order (a fixed order).  One
+// workgroup per leaf.
+__global__ __launch_bounds__(256) void k_hm_leaf_reduce(const int2* __restrict__ red,
+                                                        const size_t* __restrict__ rbase,
+                                                        const int2* __restrict__ leaves, int n, int nrhs,
+                                                        const double2* __restrict__ Ypart, double2* __restrict__ Yp) {
+  const int2 r = red[blockIdx.x];
+  const int2 lf = leaves[r.x];
+  const int rows = lf.y - lf.x;
+  const size_t len = (size_t)rows * nrhs;
+  const double2* P = Ypart + rbase[blockIdx.x];
+  for (size_t e = threadIdx.x; e < len; e += 256) {
+    double2 acc = P[e];
+    for (int s = 1; s < r.y; s++) {
+      const double2 v = P[(size_t)s * len + e];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    Yp[(size_t)(lf.x + (int)(e % rows)) + (size_t)n * (e / rows)] = acc;
   }
 }
 
@@ -448,8 +480,11 @@ static void hm_matvec_t(hipStream_t s, const HmDev& h, const S* U, const S* VH, 
   }
   if (h.vx_nred > 0)
     HX_LAUNCH(k_hm_vx_reduce, dim3(h.vx_nred), dim3(256), 0, s, h.vx_red, h.vx_rbase, h.rank, h.toff, nrhs, h.T);
-  HX_LAUNCH(k_hm_leaf<S>, dim3(h.nleaf, qg), dim3(256), 0, s, h.leaves, h.lptr, h.litem, h.ddesc, h.doff, D,
-            h.ldesc, h.rank, h.uoff, h.toff, U, h.T, Xp, n, nrhs, Yp);
+  HX_LAUNCH(k_hm_leaf<S>, dim3(h.nseg, qg), dim3(256), 0, s, h.seg, h.seg_out, h.leaves, h.litem, h.ddesc, h.doff,
+            D, h.ldesc, h.rank, h.uoff, h.toff, U, h.T, Xp, n, nrhs, Yp, h.Ypart);
+  if (h.leaf_nred > 0)
+    HX_LAUNCH(k_hm_leaf_reduce, dim3(h.leaf_nred), dim3(256), 0, s, h.leaf_red, h.leaf_rbase, h.leaves, n, nrhs,
+              h.Ypart, Yp);
   HX_LAUNCH(k_perm_scatter, g1, dim3(256), 0, s, h.perm, Yp, n, nrhs, B, alpha, beta, Y);
 }
 void hm_matvec(hipStream_t s, const HmDev& h, int n, const double2* X, int nrhs, const double2* B, double alpha,
