@@ -75,17 +75,17 @@ __device__ __forceinline__ void crecip(R ar, R ai, R& rr, R& ri) {  // 1/(ar + i
 // rows in pivot order; the final level (FINAL, diag set) also writes the
 // factored NB x NB diagonal block L\U in pivot order and ZGETRF's INFO.
 //
-// Each candidate's NB entries live in registers as a rotating window: at
-// step c, slot k holds column c + k (k < NB - c) and the slots behind them
-// hold the multipliers of steps 0..c-1.  Every step rotates the window by
-// one and appends its multiplier, so the pivot column is always slot 0 and
-// the step loop stays rolled.  One barrier per step: each wave finds its own
-// largest |re| + |im| (a DPP wave reduction) and its first lane holding it
-// (a ballot), and that lane publishes its value, row number and window
-// (retired slots zeroed) to LDS; after the barrier every thread picks the
-// winner among the four waves and reads the pivot row from LDS.  Buffers
-// alternate between steps, so a step's writes never meet the previous
-// step's reads.
+// Each candidate's NB entries live in registers, slot k = column k, and the
+// NB steps are unrolled so every slot index is static: step c touches only
+// columns c.. (an earlier rolled form rotated the whole window by one slot
+// per step to keep the pivot column in slot 0, which cost a register move
+// per slot and an update of the retired slots every step).  One barrier per
+// step: each wave finds its own largest |re| + |im| (a DPP wave reduction)
+// and its first lane holding it (a ballot), and that lane publishes its
+// value, row number and live columns to LDS; after the barrier every thread
+// picks the winner among the waves and reads the pivot row from LDS.
+// Buffers alternate between steps, so a step's writes never meet the
+// previous step's reads.
 // ---------------------------------------------------------------------------
 constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one far trailing update
 // Outer blocks from this order on.  Measured (FP64, look-ahead): N = 3,968
@@ -157,26 +157,26 @@ __global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int c
       vi[q] = use ? (double)v[q].y : 0.0;
     }
   }
-  const int nsel = min(ncols, mc);
+  const int nsel = __builtin_amdgcn_readfirstlane(min(ncols, mc));
   int mystep = -1, zero_at = 0;
   const int wave = t >> 6, lane = t & 63;
-#pragma unroll 1
-  for (int c = 0; c < nsel; c++) {
+  // The steps are unrolled, so slot k holds column k throughout: step c
+  // publishes, reads and updates only the live columns c+1.. (the pivot
+  // column c itself is read for the multiplier), and the multiplier
+  // replaces column c in place.
+#pragma unroll
+  for (int c = 0; c < kPanelNB; c++) {
+    if (c >= nsel) continue;  // block-uniform; a constant trip count lets the loop unroll
     const int buf = c & 1;
-    const double m = active ? fabs(vr[0]) + fabs(vi[0]) : -1.0;
+    const double m = active ? fabs(vr[c]) + fabs(vi[c]) : -1.0;
     const double mw = __ockl_wfred_max_f64(m);
     // ties go to the lowest row; with NaN in every lane no lane matches and
     // lane 0 answers, so each wave always publishes a valid row
     const unsigned long long hit = __ballot(m == mw);
     const int first = hit ? __builtin_ctzll(hit) : 0;
     if (lane == first) {
-      // The final level keeps the multipliers in the retired slots, so it
-      // publishes them as zeros (the update then leaves them unchanged);
-      // earlier levels only need the pivot order and publish the raw window.
-      const int live = kPanelNB - c;  // slots holding columns c..NB-1
 #pragma unroll
-      for (int k = 0; k < kPanelNB; k++)
-        wwin[buf][wave][k] = (!FINAL || k < live) ? make_double2(vr[k], vi[k]) : make_double2(0.0, 0.0);
+      for (int k = c; k < kPanelNB; k++) wwin[buf][wave][k] = make_double2(vr[k], vi[k]);
       wmax[buf][wave] = mw;
       wrow[buf][wave] = t;
     }
@@ -195,28 +195,22 @@ __global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int c
       mystep = c;
       active = false;
     }
-    if (active) {
-      double lr = vr[0], li = vi[0];
-      if (nonzero) {
-        const double2 pv = wwin[buf][w][0];
-        double ir = 0.0, ii = 0.0;
-        crecip(pv.x, pv.y, ir, ii);
-        lr = vr[0] * ir - vi[0] * ii;
-        li = vr[0] * ii + vi[0] * ir;
+    if (active && nonzero) {
+      const double2 pv = wwin[buf][w][c];
+      double ir = 0.0, ii = 0.0;
+      crecip(pv.x, pv.y, ir, ii);
+      // the fused multiply-adds are written out, so the rounding does not
+      // depend on how the compiler contracts the expression in each step
+      const double lr = fma(vr[c], ir, -(vi[c] * ii));
+      const double li = fma(vi[c], ir, vr[c] * ii);
 #pragma unroll
-        for (int k = 1; k < kPanelNB; k++) {
-          const double2 u = wwin[buf][w][k];
-          vr[k] -= lr * u.x - li * u.y;
-          vi[k] -= lr * u.y + li * u.x;
-        }
+      for (int k = c + 1; k < kPanelNB; k++) {
+        const double2 u = wwin[buf][w][k];
+        vr[k] -= fma(lr, u.x, -(li * u.y));
+        vi[k] -= fma(li, u.x, lr * u.y);
       }
-#pragma unroll
-      for (int k = 0; k < kPanelNB - 1; k++) {
-        vr[k] = vr[k + 1];
-        vi[k] = vi[k + 1];
-      }
-      vr[kPanelNB - 1] = lr;
-      vi[kPanelNB - 1] = li;
+      vr[c] = lr;
+      vi[c] = li;
     }
     if (!nonzero && zero_at == 0) zero_at = c + 1;
   }
@@ -224,13 +218,12 @@ __global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int c
   if (t < nsel) idx_out[b * kPanelNB + t] = sidx[sel[t]];
   if (t == 0) cnt_out[b] = nsel;
   if (FINAL) {
-    // the pivot row of step s holds column (k + s) mod NB in slot k
+    // the pivot row of step s: the multipliers of steps 0..s-1 in slots
+    // 0..s-1 (row s of L) and row s of U in slots s..
     if (mystep >= 0)
 #pragma unroll
-      for (int k = 0; k < kPanelNB; k++) {
-        const int q = (k + mystep) & (kPanelNB - 1);
-        if (q < ncols) diag[mystep + kPanelNB * q] = {(R)vr[k], (R)vi[k]};
-      }
+      for (int k = 0; k < kPanelNB; k++)
+        if (k < ncols) diag[mystep + kPanelNB * k] = {(R)vr[k], (R)vi[k]};
     if (t == 0 && zero_at && info[0] == 0) info[0] = col0 + zero_at;
   }
 }
