@@ -275,16 +275,23 @@ __global__ void k_pivots_to_swaps(const LuBatch bat, int buf, int wsets, int nse
   }
 }
 
-// Apply the panel's swaps to every column (and to the permutation vector).
+// Apply the swaps of pivot rows [col0, col0 + nsel) in order to the columns
+// [a0, a1) and [b0, b1) (one thread per column) and, with do_perm, to the
+// permutation vector (one more thread).
 template <typename R>
-__global__ void k_swap_rows(const LuBatch bat, int n, int col0, int nsel) {
+__global__ void k_swap_rows(const LuBatch bat, int n, int col0, int nsel, int a0, int a1, int b0, int b1,
+                            int do_perm) {
   const LuMat& mat = bat.m[blockIdx.y];
   typename CT<R>::c2* __restrict__ A = static_cast<typename CT<R>::c2*>(mat.A);
   const int lda = n;
   const int* __restrict__ ipiv = mat.ipiv;
   int* __restrict__ perm = mat.perm;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j > n) return;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, na = a1 - a0, nb_ = b1 - b0;
+  int j;
+  if (tid < na) j = a0 + tid;
+  else if (tid < na + nb_) j = b0 + (tid - na);
+  else if (do_perm && tid == na + nb_) j = n;
+  else return;
   for (int c = 0; c < nsel; c++) {
     const int r = col0 + c, p = ipiv[r] - 1;
     if (p == r) continue;
@@ -560,17 +567,20 @@ static void lu_factor_t(hipStream_t s, const LuBatch& bat, int nbat, int n, cons
       nsets = nout;
     }
     HX_LAUNCH(k_pivots_to_swaps, dim3(1, nbat), dim3(64), 0, s, bat, cur, nsets0, nb, col0);
-    // the row swaps touch every column: the previous step's deferred update must be done
-    if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
-    pending = false;
-    HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256, nbat), dim3(256), 0, s, bat, n, col0, nb);
     // Outer blocks of kOuter columns: within the block each panel updates only
     // the block's columns; at the block's end the columns to its right get
     // U = L11^-1 A12 for all of the block's rows and one rank-kOuter update,
     // so the far trailing matrix is read and written once per block instead
     // of once per panel.
     const int rest = n - col0 - nb;
-    if (outer_min <= 0 || n < outer_min) {  // one panel at a time: U12 and the update over every trailing column
+    const bool outer = !(outer_min <= 0 || n < outer_min);
+    if (!outer) {
+      // the row swaps touch every column: the previous step's deferred update must be done
+      if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
+      pending = false;
+      HX_LAUNCH(k_swap_rows<R>, dim3((n + 1 + 255) / 256, nbat), dim3(256), 0, s, bat, n, col0, nb, 0, n, 0, 0, 1);
+    }
+    if (!outer) {  // one panel at a time: U12 and the update over every trailing column
       const int nrb = (rest + 255) / 256;
       HX_LAUNCH(k_panel_solve<R>, dim3(nrb > 0 ? nrb * 2 : 1, nbat), dim3(256), 0, s, bat, n, col0, nb, nrb, 0, 1,
                 col0 + nb, n);
@@ -598,6 +608,13 @@ static void lu_factor_t(hipStream_t s, const LuBatch& bat, int nbat, int n, cons
       continue;
     }
     const int blk0 = col0 - col0 % kOuter, blk1 = min(blk0 + kOuter, n);
+    // The panel's swaps on the block's own columns (and the permutation)
+    // only: the columns left and right of the block take the block's swaps
+    // at its end, so the next block's panels never wait for the far update
+    // of this one, which runs on the second stream meanwhile (and reads the
+    // block's L columns, which these swaps do not touch).
+    HX_LAUNCH(k_swap_rows<R>, dim3((blk1 - blk0 + 1 + 255) / 256, nbat), dim3(256), 0, s, bat, n, col0, nb, blk0,
+              blk1, 0, 0, 1);
     const int near = blk1 - col0 - nb;
     const int nrb = (rest + 255) / 256, ncb = (near + 255) / 256;
     HX_LAUNCH(k_panel_solve<R>, dim3(nrb + ncb > 0 ? nrb + ncb : 1, nbat), dim3(256), 0, s, bat, n, col0, nb, nrb, 0,
@@ -607,33 +624,47 @@ static void lu_factor_t(hipStream_t s, const LuBatch& bat, int nbat, int n, cons
       HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, col0 + nb, n,
                 col0 + nb, blk1, col0, nb, tm, ntiles, 0);
     }
+    if (col0 + nb == blk1) {
+      // the block's swaps on every other column, once the previous block's
+      // far update (which covers this block's far columns) is done
+      if (pending) (void)hipStreamWaitEvent(s, aux->update, 0);
+      pending = false;
+      const int other = n - (blk1 - blk0);
+      if (other > 0)
+        HX_LAUNCH(k_swap_rows<R>, dim3((other + 255) / 256, nbat), dim3(256), 0, s, bat, n, blk0, blk1 - blk0, 0,
+                  blk0, blk1, n, 0);
+    }
     if (col0 + nb == blk1 && blk1 < n) {
-      const int far = n - blk1, fcb = (far + 255) / 256;
-      // U of the far columns, one panel's rows after another
-      for (int q0 = blk0; q0 < blk1; q0 += kPanelNB) {
-        const int qnb = min(kPanelNB, blk1 - q0);
-        if (q0 > blk0) {
-          const int tn = (far + 63) / 64;
-          HX_LAUNCH(k_trailing_update<R>, dim3(((tn + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, q0, q0 + qnb,
-                    blk1, n, blk0, q0 - blk0, 1, tn, 0);
+      const int far = n - blk1, tm = (far + 63) / 64;
+      // U = L11^-1 A12 of the block's rows over the far columns [c0, c1),
+      // one panel's rows after another
+      auto u_far = [&](hipStream_t st, int c0, int c1) {
+        const int tn = (c1 - c0 + 63) / 64, fcb = (c1 - c0 + 255) / 256;
+        for (int q0 = blk0; q0 < blk1; q0 += kPanelNB) {
+          const int qnb = min(kPanelNB, blk1 - q0);
+          if (q0 > blk0)
+            HX_LAUNCH(k_trailing_update<R>, dim3(((tn + 7) / 8) * 8, nbat), dim3(256), 0, st, bat, n, q0, q0 + qnb,
+                      c0, c1, blk0, q0 - blk0, 1, tn, 0);
+          HX_LAUNCH(k_panel_solve<R>, dim3(fcb, nbat), dim3(256), 0, st, bat, n, q0, qnb, 0, 1, 0, c0, c1);
         }
-        HX_LAUNCH(k_panel_solve<R>, dim3(fcb, nbat), dim3(256), 0, s, bat, n, q0, qnb, 0, 1, 0, blk1, n);
-      }
-      // the rank-kOuter update of the far trailing matrix
-      const int tm = (far + 63) / 64, tiles = tm;
-      if (!la || tiles <= 2) {
-        const int ntiles = tm * tiles;
-        HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, blk1, n, blk1,
-                  n, blk0, blk1 - blk0, tm, ntiles, 0);
+      };
+      // (U of all far columns stays on this stream: the second stream carries
+      // the far update, which bounds the early blocks, and a U chain there
+      // measured slower)
+      u_far(s, blk1, n);
+      if (!la || tm <= 2) {
+        HX_LAUNCH(k_trailing_update<R>, dim3(((tm * tm + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, blk1, n, blk1,
+                  n, blk0, blk1 - blk0, tm, tm * tm, 0);
       } else {
-        // look-ahead: the next outer block's columns (two tiles) here, the
-        // rest on the second stream, overlapping the next pivot selection,
-        // which reads only the next panel's columns
+        // look-ahead: the rank-kOuter update of the next outer block's
+        // columns (two tiles) here, the rest on the second stream,
+        // overlapping the next block's panels, which touch only their own
+        // columns
         HX_LAUNCH(k_trailing_update<R>, dim3(((2 * tm + 7) / 8) * 8, nbat), dim3(256), 0, s, bat, n, blk1, n, blk1,
                   n, blk0, blk1 - blk0, tm, 2 * tm, 0);
         (void)hipEventRecord(aux->panel, s);
         (void)hipStreamWaitEvent(aux->s2, aux->panel, 0);
-        const int ntiles = tm * (tiles - 2);
+        const int ntiles = tm * (tm - 2);
         HX_LAUNCH(k_trailing_update<R>, dim3(((ntiles + 7) / 8) * 8, nbat), dim3(256), 0, aux->s2, bat, n, blk1, n,
                   blk1, n, blk0, blk1 - blk0, tm, ntiles, 2);
         (void)hipEventRecord(aux->update, aux->s2);
