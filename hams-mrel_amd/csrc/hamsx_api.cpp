@@ -2336,15 +2336,31 @@ int hamsx_submit_frequency_group(hamsx_ctx** ctx, int count, const double* w1, d
     hamsx::launch::lu_factor_batch(c0->stream, b, count, n, c64, &o);
   }
   if ((rc = check_launch())) return rc;
+  // FP64 with one right-hand-side count: the group's triangular solves as one
+  // batched launch sequence too (the refined complex64 solves stay per context)
+  bool batched = !c64;
+  for (int i = 1; i < count && batched; i++) batched = ctx[i]->nrhs() == c0->nrhs();
+  if (batched) {
+    hamsx::SolveBatch sb{};
+    for (int i = 0; i < count; i++) {
+      hamsx_ctx* c = ctx[i];
+      sb.m[i] = hamsx::SolveMat{system_matrix(c), c->perm.p, c->X.p, c->Xtmp.p, c->trinv.p};
+    }
+    StageTimer tm(c0, 3);
+    hamsx::launch::lu_solve_batch(c0->stream, sb, count, n, c0->nrhs());
+    if ((rc = check_launch())) return rc;
+  }
   HX_TRY(hipEventRecord(c0->grp_ev, c0->stream));
-  // each context: its solves and force reductions on its own stream
+  // each context: its solves (unless batched above) and force reductions on its own stream
   for (int i = 0; i < count; i++) {
     hamsx_ctx* c = ctx[i];
     if ((rc = set_device(c))) return rc;
     if (i > 0) HX_TRY(hipStreamWaitEvent(c->stream, c0->grp_ev, 0));
     c->have_LU = true;
     if (!c64) c->have_A = false;  // factored in place
-    if ((rc = solve_factored(c)) || (rc = enqueue_forces(c))) return rc;
+    if (batched) c->have_X = true;
+    else if ((rc = solve_factored(c))) return rc;
+    if ((rc = enqueue_forces(c))) return rc;
     if (c->queue) HX_TRY(hipLaunchHostFunc(c->stream, notify_done, c));
     c->pending = true;
   }

@@ -189,6 +189,19 @@ struct LuMat {
 struct LuBatch {
   LuMat m[kLuBatchMax];
 };
+// one factored complex128 system of a batched solve: factors, row
+// permutation, right-hand sides (overwritten by the solution), a work copy of
+// them and the inverted diagonal blocks (tri_inverse_len(n) entries)
+struct SolveMat {
+  const double2* LU;
+  const int* perm;
+  double2* X;
+  double2* Xtmp;
+  double2* inv;
+};
+struct SolveBatch {
+  SolveMat m[kLuBatchMax];
+};
 
 // GMRES(m) Hessenberg / Givens state on the device (krylov.hip)
 struct GmDev {
@@ -230,6 +243,10 @@ void lu_factor(hipStream_t s, double2* A, int n, int* ipiv, int* perm, int* work
 size_t tri_inverse_len(int n);
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
               int nrhs);
+// lu_solve for count <= kLuBatchMax factored systems of order n at once
+// (each launch covers all of them, grid.z = system); each X is what lu_solve
+// gives it alone, bit for bit.
+void lu_solve_batch(hipStream_t s, const SolveBatch& b, int count, int n, int nrhs);
 // AMAS/BDMP/EXFC reductions into out[] (see hamsx_api.cpp for the layout)
 // Xm / mirror: the mirror half's potentials and the symmetry plane
 // (0: x = 0, 1: y = 0); mirror = -1 without symmetry.

@@ -914,6 +914,31 @@ __global__ __launch_bounds__(256) void k_trsm_super_batched(const typename CT<R>
                             Y + x_off[z], nrhs, blockIdx.x);
 }
 
+// The solve kernels for a batch of systems (blockIdx.z = system of b): the
+// same bodies on each system's own buffers.
+__global__ void k_gather_rows_batch(const SolveBatch b, int n, int nrhs) {
+  const SolveMat& m = b.m[blockIdx.z];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i < n && q < nrhs) m.Xtmp[(size_t)i + (size_t)n * q] = m.X[(size_t)m.perm[i] + (size_t)n * q];
+}
+__global__ __launch_bounds__(256) void k_tri_inverse_batch(const SolveBatch b, int n) {
+  const SolveMat& m = b.m[blockIdx.z];
+  tri_inverse_body<double>(m.LU, n, m.inv, blockIdx.x, blockIdx.y);
+}
+// LOWER: rows updated in Xtmp, solved rows into X; else updated in X, solved into Xtmp
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_super_batch(const SolveBatch b, int n, int k0, int kb, int nrhs) {
+  const SolveMat& m = b.m[blockIdx.z];
+  trsm_super_body<double, LOWER>(m.LU, m.inv, n, k0, kb, LOWER ? m.Xtmp : m.X, LOWER ? m.X : m.Xtmp, nrhs,
+                                 blockIdx.x);
+}
+__global__ void k_copy_back_batch(const SolveBatch b, size_t count) {
+  const SolveMat& m = b.m[blockIdx.z];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    m.X[i] = m.Xtmp[i];
+}
+
 __global__ void k_gather_rows(const int* __restrict__ perm, const double2* __restrict__ src, double2* __restrict__ dst,
                               int n, int nrhs, const int* __restrict__ skip) {
   if (skip && *skip) return;
@@ -1155,6 +1180,24 @@ size_t tri_inverse_len(int n) { return (size_t)2 * ((n + kPanelNB - 1) / kPanelN
 void lu_solve(hipStream_t s, const double2* LU, int n, const int* perm, double2* X, double2* Xtmp, double2* inv,
               int nrhs) {
   lu_solve_t<double>(s, LU, n, perm, X, Xtmp, inv, nrhs);
+}
+void lu_solve_batch(hipStream_t s, const SolveBatch& b, int count, int n, int nrhs) {
+  if (nrhs <= 0 || count <= 0) return;
+  const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
+  HX_LAUNCH(k_gather_rows_batch, dim3((n + 255) / 256, nrhs, count), dim3(256), 0, s, b, n, nrhs);
+  HX_LAUNCH(k_tri_inverse_batch, dim3((n + kPanelNB - 1) / kPanelNB, 2, count), dim3(256), 0, s, b, n);
+  for (int k0 = 0; k0 < n; k0 += kSuper) {
+    const int kb = n - k0 < kSuper ? n - k0 : kSuper;
+    const int rest = n - k0 - kb;
+    const int g = rest > 0 ? (rest + 255) / 256 : 1;
+    HX_LAUNCH(k_trsm_super_batch<true>, dim3(g, groups, count), dim3(256), 0, s, b, n, k0, kb, nrhs);
+  }
+  for (int k0 = ((n - 1) / kSuper) * kSuper; k0 >= 0; k0 -= kSuper) {
+    const int kb = n - k0 < kSuper ? n - k0 : kSuper;
+    const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
+    HX_LAUNCH(k_trsm_super_batch<false>, dim3(g, groups, count), dim3(256), 0, s, b, n, k0, kb, nrhs);
+  }
+  HX_LAUNCH(k_copy_back_batch, dim3(256, 1, count), dim3(256), 0, s, b, (size_t)n * nrhs);
 }
 void mirror_lower(hipStream_t s, double2* C, int n) {
   if (n <= 1) return;
