@@ -2336,18 +2336,31 @@ int hamsx_submit_frequency_group(hamsx_ctx** ctx, int count, const double* w1, d
     hamsx::launch::lu_factor_batch(c0->stream, b, count, n, c64, &o);
   }
   if ((rc = check_launch())) return rc;
-  // FP64 with one right-hand-side count: the group's triangular solves as one
-  // batched launch sequence too (the refined complex64 solves stay per context)
-  bool batched = !c64;
-  for (int i = 1; i < count && batched; i++) batched = ctx[i]->nrhs() == c0->nrhs();
+  // With one right-hand-side count (and in mixed precision one refinement
+  // setting): the group's triangular solves, or its refined complex64 solves,
+  // as one batched launch sequence too
+  bool batched = true;
+  for (int i = 1; i < count && batched; i++)
+    batched = ctx[i]->nrhs() == c0->nrhs() &&
+              (!c64 || (ctx[i]->refine_iters == c0->refine_iters && ctx[i]->refine_tol == c0->refine_tol));
   if (batched) {
-    hamsx::SolveBatch sb{};
-    for (int i = 0; i < count; i++) {
-      hamsx_ctx* c = ctx[i];
-      sb.m[i] = hamsx::SolveMat{system_matrix(c), c->perm.p, c->X.p, c->Xtmp.p, c->trinv.p};
-    }
     StageTimer tm(c0, 3);
-    hamsx::launch::lu_solve_batch(c0->stream, sb, count, n, c0->nrhs());
+    if (!c64) {
+      hamsx::SolveBatch sb{};
+      for (int i = 0; i < count; i++) {
+        hamsx_ctx* c = ctx[i];
+        sb.m[i] = hamsx::SolveMat{system_matrix(c), c->perm.p, c->X.p, c->Xtmp.p, c->trinv.p};
+      }
+      hamsx::launch::lu_solve_batch(c0->stream, sb, count, n, c0->nrhs());
+    } else {
+      hamsx::RefineBatch rb{};
+      for (int i = 0; i < count; i++) {
+        hamsx_ctx* c = ctx[i];
+        rb.m[i] = hamsx::RefineMat{system_matrix(c), c->Af.p, c->perm.p, c->X.p, c->Xtmp.p, c->trinv.p,
+                                   c->B.p, c->D.p, c->R.p, c->mvw.p, c->norms.p, c->flags.p + 2};
+      }
+      hamsx::launch::refine_solve_batch(c0->stream, rb, count, n, c0->nrhs(), c0->refine_iters, c0->refine_tol);
+    }
     if ((rc = check_launch())) return rc;
   }
   HX_TRY(hipEventRecord(c0->grp_ev, c0->stream));

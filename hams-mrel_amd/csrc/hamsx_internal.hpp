@@ -202,6 +202,28 @@ struct SolveMat {
 struct SolveBatch {
   SolveMat m[kLuBatchMax];
 };
+// one system of a batched refined complex64 solve: the FP64 matrix, its
+// complex64 factors and permutation, the iterate X (right-hand sides in,
+// solution out), the work vectors of the refinement (Xtmp, B, D, R), the
+// inverted diagonal blocks, the matvec workspace P, the residual norms and
+// the refinement state (converged flag, correction count)
+struct RefineMat {
+  const double2* M;
+  const float2* LU;
+  const int* perm;
+  double2* X;
+  double2* Xtmp;
+  double2* inv;
+  double2* B;
+  double2* D;
+  double2* R;
+  double2* P;
+  double* norms;
+  int* state;
+};
+struct RefineBatch {
+  RefineMat m[kLuBatchMax];
+};
 
 // GMRES(m) Hessenberg / Givens state on the device (krylov.hip)
 struct GmDev {
@@ -320,6 +342,11 @@ void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, cons
 // refinement state[0] = converged, state[1] = correction steps: reset, then
 // after each residual test max_q |R_q|_max / |X_q|_max <= tol
 void refine_reset(hipStream_t s, int* state);
+// The refined complex64 solve (initial solve, up to iters residual /
+// correction steps, final residual) of count <= kLuBatchMax systems of order
+// n at once; each system's X, R, norms and state are what the same steps
+// give it alone, bit for bit.
+void refine_solve_batch(hipStream_t s, const RefineBatch& b, int count, int n, int nrhs, int iters, double tol);
 void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state);
 void gm_start(hipStream_t s, const GmDev& g, int nrhs, const double* norms);
 void gm_restart(hipStream_t s, const GmDev& g, int nrhs, int m, const double* beta, double tol, double* scale,

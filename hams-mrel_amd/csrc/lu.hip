@@ -1025,14 +1025,15 @@ __global__ void k_to_c128(const float2* __restrict__ Af, double2* __restrict__ A
 // column chunk go to P and are added in a fixed order by k_matvec_reduce, so
 // the result does not depend on scheduling.
 constexpr int kMvRhs = 16, kMvChunk = 256;
-__global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restrict__ A, int n,
-                                                        const double2* __restrict__ X, int nrhs,
-                                                        double2* __restrict__ P, const int* __restrict__ skip) {
-  if (skip && *skip) return;
+// right-hand-side group qg of the pass (the kernels below pass blockIdx.z or
+// their share of it)
+__device__ __forceinline__ void matvec_partial_body(const double2* __restrict__ A, int n,
+                                                    const double2* __restrict__ X, int nrhs,
+                                                    double2* __restrict__ P, int qg) {
   __shared__ double2 red[3][64][kMvRhs];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + lane, ic = min(i, n - 1);
-  const int q0 = blockIdx.z * kMvRhs, nq = min(kMvRhs, nrhs - q0);
+  const int q0 = qg * kMvRhs, nq = min(kMvRhs, nrhs - q0);
   const int j0 = blockIdx.y * kMvChunk + w * (kMvChunk / 4);
   const int j1 = min(n, j0 + kMvChunk / 4);
   double ar[kMvRhs], ai[kMvRhs];
@@ -1065,11 +1066,16 @@ __global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restric
       }
   }
 }
-
-__global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n, int nrhs,
-                                const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y,
-                                const int* __restrict__ skip) {
+__global__ __launch_bounds__(256) void k_matvec_partial(const double2* __restrict__ A, int n,
+                                                        const double2* __restrict__ X, int nrhs,
+                                                        double2* __restrict__ P, const int* __restrict__ skip) {
   if (skip && *skip) return;
+  matvec_partial_body(A, n, X, nrhs, P, blockIdx.z);
+}
+
+__device__ __forceinline__ void matvec_reduce_body(const double2* __restrict__ P, int nchunk, int n, int nrhs,
+                                                   const double2* __restrict__ B, double alpha, double beta,
+                                                   double2* __restrict__ Y) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, tot = (size_t)n * nrhs;
   if (e >= tot) return;
   double sr = 0.0, si = 0.0;
@@ -1085,12 +1091,17 @@ __global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n
   }
   Y[e] = make_double2(yr, yi);
 }
-
-__global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms,
-                         const int* __restrict__ skip) {
+__global__ void k_matvec_reduce(const double2* __restrict__ P, int nchunk, int n, int nrhs,
+                                const double2* __restrict__ B, double alpha, double beta, double2* __restrict__ Y,
+                                const int* __restrict__ skip) {
   if (skip && *skip) return;
+  matvec_reduce_body(P, nchunk, n, nrhs, B, alpha, beta, Y);
+}
+
+__device__ __forceinline__ void colmax_body(const double2* __restrict__ V, int n, double* __restrict__ norms,
+                                            int q) {
   __shared__ double red[256];
-  const int q = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   double m = 0.0;
   for (int i = t; i < n; i += 256) {
     double2 v = V[(size_t)i + (size_t)n * q];
@@ -1103,6 +1114,11 @@ __global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double*
     __syncthreads();
   }
   if (t == 0) norms[q] = red[0];
+}
+__global__ void k_colmax(const double2* __restrict__ V, int n, int nrhs, double* __restrict__ norms,
+                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  colmax_body(V, n, norms, blockIdx.x);
 }
 
 __global__ void k_axpy(double2* __restrict__ X, const double2* __restrict__ D, size_t count,
@@ -1145,8 +1161,9 @@ __global__ void k_unpack_systems(const double2* __restrict__ P, int n, int nrhs,
 // set the converged flag when max over columns of |R|_max / |X|_max <= tol
 // (a column with X = 0 compares |R|_max itself), else count the correction
 // step that follows.
-__global__ void k_refine_check(const double* __restrict__ norms, int nrhs, double tol, int* __restrict__ state) {
-  if (threadIdx.x != 0 || state[0]) return;
+__device__ __forceinline__ void refine_check_body(const double* __restrict__ norms, int nrhs, double tol,
+                                                  int* __restrict__ state) {
+  if (state[0]) return;
   double worst = 0.0;
   for (int q = 0; q < nrhs; q++) {
     const double rn = norms[q], xn = norms[nrhs + q];
@@ -1156,6 +1173,83 @@ __global__ void k_refine_check(const double* __restrict__ norms, int nrhs, doubl
     state[0] = 1;
   else
     state[1] += 1;  // a correction step follows
+}
+__global__ void k_refine_check(const double* __restrict__ norms, int nrhs, double tol, int* __restrict__ state) {
+  if (threadIdx.x == 0) refine_check_body(norms, nrhs, tol, state);
+}
+
+// The refined complex64 solve of a batch of systems (blockIdx.z, or its
+// share of blockIdx.z, = system of b): the kernels of refine_solve's steps
+// on each system's own buffers, each system skipping what its own state
+// skips.  sel picks the vector a step works on: 0 the iterate X, 1 the
+// correction D; skip = 1 honours state[0] (converged).
+__device__ __forceinline__ double2* rb_vec(const RefineMat& m, int sel) { return sel ? m.D : m.X; }
+__device__ __forceinline__ bool rb_skip(const RefineMat& m, int skip) { return skip && m.state[0]; }
+__global__ void k_rb_start(const RefineBatch b, size_t count) {  // state reset, B = X
+  const RefineMat& m = b.m[blockIdx.z];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    m.state[0] = 0;
+    m.state[1] = 0;
+  }
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    m.B[i] = m.X[i];
+}
+__global__ __launch_bounds__(256) void k_rb_tri_inverse(const RefineBatch b, int n) {
+  const RefineMat& m = b.m[blockIdx.z];
+  tri_inverse_body<float>(m.LU, n, m.inv, blockIdx.x, blockIdx.y);
+}
+__global__ void k_rb_gather(const RefineBatch b, int n, int nrhs, int sel, int skip) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, skip)) return;
+  const double2* V = rb_vec(m, sel);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i < n && q < nrhs) m.Xtmp[(size_t)i + (size_t)n * q] = V[(size_t)m.perm[i] + (size_t)n * q];
+}
+template <bool LOWER>
+__global__ __launch_bounds__(256) void k_rb_trsm(const RefineBatch b, int n, int k0, int kb, int nrhs, int sel,
+                                                 int skip) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, skip)) return;
+  double2* V = rb_vec(m, sel);
+  trsm_super_body<float, LOWER>(m.LU, m.inv, n, k0, kb, LOWER ? m.Xtmp : V, LOWER ? V : m.Xtmp, nrhs, blockIdx.x);
+}
+__global__ void k_rb_copy_back(const RefineBatch b, size_t count, int sel, int skip) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, skip)) return;
+  double2* V = rb_vec(m, sel);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    V[i] = m.Xtmp[i];
+}
+__global__ __launch_bounds__(256) void k_rb_matvec_partial(const RefineBatch b, int n, int nrhs, int qgroups) {
+  const RefineMat& m = b.m[blockIdx.z / qgroups];
+  if (rb_skip(m, 1)) return;
+  matvec_partial_body(m.M, n, m.X, nrhs, m.P, blockIdx.z % qgroups);
+}
+// out: 0 the correction's right-hand side D, 1 the final residual R
+__global__ void k_rb_matvec_reduce(const RefineBatch b, int nchunk, int n, int nrhs, int out) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, 1)) return;
+  matvec_reduce_body(m.P, nchunk, n, nrhs, m.B, -1.0, 1.0, out ? m.R : m.D);
+}
+// which: 0 the residual just formed (out as above) into norms[0, nrhs), 1 X into norms[nrhs, 2 nrhs)
+__global__ void k_rb_colmax(const RefineBatch b, int n, int nrhs, int which, int out) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, 1)) return;
+  const double2* V = which ? m.X : (out ? m.R : m.D);
+  colmax_body(V, n, m.norms + (which ? nrhs : 0), blockIdx.x);
+}
+__global__ void k_rb_check(const RefineBatch b, int count, int nrhs, double tol) {
+  const int z = threadIdx.x;
+  if (z < count) refine_check_body(b.m[z].norms, nrhs, tol, b.m[z].state);
+}
+__global__ void k_rb_axpy(const RefineBatch b, size_t count) {
+  const RefineMat& m = b.m[blockIdx.z];
+  if (rb_skip(m, 1)) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    m.X[i].x += m.D[i].x;
+    m.X[i].y += m.D[i].y;
+  }
 }
 __global__ void k_refine_reset(int* __restrict__ state) {
   state[0] = 0;
@@ -1248,6 +1342,46 @@ void lu_solve_systems(hipStream_t s, int nsys, int nmax, const double2* LU, cons
   HX_LAUNCH(k_unpack_systems, g1, dim3(256), 0, s, Pt, n, nrhs, body, r0, Z);
 }
 void refine_reset(hipStream_t s, int* state) { HX_LAUNCH(k_refine_reset, dim3(1), dim3(1), 0, s, state); }
+void refine_solve_batch(hipStream_t s, const RefineBatch& b, int count, int n, int nrhs, int iters, double tol) {
+  if (count <= 0 || nrhs <= 0) return;
+  const size_t cnt = (size_t)n * nrhs;
+  const int groups = (nrhs + kTrsmRhs - 1) / kTrsmRhs;
+  HX_LAUNCH(k_rb_start, dim3(1024, 1, count), dim3(256), 0, s, b, cnt);
+  // the complex64 factors' inverted diagonal blocks, once for every solve below
+  HX_LAUNCH(k_rb_tri_inverse, dim3((n + kPanelNB - 1) / kPanelNB, 2, count), dim3(256), 0, s, b, n);
+  auto solve = [&](int sel, int skip) {
+    HX_LAUNCH(k_rb_gather, dim3((n + 255) / 256, nrhs, count), dim3(256), 0, s, b, n, nrhs, sel, skip);
+    for (int k0 = 0; k0 < n; k0 += kSuper) {
+      const int kb = n - k0 < kSuper ? n - k0 : kSuper;
+      const int rest = n - k0 - kb;
+      const int g = rest > 0 ? (rest + 255) / 256 : 1;
+      HX_LAUNCH(k_rb_trsm<true>, dim3(g, groups, count), dim3(256), 0, s, b, n, k0, kb, nrhs, sel, skip);
+    }
+    for (int k0 = ((n - 1) / kSuper) * kSuper; k0 >= 0; k0 -= kSuper) {
+      const int kb = n - k0 < kSuper ? n - k0 : kSuper;
+      const int g = k0 > 0 ? (k0 + 255) / 256 : 1;
+      HX_LAUNCH(k_rb_trsm<false>, dim3(g, groups, count), dim3(256), 0, s, b, n, k0, kb, nrhs, sel, skip);
+    }
+    HX_LAUNCH(k_rb_copy_back, dim3(1024, 1, count), dim3(256), 0, s, b, cnt, sel, skip);
+  };
+  const int nchunk = (n + kMvChunk - 1) / kMvChunk, qgroups = (nrhs + kMvRhs - 1) / kMvRhs;
+  auto residual = [&](int out) {
+    HX_LAUNCH(k_rb_matvec_partial, dim3((n + 63) / 64, nchunk, qgroups * count), dim3(256), 0, s, b, n, nrhs,
+              qgroups);
+    HX_LAUNCH(k_rb_matvec_reduce, dim3((unsigned)((cnt + 255) / 256), 1, count), dim3(256), 0, s, b, nchunk, n, nrhs,
+              out);
+    HX_LAUNCH(k_rb_colmax, dim3(nrhs, 1, count), dim3(256), 0, s, b, n, nrhs, 0, out);
+    HX_LAUNCH(k_rb_colmax, dim3(nrhs, 1, count), dim3(256), 0, s, b, n, nrhs, 1, out);
+  };
+  solve(0, 0);
+  for (int it = 0; it < iters; it++) {
+    residual(0);
+    HX_LAUNCH(k_rb_check, dim3(1), dim3(64), 0, s, b, count, nrhs, tol);
+    solve(1, 1);
+    HX_LAUNCH(k_rb_axpy, dim3(1024, 1, count), dim3(256), 0, s, b, cnt);
+  }
+  residual(1);
+}
 void refine_check(hipStream_t s, const double* norms, int nrhs, double tol, int* state) {
   HX_LAUNCH(k_refine_check, dim3(1), dim3(64), 0, s, norms, nrhs, tol, state);
 }
