@@ -484,6 +484,38 @@ def test_frequency_group_equals_single_solves(mesh_small, precision, outer, dept
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_frequency_group_with_mixed_heading_counts(mesh_small, precision):
+    """Contexts of one group with different heading counts (so different
+    right-hand-side counts) share the batched factorisation but solve on
+    their own streams instead of in the batched solve; each still gets its
+    own solve's coefficients bit for bit."""
+    m = mesh_small
+    heads = [np.deg2rad([0.0, 45.0]), np.deg2rad([0.0, 30.0, 90.0])]
+    ws = [0.6, 1.3]
+    say(f"[gpu] frequency group, heading counts 2 and 3, n={m.nelem} {precision}")
+    ref = []
+    for w, hd in zip(ws, heads):
+        with FrequencySolver(m, precision=precision) as s:
+            s.set_options(lu_outer_min_n=256)
+            s.set_waves(hd)
+            ref.append(s.solve_frequency(w, 40.0))
+    ctxs = [FrequencySolver(m, precision=precision) for _ in ws]
+    try:
+        for c, hd in zip(ctxs, heads):
+            c.set_options(lu_outer_min_n=256)
+            c.set_waves(hd)
+        FrequencySolver.submit_group(ctxs, ws, 40.0)
+        got = [c.collect() for c in ctxs]
+    finally:
+        for c in ctxs:
+            c.close()
+    for r, g in zip(ref, got):
+        assert r[3] == 0
+        for a, b in zip(r, g):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_frequency_group_default_outer_blocks(precision):
     """With the LU options left alone a group of two or more matrices factors
     in outer blocks from order 1024 on (single solves from 6144 on): at
