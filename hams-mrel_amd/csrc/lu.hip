@@ -95,6 +95,49 @@ constexpr int kOuter = 4 * kPanelNB;  // LU outer block: four panels share one f
 // 5.1 solves/s (the far trailing matrix is read and written once per block).
 constexpr int kOuterMinN = 6144;
 static_assert((kPanelNB & (kPanelNB - 1)) == 0, "the rotating window indexes modulo NB");
+// Turn the final pivot-row list (winners[c] = the row chosen at step c)
+// into LAPACK-style sequential swaps ipiv[col0 + c] (1-based); one thread.
+__device__ __forceinline__ void pivots_to_swaps(const int* winners, int* __restrict__ ipiv, int nsel, int col0) {
+  int who[2 * kPanelNB], pos[2 * kPanelNB], nt = 0;  // rows whose position changed
+  for (int c = 0; c < nsel; c++) {
+    int w = winners[c], p = w;
+    for (int q = 0; q < nt; q++)
+      if (who[q] == w) {
+        p = pos[q];
+        break;
+      }
+    const int dst = col0 + c;
+    ipiv[dst] = p + 1;
+    if (p != dst) {
+      // row currently at dst moves to p
+      int rowat = dst;
+      for (int q = 0; q < nt; q++)
+        if (pos[q] == dst) {
+          rowat = who[q];
+          break;
+        }
+      bool f1 = false, f2 = false;
+      for (int q = 0; q < nt; q++) {
+        if (who[q] == w) {
+          pos[q] = dst;
+          f1 = true;
+        } else if (who[q] == rowat) {
+          pos[q] = p;
+          f2 = true;
+        }
+      }
+      if (!f1) {
+        who[nt] = w;
+        pos[nt++] = dst;
+      }
+      if (!f2) {
+        who[nt] = rowat;
+        pos[nt++] = p;
+      }
+    }
+  }
+}
+
 // Batched: matrix blockIdx.y of bat; the tournament's candidate lists
 // alternate between two buffers of wsets sets in the matrix's workspace
 // (in_buf < 0: level 0, rows taken from the matrix itself).
@@ -218,6 +261,11 @@ __global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int c
   if (t < nsel) idx_out[b * kPanelNB + t] = sidx[sel[t]];
   if (t == 0) cnt_out[b] = nsel;
   if (FINAL) {
+    // the swaps of this panel (the final level is one block per matrix)
+    __shared__ int win[kPanelNB];
+    if (t < nsel) win[t] = sidx[sel[t]];
+    __syncthreads();
+    if (t == 0) pivots_to_swaps(win, mat.ipiv, nsel, col0);
     // the pivot row of step s: the multipliers of steps 0..s-1 in slots
     // 0..s-1 (row s of L) and row s of U in slots s..
     if (mystep >= 0)
@@ -225,53 +273,6 @@ __global__ __launch_bounds__(ROWS) void k_select(const LuBatch bat, int n, int c
       for (int k = 0; k < kPanelNB; k++)
         if (k < ncols) diag[mystep + kPanelNB * k] = {(R)vr[k], (R)vi[k]};
     if (t == 0 && zero_at && info[0] == 0) info[0] = col0 + zero_at;
-  }
-}
-
-// Turn the final pivot-row list into LAPACK-style sequential swaps
-// ipiv[col0 + c] (1-based) and record the first zero pivot in info.
-__global__ void k_pivots_to_swaps(const LuBatch bat, int buf, int wsets, int nsel, int col0) {
-  if (threadIdx.x != 0) return;
-  const LuMat& mat = bat.m[blockIdx.y];
-  const int* __restrict__ winners = mat.widx + (size_t)buf * wsets * kPanelNB;
-  int* __restrict__ ipiv = mat.ipiv;
-  int who[2 * kPanelNB], pos[2 * kPanelNB], nt = 0;  // rows whose position changed
-  for (int c = 0; c < nsel; c++) {
-    int w = winners[c], p = w;
-    for (int q = 0; q < nt; q++)
-      if (who[q] == w) {
-        p = pos[q];
-        break;
-      }
-    const int dst = col0 + c;
-    ipiv[dst] = p + 1;
-    if (p != dst) {
-      // row currently at dst moves to p
-      int rowat = dst;
-      for (int q = 0; q < nt; q++)
-        if (pos[q] == dst) {
-          rowat = who[q];
-          break;
-        }
-      bool f1 = false, f2 = false;
-      for (int q = 0; q < nt; q++) {
-        if (who[q] == w) {
-          pos[q] = dst;
-          f1 = true;
-        } else if (who[q] == rowat) {
-          pos[q] = p;
-          f2 = true;
-        }
-      }
-      if (!f1) {
-        who[nt] = w;
-        pos[nt++] = dst;
-      }
-      if (!f2) {
-        who[nt] = rowat;
-        pos[nt++] = p;
-      }
-    }
   }
 }
 
@@ -566,7 +567,6 @@ static void lu_factor_t(hipStream_t s, const LuBatch& bat, int nbat, int n, cons
       cur = 1 - cur;
       nsets = nout;
     }
-    HX_LAUNCH(k_pivots_to_swaps, dim3(1, nbat), dim3(64), 0, s, bat, cur, nsets0, nb, col0);
     // Outer blocks of kOuter columns: within the block each panel updates only
     // the block's columns; at the block's end the columns to its right get
     // U = L11^-1 A12 for all of the block's rows and one rank-kOuter update,
